@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""fwd+bwd ms/iter of the equirectangular Gaussian rasterizer on one MI355X.
+
+Workload: the 1.5M-Gaussian room scene at 1920x960 (FP32, SH degree 3, seeded;
+omnigs_amd.scenes.room_scene).  One step = forward (preprocess, binning,
+render) + the L1 upstream gradient sign(img - target)/(3HW) + backward
+(render backward, preprocess backward), all through the C ABI with inputs
+resident in HBM.  Prints ONE JSON line.
+
+The roofline object prices the dominant stage (by measured time) against
+HBM: achieved = its algorithmic bytes per launch / its average launch time
+(HIP events recorded around each stage on the launch stream during the
+timed region).  The CPU baseline runs the oracle (FP32, one thread) over one
+full fwd+bwd iteration of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes):
+    """Compulsory HBM bytes per launch of each stage (every dataflow tensor
+    read or written once; gathers counted once per instance)."""
+    return {
+        # means 12 + scales 12 + rotations 16 + opacity 4 + SH 192 read;
+        # radii/tiles/key 12 for all, xy 8 + conic 16 + rgb 12 + clamped 1 for visible
+        "preprocess": 236 * P + 12 * P + 37 * Pv,
+        # per pass: key read twice (count + scatter), value read, key+value written
+        "depth_sort": depth_passes * 20 * P,
+        # tiles_touched gathered through the order twice, offsets written
+        "scan": 20 * P,
+        # order, offsets, xy, radii read; tile key + id written per instance
+        "duplicate": 20 * P + 8 * R,
+        "tile_sort": tile_passes * 20 * R,
+        "ranges": 4 * R + 8 * tiles,
+        # per instance: id 4 + xy 8 + conic 16 + rgb 12; per pixel 3 colours + T + n_contrib written
+        "render": 40 * R + 8 * tiles + 20 * N,
+        "clear_grads": 44 * P,
+        # per instance gather 40; per pixel T, n_contrib, dL/dpix read; 9 gradient floats per visible Gaussian
+        "render_backward": 40 * R + 8 * tiles + 20 * N + 36 * Pv,
+        # read: means, scales, rot, SH, radii, clamped, dmean2D, dconic, dcolor (277 B);
+        # write: dmeans, dcov3D, dSH, dscale, drot (256 B)
+        "preprocess_backward": 533 * P,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--gaussians", type=int, default=1_500_000)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
+                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from omnigs_amd import capi, scenes
+
+    dev = torch.device("cuda:0")
+    sc = scenes.room_scene(P=args.gaussians)
+    P, W, H = sc["means3D"].shape[0], sc["width"], sc["height"]
+    N = W * H
+    tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    ds = {k: (torch.from_numpy(v).to(dev) if isinstance(v, np.ndarray) else v) for k, v in sc.items()}
+    ds["opacities"] = ds["opacities"].reshape(-1).contiguous()
+    target = ds["target"]
+    bufs = (capi.TorchBuffer(dev), capi.TorchBuffer(dev), capi.TorchBuffer(dev))
+    inv = 1.0 / (3.0 * H * W)
+
+    def step():
+        R, img, radii, _ = capi.forward_scene(ds, buffers=bufs)
+        dL = torch.sign(img - target).mul_(inv)
+        g = capi.backward_scene(ds, R, radii, bufs, dL)
+        return R, radii, g
+
+    for _ in range(args.warmup):
+        R, radii, _ = step()
+    torch.cuda.synchronize()
+    Pv = int((radii > 0).sum())
+
+    capi.profile_read()  # drop warm-up events
+    capi.profile_enable(True)
+    stream = torch.cuda.current_stream(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        evs[i][0].record(stream)
+        R, radii, g = step()
+        evs[i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    capi.profile_enable(False)
+    stages = capi.profile_read()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    step_ms = sorted(a.elapsed_time(b) for a, b in evs)
+    median_ms = statistics.median(step_ms)
+
+    depth_passes = 4
+    tile_passes = (int(tiles).bit_length() + 7) // 8
+    alg = algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes)
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items()}
+    dominant = max(stage_ms, key=lambda k: stage_ms[k])
+    dom_ms = stage_ms[dominant]
+    achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
+    whole_bytes = 848 * P + 124 * R + 40 * N
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dominant)
+
+    out = {
+        "metric": "fwd+bwd ms/iter, 1.5M Gaussians @1920x960 lonlat (plain SH path)",
+        "value": round(median_ms, 4),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded room scene; random-init Gaussians)",
+        "config": {"workload": "plain SH fwd+bwd, 1.5M Gaussians, 1920x960 equirectangular, tile 16x16",
+                   "gaussians": P, "width": W, "height": H, "num_rendered": R, "visible": Pv,
+                   "sh_degree": 3},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": alg[dominant], "avg_launch_ms": round(dom_ms, 5)},
+        "whole_path": {"algorithmic_bytes": whole_bytes,
+                       "achieved_GBps": round(whole_bytes / (median_ms * 1e-3) / 1e9, 2),
+                       "frac_of_hbm_roofline": round(whole_bytes / (median_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+    }
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(out))
+    return 0
+
+
+def cpu_baseline(sc) -> dict:
+    """The FP32 oracle on one host thread over one full fwd+bwd iteration of
+    the same workload (bounded: ~30 s on a 2.7 GHz Xeon)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import Oracle
+
+    o = Oracle({k: v for k, v in sc.items() if k != "target"}, 32)
+    t = time.perf_counter()
+    o.forward()
+    img = o.get("out_color").reshape(3, sc["height"], sc["width"])
+    dL = (np.sign(img - sc["target"]) / (3.0 * sc["height"] * sc["width"])).astype(np.float32)
+    o.backward(dL)
+    dt = time.perf_counter() - t
+    return {"value": round(dt * 1000.0, 1), "unit": "ms/iter", "cores": 1, "kind": "port",
+            "sample": "one full fwd+bwd iteration (1.5M Gaussians, 1920x960) of the FP32 oracle, single thread"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

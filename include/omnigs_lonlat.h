@@ -1,0 +1,192 @@
+/*
+ * C ABI of the MI355X (gfx950) equirectangular Gaussian rasterizer.
+ *
+ * Plain pointers and sizes only.  Every float pointer is DEVICE memory (FP32,
+ * C-contiguous), exactly as the reference passes tensor data_ptr()s into
+ * CudaRasterizer::LonlatRasterizer.  NULL marks an absent optional input,
+ * as an empty tensor's data_ptr() does in the reference.
+ *
+ * The entry points replace, one for one:
+ *   omnigs_lonlat_forward      <- LonlatRasterizer::forward
+ *                                 (cuda_rasterizer/rasterizer.h:103-122,
+ *                                  rasterizer_impl.cu:540-697)
+ *   omnigs_lonlat_backward     <- LonlatRasterizer::backward
+ *                                 (cuda_rasterizer/rasterizer.h:124-154,
+ *                                  rasterizer_impl.cu:701-795)
+ *   omnigs_lonlat_mark_visible <- LonlatRasterizer::markVisible
+ *                                 (cuda_rasterizer/rasterizer.h:99-101,
+ *                                  rasterizer_impl.cu:185-192)
+ * The three scratch buffers are grown through caller callbacks, the C form of
+ * the reference's std::function<char*(size_t)> resize functors
+ * (src/rasterize_points.cu:41-47).  Their contents are private to this
+ * library; the backward must receive the same three buffers and the R the
+ * forward returned.
+ *
+ * All work is enqueued on `stream` (a hipStream_t; NULL = the default
+ * stream).  The forward synchronises `stream` once, to read the number of
+ * rendered instances R back to the host, as the reference does
+ * (rasterizer_impl.cu:627-628).  The backward does not synchronise.
+ */
+#ifndef OMNIGS_LONLAT_H
+#define OMNIGS_LONLAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The LibTorch boundary maps them to the reference's
+ * exceptions (src/rasterize_points.cu:72-75, :158-161; rasterizer_impl.cu:585-588). */
+enum omnigs_status {
+  OMNIGS_OK = 0,
+  OMNIGS_ERR_INVALID_ARGUMENT = 1, /* bad sizes, missing required pointer, unsupported tile shape */
+  OMNIGS_ERR_NEED_COLORS = 2,      /* NUM_CHANNELS != 3 without precomputed colours (kept for parity) */
+  OMNIGS_ERR_ALLOC = 3,            /* a buffer callback returned NULL */
+  OMNIGS_ERR_HIP = 4               /* a HIP runtime call or launch failed */
+};
+
+/* Grows a scratch buffer to at least `nbytes` and returns its device address
+ * (NULL on failure).  `ctx` is passed through unchanged. */
+typedef char* (*omnigs_resize_fn)(void* ctx, size_t nbytes);
+
+typedef struct omnigs_buffer {
+  omnigs_resize_fn resize;
+  void* ctx;
+} omnigs_buffer;
+
+typedef struct omnigs_lonlat_forward_args {
+  int P;                       /* number of Gaussians */
+  int D;                       /* active SH degree (0..3) */
+  int M;                       /* SH coefficients per Gaussian (shs is [P,M,3]) */
+  int width, height;           /* equirectangular image size in pixels */
+  const float* background;     /* [3] */
+  const float* means3D;        /* [P,3] world positions */
+  const float* shs;            /* [P,M,3] or NULL (then colors_precomp is used) */
+  const float* colors_precomp; /* [P,3] or NULL */
+  const float* opacities;      /* [P] (activated) */
+  const float* scales;         /* [P,3] (activated) or NULL */
+  float scale_modifier;
+  const float* rotations;      /* [P,4] (w,x,y,z), used as given */
+  const float* cov3D_precomp;  /* [P,6] upper triangle or NULL */
+  const float* viewmatrix;     /* [16] Tcw^T row-major (= Tcw column-major) */
+  const float* campos;         /* [3] camera centre in world coordinates */
+  int prefiltered;             /* accepted and ignored, as in the reference's lonlat path */
+  float* out_color;            /* [3,H,W] output image (planar CHW) */
+  int* radii;                  /* [P] output radii in pixels, or NULL (kept internally) */
+} omnigs_lonlat_forward_args;
+
+typedef struct omnigs_lonlat_backward_args {
+  int P, D, M, R;              /* R = value returned by the forward */
+  int width, height;
+  const float* background;     /* [3] */
+  const float* means3D;        /* [P,3] */
+  const float* shs;            /* [P,M,3] or NULL */
+  const float* colors_precomp; /* [P,3] or NULL */
+  const float* scales;         /* [P,3] or NULL */
+  float scale_modifier;
+  const float* rotations;      /* [P,4] or NULL */
+  const float* cov3D_precomp;  /* [P,6] or NULL */
+  const float* viewmatrix;     /* [16] */
+  const float* campos;         /* [3] */
+  const int* radii;            /* [P] radii from the forward, or NULL (internal copy) */
+  char* geom_buffer;           /* the forward's three scratch buffers */
+  char* binning_buffer;
+  char* image_buffer;
+  const float* dL_dpix;        /* [3,H,W] upstream gradient of out_color */
+  /* Outputs.  The first four are accumulated by the render backward and are
+   * cleared by this call first; the rest are fully written (zero for
+   * Gaussians with radius 0). */
+  float* dL_dmean2D;           /* [P,3] NDC units (x 0.5W, 0.5H), z unused */
+  float* dL_dconic;            /* [P,4] (2x2 row-major; slots 0,1,3 used, .y holds half the off-diagonal) */
+  float* dL_dopacity;          /* [P] */
+  float* dL_dcolor;            /* [P,3] */
+  float* dL_dmean3D;           /* [P,3] */
+  float* dL_dcov3D;            /* [P,6] */
+  float* dL_dsh;               /* [P,M,3] or NULL when shs is NULL */
+  float* dL_dscale;            /* [P,3] or NULL when scales is NULL */
+  float* dL_drot;              /* [P,4] or NULL when scales is NULL */
+  float* dpx_dt;               /* [P,3] optional: d(px)/d(t) rows of the lon/lat Jacobian */
+  float* dpy_dt;               /* [P,3] optional */
+} omnigs_lonlat_backward_args;
+
+/* Forward: preprocess, binning, render.  On success *num_rendered = R. */
+int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* args, omnigs_buffer geometry_buffer,
+                          omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream,
+                          int* num_rendered);
+
+/* Backward: render backward, then preprocess backward (covariance, 2D mean,
+ * SH and scale/rotation terms in one kernel). */
+int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* args, void* stream);
+
+/* markVisible for camera_type 3: every Gaussian is visible (present[i] = 1). */
+int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream);
+
+/* Byte sizes the forward requests from each buffer callback. */
+size_t omnigs_lonlat_geometry_bytes(int P);
+size_t omnigs_lonlat_binning_bytes(int R);
+size_t omnigs_lonlat_image_bytes(int width, int height);
+
+/* Device addresses of the arrays inside the three private buffers, for
+ * inspection (parity tests compare them with a CPU restatement stage by
+ * stage).  Has no counterpart in the reference, whose buffer carving
+ * (rasterizer_impl.h:48-102) is not exported.  Any of the three buffer
+ * pointers may be NULL; the matching fields are then NULL. */
+typedef struct omnigs_lonlat_state_view {
+  /* geometry buffer, per Gaussian */
+  const uint32_t* depth_key;     /* float bits of |t|; 0xFFFFFFFF when culled */
+  const int32_t* radii;          /* internal radii (written when the forward got radii == NULL) */
+  const float* means2D;          /* [P,2] pixel coordinates */
+  const float* conic_opacity;    /* [P,4] */
+  const float* rgb;              /* [P,3] */
+  const uint8_t* clamped;        /* [P] bit c = channel c clamped */
+  const uint32_t* tiles_touched; /* [P] */
+  const uint32_t* depth_order;   /* [P] Gaussian ids by (depth bits, id) */
+  const uint32_t* offsets;       /* [P] inclusive instance offsets in depth order */
+  /* binning buffer, per instance */
+  const uint32_t* point_list;    /* [R] Gaussian id per sorted instance */
+  const uint32_t* tile_keys;     /* [R] tile id per sorted instance */
+  /* image buffer */
+  const float* final_T;          /* [H,W] */
+  const uint32_t* n_contrib;     /* [H,W] */
+  const uint32_t* ranges;        /* [tiles,2] */
+} omnigs_lonlat_state_view;
+
+int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, const char* image_buffer, int P, int R,
+                        int width, int height, omnigs_lonlat_state_view* out);
+
+/* Stage timers: when enabled, every stage of the forward and backward is
+ * bracketed by HIP events recorded on the launch stream.  read() waits for
+ * the recorded events, adds each stage's elapsed milliseconds into
+ * ms[stage] and count[stage] (arrays of OMNIGS_NUM_STAGES), and clears them.
+ * Measurement support only; no counterpart in the reference. */
+enum omnigs_stage {
+  OMNIGS_STAGE_PREPROCESS = 0,
+  OMNIGS_STAGE_DEPTH_SORT = 1,
+  OMNIGS_STAGE_SCAN = 2,
+  OMNIGS_STAGE_DUPLICATE = 3,
+  OMNIGS_STAGE_TILE_SORT = 4,
+  OMNIGS_STAGE_RANGES = 5,
+  OMNIGS_STAGE_RENDER = 6,
+  OMNIGS_STAGE_CLEAR_GRADS = 7,
+  OMNIGS_STAGE_RENDER_BACKWARD = 8,
+  OMNIGS_STAGE_PREPROCESS_BACKWARD = 9,
+  OMNIGS_NUM_STAGES = 10
+};
+int omnigs_profile_enable(int on);
+int omnigs_profile_read(double* ms, int* count);
+const char* omnigs_stage_name(int stage);
+
+/* Human-readable text of a status code, and of the last HIP error seen. */
+const char* omnigs_status_string(int status);
+const char* omnigs_last_error(void);
+
+/* Library version string. */
+const char* omnigs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OMNIGS_LONLAT_H */

@@ -1,0 +1,107 @@
+"""Builds the native parts in-tree (no JIT cache):
+
+* ``omnigs_amd/lib/libomnigs_lonlat.so`` -- the gfx950 HIP kernels behind the
+  C ABI of ``include/omnigs_lonlat.h`` (hipcc, one object per stage).
+* ``omnigs_amd/lib/_omnigs_torch*.so`` -- the LibTorch-ROCm boundary
+  (RasterizeGaussiansCUDA / RasterizeGaussiansBackwardCUDA / markVisible and
+  the autograd Function), a C++ extension that calls the C ABI.
+
+Usage: ``python -m omnigs_amd.build [--hip-only]``.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+OBJDIR = os.path.join(ROOT, "_build", "obj")
+HIP_LIB = os.path.join(LIBDIR, "libomnigs_lonlat.so")
+
+HIP_SOURCES = ["preprocess.hip", "sort.hip", "render.hip", "preprocess_backward.hip", "capi.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed-FP32 VALU ops are kept out of every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"command failed ({r.returncode}): {cmd[0]} ... {cmd[-1]}")
+
+
+def build_hip(jobs: int = 8) -> str:
+    os.makedirs(OBJDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+    tmp = HIP_LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_torch_ext() -> str | None:
+    """The LibTorch boundary extension (C++, g++), linked against the HIP library."""
+    src = os.path.join(HERE, "torch_ext", "rasterize_points.cpp")
+    if not os.path.exists(src):
+        return None
+    import torch  # noqa: F401  (paths only)
+    from torch.utils import cpp_extension as ce
+
+    ext_suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(LIBDIR, "_omnigs_torch" + ext_suffix)
+    torch_lib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    inc = ce.include_paths() + [sysconfig.get_paths()["include"], os.path.join(ROOT, "include"), "/opt/rocm/include"]
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    flags = ["-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+             "-DTORCH_EXTENSION_NAME=_omnigs_torch", "-DTORCH_API_INCLUDE_EXTENSION_H", "-D__HIP_PLATFORM_AMD__=1",
+             "-DUSE_ROCM=1", "-Wno-unused-variable"]
+    srcs = [os.path.join(HERE, "torch_ext", f) for f in ("rasterize_points.cpp", "gaussian_rasterizer.cpp",
+                                                          "bindings.cpp")
+            if os.path.exists(os.path.join(HERE, "torch_ext", f))]
+    cmd = [os.environ.get("CXX", "g++"), *flags, *[f"-I{p}" for p in inc], *srcs, "-o", out + ".tmp",
+           f"-L{torch_lib}", f"-L{LIBDIR}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", "-lc10_hip",
+           "-ltorch_hip", "-lomnigs_lonlat", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{torch_lib}"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def clean() -> None:
+    shutil.rmtree(os.path.join(ROOT, "_build"), ignore_errors=True)
+    if os.path.isdir(LIBDIR):
+        for f in os.listdir(LIBDIR):
+            if f.endswith(".so"):
+                os.remove(os.path.join(LIBDIR, f))
+
+
+def build_all(hip_only: bool = False) -> None:
+    clean()
+    build_hip()
+    if not hip_only:
+        build_torch_ext()
+
+
+if __name__ == "__main__":
+    build_all(hip_only="--hip-only" in sys.argv)
+    print("built:", ", ".join(sorted(os.listdir(LIBDIR))))

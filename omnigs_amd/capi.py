@@ -1,0 +1,299 @@
+"""ctypes binding of the C ABI in include/omnigs_lonlat.h.
+
+Device memory comes from torch (ROCm) tensors; the library itself is plain
+HIP.  ``import torch`` happens before the library is loaded so that one HIP
+runtime (torch's) serves both.  There is no fallback: if the library is
+missing the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libomnigs_lonlat.so")
+
+OMNIGS_OK = 0
+OMNIGS_ERR_INVALID_ARGUMENT = 1
+OMNIGS_ERR_NEED_COLORS = 2
+OMNIGS_ERR_ALLOC = 3
+OMNIGS_ERR_HIP = 4
+
+EXPORTED_SYMBOLS = (
+    "omnigs_lonlat_forward", "omnigs_lonlat_backward", "omnigs_lonlat_mark_visible",
+    "omnigs_lonlat_geometry_bytes", "omnigs_lonlat_binning_bytes", "omnigs_lonlat_image_bytes",
+    "omnigs_lonlat_state", "omnigs_profile_enable", "omnigs_profile_read", "omnigs_stage_name",
+    "omnigs_status_string", "omnigs_last_error", "omnigs_version",
+)
+NUM_STAGES = 10
+
+_STATE_FIELDS = ("depth_key", "radii", "means2D", "conic_opacity", "rgb", "clamped", "tiles_touched",
+                 "depth_order", "offsets", "point_list", "tile_keys", "final_T", "n_contrib", "ranges")
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+RESIZE_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+
+
+class Buffer(ctypes.Structure):
+    _fields_ = [("resize", RESIZE_FN), ("ctx", ctypes.c_void_p)]
+
+
+class ForwardArgs(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int), ("D", ctypes.c_int), ("M", ctypes.c_int),
+        ("width", ctypes.c_int), ("height", ctypes.c_int),
+        ("background", ctypes.c_void_p), ("means3D", ctypes.c_void_p), ("shs", ctypes.c_void_p),
+        ("colors_precomp", ctypes.c_void_p), ("opacities", ctypes.c_void_p), ("scales", ctypes.c_void_p),
+        ("scale_modifier", ctypes.c_float), ("rotations", ctypes.c_void_p), ("cov3D_precomp", ctypes.c_void_p),
+        ("viewmatrix", ctypes.c_void_p), ("campos", ctypes.c_void_p), ("prefiltered", ctypes.c_int),
+        ("out_color", ctypes.c_void_p), ("radii", ctypes.c_void_p),
+    ]
+
+
+class BackwardArgs(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int), ("D", ctypes.c_int), ("M", ctypes.c_int), ("R", ctypes.c_int),
+        ("width", ctypes.c_int), ("height", ctypes.c_int),
+        ("background", ctypes.c_void_p), ("means3D", ctypes.c_void_p), ("shs", ctypes.c_void_p),
+        ("colors_precomp", ctypes.c_void_p), ("scales", ctypes.c_void_p), ("scale_modifier", ctypes.c_float),
+        ("rotations", ctypes.c_void_p), ("cov3D_precomp", ctypes.c_void_p), ("viewmatrix", ctypes.c_void_p),
+        ("campos", ctypes.c_void_p), ("radii", ctypes.c_void_p),
+        ("geom_buffer", ctypes.c_void_p), ("binning_buffer", ctypes.c_void_p), ("image_buffer", ctypes.c_void_p),
+        ("dL_dpix", ctypes.c_void_p),
+        ("dL_dmean2D", ctypes.c_void_p), ("dL_dconic", ctypes.c_void_p), ("dL_dopacity", ctypes.c_void_p),
+        ("dL_dcolor", ctypes.c_void_p), ("dL_dmean3D", ctypes.c_void_p), ("dL_dcov3D", ctypes.c_void_p),
+        ("dL_dsh", ctypes.c_void_p), ("dL_dscale", ctypes.c_void_p), ("dL_drot", ctypes.c_void_p),
+        ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p),
+    ]
+
+
+class StateView(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in _STATE_FIELDS]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libomnigs_lonlat.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m omnigs_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.omnigs_lonlat_forward.argtypes = [ctypes.POINTER(ForwardArgs), Buffer, Buffer, Buffer, ctypes.c_void_p,
+                                          _ip]
+    lib.omnigs_lonlat_forward.restype = ctypes.c_int
+    lib.omnigs_lonlat_backward.argtypes = [ctypes.POINTER(BackwardArgs), ctypes.c_void_p]
+    lib.omnigs_lonlat_backward.restype = ctypes.c_int
+    lib.omnigs_lonlat_mark_visible.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.omnigs_lonlat_mark_visible.restype = ctypes.c_int
+    for name in ("omnigs_lonlat_geometry_bytes", "omnigs_lonlat_binning_bytes"):
+        getattr(lib, name).argtypes = [ctypes.c_int]
+        getattr(lib, name).restype = ctypes.c_size_t
+    lib.omnigs_lonlat_image_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.omnigs_lonlat_image_bytes.restype = ctypes.c_size_t
+    lib.omnigs_lonlat_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(StateView)]
+    lib.omnigs_lonlat_state.restype = ctypes.c_int
+    lib.omnigs_profile_enable.argtypes = [ctypes.c_int]
+    lib.omnigs_profile_enable.restype = ctypes.c_int
+    lib.omnigs_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    lib.omnigs_profile_read.restype = ctypes.c_int
+    lib.omnigs_stage_name.argtypes = [ctypes.c_int]
+    lib.omnigs_stage_name.restype = ctypes.c_char_p
+    lib.omnigs_status_string.argtypes = [ctypes.c_int]
+    lib.omnigs_status_string.restype = ctypes.c_char_p
+    lib.omnigs_last_error.restype = ctypes.c_char_p
+    lib.omnigs_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+class OmnigsError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        lib = load()
+        msg = lib.omnigs_status_string(status).decode()
+        detail = lib.omnigs_last_error().decode()
+        super().__init__(f"{where}: {msg} ({detail})")
+        self.status = status
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None or t.numel() == 0:
+        return None
+    assert t.is_contiguous(), "tensors passed to the C ABI must be contiguous"
+    return t.data_ptr()
+
+
+class TorchBuffer:
+    """A uint8 device tensor grown by the library through the resize callback
+    (the C form of the reference's resizeFunctional, rasterize_points.cu:41-47)."""
+
+    def __init__(self, device: torch.device):
+        self.device = device
+        self.tensor = torch.empty(0, dtype=torch.uint8, device=device)
+
+        def _resize(_ctx, nbytes):
+            try:
+                if self.tensor.numel() < nbytes:
+                    self.tensor = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+                return self.tensor.data_ptr() if nbytes else self.tensor.data_ptr() or None
+            except Exception:  # allocation failure surfaces as OMNIGS_ERR_ALLOC
+                return None
+
+        self._cb = RESIZE_FN(_resize)
+        self.c = Buffer(self._cb, None)
+
+
+def stream_handle(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def forward(*, background, means3D, opacities, viewmatrix, campos, width, height, shs=None, colors_precomp=None,
+            scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0, prefiltered=False,
+            buffers=None):
+    """Runs omnigs_lonlat_forward.  Returns (R, out_color [3,H,W], radii [P], (geom, binning, image))."""
+    lib = load()
+    dev = means3D.device
+    P = means3D.shape[0]
+    M = shs.shape[1] if (shs is not None and shs.numel()) else 0
+    # every pixel and every radius is written when P > 0; P == 0 leaves zeros (reference behaviour)
+    alloc = torch.empty if P > 0 else torch.zeros
+    out = alloc((3, height, width), dtype=torch.float32, device=dev)
+    radii = alloc((P,), dtype=torch.int32, device=dev)
+    bufs = buffers or (TorchBuffer(dev), TorchBuffer(dev), TorchBuffer(dev))
+    a = ForwardArgs(P, degree, M, width, height, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
+                    _ptr(opacities), _ptr(scales), float(scale_modifier), _ptr(rotations), _ptr(cov3D_precomp),
+                    _ptr(viewmatrix), _ptr(campos), int(prefiltered), _ptr(out), _ptr(radii))
+    R = ctypes.c_int(0)
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), bufs[0].c, bufs[1].c, bufs[2].c, stream_handle(dev),
+                                   ctypes.byref(R))
+    if st != OMNIGS_OK:
+        raise OmnigsError(st, "omnigs_lonlat_forward")
+    return R.value, out, radii, bufs
+
+
+def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buffers, shs=None,
+             colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0,
+             want_jacobian=False):
+    """Runs omnigs_lonlat_backward; returns a dict of gradient tensors."""
+    lib = load()
+    dev = means3D.device
+    P = means3D.shape[0]
+    H, W = dL_dpix.shape[1], dL_dpix.shape[2]
+    M = shs.shape[1] if (shs is not None and shs.numel()) else 0
+    f = dict(dtype=torch.float32, device=dev)
+    g = {
+        "dL_dmeans2D": torch.empty((P, 3), **f), "dL_dconic": torch.empty((P, 2, 2), **f),
+        "dL_dopacity": torch.empty((P, 1), **f), "dL_dcolors": torch.empty((P, 3), **f),
+        "dL_dmeans3D": torch.empty((P, 3), **f), "dL_dcov3D": torch.empty((P, 6), **f),
+        # written in full by the kernels when their inputs are given, zeros otherwise
+        "dL_dsh": (torch.empty if shs is not None else torch.zeros)((P, M, 3), **f),
+        "dL_dscales": (torch.empty if scales is not None else torch.zeros)((P, 3), **f),
+        "dL_drotations": (torch.empty if scales is not None else torch.zeros)((P, 4), **f),
+    }
+    if want_jacobian:
+        g["dpx_dt"] = torch.empty((P, 3), **f)
+        g["dpy_dt"] = torch.empty((P, 3), **f)
+    geom, binning, image = buffers
+    a = BackwardArgs(P, degree, M, R, W, H, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
+                     _ptr(scales), float(scale_modifier), _ptr(rotations), _ptr(cov3D_precomp), _ptr(viewmatrix),
+                     _ptr(campos), _ptr(radii), geom.tensor.data_ptr(),
+                     binning.tensor.data_ptr() if binning.tensor.numel() else None, image.tensor.data_ptr(),
+                     _ptr(dL_dpix.contiguous()), _ptr(g["dL_dmeans2D"]), _ptr(g["dL_dconic"]),
+                     _ptr(g["dL_dopacity"]), _ptr(g["dL_dcolors"]), _ptr(g["dL_dmeans3D"]), _ptr(g["dL_dcov3D"]),
+                     _ptr(g["dL_dsh"]) if shs is not None else None,
+                     _ptr(g["dL_dscales"]) if scales is not None else None,
+                     _ptr(g["dL_drotations"]) if scales is not None else None,
+                     _ptr(g.get("dpx_dt")), _ptr(g.get("dpy_dt")))
+    st = lib.omnigs_lonlat_backward(ctypes.byref(a), stream_handle(dev))
+    if st != OMNIGS_OK:
+        raise OmnigsError(st, "omnigs_lonlat_backward")
+    return g
+
+
+def state(buffers, P: int, R: int, width: int, height: int) -> dict:
+    """Views (torch tensors aliasing the private buffers) of the forward's
+    internal arrays, decoded with omnigs_lonlat_state."""
+    lib = load()
+    geom, binning, image = buffers
+    sv = StateView()
+    bases = [b.tensor.data_ptr() if b.tensor.numel() else None for b in buffers]
+    st = lib.omnigs_lonlat_state(bases[0], bases[1], bases[2], P, R, width, height, ctypes.byref(sv))
+    if st != OMNIGS_OK:
+        raise OmnigsError(st, "omnigs_lonlat_state")
+    tiles = ((width + 15) // 16) * ((height + 15) // 16)
+    N = width * height
+    spec = {  # field: (buffer, dtype, count)
+        "depth_key": (geom, torch.int32, P), "radii": (geom, torch.int32, P), "means2D": (geom, torch.float32, 2 * P),
+        "conic_opacity": (geom, torch.float32, 4 * P), "rgb": (geom, torch.float32, 3 * P),
+        "clamped": (geom, torch.uint8, P), "tiles_touched": (geom, torch.int32, P),
+        "depth_order": (geom, torch.int32, P), "offsets": (geom, torch.int32, P),
+        "point_list": (binning, torch.int32, R), "tile_keys": (binning, torch.int32, R),
+        "final_T": (image, torch.float32, N), "n_contrib": (image, torch.int32, N), "ranges": (image, torch.int32, 2 * tiles),
+    }
+    out = {}
+    for name, (buf, dt, n) in spec.items():
+        addr = getattr(sv, name)
+        if not addr or n == 0:
+            continue
+        off = addr - buf.tensor.data_ptr()
+        nbytes = n * torch.empty((), dtype=dt).element_size()
+        assert 0 <= off and off + nbytes <= buf.tensor.numel(), name
+        out[name] = buf.tensor[off:off + nbytes].view(dt)
+    return out
+
+
+def mark_visible(P: int, device) -> torch.Tensor:
+    lib = load()
+    present = torch.zeros((P,), dtype=torch.bool, device=device)
+    st = lib.omnigs_lonlat_mark_visible(P, _ptr(present), stream_handle(torch.device(device)))
+    if st != OMNIGS_OK:
+        raise OmnigsError(st, "omnigs_lonlat_mark_visible")
+    return present
+
+
+def scene_to_device(scene: dict, device="cuda:0") -> dict:
+    """Moves the float arrays of a scene (numpy) to contiguous device tensors."""
+    out = {}
+    for k, v in scene.items():
+        if hasattr(v, "dtype") and hasattr(v, "shape"):
+            out[k] = torch.from_numpy(v).to(device).contiguous()
+        else:
+            out[k] = v
+    return out
+
+
+def forward_scene(ds: dict, **kw):
+    """Convenience: forward of a device scene dict from omnigs_amd.scenes."""
+    return forward(background=ds["background"], means3D=ds["means3D"], opacities=ds["opacities"].reshape(-1),
+                   viewmatrix=ds["viewmatrix"], campos=ds["campos"], width=ds["width"], height=ds["height"],
+                   shs=ds.get("shs"), scales=ds.get("scales"), rotations=ds.get("rotations"),
+                   degree=ds.get("degree", 3), scale_modifier=ds.get("scale_modifier", 1.0), **kw)
+
+
+def backward_scene(ds: dict, R, radii, buffers, dL_dpix, **kw):
+    return backward(R=R, background=ds["background"], means3D=ds["means3D"], radii=radii,
+                    viewmatrix=ds["viewmatrix"], campos=ds["campos"], dL_dpix=dL_dpix, buffers=buffers,
+                    shs=ds.get("shs"), scales=ds.get("scales"), rotations=ds.get("rotations"),
+                    degree=ds.get("degree", 3), scale_modifier=ds.get("scale_modifier", 1.0), **kw)
+
+
+def profile_enable(on: bool = True) -> None:
+    load().omnigs_profile_enable(1 if on else 0)
+
+
+def profile_read() -> dict:
+    """{stage name: (total ms, launches)} of the events recorded since the last read."""
+    lib = load()
+    ms = (ctypes.c_double * NUM_STAGES)()
+    cnt = (ctypes.c_int * NUM_STAGES)()
+    st = lib.omnigs_profile_read(ms, cnt)
+    if st != OMNIGS_OK:
+        raise OmnigsError(st, "omnigs_profile_read")
+    return {lib.omnigs_stage_name(i).decode(): (ms[i], cnt[i]) for i in range(NUM_STAGES)}

@@ -1,0 +1,315 @@
+// C ABI entry points (include/omnigs_lonlat.h): host orchestration of the
+// forward (preprocess -> depth sort -> scan -> expand -> tile sort -> ranges
+// -> render) and the backward (render backward -> fused preprocess backward).
+// Stage order and buffer contract follow the reference's
+// LonlatRasterizer::forward / backward (rasterizer_impl.cu:540-697, :701-795).
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/omnigs_lonlat.h"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+int fail_hip(hipError_t e, const char* where) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s", where, hipGetErrorString(e));
+  return OMNIGS_ERR_HIP;
+}
+int fail_arg(const char* what) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "invalid argument: %s", what);
+  return OMNIGS_ERR_INVALID_ARGUMENT;
+}
+
+#define OMNIGS_TRY(expr, where)                 \
+  do {                                          \
+    hipError_t _e = (expr);                     \
+    if (_e != hipSuccess) return fail_hip(_e, where); \
+  } while (0)
+
+// ---- stage timers (HIP events on the launch stream) ----
+struct StageEvent {
+  int stage;
+  hipEvent_t start, stop;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<StageEvent> g_prof_events;
+
+struct StageTimer {
+  int stage;
+  hipStream_t s;
+  StageEvent ev{};
+  bool on;
+  StageTimer(int st, hipStream_t stream) : stage(st), s(stream) {
+    {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      on = g_prof_on;
+    }
+    if (on) {
+      ev.stage = st;
+      if (hipEventCreate(&ev.start) != hipSuccess || hipEventCreate(&ev.stop) != hipSuccess ||
+          hipEventRecord(ev.start, s) != hipSuccess)
+        on = false;
+    }
+  }
+  ~StageTimer() {
+    if (!on) return;
+    if (hipEventRecord(ev.stop, s) == hipSuccess) {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      g_prof_events.push_back(ev);
+    }
+  }
+};
+
+char* grow(omnigs_buffer b, size_t bytes) {
+  if (!b.resize) return nullptr;
+  return b.resize(b.ctx, bytes);
+}
+
+}  // namespace
+
+using namespace omnigs;
+
+extern "C" {
+
+size_t omnigs_lonlat_geometry_bytes(int P) { return carve_geom(nullptr, (size_t)(P < 0 ? 0 : P)).bytes; }
+size_t omnigs_lonlat_binning_bytes(int R) { return carve_bin(nullptr, (size_t)(R < 0 ? 0 : R)).bytes; }
+size_t omnigs_lonlat_image_bytes(int width, int height) {
+  if (width <= 0 || height <= 0) return 0;
+  return carve_img(nullptr, width, height).bytes;
+}
+
+const char* omnigs_status_string(int status) {
+  switch (status) {
+    case OMNIGS_OK: return "ok";
+    case OMNIGS_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case OMNIGS_ERR_NEED_COLORS: return "For non-RGB, provide precomputed Gaussian colors!";
+    case OMNIGS_ERR_ALLOC: return "buffer allocation failed";
+    case OMNIGS_ERR_HIP: return "HIP error";
+  }
+  return "unknown status";
+}
+
+const char* omnigs_last_error(void) { return g_last_error; }
+
+const char* omnigs_version(void) { return "omnigs-mi355x 0.1 (gfx950)"; }
+
+int omnigs_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+  return OMNIGS_OK;
+}
+
+int omnigs_profile_read(double* ms, int* count) {
+  std::vector<StageEvent> evs;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    evs.swap(g_prof_events);
+  }
+  int status = OMNIGS_OK;
+  for (auto& e : evs) {
+    float t = 0.f;
+    hipError_t err = hipEventSynchronize(e.stop);
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e.start, e.stop);
+    if (err != hipSuccess) status = fail_hip(err, "profile_read");
+    else if (e.stage >= 0 && e.stage < OMNIGS_NUM_STAGES) {
+      if (ms) ms[e.stage] += t;
+      if (count) count[e.stage] += 1;
+    }
+    (void)hipEventDestroy(e.start);
+    (void)hipEventDestroy(e.stop);
+  }
+  return status;
+}
+
+const char* omnigs_stage_name(int stage) {
+  static const char* names[OMNIGS_NUM_STAGES] = {"preprocess", "depth_sort", "scan", "duplicate", "tile_sort",
+                                                 "ranges", "render", "clear_grads", "render_backward",
+                                                 "preprocess_backward"};
+  return (stage >= 0 && stage < OMNIGS_NUM_STAGES) ? names[stage] : "unknown";
+}
+
+int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, const char* image_buffer, int P, int R,
+                        int width, int height, omnigs_lonlat_state_view* out) {
+  if (!out || P < 0 || R < 0 || width <= 0 || height <= 0) return fail_arg("state view");
+  std::memset(out, 0, sizeof(*out));
+  if (geom_buffer) {
+    const GeomState g = carve_geom(const_cast<char*>(geom_buffer), P);
+    out->depth_key = g.depth_key;
+    out->radii = g.radii;
+    out->means2D = reinterpret_cast<const float*>(g.xy);
+    out->conic_opacity = reinterpret_cast<const float*>(g.conic_opacity);
+    out->rgb = g.rgb;
+    out->clamped = g.clamped;
+    out->tiles_touched = g.tiles_touched;
+    out->depth_order = g.order;
+    out->offsets = g.offsets;
+  }
+  if (binning_buffer && R > 0) {
+    const BinState b = carve_bin(const_cast<char*>(binning_buffer), R);
+    const bool alt = radix_passes(bit_width_u32((uint32_t)tile_grid(width, height).tiles())) % 2;
+    out->point_list = alt ? b.gid_alt : b.gid;
+    out->tile_keys = alt ? b.tile_key_alt : b.tile_key;
+  }
+  if (image_buffer) {
+    const ImgState s = carve_img(const_cast<char*>(image_buffer), width, height);
+    out->final_T = s.final_T;
+    out->n_contrib = s.n_contrib;
+    out->ranges = reinterpret_cast<const uint32_t*>(s.ranges);
+  }
+  return OMNIGS_OK;
+}
+
+int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream) {
+  if (P < 0) return fail_arg("P < 0");
+  if (P > 0 && !present) return fail_arg("present is NULL");
+  OMNIGS_TRY(launch_mark_visible(P, present, (hipStream_t)stream), "mark_visible");
+  return OMNIGS_OK;
+}
+
+int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,
+                          omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream_,
+                          int* num_rendered) {
+  if (!a || !num_rendered) return fail_arg("NULL args");
+  *num_rendered = 0;
+  const int P = a->P, W = a->width, H = a->height;
+  if (P < 0 || W <= 0 || H <= 0) return fail_arg("P < 0 or empty image");
+  if (P == 0) return OMNIGS_OK;  // the reference boundary skips the rasterizer entirely
+  if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || !a->out_color)
+    return fail_arg("missing required pointer");
+  if (!a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  if (!a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
+    return fail_arg("SH degree / coefficient count");
+  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
+  hipStream_t s = (hipStream_t)stream_;
+  const TileGrid grid = tile_grid(W, H);
+
+  char* gbuf = grow(geometry_buffer, carve_geom(nullptr, P).bytes);
+  char* ibuf = grow(image_buffer, carve_img(nullptr, W, H).bytes);
+  if (!gbuf || !ibuf) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "geometry/image buffer allocation failed");
+    return OMNIGS_ERR_ALLOC;
+  }
+  const GeomState g = carve_geom(gbuf, P);
+  const ImgState img = carve_img(ibuf, W, H);
+  int* radii = a->radii ? a->radii : g.radii;
+
+  PreprocessArgs pa{P, a->D, a->M, W, H, a->means3D, a->shs, a->colors_precomp, a->opacities, a->scales,
+                    a->scale_modifier, a->rotations, a->cov3D_precomp, a->viewmatrix, a->campos};
+  {
+    StageTimer tm(OMNIGS_STAGE_PREPROCESS, s);
+    OMNIGS_TRY(launch_preprocess(pa, g, radii, s), "preprocess");
+  }
+
+  // Gaussians by (depth bits, id): 32-bit stable radix sort with index values.
+  SortResult ds;
+  {
+    StageTimer tm(OMNIGS_STAGE_DEPTH_SORT, s);
+    OMNIGS_TRY(launch_radix_sort(g.depth_key, g.key_alt, nullptr, g.order, g.order_alt, (size_t)P, 32,
+                                 g.sort_counts, g.scan_partials, s, &ds),
+               "depth sort");
+  }
+  // Instance offsets in depth order.
+  {
+    StageTimer tm(OMNIGS_STAGE_SCAN, s);
+    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s), "scan");
+  }
+
+  uint32_t R = 0;
+  OMNIGS_TRY(hipMemcpyAsync(&R, g.offsets + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
+  OMNIGS_TRY(hipStreamSynchronize(s), "sync R");
+  if (R > 0x7fffffffu) return fail_arg("instance count exceeds int range");
+
+  char* bbuf = grow(binning_buffer, carve_bin(nullptr, R).bytes);
+  if (!bbuf) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "binning buffer allocation failed");
+    return OMNIGS_ERR_ALLOC;
+  }
+  const BinState b = carve_bin(bbuf, R);
+  {
+    StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
+    OMNIGS_TRY(launch_duplicate(P, ds.vals, g.offsets, g.xy, radii, grid, b.tile_key, b.gid, s), "duplicate");
+  }
+  SortResult ts{b.tile_key, b.gid};
+  if (R > 0) {
+    StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
+    const int bits = bit_width_u32((uint32_t)grid.tiles());
+    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, R, bits, b.sort_counts,
+                                 b.scan_partials, s, &ts),
+               "tile sort");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_RANGES, s);
+    OMNIGS_TRY(launch_tile_ranges(ts.keys, R, img.ranges, grid.tiles(), s), "ranges");
+  }
+  const float* colors = a->colors_precomp ? a->colors_precomp : g.rgb;
+  {
+    StageTimer tm(OMNIGS_STAGE_RENDER, s);
+    OMNIGS_TRY(launch_render_forward(grid, img.ranges, ts.vals, W, H, g.xy, colors, g.conic_opacity, img.final_T,
+                                     img.n_contrib, a->background, a->out_color, s),
+               "render");
+  }
+  *num_rendered = (int)R;
+  return OMNIGS_OK;
+}
+
+int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) {
+  if (!a) return fail_arg("NULL args");
+  const int P = a->P, W = a->width, H = a->height, R = a->R;
+  if (P < 0 || R < 0 || W <= 0 || H <= 0) return fail_arg("negative size or empty image");
+  if (P == 0) return OMNIGS_OK;
+  if (!a->means3D || !a->viewmatrix || !a->campos || !a->background || !a->dL_dpix || !a->geom_buffer ||
+      !a->image_buffer || (R > 0 && !a->binning_buffer))
+    return fail_arg("missing required pointer");
+  if (!a->dL_dmean2D || !a->dL_dconic || !a->dL_dopacity || !a->dL_dcolor || !a->dL_dmean3D || !a->dL_dcov3D)
+    return fail_arg("missing gradient output");
+  if (a->shs && !a->dL_dsh) return fail_arg("dL_dsh is NULL while shs is given");
+  if (!a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
+  if (a->scales && (!a->dL_dscale || !a->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
+  hipStream_t s = (hipStream_t)stream_;
+  const TileGrid grid = tile_grid(W, H);
+  const GeomState g = carve_geom(a->geom_buffer, P);
+  const ImgState img = carve_img(a->image_buffer, W, H);
+  const int* radii = a->radii ? a->radii : g.radii;
+
+  // The point list ends in gid (even pass count) or gid_alt (odd).
+  const uint32_t* point_list = nullptr;
+  if (R > 0) {
+    const BinState b = carve_bin(a->binning_buffer, R);
+    const int passes = radix_passes(bit_width_u32((uint32_t)grid.tiles()));
+    point_list = (passes % 2) ? b.gid_alt : b.gid;
+  }
+
+  {
+    StageTimer tm(OMNIGS_STAGE_CLEAR_GRADS, s);
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dmean2D), 3 * (size_t)P, 0u, s), "clear");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dconic), 4 * (size_t)P, 0u, s), "clear");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dopacity), (size_t)P, 0u, s), "clear");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dcolor), 3 * (size_t)P, 0u, s), "clear");
+  }
+
+  const float* colors = a->colors_precomp ? a->colors_precomp : g.rgb;
+  if (R > 0) {
+    StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
+    OMNIGS_TRY(launch_render_backward(grid, img.ranges, point_list, W, H, a->background, g.xy, g.conic_opacity,
+                                      colors, img.final_T, img.n_contrib, a->dL_dpix, a->dL_dmean2D, a->dL_dconic,
+                                      a->dL_dopacity, a->dL_dcolor, s),
+               "render backward");
+  }
+  PreprocessBackwardArgs pb{P, a->D, a->M, W, H, a->means3D, a->shs, a->scales, a->scale_modifier, a->rotations,
+                            a->cov3D_precomp, a->viewmatrix, a->campos, radii, g.clamped, a->dL_dmean2D,
+                            a->dL_dconic, a->dL_dcolor, a->dL_dmean3D, a->dL_dcov3D, a->dL_dsh,
+                            a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt};
+  {
+    StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
+    OMNIGS_TRY(launch_preprocess_backward(pb, s), "preprocess backward");
+  }
+  return OMNIGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+// Shared device helpers and buffer layouts of the gfx950 equirectangular
+// rasterizer.  Written for CDNA4: 64-lane wavefronts, 160 KB LDS per CU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace omnigs {
+
+constexpr int kWave = 64;
+constexpr int kTileX = 16;  // tile width in pixels (reference config.h:26)
+constexpr int kTileY = 16;  // tile height in pixels (reference config.h:27)
+constexpr int kTilePixels = kTileX * kTileY;
+constexpr int kChannels = 3;
+
+// Real SH constants (textbook values; the reference lists them at
+// cuda_rasterizer/auxiliary.h:32-49).
+constexpr float kSH0 = 0.28209479177387814f;
+constexpr float kSH1 = 0.4886025119029199f;
+constexpr float kSH2_0 = 1.0925484305920792f;
+constexpr float kSH2_1 = -1.0925484305920792f;
+constexpr float kSH2_2 = 0.31539156525252005f;
+constexpr float kSH2_3 = -1.0925484305920792f;
+constexpr float kSH2_4 = 0.5462742152960396f;
+constexpr float kSH3_0 = -0.5900435899266435f;
+constexpr float kSH3_1 = 2.890611442640554f;
+constexpr float kSH3_2 = -0.4570457994644658f;
+constexpr float kSH3_3 = 0.3731763325901154f;
+constexpr float kSH3_4 = -0.4570457994644658f;
+constexpr float kSH3_5 = 1.445305721320277f;
+constexpr float kSH3_6 = -0.5900435899266435f;
+
+constexpr float kInvPi = 0.318309886183790671537767526745028724f;
+constexpr float kTwoInvPi = 0.636619772367581343075535053490057448f;
+
+struct TileGrid {
+  int gx, gy;
+  __host__ __device__ int tiles() const { return gx * gy; }
+};
+inline TileGrid tile_grid(int W, int H) { return {(W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY}; }
+
+// Tile rectangle [x0,x1) x [y0,y1) of a Gaussian at pixel (px,py) with integer
+// radius r.  The float evaluation order is the reference's getRect
+// (auxiliary.h:56-66) so that tile sets agree bit for bit:
+// lo = (int)((p - r) / 16), hi = (int)((((p + r) + 16) - 1) / 16), clamped.
+struct Rect {
+  int x0, y0, x1, y1;
+};
+__device__ __forceinline__ int rect_lo(float p, int r, int tile, int g) {
+  int v = (int)((p - (float)r) / (float)tile);
+  return min(g, max(0, v));
+}
+__device__ __forceinline__ int rect_hi(float p, int r, int tile, int g) {
+  int v = (int)((((p + (float)r) + (float)tile) - 1.0f) / (float)tile);
+  return min(g, max(0, v));
+}
+__device__ __forceinline__ Rect tile_rect(float2 p, int r, TileGrid g) {
+  return {rect_lo(p.x, r, kTileX, g.gx), rect_lo(p.y, r, kTileY, g.gy), rect_hi(p.x, r, kTileX, g.gx),
+          rect_hi(p.y, r, kTileY, g.gy)};
+}
+
+// ---------------------------------------------------------------------------
+// Buffer layouts.  The three byte buffers are private to this library; the
+// backward re-derives every array's offset from (P), (R, tiles) and (W, H),
+// like the reference's GeometryState / BinningState / ImageState carving
+// (rasterizer_impl.h:48-102), but with this library's own contents.
+// ---------------------------------------------------------------------------
+constexpr size_t kAlign = 256;
+__host__ __device__ inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// Aligns absolute addresses (like the reference's obtain(), rasterizer_impl.h:37-43);
+// sizes computed from a NULL base carry kAlign bytes of slack for that.
+template <class T> struct Carver {
+  size_t off = 0;
+  char* base;
+  __host__ __device__ explicit Carver(char* b) : base(b) {}
+  template <class U> U* take(size_t count) {
+    const uintptr_t a = align_up(reinterpret_cast<uintptr_t>(base) + off);
+    off = a - reinterpret_cast<uintptr_t>(base);
+    U* p = reinterpret_cast<U*>(a);
+    off += count * sizeof(U);
+    return p;
+  }
+};
+
+// Radix sort geometry: 256 threads x 16 keys per block, 8-bit digits.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+inline size_t sort_blocks(size_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+// Scan geometry: 256 threads x 16 items per block.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+inline size_t scan_blocks(size_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+// Per-Gaussian state written by the forward preprocess.
+struct GeomState {
+  uint32_t* depth_key;     // float bits of the camera distance (sort key); ~0 if culled
+  int32_t* radii;          // internal radii (used when the caller passes none)
+  float2* xy;              // pixel-space mean
+  float4* conic_opacity;   // (A, B, C, opacity)
+  float* rgb;              // [P,3] SH colour
+  uint8_t* clamped;        // bit c set when channel c was clamped at 0
+  uint32_t* tiles_touched; // rect area
+  uint32_t* order;         // Gaussians sorted by (depth bits, index)
+  uint32_t* order_alt;     // ping-pong for the depth sort
+  uint32_t* key_alt;       // ping-pong keys for the depth sort
+  uint32_t* offsets;       // inclusive scan of tiles_touched in depth order
+  uint32_t* scan_partials; // scan block sums
+  uint32_t* sort_counts;   // radix digit counts [kRadix, blocks]
+  size_t bytes;
+};
+
+inline GeomState carve_geom(char* base, size_t P) {
+  GeomState g;
+  Carver<char> c(base);
+  g.depth_key = c.take<uint32_t>(P);
+  g.radii = c.take<int32_t>(P);
+  g.xy = c.take<float2>(P);
+  g.conic_opacity = c.take<float4>(P);
+  g.rgb = c.take<float>(3 * P);
+  g.clamped = c.take<uint8_t>(P);
+  g.tiles_touched = c.take<uint32_t>(P);
+  g.order = c.take<uint32_t>(P);
+  g.order_alt = c.take<uint32_t>(P);
+  g.key_alt = c.take<uint32_t>(P);
+  g.offsets = c.take<uint32_t>(P);
+  g.scan_partials = c.take<uint32_t>(scan_blocks(P) + 1);
+  g.sort_counts = c.take<uint32_t>((size_t)kRadix * sort_blocks(P) + 1);
+  g.bytes = align_up(c.off) + kAlign;
+  return g;
+}
+
+// Per-instance state: tile keys and Gaussian ids, ping-pong for the tile sort.
+struct BinState {
+  uint32_t* tile_key;
+  uint32_t* tile_key_alt;
+  uint32_t* gid;
+  uint32_t* gid_alt;
+  uint32_t* sort_counts;  // [kRadix, blocks]
+  uint32_t* scan_partials;
+  size_t bytes;
+};
+
+inline BinState carve_bin(char* base, size_t R) {
+  BinState b;
+  Carver<char> c(base);
+  b.tile_key = c.take<uint32_t>(R);
+  b.tile_key_alt = c.take<uint32_t>(R);
+  b.gid = c.take<uint32_t>(R);
+  b.gid_alt = c.take<uint32_t>(R);
+  const size_t cnt = (size_t)kRadix * sort_blocks(R) + 1;
+  b.sort_counts = c.take<uint32_t>(cnt);
+  b.scan_partials = c.take<uint32_t>(scan_blocks(cnt) + 1);
+  b.bytes = align_up(c.off) + kAlign;
+  return b;
+}
+
+// Per-pixel / per-tile state.
+struct ImgState {
+  float* final_T;
+  uint32_t* n_contrib;
+  uint2* ranges;
+  size_t bytes;
+};
+
+inline ImgState carve_img(char* base, int W, int H) {
+  ImgState s;
+  Carver<char> c(base);
+  const size_t N = (size_t)W * H;
+  s.final_T = c.take<float>(N);
+  s.n_contrib = c.take<uint32_t>(N);
+  s.ranges = c.take<uint2>((size_t)tile_grid(W, H).tiles());
+  s.bytes = align_up(c.off) + kAlign;
+  return s;
+}
+
+// Number of 8-bit passes for a key range of `bits` bits, and whether the
+// sorted result ends in the *_alt arrays (odd pass count).
+inline int radix_passes(int bits) { return (bits + kRadixBits - 1) / kRadixBits; }
+
+// bit count of n (highest set bit + 1); the reference's getHigherMsb
+// (rasterizer_impl.cu:47-62) computes the same quantity.
+inline int bit_width_u32(uint32_t n) {
+  int b = 0;
+  while (b < 32 && (n >> b)) ++b;
+  return b;
+}
+
+}  // namespace omnigs

@@ -1,0 +1,86 @@
+// Host-side launchers of the gfx950 kernels (one translation unit per stage).
+#pragma once
+
+#include "common.hpp"
+
+namespace omnigs {
+
+struct PreprocessArgs {
+  int P, D, M, W, H;
+  const float* means3D;
+  const float* shs;
+  const float* colors_precomp;
+  const float* opacities;
+  const float* scales;
+  float scale_modifier;
+  const float* rotations;
+  const float* cov3D_precomp;
+  const float* viewmatrix;
+  const float* campos;
+};
+
+struct PreprocessBackwardArgs {
+  int P, D, M, W, H;
+  const float* means3D;
+  const float* shs;
+  const float* scales;
+  float scale_modifier;
+  const float* rotations;
+  const float* cov3D_precomp;
+  const float* viewmatrix;
+  const float* campos;
+  const int* radii;
+  const uint8_t* clamped;
+  const float* dL_dmean2D;  // [P,3]
+  const float* dL_dconic;   // [P,4]
+  const float* dL_dcolor;   // [P,3]
+  float* dL_dmean3D;
+  float* dL_dcov3D;
+  float* dL_dsh;
+  float* dL_dscale;
+  float* dL_drot;
+  float* dpx_dt;
+  float* dpy_dt;
+};
+
+// preprocess.hip
+hipError_t launch_preprocess(const PreprocessArgs& a, const GeomState& g, int* radii, hipStream_t s);
+hipError_t launch_mark_visible(int P, uint8_t* present, hipStream_t s);
+
+// sort.hip
+// Inclusive (or exclusive) scan of n u32 values, optionally gathered through
+// idx (out[i] = sum of in[idx[j]] for j <= i).  partials needs scan_blocks(n)+1 entries.
+hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
+                       uint32_t* partials, hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
+// vals_in is NULL the values are the element indices.  The result is in
+// (keys_out, vals_out) if the pass count is even... callers use the returned
+// pointers.  counts: kRadix*sort_blocks(n)+1 entries, partials: scan_blocks of that.
+struct SortResult {
+  uint32_t* keys;
+  uint32_t* vals;
+};
+hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
+                             uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
+                             hipStream_t s, SortResult* out);
+hipError_t launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float2* xy,
+                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
+hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
+                              hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+// render.hip
+hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
+                                 const float2* xy, const float* colors, const float4* conic_opacity,
+                                 float* final_T, uint32_t* n_contrib, const float* bg, float* out_color,
+                                 hipStream_t s);
+hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
+                                  const float* bg, const float2* xy, const float4* conic_opacity,
+                                  const float* colors, const float* final_T, const uint32_t* n_contrib,
+                                  const float* dL_dpix, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
+                                  float* dL_dcolors, hipStream_t s);
+
+// preprocess_backward.hip
+hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s);
+
+}  // namespace omnigs

@@ -1,0 +1,125 @@
+// Forward preprocess: one thread per Gaussian.
+//
+// Follows the semantics of the reference's preprocessLonlatCUDA
+// (cuda_rasterizer/forward.cu:593-703): distance cull at |t|^2 <= 0.04,
+// lon/lat projection, 3D covariance, EWA 2D covariance through the lon/lat
+// Jacobian (+0.3 I, no clamping), conic, 3-sigma radius, clamped tile rect
+// (no longitude-seam wrap), SH colour with clamp-at-zero flags.  The sort key
+// written here is the float bit pattern of the camera distance |t|.
+//
+// HBM-bound: ~236 B read (means, scales, rotations, opacity, 48 SH floats)
+// and ~45 B written per Gaussian.
+#include "kernels.hpp"
+#include "lonlat_math.hpp"
+
+namespace omnigs {
+
+__global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomState g, int* __restrict__ radii_out,
+                                                         TileGrid grid) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+
+  radii_out[idx] = 0;
+  g.tiles_touched[idx] = 0;
+  g.depth_key[idx] = 0xFFFFFFFFu;  // culled Gaussians sort last and emit nothing
+
+  const View v = load_view(a.viewmatrix);
+  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+  const float3 t = to_camera(v, p);
+  const float rr = t.x * t.x + t.y * t.y + t.z * t.z;
+  if (rr <= 0.04f) return;
+  const float r = sqrtf(rr);
+  const float inv_r = 1.0f / (r + 1e-7f);
+  const float lon = atan2f(t.x, t.z);
+  const float lat = asinf(t.y * inv_r);
+  const float sx = lon * kInvPi;
+  const float sy = lat * kTwoInvPi;
+
+  Sym3 S;
+  if (a.cov3D_precomp) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
+  } else {
+    const float3 s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
+    const float4 q = reinterpret_cast<const float4*>(a.rotations)[idx];
+    S = cov3d_from(s, a.scale_modifier, q);
+  }
+
+  const Jac j = lonlat_jacobian(t, a.W, a.H);
+  const Proj2 A = project_rows(j, v);
+  const float cxx = quad_form(S, A.ax, A.ax) + 0.3f;
+  const float cxy = quad_form(S, A.ax, A.ay);
+  const float cyy = quad_form(S, A.ay, A.ay) + 0.3f;
+
+  const float det = cxx * cyy - cxy * cxy;
+  if (det == 0.0f) return;
+  const float det_inv = 1.f / det;
+  const float4 conic = make_float4(cyy * det_inv, -cxy * det_inv, cxx * det_inv, a.opacities[idx]);
+
+  const float mid = 0.5f * (cxx + cyy);
+  const float disc = sqrtf(fmaxf(0.1f, mid * mid - det));
+  const float lambda = fmaxf(mid + disc, mid - disc);
+  const int radius = (int)ceilf(3.f * sqrtf(lambda));
+
+  // ndc -> pixel in FP64, rounded to float (reference ndc2Pix evaluates in double).
+  const float px = (float)((((double)sx + 1.0) * (double)a.W - 1.0) * 0.5);
+  const float py = (float)((((double)sy + 1.0) * (double)a.H - 1.0) * 0.5);
+  const float2 pix = make_float2(px, py);
+  const Rect rc = tile_rect(pix, radius, grid);
+  const int area = (rc.x1 - rc.x0) * (rc.y1 - rc.y0);
+  if (area == 0) return;
+
+  if (!a.colors_precomp) {
+    const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
+    float3 d = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
+    const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    d = make_float3(d.x / len, d.y / len, d.z / len);
+    float b[16];
+    sh_basis(a.D, d.x, d.y, d.z, b);
+    const int nc = min(a.M, sh_coeffs(a.D));
+    float3 c[16];
+    load_sh(a.shs, a.M, idx, nc, c);
+    float3 rgb = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i < nc) {
+        rgb.x += b[i] * c[i].x;
+        rgb.y += b[i] * c[i].y;
+        rgb.z += b[i] * c[i].z;
+      }
+    }
+    rgb.x += 0.5f;
+    rgb.y += 0.5f;
+    rgb.z += 0.5f;
+    g.clamped[idx] = (uint8_t)((rgb.x < 0.f) | ((rgb.y < 0.f) << 1) | ((rgb.z < 0.f) << 2));
+    g.rgb[3 * idx] = fmaxf(rgb.x, 0.f);
+    g.rgb[3 * idx + 1] = fmaxf(rgb.y, 0.f);
+    g.rgb[3 * idx + 2] = fmaxf(rgb.z, 0.f);
+  }
+
+  g.depth_key[idx] = __float_as_uint(r);
+  radii_out[idx] = radius;
+  g.xy[idx] = pix;
+  g.conic_opacity[idx] = conic;
+  g.tiles_touched[idx] = (uint32_t)area;
+}
+
+__global__ void mark_visible_kernel(int P, uint8_t* __restrict__ present) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < P) present[idx] = 1;
+}
+
+hipError_t launch_preprocess(const PreprocessArgs& a, const GeomState& g, int* radii, hipStream_t s) {
+  if (a.P == 0) return hipSuccess;
+  const TileGrid grid = tile_grid(a.W, a.H);
+  hipLaunchKernelGGL(preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, g, radii, grid);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_visible(int P, uint8_t* present, hipStream_t s) {
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(mark_visible_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, present);
+  return hipGetLastError();
+}
+
+}  // namespace omnigs

@@ -1,0 +1,296 @@
+// Backward preprocess, one thread per Gaussian, fusing what the reference
+// splits into computeCov2DLonLatCUDA (backward.cu:297-485) and
+// preprocessLonLatCUDA (backward.cu:613-669):
+//   dL/dconic -> dL/d(a,b,c) -> dL/dcov3D and dL/dJ -> dL/dt via the second
+//     derivatives of the lon/lat projection (supp.pdf Eq. 43-55),
+//   + the 2D-mean term (dL_dmean2D is in NDC units, scaled by 2/W, 2/H),
+//   + the SH view-direction term, dL/dsh, and dL/dscale, dL/drot
+//     (w.r.t. the quaternion as given; normalisation is the caller's).
+// The reference's dpx_dt / dpy_dt round trip through HBM disappears: the
+// Jacobian stays in registers (it is written only when asked for).
+// HBM-bound: ~290 B read and ~250 B written per visible Gaussian.
+#include "kernels.hpp"
+#include "lonlat_math.hpp"
+
+namespace omnigs {
+
+// d(basis_i)/d(x,y,z) of the degree <= 3 real SH basis (unused tail = 0).
+__device__ __forceinline__ void sh_basis_grad(int deg, float x, float y, float z, float bx[16], float by[16],
+                                              float bz[16]) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) bx[i] = by[i] = bz[i] = 0.f;
+  if (deg > 0) {
+    by[1] = -kSH1;
+    bz[2] = kSH1;
+    bx[3] = -kSH1;
+    if (deg > 1) {
+      const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+      bx[4] = kSH2_0 * y;          by[4] = kSH2_0 * x;
+      by[5] = kSH2_1 * z;          bz[5] = kSH2_1 * y;
+      bx[6] = kSH2_2 * -2.f * x;   by[6] = kSH2_2 * -2.f * y;   bz[6] = kSH2_2 * 4.f * z;
+      bx[7] = kSH2_3 * z;          bz[7] = kSH2_3 * x;
+      bx[8] = kSH2_4 * 2.f * x;    by[8] = kSH2_4 * -2.f * y;
+      if (deg > 2) {
+        bx[9] = kSH3_0 * 6.f * xy;                  by[9] = kSH3_0 * 3.f * (xx - yy);
+        bx[10] = kSH3_1 * yz;                       by[10] = kSH3_1 * xz;          bz[10] = kSH3_1 * xy;
+        bx[11] = kSH3_2 * -2.f * xy;                by[11] = kSH3_2 * (4.f * zz - xx - 3.f * yy);
+        bz[11] = kSH3_2 * 8.f * yz;
+        bx[12] = kSH3_3 * -6.f * xz;                by[12] = kSH3_3 * -6.f * yz;
+        bz[12] = kSH3_3 * 3.f * (2.f * zz - xx - yy);
+        bx[13] = kSH3_4 * (4.f * zz - 3.f * xx - yy); by[13] = kSH3_4 * -2.f * xy; bz[13] = kSH3_4 * 8.f * xz;
+        bx[14] = kSH3_5 * 2.f * xz;                 by[14] = kSH3_5 * -2.f * yz;   bz[14] = kSH3_5 * (xx - yy);
+        bx[15] = kSH3_6 * 3.f * (xx - yy);          by[15] = kSH3_6 * -6.f * xy;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBackwardArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+  const int M = a.M;
+
+  if (!(a.radii[idx] > 0)) {
+    a.dL_dmean3D[3 * idx] = 0.f;
+    a.dL_dmean3D[3 * idx + 1] = 0.f;
+    a.dL_dmean3D[3 * idx + 2] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = 0.f;
+    if (a.dL_dsh)
+      for (int k = 0; k < 3 * M; ++k) a.dL_dsh[(size_t)3 * M * idx + k] = 0.f;
+    if (a.dL_dscale) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.dL_dscale[3 * idx + k] = 0.f;
+    }
+    if (a.dL_drot) reinterpret_cast<float4*>(a.dL_drot)[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a.dpx_dt) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.dpx_dt[3 * idx + k] = 0.f;
+    }
+    if (a.dpy_dt) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.dpy_dt[3 * idx + k] = 0.f;
+    }
+    return;
+  }
+
+  const View v = load_view(a.viewmatrix);
+  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+  const float3 t = to_camera(v, p);
+
+  // ---- 3D covariance (recomputed; identical to the forward's) ----
+  Sym3 S;
+  float3 s = make_float3(0.f, 0.f, 0.f);
+  float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
+  if (a.cov3D_precomp) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
+  } else {
+    s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
+    q = reinterpret_cast<const float4*>(a.rotations)[idx];
+    S = cov3d_from(s, a.scale_modifier, q);
+  }
+
+  // ---- covariance term (computeCov2DLonLatCUDA) ----
+  const Jac j = lonlat_jacobian(t, a.W, a.H);
+  if (a.dpx_dt) {
+    a.dpx_dt[3 * idx] = j.px_x;
+    a.dpx_dt[3 * idx + 1] = 0.f;
+    a.dpx_dt[3 * idx + 2] = j.px_z;
+  }
+  if (a.dpy_dt) {
+    a.dpy_dt[3 * idx] = j.py_x;
+    a.dpy_dt[3 * idx + 1] = j.py_y;
+    a.dpy_dt[3 * idx + 2] = j.py_z;
+  }
+  const Proj2 A = project_rows(j, v);
+  const float ca = quad_form(S, A.ax, A.ax) + 0.3f;
+  const float cb = quad_form(S, A.ax, A.ay);
+  const float cc = quad_form(S, A.ay, A.ay) + 0.3f;
+
+  const float4 dcon4 = reinterpret_cast<const float4*>(a.dL_dconic)[idx];
+  const float dcx = dcon4.x, dcy = dcon4.y, dcz = dcon4.w;
+  const float denom = ca * cc - cb * cb;
+  const float d2inv = 1.0f / ((denom * denom) + 1e-7f);
+  float dL_da = 0.f, dL_db = 0.f, dL_dc = 0.f;
+  float dcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const float* ax = A.ax;
+  const float* ay = A.ay;
+  if (d2inv != 0.f) {
+    dL_da = d2inv * (-cc * cc * dcx + 2.f * cb * cc * dcy + (denom - ca * cc) * dcz);
+    dL_dc = d2inv * (-ca * ca * dcz + 2.f * ca * cb * dcy + (denom - ca * cc) * dcx);
+    dL_db = d2inv * 2.f * (cb * cc * dcx - (denom + 2.f * cb * cb) * dcy + ca * cb * dcz);
+    dcov[0] = ax[0] * ax[0] * dL_da + ax[0] * ay[0] * dL_db + ay[0] * ay[0] * dL_dc;
+    dcov[3] = ax[1] * ax[1] * dL_da + ax[1] * ay[1] * dL_db + ay[1] * ay[1] * dL_dc;
+    dcov[5] = ax[2] * ax[2] * dL_da + ax[2] * ay[2] * dL_db + ay[2] * ay[2] * dL_dc;
+    dcov[1] = 2.f * ax[0] * ax[1] * dL_da + (ax[0] * ay[1] + ax[1] * ay[0]) * dL_db + 2.f * ay[0] * ay[1] * dL_dc;
+    dcov[2] = 2.f * ax[0] * ax[2] * dL_da + (ax[0] * ay[2] + ax[2] * ay[0]) * dL_db + 2.f * ay[0] * ay[2] * dL_dc;
+    dcov[4] = 2.f * ax[2] * ax[1] * dL_da + (ax[1] * ay[2] + ax[2] * ay[1]) * dL_db + 2.f * ay[1] * ay[2] * dL_dc;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = dcov[k];
+
+  // dL/dA (rows of J R), then dL/dJ = dL/dA R^T
+  float dAx[3], dAy[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float sx = S.at(k, 0) * ax[0] + S.at(k, 1) * ax[1] + S.at(k, 2) * ax[2];
+    const float sy = S.at(k, 0) * ay[0] + S.at(k, 1) * ay[1] + S.at(k, 2) * ay[2];
+    dAx[k] = 2.f * sx * dL_da + sy * dL_db;
+    dAy[k] = 2.f * sy * dL_dc + sx * dL_db;
+  }
+  const float dJ00 = v.r[0][0] * dAx[0] + v.r[0][1] * dAx[1] + v.r[0][2] * dAx[2];
+  const float dJ02 = v.r[2][0] * dAx[0] + v.r[2][1] * dAx[1] + v.r[2][2] * dAx[2];
+  const float dJ10 = v.r[0][0] * dAy[0] + v.r[0][1] * dAy[1] + v.r[0][2] * dAy[2];
+  const float dJ11 = v.r[1][0] * dAy[0] + v.r[1][1] * dAy[1] + v.r[1][2] * dAy[2];
+  const float dJ12 = v.r[2][0] * dAy[0] + v.r[2][1] * dAy[1] + v.r[2][2] * dAy[2];
+
+  // second derivatives of (px, py) w.r.t. t
+  const float txtx = t.x * t.x, tyty = t.y * t.y, tztz = t.z * t.z;
+  const float rxz2 = txtx + tztz;
+  const float inv_rxz2 = 1.0f / (rxz2 + 1e-7f);
+  const float inv_rxz4 = inv_rxz2 * inv_rxz2;
+  const float rxz = sqrtf(rxz2);
+  const float inv_rxz = 1.0f / (rxz + 1e-7f);
+  const float r2 = rxz2 + tyty;
+  const float inv_r2 = 1.0f / (r2 + 1e-7f);
+  const float inv_r4 = inv_r2 * inv_r2;
+  const float inv_rxz_r4 = inv_rxz * inv_r4;
+  const float inv_rxz3_r4 = inv_rxz2 * inv_rxz_r4;
+  const float Hp = j.hpi, Wp = j.w2pi;
+  const float temp1 = Hp * (tyty - rxz2) * inv_rxz_r4;
+  const float temp2 = Hp * (t.x * t.y * t.z) * (r2 + 2.f * rxz2) * inv_rxz3_r4;
+  const float temp3 = Wp * (txtx - tztz) * inv_rxz4;
+  const float temp4 = Wp * 2.f * t.x * t.z * inv_rxz4;
+  const float temp5 = Hp * t.y * inv_rxz3_r4;
+  const float dtx = -dJ00 * temp4 + dJ02 * temp3 + dJ10 * temp5 * (2.f * txtx * rxz2 - tztz * r2) +
+                    dJ11 * t.x * temp1 + dJ12 * temp2;
+  const float dty = dJ10 * t.x * temp1 - dJ11 * Hp * 2.f * rxz * t.y * inv_r4 + dJ12 * t.z * temp1;
+  const float dtz = dJ00 * temp3 + dJ02 * temp4 + dJ10 * temp2 + dJ11 * t.z * temp1 +
+                    dJ12 * temp5 * (2.f * tztz * rxz2 - txtx * r2);
+  float3 dmean = to_world_dir(v, make_float3(dtx, dty, dtz));
+
+  // ---- 2D-mean term (NDC -> pixel: x 2/W, 2/H) ----
+  {
+    const float dL_dpx = a.dL_dmean2D[3 * idx] * (2.0f / (float)a.W);
+    const float dL_dpy = a.dL_dmean2D[3 * idx + 1] * (2.0f / (float)a.H);
+    const float3 dt2 = make_float3(dL_dpx * j.px_x + dL_dpy * j.py_x, dL_dpy * j.py_y,
+                                   dL_dpx * j.px_z + dL_dpy * j.py_z);
+    const float3 dm2 = to_world_dir(v, dt2);
+    dmean.x += dm2.x;
+    dmean.y += dm2.y;
+    dmean.z += dm2.z;
+  }
+
+  // ---- SH term ----
+  if (a.shs) {
+    const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
+    const float3 d0 = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
+    const float len = sqrtf(d0.x * d0.x + d0.y * d0.y + d0.z * d0.z);
+    const float x = d0.x / len, y = d0.y / len, z = d0.z / len;
+    const uint8_t cm = a.clamped[idx];
+    const float gx = (cm & 1) ? 0.f : a.dL_dcolor[3 * idx];
+    const float gy = (cm & 2) ? 0.f : a.dL_dcolor[3 * idx + 1];
+    const float gz = (cm & 4) ? 0.f : a.dL_dcolor[3 * idx + 2];
+    float b[16];
+    sh_basis(a.D, x, y, z, b);
+    const int nc = min(M, sh_coeffs(a.D));
+    float* dsh = a.dL_dsh + (size_t)3 * M * idx;
+    if (M == 16) {
+      float f[48];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const bool on = i < nc;
+        f[3 * i] = on ? b[i] * gx : 0.f;
+        f[3 * i + 1] = on ? b[i] * gy : 0.f;
+        f[3 * i + 2] = on ? b[i] * gz : 0.f;
+      }
+      float4* d4 = reinterpret_cast<float4*>(dsh);
+#pragma unroll
+      for (int i = 0; i < 12; ++i) d4[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+    } else {
+      for (int i = 0; i < M; ++i) {
+        const bool on = i < nc;
+        dsh[3 * i] = on ? b[i] * gx : 0.f;
+        dsh[3 * i + 1] = on ? b[i] * gy : 0.f;
+        dsh[3 * i + 2] = on ? b[i] * gz : 0.f;
+      }
+    }
+    if (a.D > 0) {
+      float3 c[16];
+      load_sh(a.shs, M, idx, nc, c);
+      float bx[16], by[16], bz[16];
+      sh_basis_grad(a.D, x, y, z, bx, by, bz);
+      float ddx = 0.f, ddy = 0.f, ddz = 0.f;
+#pragma unroll
+      for (int i = 1; i < 16; ++i) {
+        if (i < nc) {
+          const float sg = c[i].x * gx + c[i].y * gy + c[i].z * gz;
+          ddx += bx[i] * sg;
+          ddy += by[i] * sg;
+          ddz += bz[i] * sg;
+        }
+      }
+      // derivative of the normalisation d0/|d0| (dnormvdv, auxiliary.h:134-144)
+      const float sum2 = d0.x * d0.x + d0.y * d0.y + d0.z * d0.z;
+      const float invsum32 = 1.0f / sqrtf(sum2 * sum2 * sum2);
+      dmean.x += ((sum2 - d0.x * d0.x) * ddx - d0.y * d0.x * ddy - d0.z * d0.x * ddz) * invsum32;
+      dmean.y += (-d0.x * d0.y * ddx + (sum2 - d0.y * d0.y) * ddy - d0.z * d0.y * ddz) * invsum32;
+      dmean.z += (-d0.x * d0.z * ddx - d0.y * d0.z * ddy + (sum2 - d0.z * d0.z) * ddz) * invsum32;
+    }
+  }
+  a.dL_dmean3D[3 * idx] = dmean.x;
+  a.dL_dmean3D[3 * idx + 1] = dmean.y;
+  a.dL_dmean3D[3 * idx + 2] = dmean.z;
+
+  // ---- scale / rotation term (computeCov3D backward, backward.cu:489-552) ----
+  if (a.scales && a.dL_dscale && a.dL_drot) {
+    float R[3][3];
+    quat_to_rot(q, R);
+    const float sv[3] = {a.scale_modifier * s.x, a.scale_modifier * s.y, a.scale_modifier * s.z};
+    // dL/dSigma with halved off-diagonals
+    float D[3][3];
+    D[0][0] = dcov[0];
+    D[1][1] = dcov[3];
+    D[2][2] = dcov[5];
+    D[0][1] = D[1][0] = 0.5f * dcov[1];
+    D[0][2] = D[2][0] = 0.5f * dcov[2];
+    D[1][2] = D[2][1] = 0.5f * dcov[4];
+    // M[k][i] = s_k R[i][k];  dM = 2 M D;  dscale_k = sum_i R[i][k] dM[k][i]
+    float E[3][3];
+    float dsc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float acc = 0.f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float dMki =
+            2.f * (sv[k] * R[0][k] * D[0][i] + sv[k] * R[1][k] * D[1][i] + sv[k] * R[2][k] * D[2][i]);
+        acc += R[i][k] * dMki;
+        E[k][i] = sv[k] * dMki;
+      }
+      dsc[k] = acc;
+    }
+    a.dL_dscale[3 * idx] = dsc[0];
+    a.dL_dscale[3 * idx + 1] = dsc[1];
+    a.dL_dscale[3 * idx + 2] = dsc[2];
+    const float w = q.x, x = q.y, y = q.z, z = q.w;
+    float4 dq;
+    dq.x = 2.f * z * (E[0][1] - E[1][0]) + 2.f * y * (E[2][0] - E[0][2]) + 2.f * x * (E[1][2] - E[2][1]);
+    dq.y = 2.f * y * (E[1][0] + E[0][1]) + 2.f * z * (E[2][0] + E[0][2]) + 2.f * w * (E[1][2] - E[2][1]) -
+           4.f * x * (E[2][2] + E[1][1]);
+    dq.z = 2.f * x * (E[1][0] + E[0][1]) + 2.f * w * (E[2][0] - E[0][2]) + 2.f * z * (E[1][2] + E[2][1]) -
+           4.f * y * (E[2][2] + E[0][0]);
+    dq.w = 2.f * w * (E[0][1] - E[1][0]) + 2.f * x * (E[2][0] + E[0][2]) + 2.f * y * (E[1][2] + E[2][1]) -
+           4.f * z * (E[1][1] + E[0][0]);
+    reinterpret_cast<float4*>(a.dL_drot)[idx] = dq;
+  }
+}
+
+hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s) {
+  if (a.P == 0) return hipSuccess;
+  hipLaunchKernelGGL(preprocess_backward_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace omnigs

@@ -1,0 +1,285 @@
+// Tile render, forward and backward.  One 256-thread workgroup per 16x16
+// tile; each 64-lane wavefront owns an 8x8 quadrant of the tile so that a
+// wave's pixels are spatially compact (more waves skip a small Gaussian).
+//
+// Per-pixel semantics follow the reference's renderCUDA forward
+// (forward.cu:346-467) and backward (backward.cu:672-843):
+//   power = -0.5 (A dx^2 + C dy^2) - B dx dy   (skip if > 0)
+//   alpha = min(0.99, o exp(power))            (skip if < 1/255)
+//   stop before blending once T (1 - alpha) < 1e-4
+//   out = sum c alpha T + T_final bg;  n_contrib = 1-based index of the last blended instance.
+// The backward walks each tile's list from the back, recovers T by division,
+// and reduces every Gaussian's nine gradient terms across the wave with DPP
+// adds, then across the four waves in LDS, before one global atomic per
+// (tile, Gaussian, term).
+#include "kernels.hpp"
+
+namespace omnigs {
+
+constexpr int kBatch = kTilePixels;  // instances staged per LDS round
+constexpr int kWavesPerTile = kTilePixels / kWave;
+
+__device__ __forceinline__ void pixel_of(int tile, int gx, int tid, int& px, int& py) {
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  const int tx = tile % gx, ty = tile / gx;
+  px = tx * kTileX + (w & 1) * 8 + (lane & 7);
+  py = ty * kTileY + (w >> 1) * 8 + (lane >> 3);
+}
+
+__global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
+    int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
+    const float2* __restrict__ xy, const float* __restrict__ colors, const float4* __restrict__ conic_opacity,
+    float* __restrict__ final_T, uint32_t* __restrict__ n_contrib, const float* __restrict__ bg,
+    float* __restrict__ out_color) {
+  __shared__ float2 s_xy[kBatch];
+  __shared__ float4 s_co[kBatch];
+  __shared__ float s_rgb[kChannels][kBatch];
+
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x;
+  int px, py;
+  pixel_of(tile, gx, tid, px, py);
+  const bool inside = px < W && py < H;
+  const float fx = (float)px, fy = (float)py;
+  const uint2 range = ranges[tile];
+  const int n = (int)(range.y - range.x);
+
+  bool done = !inside;
+  float T = 1.0f;
+  float C0 = 0.f, C1 = 0.f, C2 = 0.f;
+  uint32_t contributor = 0, last = 0;
+
+  for (int base = 0; base < n; base += kBatch) {
+    if (__syncthreads_count(done) == kTilePixels) break;
+    const int k = base + tid;
+    if (k < n) {
+      const uint32_t g = point_list[range.x + k];
+      s_xy[tid] = xy[g];
+      s_co[tid] = conic_opacity[g];
+      s_rgb[0][tid] = colors[3 * g];
+      s_rgb[1][tid] = colors[3 * g + 1];
+      s_rgb[2][tid] = colors[3 * g + 2];
+    }
+    __syncthreads();
+    const int cnt = min(kBatch, n - base);
+    for (int j = 0; !done && j < cnt; ++j) {
+      ++contributor;
+      const float2 m = s_xy[j];
+      const float4 co = s_co[j];
+      const float dx = m.x - fx, dy = m.y - fy;
+      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+      if (power > 0.0f) continue;
+      const float alpha = fminf(0.99f, co.w * expf(power));
+      if (alpha < 1.0f / 255.0f) continue;
+      const float test_T = T * (1.0f - alpha);
+      if (test_T < 0.0001f) {
+        done = true;
+        continue;
+      }
+      const float wgt = alpha * T;
+      C0 += s_rgb[0][j] * wgt;
+      C1 += s_rgb[1][j] * wgt;
+      C2 += s_rgb[2][j] * wgt;
+      T = test_T;
+      last = contributor;
+    }
+  }
+
+  if (inside) {
+    const size_t pid = (size_t)py * W + px;
+    const size_t N = (size_t)W * H;
+    final_T[pid] = T;
+    n_contrib[pid] = last;
+    out_color[pid] = C0 + T * bg[0];
+    out_color[N + pid] = C1 + T * bg[1];
+    out_color[2 * N + pid] = C2 + T * bg[2];
+  }
+}
+
+// Sum of v over the 64 lanes, valid in lane 63 (DPP: quad swaps, row shifts,
+// row broadcasts 15 and 31).
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ float dpp_shift(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, RowMask, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum_lane63(float v) {
+  v += dpp_shift<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_shift<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_shift<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_shift<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_shift<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_shift<0x143, 0xc>(v);  // row_bcast:31
+  return v;
+}
+
+constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
+
+__global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
+    int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
+    const float* __restrict__ bg, const float2* __restrict__ xy, const float4* __restrict__ conic_opacity,
+    const float* __restrict__ colors, const float* __restrict__ final_Ts, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ dL_dpix, float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
+    float* __restrict__ dL_dopacity, float* __restrict__ dL_dcolors) {
+  __shared__ uint32_t s_gid[kBatch];
+  __shared__ float2 s_xy[kBatch];
+  __shared__ float4 s_co[kBatch];
+  __shared__ float s_rgb[kChannels][kBatch];
+  __shared__ float s_acc[kWavesPerTile][kGradTerms][kBatch];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  const int tile = blockIdx.x;
+  int px, py;
+  pixel_of(tile, gx, tid, px, py);
+  const bool inside = px < W && py < H;
+  const float fx = (float)px, fy = (float)py;
+  const uint2 range = ranges[tile];
+  const int n = (int)(range.y - range.x);
+  const size_t N = (size_t)W * H;
+  const size_t pid = (size_t)py * W + px;
+
+  const float T_final = inside ? final_Ts[pid] : 0.f;
+  float T = T_final;
+  const int last = inside ? (int)n_contrib[pid] : 0;
+  float dp0 = 0.f, dp1 = 0.f, dp2 = 0.f;
+  if (inside) {
+    dp0 = dL_dpix[pid];
+    dp1 = dL_dpix[N + pid];
+    dp2 = dL_dpix[2 * N + pid];
+  }
+  const float bg_dot = bg[0] * dp0 + bg[1] * dp1 + bg[2] * dp2;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+  float lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
+  float last_alpha = 0.f;
+  const float ddelx_dx = 0.5f * (float)W, ddely_dy = 0.5f * (float)H;
+
+  for (int base = 0; base < n; base += kBatch) {
+    __syncthreads();
+    const int k = base + tid;
+    if (k < n) {
+      const uint32_t g = point_list[range.y - 1 - k];
+      s_gid[tid] = g;
+      s_xy[tid] = xy[g];
+      s_co[tid] = conic_opacity[g];
+      s_rgb[0][tid] = colors[3 * g];
+      s_rgb[1][tid] = colors[3 * g + 1];
+      s_rgb[2][tid] = colors[3 * g + 2];
+    }
+#pragma unroll
+    for (int ww = 0; ww < kWavesPerTile; ++ww)
+#pragma unroll
+      for (int q = 0; q < kGradTerms; ++q) s_acc[ww][q][tid] = 0.f;
+    __syncthreads();
+
+    const int cnt = min(kBatch, n - base);
+    for (int j = 0; j < cnt; ++j) {
+      // 0-based position of this instance in the tile's front-to-back list
+      const int pos = n - 1 - (base + j);
+      bool ok = inside && pos < last;
+      float g_c0 = 0.f, g_c1 = 0.f, g_c2 = 0.f, g_mx = 0.f, g_my = 0.f, g_a = 0.f, g_b = 0.f, g_cc = 0.f,
+            g_o = 0.f;
+      if (ok) {
+        const float2 m = s_xy[j];
+        const float4 co = s_co[j];
+        const float dx = m.x - fx, dy = m.y - fy;
+        const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+        const float G = expf(power);
+        const float alpha = fminf(0.99f, co.w * G);
+        ok = power <= 0.0f && alpha >= 1.0f / 255.0f;
+        if (ok) {
+          T = T / (1.f - alpha);
+          const float dchannel_dcolor = alpha * T;
+          const float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
+          acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
+          acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
+          acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
+          lc0 = c0;
+          lc1 = c1;
+          lc2 = c2;
+          float dL_dalpha = (c0 - acc0) * dp0 + (c1 - acc1) * dp1 + (c2 - acc2) * dp2;
+          g_c0 = dchannel_dcolor * dp0;
+          g_c1 = dchannel_dcolor * dp1;
+          g_c2 = dchannel_dcolor * dp2;
+          dL_dalpha *= T;
+          last_alpha = alpha;
+          dL_dalpha += (-T_final / (1.f - alpha)) * bg_dot;
+          const float dL_dG = co.w * dL_dalpha;
+          const float gdx = G * dx, gdy = G * dy;
+          const float dG_ddelx = -gdx * co.x - gdy * co.y;
+          const float dG_ddely = -gdy * co.z - gdx * co.y;
+          g_mx = dL_dG * dG_ddelx * ddelx_dx;
+          g_my = dL_dG * dG_ddely * ddely_dy;
+          g_a = -0.5f * gdx * dx * dL_dG;
+          g_b = -0.5f * gdx * dy * dL_dG;
+          g_cc = -0.5f * gdy * dy * dL_dG;
+          g_o = G * dL_dalpha;
+        }
+      }
+      if (__ballot(ok) == 0ull) continue;  // wave-uniform
+      g_c0 = wave_sum_lane63(g_c0);
+      g_c1 = wave_sum_lane63(g_c1);
+      g_c2 = wave_sum_lane63(g_c2);
+      g_mx = wave_sum_lane63(g_mx);
+      g_my = wave_sum_lane63(g_my);
+      g_a = wave_sum_lane63(g_a);
+      g_b = wave_sum_lane63(g_b);
+      g_cc = wave_sum_lane63(g_cc);
+      g_o = wave_sum_lane63(g_o);
+      if (lane == kWave - 1) {
+        s_acc[w][0][j] = g_c0;
+        s_acc[w][1][j] = g_c1;
+        s_acc[w][2][j] = g_c2;
+        s_acc[w][3][j] = g_mx;
+        s_acc[w][4][j] = g_my;
+        s_acc[w][5][j] = g_a;
+        s_acc[w][6][j] = g_b;
+        s_acc[w][7][j] = g_cc;
+        s_acc[w][8][j] = g_o;
+      }
+    }
+    __syncthreads();
+    if (tid < cnt) {
+      float v[kGradTerms];
+#pragma unroll
+      for (int q = 0; q < kGradTerms; ++q) {
+        float s = 0.f;
+#pragma unroll
+        for (int ww = 0; ww < kWavesPerTile; ++ww) s += s_acc[ww][q][tid];
+        v[q] = s;
+      }
+      const uint32_t g = s_gid[tid];
+      float* dst[kGradTerms] = {&dL_dcolors[3 * g],     &dL_dcolors[3 * g + 1], &dL_dcolors[3 * g + 2],
+                                &dL_dmean2D[3 * g],     &dL_dmean2D[3 * g + 1], &dL_dconic[4 * g],
+                                &dL_dconic[4 * g + 1],  &dL_dconic[4 * g + 3],  &dL_dopacity[g]};
+#pragma unroll
+      for (int q = 0; q < kGradTerms; ++q)
+        if (v[q] != 0.f) unsafeAtomicAdd(dst[q], v[q]);
+    }
+  }
+}
+
+hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
+                                 const float2* xy, const float* colors, const float4* conic_opacity,
+                                 float* final_T, uint32_t* n_contrib, const float* bg, float* out_color,
+                                 hipStream_t s) {
+  const int tiles = grid.tiles();
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(render_forward_kernel, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list, W, H,
+                     xy, colors, conic_opacity, final_T, n_contrib, bg, out_color);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
+                                  const float* bg, const float2* xy, const float4* conic_opacity,
+                                  const float* colors, const float* final_T, const uint32_t* n_contrib,
+                                  const float* dL_dpix, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
+                                  float* dL_dcolors, hipStream_t s) {
+  const int tiles = grid.tiles();
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(render_backward_kernel, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list, W,
+                     H, bg, xy, conic_opacity, colors, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic,
+                     dL_dopacity, dL_dcolors);
+  return hipGetLastError();
+}
+
+}  // namespace omnigs

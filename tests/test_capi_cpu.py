@@ -1,0 +1,93 @@
+"""CPU-only checks of the C ABI library: it loads without a GPU, exports every
+function include/omnigs_lonlat.h declares, and its host-side validation and
+size queries behave (no GPU compute is called here)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "omnigs_lonlat.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(omnigs_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from omnigs_amd import capi
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.fail("libomnigs_lonlat.so not built (run __graft_entry__.build())")
+    return capi.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    declared = _declared_functions()
+    assert len(declared) >= 10
+    from omnigs_amd import capi
+    assert set(declared) == set(capi.EXPORTED_SYMBOLS), set(declared) ^ set(capi.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_version_and_status_strings(lib):
+    assert b"gfx950" in lib.omnigs_version()
+    assert lib.omnigs_status_string(0) == b"ok"
+    assert b"precomputed" in lib.omnigs_status_string(2)
+
+
+def test_buffer_sizes_scale(lib):
+    g1, g2 = lib.omnigs_lonlat_geometry_bytes(1000), lib.omnigs_lonlat_geometry_bytes(2000)
+    assert 60 * 1000 < g1 < g2
+    assert lib.omnigs_lonlat_binning_bytes(0) > 0
+    b1, b2 = lib.omnigs_lonlat_binning_bytes(10000), lib.omnigs_lonlat_binning_bytes(20000)
+    assert 16 * 10000 <= b1 < b2
+    assert lib.omnigs_lonlat_image_bytes(64, 32) >= 64 * 32 * 8 + 8 * 4 * 2
+    assert lib.omnigs_lonlat_image_bytes(0, 32) == 0
+
+
+def test_argument_validation_needs_no_gpu(lib):
+    from omnigs_amd import capi
+    a = capi.ForwardArgs()
+    a.P, a.width, a.height = -1, 64, 32
+    R = ctypes.c_int(7)
+    null_buf = capi.Buffer(capi.RESIZE_FN(0), None)
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
+    assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT and R.value == 0
+    assert b"P < 0" in lib.omnigs_last_error()
+    a.P = 0  # the reference skips the rasterizer for P == 0
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
+    assert st == capi.OMNIGS_OK and R.value == 0
+    a.P = 10  # required pointers missing
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
+    assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    b = capi.BackwardArgs()
+    b.P, b.R, b.width, b.height = 10, 5, 64, 32
+    assert lib.omnigs_lonlat_backward(ctypes.byref(b), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    assert lib.omnigs_lonlat_mark_visible(-1, None, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    assert lib.omnigs_lonlat_mark_visible(0, None, None) == capi.OMNIGS_OK
+
+
+def test_state_view_offsets_are_inside_buffers(lib):
+    from omnigs_amd import capi
+    P, R, W, H = 1000, 5000, 200, 90
+    base_g, base_b, base_i = 0x100000, 0x900000, 0x2000000
+    sv = capi.StateView()
+    assert lib.omnigs_lonlat_state(base_g, base_b, base_i, P, R, W, H, ctypes.byref(sv)) == 0
+    gsz = lib.omnigs_lonlat_geometry_bytes(P)
+    bsz = lib.omnigs_lonlat_binning_bytes(R)
+    isz = lib.omnigs_lonlat_image_bytes(W, H)
+    for f in ("depth_key", "radii", "means2D", "conic_opacity", "rgb", "clamped", "tiles_touched",
+              "depth_order", "offsets"):
+        v = getattr(sv, f)
+        assert base_g <= v < base_g + gsz and v % 256 == 0, f
+    for f in ("point_list", "tile_keys"):
+        assert base_b <= getattr(sv, f) < base_b + bsz, f
+    for f in ("final_T", "n_contrib", "ranges"):
+        assert base_i <= getattr(sv, f) < base_i + isz, f

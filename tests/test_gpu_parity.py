@@ -1,0 +1,303 @@
+"""Parity of the gfx950 HIP path (called through the C ABI) with the CPU oracle.
+
+Bars:
+* integer work (radii, tile counts, instance lists, tile ranges, n_contrib):
+  bit-exact given identical upstream outputs;
+* image / final_T: |gpu - oracle| <= 1e-4 absolute;
+* gradients: |gpu - oracle| <= 1e-3 |oracle| + 1e-4 max|oracle| per element
+  (the second term absorbs float-atomic reordering on cancelling sums);
+* the GPU's error against the FP64 oracle is reported beside the FP32 bar.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from omnigs_amd import scenes
+from tests.helpers import allclose_grad
+
+pytestmark = pytest.mark.gpu
+
+IMG_ATOL = 1e-4
+GRAD_RTOL = 1e-3
+GRAD_SCALE_ATOL = 1e-4
+
+
+@pytest.fixture(autouse=True)
+def _name(request):
+    from tests.gpu_util import announce
+    announce(request.node.nodeid)
+    yield
+
+
+def _oracle(sc, prec=32):
+    from oracle import Oracle
+    return Oracle(sc, prec)
+
+
+def _setup(sc):
+    from tests.gpu_util import gpu_state, run_forward, to_dev
+    ds = to_dev(sc)
+    R, img, radii, bufs = run_forward(ds)
+    st = gpu_state(bufs, sc["means3D"].shape[0], R, sc["width"], sc["height"])
+    return ds, R, img, radii, bufs, st
+
+
+def _radii_agree(gr, orr, oracle_obj=None):
+    mism = np.nonzero(gr != orr)[0]
+    assert len(mism) <= max(2, int(1e-4 * len(gr))), f"{len(mism)} radii differ"
+    assert np.all(np.abs(gr[mism].astype(np.int64) - orr[mism]) <= 1) or np.all((gr[mism] == 0) | (orr[mism] == 0))
+    return mism
+
+
+def test_c_abi_loads_and_reports(gpu):
+    from omnigs_amd import capi
+    lib = capi.load()
+    assert b"gfx950" in lib.omnigs_version()
+    present = capi.mark_visible(1000, gpu)
+    assert bool(present.all())
+
+
+@pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0), (3000, 200, 90, 1)])
+def test_preprocess_parity(gpu, P, W, H, seed):
+    from tests.gpu_util import np_
+    sc = scenes.small_scene(P=P, width=W, height=H, seed=seed)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    o = _oracle(sc).run("preprocess")
+    gr, orr = np_(radii), o.get("radii")
+    mism = _radii_agree(gr, orr)
+    vis = (gr > 0) & (orr > 0)
+    vis[mism] = False
+    np.testing.assert_array_equal(st["tiles_touched"][vis], o.get("tiles_touched")[vis])
+    np.testing.assert_allclose(st["means2D"].reshape(-1, 2)[vis], o.get("xy").reshape(-1, 2)[vis], atol=2e-3)
+    gc, oc = st["conic_opacity"].reshape(-1, 4)[vis], o.get("conic").reshape(-1, 4)[vis]
+    np.testing.assert_allclose(gc, oc, rtol=2e-4, atol=2e-4 * np.abs(oc).max())
+    np.testing.assert_allclose(st["rgb"].reshape(-1, 3)[vis], o.get("rgb").reshape(-1, 3)[vis], atol=2e-5)
+    np.testing.assert_allclose(st["depth"][vis], o.get("depth")[vis], rtol=1e-6)
+    cl = st["clamped"][vis]
+    ocl = o.get("clamped").reshape(-1, 3)[vis]
+    ocl_bits = ocl[:, 0] | (ocl[:, 1] << 1) | (ocl[:, 2] << 2)
+    assert np.mean(cl != ocl_bits) < 1e-4
+
+
+@pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0), (3000, 200, 90, 1), (800, 64, 32, 2)])
+def test_binning_bit_exact_given_identical_preprocess(gpu, P, W, H, seed):
+    sc = scenes.small_scene(P=P, width=W, height=H, seed=seed)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    from tests.gpu_util import np_
+    o = _oracle(sc).run("preprocess")
+    o.set("radii", np_(radii))
+    o.set("xy", st["means2D"])
+    o.set("depth", st["depth"])
+    o.set("tiles_touched", st["tiles_touched"])
+    o.run("binning")
+    assert R == o.num_rendered
+    np.testing.assert_array_equal(st["point_list"], o.get("point_list"))
+    np.testing.assert_array_equal(st["ranges"], o.get("ranges"))
+    np.testing.assert_array_equal(st["tile_keys"], (o.get("keys") >> np.uint64(32)).astype(np.uint32))
+
+
+@pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0), (3000, 200, 90, 1)])
+def test_render_forward_given_identical_binning(gpu, P, W, H, seed):
+    sc = scenes.small_scene(P=P, width=W, height=H, seed=seed)
+    sc["background"] = np.array([0.1, 0.2, 0.3], dtype=np.float32)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    from tests.gpu_util import np_
+    o = _oracle(sc).run("preprocess")
+    for k, v in (("xy", st["means2D"]), ("conic", st["conic_opacity"]), ("rgb", st["rgb"]),
+                 ("point_list", st["point_list"]), ("ranges", st["ranges"])):
+        o.set(k, v)
+    o.run("render")
+    np.testing.assert_allclose(np_(img).reshape(-1), o.get("out_color"), atol=IMG_ATOL)
+    np.testing.assert_allclose(st["final_T"], o.get("final_T"), atol=IMG_ATOL)
+    nc_mism = np.mean(st["n_contrib"] != o.get("n_contrib"))
+    assert nc_mism < 1e-4, nc_mism
+
+
+def _grad_check(name, g, o_val):
+    ok, worst = allclose_grad(g, o_val, GRAD_RTOL, GRAD_SCALE_ATOL)
+    assert ok, f"{name}: worst tolerance ratio {worst:.3g}"
+
+
+@pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0), (3000, 200, 90, 1)])
+def test_render_backward_given_identical_forward(gpu, P, W, H, seed):
+    import torch
+    sc = scenes.small_scene(P=P, width=W, height=H, seed=seed)
+    sc["background"] = np.array([0.1, 0.2, 0.3], dtype=np.float32)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    from tests.gpu_util import np_, run_backward
+    dL = torch.from_numpy(sc["dL_dpix"]).to(gpu)
+    g = run_backward(ds, R, radii, bufs, dL)
+    o = _oracle(sc).run("preprocess")
+    for k, v in (("xy", st["means2D"]), ("conic", st["conic_opacity"]), ("rgb", st["rgb"]),
+                 ("point_list", st["point_list"]), ("ranges", st["ranges"]), ("final_T", st["final_T"]),
+                 ("n_contrib", st["n_contrib"])):
+        o.set(k, v)
+    o.run("render_backward", sc["dL_dpix"])
+    _grad_check("dL_dmean2D", np_(g["dL_dmeans2D"]).reshape(-1), o.get("dL_dmean2D"))
+    _grad_check("dL_dconic", np_(g["dL_dconic"]).reshape(-1), o.get("dL_dconic"))
+    _grad_check("dL_dopacity", np_(g["dL_dopacity"]).reshape(-1), o.get("dL_dopacity"))
+    _grad_check("dL_dcolors", np_(g["dL_dcolors"]).reshape(-1), o.get("dL_dcolors"))
+
+
+@pytest.mark.parametrize("P,W,H,seed,degree", [(20000, 512, 256, 0, 3), (3000, 200, 90, 1, 1)])
+def test_preprocess_backward_given_identical_inputs(gpu, P, W, H, seed, degree):
+    import torch
+    sc = scenes.small_scene(P=P, width=W, height=H, seed=seed)
+    sc["degree"] = degree
+    ds, R, img, radii, bufs, st = _setup(sc)
+    from tests.gpu_util import np_, run_backward
+    dL = torch.from_numpy(sc["dL_dpix"]).to(gpu)
+    g = run_backward(ds, R, radii, bufs, dL)
+    o = _oracle(sc).run("preprocess")
+    clamped = st["clamped"]
+    o.set("radii", np_(radii))
+    o.set("clamped", np.stack([clamped & 1, (clamped >> 1) & 1, (clamped >> 2) & 1], 1).astype(np.uint8))
+    o.set("dL_dmean2D", np_(g["dL_dmeans2D"]))
+    o.set("dL_dconic", np_(g["dL_dconic"]))
+    o.set("dL_dcolors", np_(g["dL_dcolors"]))
+    o.set("dL_dopacity", np_(g["dL_dopacity"]))
+    o.run("cov2d_backward").run("preprocess_backward")
+    for mine, ref in (("dpx_dt", "dpx_dt"), ("dpy_dt", "dpy_dt"), ("dL_dcov3D", "dL_dcov3D"),
+                      ("dL_dmeans3D", "dL_dmeans3D"), ("dL_dsh", "dL_dsh"), ("dL_dscales", "dL_dscales"),
+                      ("dL_drotations", "dL_drot")):
+        _grad_check(mine, np_(g[mine]).reshape(-1), o.get(ref))
+
+
+@pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0)])
+def test_end_to_end_against_fp32_and_fp64(gpu, P, W, H, seed):
+    """The 20k-Gaussian 512x256 case, fwd + bwd, all stages on the GPU."""
+    import torch
+    sc = scenes.small_scene(P=P, width=W, height=H, seed=seed)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    from tests.gpu_util import np_, run_backward
+    g = run_backward(ds, R, radii, bufs, torch.from_numpy(sc["dL_dpix"]).to(gpu))
+    o32 = _oracle(sc, 32).forward().backward(sc["dL_dpix"])
+    o64 = _oracle(sc, 64).forward().backward(sc["dL_dpix"])
+    assert R == o32.num_rendered
+    e32 = np.abs(np_(img).reshape(-1) - o32.get("out_color")).max()
+    e64 = np.abs(np_(img).reshape(-1) - o64.get("out_color")).max()
+    print(f"image max|gpu-fp32|={e32:.3g} max|gpu-fp64|={e64:.3g}")
+    assert e32 <= IMG_ATOL
+    for mine, ref in (("dL_dmeans3D", "dL_dmeans3D"), ("dL_dsh", "dL_dsh"), ("dL_dscales", "dL_dscales"),
+                      ("dL_drotations", "dL_drot"), ("dL_dopacity", "dL_dopacity"), ("dL_dmeans2D", "dL_dmean2D")):
+        a = np_(g[mine]).reshape(-1)
+        _grad_check(mine, a, o32.get(ref))
+        b = o64.get(ref)
+        print(f"{mine}: max|gpu-fp64|/max|fp64| = {np.abs(a - b).max() / max(np.abs(b).max(), 1e-30):.3g}")
+
+
+def test_edge_cases_empty_and_all_culled(gpu):
+    import torch
+    from tests.gpu_util import np_, run_backward, run_forward, to_dev
+    sc = scenes.small_scene(P=10, width=64, height=32, seed=3)
+    # P == 0: no work, image untouched (zeros), R = 0
+    empty = {k: (v[:0] if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[0] == 10 else v)
+             for k, v in sc.items()}
+    ds = to_dev(empty)
+    R, img, radii, bufs = run_forward(ds)
+    assert R == 0 and float(img.abs().sum()) == 0.0 and radii.numel() == 0
+    # every Gaussian inside the 0.2 m cull radius: R = 0, image = background
+    sc2 = dict(sc)
+    sc2["means3D"] = (sc["means3D"] / np.linalg.norm(sc["means3D"], axis=1, keepdims=True) * 0.1).astype(np.float32)
+    sc2["background"] = np.array([0.25, 0.5, 0.75], dtype=np.float32)
+    ds2 = to_dev(sc2)
+    R, img, radii, bufs = run_forward(ds2)
+    assert R == 0 and int(radii.abs().sum()) == 0
+    for c in range(3):
+        assert torch.all(img[c] == float(sc2["background"][c]))
+    g = run_backward(ds2, R, radii, bufs, torch.ones((3, 32, 64), device=gpu))
+    for k, v in g.items():
+        assert float(v.abs().sum()) == 0.0, k
+
+
+@pytest.mark.parametrize("variant", ["colors_precomp", "cov3D_precomp", "degree0", "M9_degree2", "whale",
+                                     "seam"])
+def test_variants_end_to_end(gpu, variant):
+    import torch
+    from tests.gpu_util import np_, run_backward
+    sc = scenes.small_scene(P=2000, width=256, height=128, seed=21)
+    if variant == "colors_precomp":
+        sc.pop("shs")
+        sc["colors_precomp"] = np.random.default_rng(0).uniform(0, 1, size=(2000, 3)).astype(np.float32)
+    elif variant == "cov3D_precomp":
+        q, s = sc.pop("rotations").astype(np.float64), sc.pop("scales").astype(np.float64)
+        w, x, y, z = q.T
+        R_ = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                       np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                       np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+        Sg = np.einsum("pij,pj,pkj->pik", R_, s ** 2, R_)
+        sc["cov3D_precomp"] = np.stack([Sg[:, 0, 0], Sg[:, 0, 1], Sg[:, 0, 2], Sg[:, 1, 1], Sg[:, 1, 2],
+                                        Sg[:, 2, 2]], -1).astype(np.float32)
+    elif variant == "degree0":
+        sc["degree"] = 0
+    elif variant == "M9_degree2":
+        sc["shs"] = np.ascontiguousarray(sc["shs"][:, :9, :])
+        sc["degree"] = 2
+    elif variant == "whale":
+        sc["means3D"][:5] = np.array([[0.0, 0.0, 0.5], [0.3, 0.1, -0.4], [0.0, 0.45, 0.1], [-0.4, -0.2, 0.0],
+                                      [0.25, 0.0, 0.25]], dtype=np.float32)
+        sc["scales"][:5] = 0.3
+    elif variant == "seam":
+        rng = np.random.default_rng(5)
+        lon = np.pi + rng.normal(0, 0.03, size=2000)
+        lat = rng.uniform(-0.5, 0.5, size=2000)
+        rho = rng.uniform(1, 4, size=(2000, 1))
+        sc["means3D"] = (rho * np.stack([np.cos(lat) * np.sin(lon), np.sin(lat), np.cos(lat) * np.cos(lon)],
+                                        1)).astype(np.float32)
+    sc["background"] = np.array([0.1, 0.2, 0.3], dtype=np.float32)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    g = run_backward(ds, R, radii, bufs, torch.from_numpy(sc["dL_dpix"]).to(gpu))
+    o = _oracle(sc, 32).forward().backward(sc["dL_dpix"])
+    assert R == o.num_rendered
+    np.testing.assert_allclose(np_(img).reshape(-1), o.get("out_color"), atol=IMG_ATOL)
+    names = [("dL_dmeans3D", "dL_dmeans3D"), ("dL_dopacity", "dL_dopacity"), ("dL_dcov3D", "dL_dcov3D")]
+    if variant == "colors_precomp":
+        names.append(("dL_dcolors", "dL_dcolors"))
+    else:
+        names.append(("dL_dsh", "dL_dsh"))
+    if variant != "cov3D_precomp":
+        names += [("dL_dscales", "dL_dscales"), ("dL_drotations", "dL_drot")]
+    for mine, ref in names:
+        _grad_check(f"{variant}:{mine}", np_(g[mine]).reshape(-1), o.get(ref))
+
+
+def test_full_size_properties(gpu):
+    """1.5M Gaussians at 1920x960 (the room scene): the integer path is checked
+    bit-exactly against the oracle's preprocess-fed binning, and
+    size-independent properties are checked on the GPU outputs."""
+    import torch
+    from tests.gpu_util import gpu_state, np_, run_backward, run_forward, to_dev
+    sc = scenes.room_scene()
+    P, W, H = sc["means3D"].shape[0], sc["width"], sc["height"]
+    ds = to_dev(sc)
+    R, img, radii, bufs = run_forward(ds)
+    st = gpu_state(bufs, P, R, W, H)
+    tt = st["tiles_touched"].astype(np.int64)
+    assert R == int(tt.sum())
+    # binning bit-exact given the GPU's own preprocess outputs
+    o = _oracle({k: v for k, v in sc.items() if k != "target"}, 32)
+    o.set("radii", np_(radii)); o.set("xy", st["means2D"]); o.set("depth", st["depth"])
+    o.set("tiles_touched", st["tiles_touched"])
+    o.run("binning")
+    assert o.num_rendered == R
+    np.testing.assert_array_equal(st["point_list"], o.get("point_list"))
+    np.testing.assert_array_equal(st["ranges"], o.get("ranges"))
+    # determinism: a second forward is bit-identical
+    R2, img2, radii2, bufs2 = run_forward(ds)
+    assert R2 == R and torch.equal(img, img2) and torch.equal(radii, radii2)
+    # background identity: image(bg=1) - image(bg=0) = final_T (within 1 ulp of 1)
+    ds1 = dict(ds); ds1["background"] = torch.ones(3, device=gpu)
+    _, img1, _, _ = run_forward(ds1)
+    T = torch.from_numpy(st["final_T"]).to(gpu).reshape(H, W)
+    assert float((img1 - img - T).abs().max()) <= 2.5e-7
+    assert float(T.min()) >= 0.9999e-4 and float(T.max()) <= 1.0
+    # backward is linear in dL/dpix
+    target = torch.from_numpy(sc["target"]).to(gpu)
+    dL = torch.sign(img - target) / (3.0 * H * W)
+    g1 = run_backward(ds, R, radii, bufs, dL, want_jacobian=False)
+    g2 = run_backward(ds, R, radii, bufs, 2.0 * dL, want_jacobian=False)
+    for k in ("dL_dmeans3D", "dL_dsh", "dL_dopacity", "dL_dscales", "dL_drotations"):
+        a, b = g1[k], g2[k]
+        err = float((2 * a - b).abs().max()) / max(float(b.abs().max()), 1e-30)
+        assert err < 1e-5, (k, err)
