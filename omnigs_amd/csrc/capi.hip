@@ -232,7 +232,7 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geo
   const BinState b = carve_bin(bbuf, R);
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
-    OMNIGS_TRY(launch_duplicate(P, ds.vals, g.offsets, g.xy, radii, grid, b.tile_key, b.gid, s), "duplicate");
+    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.xy, radii, grid, b.tile_key, b.gid, s), "duplicate");
   }
   SortResult ts{b.tile_key, b.gid};
   if (R > 0) {

@@ -86,7 +86,7 @@ template <class T> struct Carver {
 
 // Radix sort geometry: 256 threads x 16 keys per block, 8-bit digits.
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
@@ -112,7 +112,7 @@ struct GeomState {
   uint32_t* key_alt;       // ping-pong keys for the depth sort
   uint32_t* offsets;       // inclusive scan of tiles_touched in depth order
   uint32_t* scan_partials; // scan block sums
-  uint32_t* sort_counts;   // radix digit counts [kRadix, blocks]
+  uint32_t* sort_counts;   // radix digit counts [kRadix, blocks] + [kRadix] totals
   size_t bytes;
 };
 
@@ -131,7 +131,7 @@ inline GeomState carve_geom(char* base, size_t P) {
   g.key_alt = c.take<uint32_t>(P);
   g.offsets = c.take<uint32_t>(P);
   g.scan_partials = c.take<uint32_t>(scan_blocks(P) + 1);
-  g.sort_counts = c.take<uint32_t>((size_t)kRadix * sort_blocks(P) + 1);
+  g.sort_counts = c.take<uint32_t>((size_t)kRadix * (sort_blocks(P) + 1));
   g.bytes = align_up(c.off) + kAlign;
   return g;
 }
@@ -154,9 +154,8 @@ inline BinState carve_bin(char* base, size_t R) {
   b.tile_key_alt = c.take<uint32_t>(R);
   b.gid = c.take<uint32_t>(R);
   b.gid_alt = c.take<uint32_t>(R);
-  const size_t cnt = (size_t)kRadix * sort_blocks(R) + 1;
-  b.sort_counts = c.take<uint32_t>(cnt);
-  b.scan_partials = c.take<uint32_t>(scan_blocks(cnt) + 1);
+  b.sort_counts = c.take<uint32_t>((size_t)kRadix * (sort_blocks(R) + 1));
+  b.scan_partials = c.take<uint32_t>(1);
   b.bytes = align_up(c.off) + kAlign;
   return b;
 }

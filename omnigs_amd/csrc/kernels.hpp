@@ -63,7 +63,7 @@ struct SortResult {
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
                              hipStream_t s, SortResult* out);
-hipError_t launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float2* xy,
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float2* xy,
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s);

@@ -26,6 +26,46 @@ __device__ __forceinline__ void pixel_of(int tile, int gx, int tid, int& px, int
   py = ty * kTileY + (w >> 1) * 8 + (lane >> 3);
 }
 
+// Conservative 4-bit mask of the tile's 8x8 wave quadrants that Gaussian
+// (m, co) can reach with alpha >= 1/255: outside the ellipse
+// 0.5 d^T Q d <= ln(255 o) the alpha test fails for every pixel, so skipping
+// those (wave, instance) pairs changes no result.  The box is padded by
+// 1% + 0.05 px against rounding; degenerate conics get the full mask.
+__device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile, int gx) {
+  const float det = co.x * co.z - co.y * co.y;
+  const float o255 = 255.0f * co.w;
+  if (!(o255 > 0.9999f)) return 0u;  // alpha < 1/255 everywhere
+  if (!(det > 0.f) || !(co.x > 0.f) || !(co.z > 0.f)) return 0xFu;
+  const float tau2 = fmaxf(2.0f * __logf(o255) * 1.0001f, 0.f) + 1e-3f;
+  const float ex = sqrtf(tau2 * co.z / det) * 1.01f + 0.05f;
+  const float ey = sqrtf(tau2 * co.x / det) * 1.01f + 0.05f;
+  const float x0 = (float)((tile % gx) * kTileX), y0 = (float)((tile / gx) * kTileY);
+  uint32_t mk = 0;
+#pragma unroll
+  for (int q = 0; q < kWavesPerTile; ++q) {
+    const float qx = x0 + (float)((q & 1) * 8), qy = y0 + (float)((q >> 1) * 8);
+    const bool hit = (m.x + ex >= qx) && (m.x - ex <= qx + 7.f) && (m.y + ey >= qy) && (m.y - ey <= qy + 7.f);
+    mk |= hit ? (1u << q) : 0u;
+  }
+  return mk;
+}
+
+// Each wave compacts the batch positions whose mask has its bit set into its
+// own ascending list; returns the list length (wave-uniform).
+__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* list, int cnt, int w, int lane) {
+  int count = 0;
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int c = 0; c < kBatch / kWave; ++c) {
+    const int j = c * kWave + lane;
+    const bool on = j < cnt && ((s_mask[j] >> w) & 1u);
+    const uint64_t bal = __ballot(on);
+    if (on) list[count + __popcll(bal & below)] = (uint8_t)j;
+    count += __popcll(bal);
+  }
+  return count;
+}
+
 __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
     const float2* __restrict__ xy, const float* __restrict__ colors, const float4* __restrict__ conic_opacity,
@@ -34,8 +74,11 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_co[kBatch];
   __shared__ float s_rgb[kChannels][kBatch];
+  __shared__ uint8_t s_mask[kBatch];
+  __shared__ uint8_t s_list[kWavesPerTile][kBatch];
 
   const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), w = tid / kWave;
   const int tile = blockIdx.x;
   int px, py;
   pixel_of(tile, gx, tid, px, py);
@@ -47,23 +90,27 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   bool done = !inside;
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
-  uint32_t contributor = 0, last = 0;
+  uint32_t last = 0;
 
   for (int base = 0; base < n; base += kBatch) {
     if (__syncthreads_count(done) == kTilePixels) break;
     const int k = base + tid;
     if (k < n) {
       const uint32_t g = point_list[range.x + k];
-      s_xy[tid] = xy[g];
-      s_co[tid] = conic_opacity[g];
+      const float2 m = xy[g];
+      const float4 co = conic_opacity[g];
+      s_xy[tid] = m;
+      s_co[tid] = co;
       s_rgb[0][tid] = colors[3 * g];
       s_rgb[1][tid] = colors[3 * g + 1];
       s_rgb[2][tid] = colors[3 * g + 2];
+      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
     }
     __syncthreads();
     const int cnt = min(kBatch, n - base);
-    for (int j = 0; !done && j < cnt; ++j) {
-      ++contributor;
+    const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane);
+    for (int ii = 0; ii < len && !done; ++ii) {
+      const int j = s_list[w][ii];
       const float2 m = s_xy[j];
       const float4 co = s_co[j];
       const float dx = m.x - fx, dy = m.y - fy;
@@ -81,7 +128,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       C1 += s_rgb[1][j] * wgt;
       C2 += s_rgb[2][j] * wgt;
       T = test_T;
-      last = contributor;
+      last = (uint32_t)(base + j + 1);  // 1-based position in the tile's list
     }
   }
 
@@ -125,6 +172,8 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   __shared__ float4 s_co[kBatch];
   __shared__ float s_rgb[kChannels][kBatch];
   __shared__ float s_acc[kWavesPerTile][kGradTerms][kBatch];
+  __shared__ uint8_t s_mask[kBatch];
+  __shared__ uint8_t s_list[kWavesPerTile][kBatch];
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), w = tid / kWave;
@@ -158,12 +207,15 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     const int k = base + tid;
     if (k < n) {
       const uint32_t g = point_list[range.y - 1 - k];
+      const float2 m = xy[g];
+      const float4 co = conic_opacity[g];
       s_gid[tid] = g;
-      s_xy[tid] = xy[g];
-      s_co[tid] = conic_opacity[g];
+      s_xy[tid] = m;
+      s_co[tid] = co;
       s_rgb[0][tid] = colors[3 * g];
       s_rgb[1][tid] = colors[3 * g + 1];
       s_rgb[2][tid] = colors[3 * g + 2];
+      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
     }
 #pragma unroll
     for (int ww = 0; ww < kWavesPerTile; ++ww)
@@ -172,7 +224,9 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     __syncthreads();
 
     const int cnt = min(kBatch, n - base);
-    for (int j = 0; j < cnt; ++j) {
+    const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane);
+    for (int ii = 0; ii < len; ++ii) {
+      const int j = s_list[w][ii];
       // 0-based position of this instance in the tile's front-to-back list
       const int pos = n - 1 - (base + j);
       bool ok = inside && pos < last;

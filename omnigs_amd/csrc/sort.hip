@@ -119,9 +119,14 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, s
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits.  Per pass: digit counts per block,
-// an exclusive scan of the digit-major count table, then a stable scatter in
-// which every block ranks its 4096 keys in input order with 64-lane ballots.
+// Stable LSD radix sort, 8-bit digits.  Per pass three kernels:
+//   count   -- digit histogram of every block's tile of keys, digit-major table;
+//   rowscan -- one block per digit: exclusive scan of that digit's row over the
+//              blocks, and the digit's total;
+//   scatter -- every block turns the 256 digit totals into digit bases (LDS
+//              scan), then ranks its keys in input order with 64-lane ballots
+//              and writes each to base[d] + row prefix + rank (stable).
+// counts holds kRadix * blocks row entries followed by kRadix digit totals.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const uint32_t* __restrict__ keys, size_t n,
                                                                    int shift, uint32_t mask,
@@ -139,19 +144,38 @@ __global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const uint32_
   for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[(size_t)d * nblocks + blockIdx.x] = hist[d];
 }
 
+__global__ __launch_bounds__(kSortThreads) void radix_rowscan_kernel(uint32_t* __restrict__ counts,
+                                                                     uint32_t nblocks) {
+  const uint32_t d = blockIdx.x;
+  uint32_t* row = counts + (size_t)d * nblocks;
+  uint32_t carry = 0;
+  for (uint32_t c = 0; c < nblocks; c += kSortThreads) {
+    const uint32_t i = c + threadIdx.x;
+    const uint32_t v = i < nblocks ? row[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, &tot);
+    if (i < nblocks) row[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) counts[(size_t)kRadix * nblocks + d] = carry;
+}
+
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, size_t n, int shift, int bits, const uint32_t* __restrict__ offs,
+    uint32_t* __restrict__ vout, size_t n, int shift, int bits, const uint32_t* __restrict__ counts,
     uint32_t nblocks) {
   constexpr int kWaves = kSortThreads / kWave;
+  static_assert(kRadix == kSortThreads, "one digit per thread in the base scan");
   __shared__ uint32_t run[kRadix];
   __shared__ uint32_t wcnt[kWaves][kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
-  for (int d = tid; d < kRadix; d += kSortThreads) {
-    run[d] = offs[(size_t)d * nblocks + blockIdx.x];
+  {
+    uint32_t tot;
+    const uint32_t digit_base = block_exclusive_scan(counts[(size_t)kRadix * nblocks + tid], &tot);
+    run[tid] = digit_base + counts[(size_t)tid * nblocks + blockIdx.x];
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][d] = 0;
+    for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][tid] = 0;
   }
   __syncthreads();
   const uint64_t lanes_below = (1ull << lane) - 1ull;
@@ -180,21 +204,21 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
       vout[pos] = val;
     }
     __syncthreads();
-    for (int dd = tid; dd < kRadix; dd += kSortThreads) {
+    {
       uint32_t add = 0;
 #pragma unroll
       for (int ww = 0; ww < kWaves; ++ww) {
-        add += wcnt[ww][dd];
-        wcnt[ww][dd] = 0;
+        add += wcnt[ww][tid];
+        wcnt[ww][tid] = 0;
       }
-      run[dd] += add;
+      run[tid] += add;
     }
     __syncthreads();
   }
 }
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
-                             uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
+                             uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* /*partials*/,
                              hipStream_t s, SortResult* out) {
   // Pass p reads (src) and writes (dst); the first pass reads vals_in (or
   // indices) and writes vals_alt, so an odd pass count ends in the *_alt arrays.
@@ -210,17 +234,14 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
   }
   if (bits < 1) bits = 1;
   const uint32_t nb = (uint32_t)sort_blocks(n);
-  const size_t ncounts = (size_t)kRadix * nb;
   for (int shift = 0; shift < bits; shift += kRadixBits) {
     const int pb = bits - shift < kRadixBits ? bits - shift : kRadixBits;
     const uint32_t mask = (1u << pb) - 1u;
     hipLaunchKernelGGL(radix_count_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, n, shift,
                        mask, counts, nb);
-    hipError_t e = launch_scan(counts, nullptr, counts, ncounts, true, partials, s);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(radix_rowscan_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts, nb);
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, vsrc, kdst,
                        vdst, n, shift, pb, (const uint32_t*)counts, nb);
-    // swap
     uint32_t* kt = ksrc;
     ksrc = kdst;
     kdst = kt;
@@ -235,63 +256,85 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
 }
 
 // ---------------------------------------------------------------------------
-// Instance expansion in depth order.  Block b owns sorted positions
-// [256 b, 256 b + 256); its instances [begin, end) are spread over all
-// threads (load-balanced regardless of how many tiles each Gaussian covers),
-// each finding its owner by binary search over the block's inclusive offsets.
-// Tiles of one Gaussian are emitted row-major over its rect, as the reference's
-// duplicateWithKeys does (rasterizer_impl.cu:127-138).
+// Instance expansion in depth order, balanced over instances: block b owns
+// instances [kDupChunk b, kDupChunk (b+1)).  Thread 0 finds the first and last
+// depth-ordered Gaussian owning them (binary search over the inclusive
+// offsets); their offsets and rects are staged in LDS (every Gaussian in that
+// span covers >= 1 tile, so the span is <= kDupChunk long) and each instance
+// finds its owner by binary search in LDS.  Near-camera Gaussians covering
+// thousands of tiles therefore spread over many blocks instead of serialising
+// one.  Tiles of one Gaussian are emitted row-major over its rect, as the
+// reference's duplicateWithKeys does (rasterizer_impl.cu:127-138).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void duplicate_kernel(int P, const uint32_t* __restrict__ order,
-                                                        const uint32_t* __restrict__ offsets,
-                                                        const float2* __restrict__ xy, const int* __restrict__ radii,
-                                                        TileGrid grid, uint32_t* __restrict__ tile_key,
-                                                        uint32_t* __restrict__ gid) {
-  __shared__ uint32_t s_end[256];
-  __shared__ uint32_t s_gid[256];
-  __shared__ int s_x0[256], s_y0[256], s_w[256];
+constexpr int kDupThreads = 256;
+constexpr int kDupChunk = 2048;
+
+__device__ __forceinline__ uint32_t first_greater(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;  // first i with a[i] > v (a non-decreasing)
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] > v) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t R, const uint32_t* __restrict__ order,
+                                                                const uint32_t* __restrict__ offsets,
+                                                                const float2* __restrict__ xy,
+                                                                const int* __restrict__ radii, TileGrid grid,
+                                                                uint32_t* __restrict__ tile_key,
+                                                                uint32_t* __restrict__ gid) {
+  __shared__ uint32_t s_end[kDupChunk];
+  __shared__ uint32_t s_gid[kDupChunk];
+  __shared__ uint16_t s_x0[kDupChunk], s_y0[kDupChunk], s_w[kDupChunk];
+  __shared__ float s_inv_w[kDupChunk];
+  __shared__ uint32_t s_span[2];
   const int tid = threadIdx.x;
-  const int s0 = blockIdx.x * 256;
-  const int nloc = min(256, P - s0);
-  if (tid < nloc) {
-    const uint32_t gg = order[s0 + tid];
-    s_end[tid] = offsets[s0 + tid];
-    s_gid[tid] = gg;
-    const int r = radii[gg];
-    if (r > 0) {
-      const Rect rc = tile_rect(xy[gg], r, grid);
-      s_x0[tid] = rc.x0;
-      s_y0[tid] = rc.y0;
-      s_w[tid] = rc.x1 - rc.x0;
-    } else {
-      s_x0[tid] = 0;
-      s_y0[tid] = 0;
-      s_w[tid] = 1;
-    }
+  const uint32_t k0 = blockIdx.x * (uint32_t)kDupChunk;
+  const uint32_t k1 = min(R, k0 + (uint32_t)kDupChunk);
+  if (tid == 0) {
+    s_span[0] = first_greater(offsets, (uint32_t)P, k0);
+    s_span[1] = first_greater(offsets, (uint32_t)P, k1 - 1);
   }
   __syncthreads();
-  const uint32_t begin = s0 == 0 ? 0u : offsets[s0 - 1];
-  const uint32_t end = s_end[nloc - 1];
-  for (uint32_t k = begin + tid; k < end; k += 256) {
-    int lo = 0, hi = nloc - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (s_end[mid] > k) hi = mid; else lo = mid + 1;
-    }
-    const uint32_t first = lo == 0 ? begin : s_end[lo - 1];
+  const uint32_t sb = s_span[0];
+  const uint32_t ns = s_span[1] - sb + 1;  // <= kDupChunk
+  const uint32_t begin0 = sb == 0 ? 0u : offsets[sb - 1];
+  for (uint32_t j = tid; j < ns; j += kDupThreads) {
+    const uint32_t s = sb + j;
+    const uint32_t gg = order[s];
+    s_end[j] = offsets[s];
+    s_gid[j] = gg;
+    const Rect rc = tile_rect(xy[gg], radii[gg], grid);
+    const int w = max(1, rc.x1 - rc.x0);
+    s_x0[j] = (uint16_t)rc.x0;
+    s_y0[j] = (uint16_t)rc.y0;
+    s_w[j] = (uint16_t)w;
+    s_inv_w[j] = 1.0f / (float)w;
+  }
+  __syncthreads();
+  for (uint32_t k = k0 + tid; k < k1; k += kDupThreads) {
+    const uint32_t j = first_greater(s_end, ns, k);
+    const uint32_t first = j == 0 ? begin0 : s_end[j - 1];
     const uint32_t local = k - first;
-    const uint32_t w = (uint32_t)s_w[lo];
-    const uint32_t y = (uint32_t)s_y0[lo] + local / w;
-    const uint32_t x = (uint32_t)s_x0[lo] + local % w;
+    const uint32_t w = s_w[j];
+    // local / w via a float estimate and one integer correction (exact for local < 2^23)
+    uint32_t q = (uint32_t)((float)local * s_inv_w[j]);
+    int r = (int)local - (int)(q * w);
+    if (r < 0) { --q; r += (int)w; }
+    else if (r >= (int)w) { ++q; r -= (int)w; }
+    const uint32_t y = (uint32_t)s_y0[j] + q;
+    const uint32_t x = (uint32_t)s_x0[j] + (uint32_t)r;
     tile_key[k] = y * (uint32_t)grid.gx + x;
-    gid[k] = s_gid[lo];
+    gid[k] = s_gid[j];
   }
 }
 
-hipError_t launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float2* xy,
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float2* xy,
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
-  if (P == 0) return hipSuccess;
-  hipLaunchKernelGGL(duplicate_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, order, offsets, xy, radii, grid,
+  if (P == 0 || R == 0) return hipSuccess;
+  const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
+  hipLaunchKernelGGL(duplicate_kernel, dim3(blocks), dim3(kDupThreads), 0, s, P, R, order, offsets, xy, radii, grid,
                      tile_key, gid);
   return hipGetLastError();
 }
