@@ -216,7 +216,9 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geo
   // Instance offsets in depth order.
   {
     StageTimer tm(OMNIGS_STAGE_SCAN, s);
-    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s), "scan");
+    // key_alt is free after the depth sort: it holds tiles_touched in depth order
+    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s, g.key_alt),
+               "scan");
   }
 
   uint32_t R = 0;

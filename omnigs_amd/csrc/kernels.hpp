@@ -51,7 +51,7 @@ hipError_t launch_mark_visible(int P, uint8_t* present, hipStream_t s);
 // Inclusive (or exclusive) scan of n u32 values, optionally gathered through
 // idx (out[i] = sum of in[idx[j]] for j <= i).  partials needs scan_blocks(n)+1 entries.
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
-                       uint32_t* partials, hipStream_t s);
+                       uint32_t* partials, hipStream_t s, uint32_t* gather_buf = nullptr);
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned

@@ -51,14 +51,17 @@ __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile,
 }
 
 // Each wave compacts the batch positions whose mask has its bit set into its
-// own ascending list; returns the list length (wave-uniform).
-__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* list, int cnt, int w, int lane) {
+// own ascending list; returns the list length (wave-uniform).  Slots below
+// `first_slot` are dropped (backward: positions past every lane's last
+// contributor).
+__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* list, int cnt, int w, int lane,
+                                               int first_slot = 0) {
   int count = 0;
   const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int c = 0; c < kBatch / kWave; ++c) {
     const int j = c * kWave + lane;
-    const bool on = j < cnt && ((s_mask[j] >> w) & 1u);
+    const bool on = j >= first_slot && j < cnt && ((s_mask[j] >> w) & 1u);
     const uint64_t bal = __ballot(on);
     if (on) list[count + __popcll(bal & below)] = (uint8_t)j;
     count += __popcll(bal);
@@ -143,19 +146,56 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   }
 }
 
-// Sum of v over the 64 lanes, valid in lane 63 (DPP: quad swaps, row shifts,
-// row broadcasts 15 and 31).
-template <int Ctrl, int RowMask>
-__device__ __forceinline__ float dpp_shift(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, RowMask, 0xf, false));
+// Cross-lane helpers (gfx950): DPP moves, and the permlane swaps that trade
+// the upper 32 lanes of one VGPR with the lower 32 of another (32) or the odd
+// 16-lane rows of one with the even rows of another (16).
+template <int Ctrl>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float wave_sum_lane63(float v) {
-  v += dpp_shift<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
-  v += dpp_shift<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
-  v += dpp_shift<0x114, 0xf>(v);  // row_shr:4
-  v += dpp_shift<0x118, 0xf>(v);  // row_shr:8
-  v += dpp_shift<0x142, 0xa>(v);  // row_bcast:15
-  v += dpp_shift<0x143, 0xc>(v);  // row_bcast:31
+// x + y of a 32-lane swap: lanes 0-31 hold x[i] + x[i+32], lanes 32-63 hold y[i-32] + y[i].
+__device__ __forceinline__ float fold32(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(y), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+// Rows 0,2 hold x summed over row pairs (0+1, 2+3); rows 1,3 hold y likewise.
+__device__ __forceinline__ float fold16(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(y), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+
+// Reduces eight per-lane terms t[0..7] over the wave by recursive halving:
+// on return every lane of 8-lane group g = lane >> 3 holds sum(t[g]).
+__device__ __forceinline__ float wave_reduce8(const float t[8], int lane) {
+  // lanes 0-31 carry t[0..3], lanes 32-63 carry t[4..7]
+  const float a0 = fold32(t[0], t[4]), a1 = fold32(t[1], t[5]);
+  const float a2 = fold32(t[2], t[6]), a3 = fold32(t[3], t[7]);
+  // bit 4 of the lane picks between (a0, a2) and (a1, a3)
+  const float b0 = fold16(a0, a2), b1 = fold16(a1, a3);
+  // bit 3 of the lane keeps b1 (else b0); the partner 8 lanes away gives the other
+  const bool hi = (lane >> 3) & 1;
+  const float keep = hi ? b1 : b0, give = hi ? b0 : b1;
+  float c = keep + dpp<0x128>(give);  // row_ror:8 (lane i <-> i^8 within a 16-lane row)
+  c += dpp<0x141>(c);                 // row_half_mirror: i + (7 - i) within 8 lanes
+  c += dpp<0x4e>(c);                  // quad_perm [2,3,0,1]
+  c += dpp<0xb1>(c);                  // quad_perm [1,0,3,2]
+  return c;
+}
+
+// Full wave sum of one term; every lane receives it.
+__device__ __forceinline__ float wave_reduce1(float v) {
+  float c = fold32(v, v);
+  c = fold16(c, c);
+  c += dpp<0x128>(c);
+  c += dpp<0x141>(c);
+  c += dpp<0x4e>(c);
+  c += dpp<0xb1>(c);
+  return c;
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) v = max(v, __shfl_xor(v, o, kWave));
   return v;
 }
 
@@ -171,9 +211,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_co[kBatch];
   __shared__ float s_rgb[kChannels][kBatch];
-  __shared__ float s_acc[kWavesPerTile][kGradTerms][kBatch];
+  __shared__ float s_acc[kGradTerms][kBatch];
   __shared__ uint8_t s_mask[kBatch];
   __shared__ uint8_t s_list[kWavesPerTile][kBatch];
+  __shared__ int s_nlive;
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), w = tid / kWave;
@@ -183,13 +224,21 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
   const uint2 range = ranges[tile];
-  const int n = (int)(range.y - range.x);
   const size_t N = (size_t)W * H;
   const size_t pid = (size_t)py * W + px;
 
   const float T_final = inside ? final_Ts[pid] : 0.f;
   float T = T_final;
   const int last = inside ? (int)n_contrib[pid] : 0;
+  // Positions >= max(last) over the tile blend into no pixel: the traversal
+  // starts there instead of at the end of the list.  Per wave, positions
+  // >= the wave's own max(last) are dropped from its lists.
+  if (tid == 0) s_nlive = 0;
+  __syncthreads();
+  const int wlast = wave_max_int(last);
+  if (lane == 0) atomicMax(&s_nlive, wlast);
+  __syncthreads();
+  const int n = s_nlive;
   float dp0 = 0.f, dp1 = 0.f, dp2 = 0.f;
   if (inside) {
     dp0 = dL_dpix[pid];
@@ -206,7 +255,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     __syncthreads();
     const int k = base + tid;
     if (k < n) {
-      const uint32_t g = point_list[range.y - 1 - k];
+      const uint32_t g = point_list[range.x + (n - 1 - k)];
       const float2 m = xy[g];
       const float4 co = conic_opacity[g];
       s_gid[tid] = g;
@@ -218,13 +267,13 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
     }
 #pragma unroll
-    for (int ww = 0; ww < kWavesPerTile; ++ww)
-#pragma unroll
-      for (int q = 0; q < kGradTerms; ++q) s_acc[ww][q][tid] = 0.f;
+    for (int q = 0; q < kGradTerms; ++q) s_acc[q][tid] = 0.f;
     __syncthreads();
 
     const int cnt = min(kBatch, n - base);
-    const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane);
+    // slot j holds position n - 1 - (base + j); keep positions < wlast
+    const int first_slot = n - base - wlast;
+    const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane, first_slot);
     for (int ii = 0; ii < len; ++ii) {
       const int j = s_list[w][ii];
       // 0-based position of this instance in the tile's front-to-back list
@@ -241,7 +290,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
         const float alpha = fminf(0.99f, co.w * G);
         ok = power <= 0.0f && alpha >= 1.0f / 255.0f;
         if (ok) {
-          T = T / (1.f - alpha);
+          // T and 1/(1 - alpha) only scale gradients here (no decision depends
+          // on them), so the hardware reciprocal replaces two IEEE divisions.
+          const float inv_one_minus = __builtin_amdgcn_rcpf(1.f - alpha);
+          T = T * inv_one_minus;
           const float dchannel_dcolor = alpha * T;
           const float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
           acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
@@ -256,7 +308,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
           g_c2 = dchannel_dcolor * dp2;
           dL_dalpha *= T;
           last_alpha = alpha;
-          dL_dalpha += (-T_final / (1.f - alpha)) * bg_dot;
+          dL_dalpha += (-T_final * inv_one_minus) * bg_dot;
           const float dL_dG = co.w * dL_dalpha;
           const float gdx = G * dx, gdy = G * dy;
           const float dG_ddelx = -gdx * co.x - gdy * co.y;
@@ -270,37 +322,17 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
         }
       }
       if (__ballot(ok) == 0ull) continue;  // wave-uniform
-      g_c0 = wave_sum_lane63(g_c0);
-      g_c1 = wave_sum_lane63(g_c1);
-      g_c2 = wave_sum_lane63(g_c2);
-      g_mx = wave_sum_lane63(g_mx);
-      g_my = wave_sum_lane63(g_my);
-      g_a = wave_sum_lane63(g_a);
-      g_b = wave_sum_lane63(g_b);
-      g_cc = wave_sum_lane63(g_cc);
-      g_o = wave_sum_lane63(g_o);
-      if (lane == kWave - 1) {
-        s_acc[w][0][j] = g_c0;
-        s_acc[w][1][j] = g_c1;
-        s_acc[w][2][j] = g_c2;
-        s_acc[w][3][j] = g_mx;
-        s_acc[w][4][j] = g_my;
-        s_acc[w][5][j] = g_a;
-        s_acc[w][6][j] = g_b;
-        s_acc[w][7][j] = g_cc;
-        s_acc[w][8][j] = g_o;
-      }
+      const float t[8] = {g_c0, g_c1, g_c2, g_mx, g_my, g_a, g_b, g_cc};
+      const float grp = wave_reduce8(t, lane);
+      const float op = wave_reduce1(g_o);
+      if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][j], grp);
+      if (lane == 0) atomicAdd(&s_acc[8][j], op);
     }
     __syncthreads();
     if (tid < cnt) {
       float v[kGradTerms];
 #pragma unroll
-      for (int q = 0; q < kGradTerms; ++q) {
-        float s = 0.f;
-#pragma unroll
-        for (int ww = 0; ww < kWavesPerTile; ++ww) s += s_acc[ww][q][tid];
-        v[q] = s;
-      }
+      for (int q = 0; q < kGradTerms; ++q) v[q] = s_acc[q][tid];
       const uint32_t g = s_gid[tid];
       float* dst[kGradTerms] = {&dL_dcolors[3 * g],     &dL_dcolors[3 * g + 1], &dL_dcolors[3 * g + 2],
                                 &dL_dmean2D[3 * g],     &dL_dmean2D[3 * g + 1], &dL_dconic[4 * g],

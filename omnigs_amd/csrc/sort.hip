@@ -53,15 +53,22 @@ __device__ __forceinline__ uint32_t load_in(const uint32_t* __restrict__ in, con
   return idx ? in[idx[i]] : in[i];
 }
 
+// Block sums; with a gather index the gathered values are also written to
+// `gathered` so that the downsweep reads them contiguously.
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                    const uint32_t* __restrict__ idx, size_t n,
-                                                                   uint32_t* __restrict__ partials) {
+                                                                   uint32_t* __restrict__ partials,
+                                                                   uint32_t* __restrict__ gathered) {
   const size_t base = (size_t)blockIdx.x * kScanTile;
   uint32_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     const size_t i = base + (size_t)k * kScanThreads + threadIdx.x;
-    if (i < n) s += load_in(in, idx, i);
+    if (i < n) {
+      const uint32_t v = load_in(in, idx, i);
+      if (gathered) gathered[i] = v;
+      s += v;
+    }
   }
   uint32_t tot;
   block_exclusive_scan(s, &tot);
@@ -108,12 +115,15 @@ __global__ __launch_bounds__(kScanThreads) void scan_downsweep_kernel(const uint
 }
 
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
-                       uint32_t* partials, hipStream_t s) {
+                       uint32_t* partials, hipStream_t s, uint32_t* gather_buf) {
   if (n == 0) return hipSuccess;
   const size_t nb = scan_blocks(n);
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, in, idx, n, partials);
+  uint32_t* gathered = idx ? gather_buf : nullptr;
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, in, idx, n, partials, gathered);
   hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, nb);
-  hipLaunchKernelGGL(scan_downsweep_kernel, dim3(nb), dim3(kScanThreads), 0, s, in, idx, n,
+  const uint32_t* din = gathered ? gathered : in;
+  const uint32_t* didx = gathered ? nullptr : idx;
+  hipLaunchKernelGGL(scan_downsweep_kernel, dim3(nb), dim3(kScanThreads), 0, s, din, didx, n,
                      (const uint32_t*)partials, out, exclusive ? 1 : 0);
   return hipGetLastError();
 }
@@ -165,28 +175,46 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     uint32_t* __restrict__ vout, size_t n, int shift, int bits, const uint32_t* __restrict__ counts,
     uint32_t nblocks) {
   constexpr int kWaves = kSortThreads / kWave;
-  static_assert(kRadix == kSortThreads, "one digit per thread in the base scan");
-  __shared__ uint32_t run[kRadix];
+  static_assert(kRadix == kSortThreads, "one digit per thread in the base scans");
+  __shared__ uint32_t s_key[kSortTile];
+  __shared__ uint32_t s_val[kSortTile];
+  __shared__ uint32_t gbase[kRadix];   // global position of the block's first key of digit d
+  __shared__ uint32_t lstart[kRadix];  // first slot of digit d in the block's locally sorted tile
+  __shared__ uint32_t run[kRadix];     // keys of digit d placed so far
   __shared__ uint32_t wcnt[kWaves][kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
+  const size_t base = (size_t)blockIdx.x * kSortTile;
+  const uint32_t nloc = (uint32_t)min((size_t)kSortTile, n - base);
+
+  // keys and values of this block, kept in registers for the ranking rounds
+  uint32_t keys[kSortItems], vals[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t li = (uint32_t)(k * kSortThreads + tid);
+    const size_t i = base + li;
+    keys[k] = li < nloc ? kin[i] : 0u;
+    vals[k] = li < nloc ? (vin ? vin[i] : (uint32_t)i) : 0u;
+  }
   {
+    const uint32_t total = counts[(size_t)kRadix * nblocks + tid];
+    const uint32_t rowp = counts[(size_t)tid * nblocks + blockIdx.x];
+    const uint32_t rown = blockIdx.x + 1 < nblocks ? counts[(size_t)tid * nblocks + blockIdx.x + 1] : total;
     uint32_t tot;
-    const uint32_t digit_base = block_exclusive_scan(counts[(size_t)kRadix * nblocks + tid], &tot);
-    run[tid] = digit_base + counts[(size_t)tid * nblocks + blockIdx.x];
+    const uint32_t dbase = block_exclusive_scan(total, &tot);
+    gbase[tid] = dbase + rowp;
+    lstart[tid] = block_exclusive_scan(rown - rowp, &tot);
+    run[tid] = 0;
 #pragma unroll
     for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][tid] = 0;
   }
   __syncthreads();
   const uint64_t lanes_below = (1ull << lane) - 1ull;
-  const size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    const size_t i = base + (size_t)k * kSortThreads + tid;
-    if (base + (size_t)k * kSortThreads >= n) break;  // uniform across the block
-    const bool valid = i < n;
-    const uint32_t key = valid ? kin[i] : 0u;
-    const uint32_t val = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
-    const uint32_t d = (key >> shift) & mask;
+    if ((uint32_t)(k * kSortThreads) >= nloc) break;  // uniform across the block
+    const bool valid = (uint32_t)(k * kSortThreads + tid) < nloc;
+    const uint32_t d = (keys[k] >> shift) & mask;
     // lanes of this wave holding the same digit
     uint64_t same = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
@@ -198,10 +226,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(same);
     __syncthreads();
     if (valid) {
-      uint32_t pos = run[d] + rank;
-      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
-      kout[pos] = key;
-      vout[pos] = val;
+      uint32_t lp = lstart[d] + run[d] + rank;
+      for (int ww = 0; ww < w; ++ww) lp += wcnt[ww][d];
+      s_key[lp] = keys[k];
+      s_val[lp] = vals[k];
     }
     __syncthreads();
     {
@@ -214,6 +242,15 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
       run[tid] += add;
     }
     __syncthreads();
+  }
+  // the block's keys are now grouped by digit in LDS: write each digit's run
+  // contiguously to its global slot (consecutive threads, consecutive addresses)
+  for (uint32_t i = tid; i < nloc; i += kSortThreads) {
+    const uint32_t key = s_key[i];
+    const uint32_t d = (key >> shift) & mask;
+    const uint32_t pos = gbase[d] + (i - lstart[d]);
+    kout[pos] = key;
+    vout[pos] = s_val[i];
   }
 }
 
