@@ -116,6 +116,18 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* args, omnigs_buffer 
                           omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream,
                           int* num_rendered);
 
+/* Checkpointed form: re-runs preprocess and binning of a forward whose
+ * geometry and binning buffers were released, into fresh buffers, against
+ * the forward's kept image buffer (its per-pixel final_T and n_contrib are
+ * left as the forward wrote them; the per-tile ranges are rewritten with the
+ * same values).  `args` must be the forward's arguments (out_color unused).
+ * All kernels are deterministic, so *num_rendered equals the forward's R.
+ * Has no counterpart in the reference, whose autograd wrapper keeps all three
+ * buffers (src/gaussian_rasterizer.cpp:85-98); it serves memory-bound
+ * training callers. */
+int omnigs_lonlat_recompute(const omnigs_lonlat_forward_args* args, omnigs_buffer geometry_buffer,
+                            omnigs_buffer binning_buffer, char* image_buffer, void* stream, int* num_rendered);
+
 /* Backward: render backward, then preprocess backward (covariance, 2D mean,
  * SH and scale/rotation terms in one kernel). */
 int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* args, void* stream);

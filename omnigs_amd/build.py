@@ -60,12 +60,12 @@ def build_hip(jobs: int = 8) -> str:
     return HIP_LIB
 
 
-def build_torch_ext() -> str | None:
-    """The LibTorch boundary extension (C++, g++), linked against the HIP library."""
-    src = os.path.join(HERE, "torch_ext", "rasterize_points.cpp")
-    if not os.path.exists(src):
-        return None
-    import torch  # noqa: F401  (paths only)
+def build_torch_ext(jobs: int = 4) -> str | None:
+    """The LibTorch boundary extension (C++ via g++, no HIP device code), linked
+    against the HIP library with an $ORIGIN rpath."""
+    src_dir = os.path.join(HERE, "torch_ext")
+    srcs = [os.path.join(src_dir, f) for f in ("rasterize_points.cpp", "gaussian_rasterizer.cpp", "bindings.cpp")]
+    import torch
     from torch.utils import cpp_extension as ce
 
     ext_suffix = sysconfig.get_config_var("EXT_SUFFIX")
@@ -73,16 +73,23 @@ def build_torch_ext() -> str | None:
     torch_lib = os.path.join(os.path.dirname(torch.__file__), "lib")
     inc = ce.include_paths() + [sysconfig.get_paths()["include"], os.path.join(ROOT, "include"), "/opt/rocm/include"]
     abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
-    flags = ["-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
-             "-DTORCH_EXTENSION_NAME=_omnigs_torch", "-DTORCH_API_INCLUDE_EXTENSION_H", "-D__HIP_PLATFORM_AMD__=1",
-             "-DUSE_ROCM=1", "-Wno-unused-variable"]
-    srcs = [os.path.join(HERE, "torch_ext", f) for f in ("rasterize_points.cpp", "gaussian_rasterizer.cpp",
-                                                          "bindings.cpp")
-            if os.path.exists(os.path.join(HERE, "torch_ext", f))]
-    cmd = [os.environ.get("CXX", "g++"), *flags, *[f"-I{p}" for p in inc], *srcs, "-o", out + ".tmp",
-           f"-L{torch_lib}", f"-L{LIBDIR}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", "-lc10_hip",
-           "-ltorch_hip", "-lomnigs_lonlat", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{torch_lib}"]
-    _run(cmd)
+    flags = ["-O2", "-std=c++17", "-fPIC", f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DTORCH_EXTENSION_NAME=_omnigs_torch",
+             "-DTORCH_API_INCLUDE_EXTENSION_H", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-Wno-unused-variable",
+             "-Wno-deprecated-declarations"]
+    cxx = os.environ.get("CXX", "g++")
+    objdir = os.path.join(OBJDIR, "torch_ext")
+    os.makedirs(objdir, exist_ok=True)
+    objs, cmds = [], []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src).replace(".cpp", ".o"))
+        objs.append(obj)
+        cmds.append([cxx, *flags, *[f"-I{p}" for p in inc], "-c", src, "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+    _run([cxx, "-shared", *objs, "-o", out + ".tmp", f"-L{torch_lib}", f"-L{LIBDIR}", "-lc10", "-ltorch",
+          "-ltorch_cpu", "-ltorch_python", "-lc10_hip", "-ltorch_hip", "-lomnigs_lonlat", "-Wl,-rpath,$ORIGIN",
+          f"-Wl,-rpath,{torch_lib}"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -22,7 +22,7 @@ OMNIGS_ERR_ALLOC = 3
 OMNIGS_ERR_HIP = 4
 
 EXPORTED_SYMBOLS = (
-    "omnigs_lonlat_forward", "omnigs_lonlat_backward", "omnigs_lonlat_mark_visible",
+    "omnigs_lonlat_forward", "omnigs_lonlat_recompute", "omnigs_lonlat_backward", "omnigs_lonlat_mark_visible",
     "omnigs_lonlat_geometry_bytes", "omnigs_lonlat_binning_bytes", "omnigs_lonlat_image_bytes",
     "omnigs_lonlat_state", "omnigs_profile_enable", "omnigs_profile_read", "omnigs_stage_name",
     "omnigs_status_string", "omnigs_last_error", "omnigs_version",
@@ -88,6 +88,9 @@ def load() -> ctypes.CDLL:
     lib.omnigs_lonlat_forward.argtypes = [ctypes.POINTER(ForwardArgs), Buffer, Buffer, Buffer, ctypes.c_void_p,
                                           _ip]
     lib.omnigs_lonlat_forward.restype = ctypes.c_int
+    lib.omnigs_lonlat_recompute.argtypes = [ctypes.POINTER(ForwardArgs), Buffer, Buffer, ctypes.c_void_p,
+                                            ctypes.c_void_p, _ip]
+    lib.omnigs_lonlat_recompute.restype = ctypes.c_int
     lib.omnigs_lonlat_backward.argtypes = [ctypes.POINTER(BackwardArgs), ctypes.c_void_p]
     lib.omnigs_lonlat_backward.restype = ctypes.c_int
     lib.omnigs_lonlat_mark_visible.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]

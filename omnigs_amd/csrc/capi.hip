@@ -171,16 +171,22 @@ int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream) {
   return OMNIGS_OK;
 }
 
-int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,
-                          omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream_,
-                          int* num_rendered) {
+}  // extern "C"
+
+namespace {
+
+// Shared by forward (render = true) and recompute (render = false, image
+// buffer given and kept).
+int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer, omnigs_buffer binning_buffer,
+                 omnigs_buffer image_buffer, char* kept_image, void* stream_, int* num_rendered, bool render) {
   if (!a || !num_rendered) return fail_arg("NULL args");
   *num_rendered = 0;
   const int P = a->P, W = a->width, H = a->height;
   if (P < 0 || W <= 0 || H <= 0) return fail_arg("P < 0 or empty image");
   if (P == 0) return OMNIGS_OK;  // the reference boundary skips the rasterizer entirely
-  if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || !a->out_color)
+  if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || (render && !a->out_color))
     return fail_arg("missing required pointer");
+  if (!render && !kept_image) return fail_arg("recompute needs the forward's image buffer");
   if (!a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
   if (!a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
     return fail_arg("SH degree / coefficient count");
@@ -189,7 +195,7 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geo
   const TileGrid grid = tile_grid(W, H);
 
   char* gbuf = grow(geometry_buffer, carve_geom(nullptr, P).bytes);
-  char* ibuf = grow(image_buffer, carve_img(nullptr, W, H).bytes);
+  char* ibuf = render ? grow(image_buffer, carve_img(nullptr, W, H).bytes) : kept_image;
   if (!gbuf || !ibuf) {
     std::snprintf(g_last_error, sizeof(g_last_error), "geometry/image buffer allocation failed");
     return OMNIGS_ERR_ALLOC;
@@ -248,6 +254,8 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geo
     StageTimer tm(OMNIGS_STAGE_RANGES, s);
     OMNIGS_TRY(launch_tile_ranges(ts.keys, R, img.ranges, grid.tiles(), s), "ranges");
   }
+  *num_rendered = (int)R;
+  if (!render) return OMNIGS_OK;
   const float* colors = a->colors_precomp ? a->colors_precomp : g.rgb;
   {
     StageTimer tm(OMNIGS_STAGE_RENDER, s);
@@ -255,8 +263,22 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geo
                                      img.n_contrib, a->background, a->out_color, s),
                "render");
   }
-  *num_rendered = (int)R;
   return OMNIGS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,
+                          omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream, int* num_rendered) {
+  return forward_impl(a, geometry_buffer, binning_buffer, image_buffer, nullptr, stream, num_rendered, true);
+}
+
+int omnigs_lonlat_recompute(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,
+                            omnigs_buffer binning_buffer, char* image_buffer, void* stream, int* num_rendered) {
+  return forward_impl(a, geometry_buffer, binning_buffer, omnigs_buffer{nullptr, nullptr}, image_buffer, stream,
+                      num_rendered, false);
 }
 
 int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) {
