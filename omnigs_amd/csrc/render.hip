@@ -26,6 +26,19 @@ __device__ __forceinline__ void pixel_of(int tile, int gx, int tid, int& px, int
   py = ty * kTileY + (w >> 1) * 8 + (lane >> 3);
 }
 
+// exp(power) is evaluated as exp2 of a power prescaled by log2(e) at staging:
+// power2 = a dx^2 + c dy^2 + b dx dy with (a, b, c) = -log2(e) (A/2, B, C/2),
+// G = 2^power2 on the hardware exp2.  The forward and the backward use the
+// same staged values, so every alpha decision agrees between them.
+constexpr float kLog2e = 1.4426950408889634f;
+__device__ __forceinline__ float3 prescale_conic(float4 co) {
+  return make_float3(-0.5f * kLog2e * co.x, -kLog2e * co.y, -0.5f * kLog2e * co.z);
+}
+__device__ __forceinline__ float power2_of(float3 q, float dx, float dy) {
+  return dx * (q.x * dx + q.y * dy) + q.z * dy * dy;
+}
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+
 // Conservative 4-bit mask of the tile's 8x8 wave quadrants that Gaussian
 // (m, co) can reach with alpha >= 1/255: outside the ellipse
 // 0.5 d^T Q d <= ln(255 o) the alpha test fails for every pixel, so skipping
@@ -75,7 +88,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     float* __restrict__ final_T, uint32_t* __restrict__ n_contrib, const float* __restrict__ bg,
     float* __restrict__ out_color) {
   __shared__ float2 s_xy[kBatch];
-  __shared__ float4 s_co[kBatch];
+  __shared__ float4 s_q[kBatch];  // prescaled conic (a, b, c) and opacity
   __shared__ float s_rgb[kChannels][kBatch];
   __shared__ uint8_t s_mask[kBatch];
   __shared__ uint8_t s_list[kWavesPerTile][kBatch];
@@ -102,8 +115,9 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const uint32_t g = point_list[range.x + k];
       const float2 m = xy[g];
       const float4 co = conic_opacity[g];
+      const float3 q = prescale_conic(co);
       s_xy[tid] = m;
-      s_co[tid] = co;
+      s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
       s_rgb[0][tid] = colors[3 * g];
       s_rgb[1][tid] = colors[3 * g + 1];
       s_rgb[2][tid] = colors[3 * g + 2];
@@ -115,11 +129,11 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     for (int ii = 0; ii < len && !done; ++ii) {
       const int j = s_list[w][ii];
       const float2 m = s_xy[j];
-      const float4 co = s_co[j];
+      const float4 q = s_q[j];
       const float dx = m.x - fx, dy = m.y - fy;
-      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-      if (power > 0.0f) continue;
-      const float alpha = fminf(0.99f, co.w * expf(power));
+      const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
+      if (power2 > 0.0f) continue;
+      const float alpha = fminf(0.99f, q.w * exp2_hw(power2));
       if (alpha < 1.0f / 255.0f) continue;
       const float test_T = T * (1.0f - alpha);
       if (test_T < 0.0001f) {
@@ -200,6 +214,9 @@ __device__ __forceinline__ int wave_max_int(int v) {
 }
 
 constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
+// A wave with at most this many contributing lanes adds their terms to LDS
+// directly instead of running the cross-lane reduction.
+constexpr int kSparseLanes = 6;
 
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
@@ -210,6 +227,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   __shared__ uint32_t s_gid[kBatch];
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_co[kBatch];
+  __shared__ float4 s_q[kBatch];  // prescaled conic, as in the forward
   __shared__ float s_rgb[kChannels][kBatch];
   __shared__ float s_acc[kGradTerms][kBatch];
   __shared__ uint8_t s_mask[kBatch];
@@ -258,9 +276,11 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const uint32_t g = point_list[range.x + (n - 1 - k)];
       const float2 m = xy[g];
       const float4 co = conic_opacity[g];
+      const float3 q = prescale_conic(co);
       s_gid[tid] = g;
       s_xy[tid] = m;
       s_co[tid] = co;
+      s_q[tid] = make_float4(q.x, q.y, q.z, 0.f);
       s_rgb[0][tid] = colors[3 * g];
       s_rgb[1][tid] = colors[3 * g + 1];
       s_rgb[2][tid] = colors[3 * g + 2];
@@ -284,11 +304,12 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       if (ok) {
         const float2 m = s_xy[j];
         const float4 co = s_co[j];
+        const float4 q = s_q[j];
         const float dx = m.x - fx, dy = m.y - fy;
-        const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-        const float G = expf(power);
+        const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
+        const float G = exp2_hw(power2);
         const float alpha = fminf(0.99f, co.w * G);
-        ok = power <= 0.0f && alpha >= 1.0f / 255.0f;
+        ok = power2 <= 0.0f && alpha >= 1.0f / 255.0f;
         if (ok) {
           // T and 1/(1 - alpha) only scale gradients here (no decision depends
           // on them), so the hardware reciprocal replaces two IEEE divisions.
@@ -321,12 +342,28 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
           g_o = G * dL_dalpha;
         }
       }
-      if (__ballot(ok) == 0ull) continue;  // wave-uniform
-      const float t[8] = {g_c0, g_c1, g_c2, g_mx, g_my, g_a, g_b, g_cc};
-      const float grp = wave_reduce8(t, lane);
-      const float op = wave_reduce1(g_o);
-      if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][j], grp);
-      if (lane == 0) atomicAdd(&s_acc[8][j], op);
+      const uint64_t bal = __ballot(ok);
+      if (bal == 0ull) continue;  // wave-uniform
+      if (__popcll(bal) <= kSparseLanes) {
+        // few pixels hit: their terms go straight into LDS
+        if (ok) {
+          atomicAdd(&s_acc[0][j], g_c0);
+          atomicAdd(&s_acc[1][j], g_c1);
+          atomicAdd(&s_acc[2][j], g_c2);
+          atomicAdd(&s_acc[3][j], g_mx);
+          atomicAdd(&s_acc[4][j], g_my);
+          atomicAdd(&s_acc[5][j], g_a);
+          atomicAdd(&s_acc[6][j], g_b);
+          atomicAdd(&s_acc[7][j], g_cc);
+          atomicAdd(&s_acc[8][j], g_o);
+        }
+      } else {
+        const float t[8] = {g_c0, g_c1, g_c2, g_mx, g_my, g_a, g_b, g_cc};
+        const float grp = wave_reduce8(t, lane);
+        const float op = wave_reduce1(g_o);
+        if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][j], grp);
+        if (lane == 0) atomicAdd(&s_acc[8][j], op);
+      }
     }
     __syncthreads();
     if (tid < cnt) {

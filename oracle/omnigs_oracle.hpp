@@ -155,6 +155,10 @@ template <class R> struct State {
   std::vector<uint32_t> ranges;  // [tiles,2]
   std::vector<R> final_T, out_color;
   std::vector<uint32_t> n_contrib;
+  // per pixel: smallest relative distance of any alpha >= 1/255 or T >= 1e-4
+  // decision to its threshold (a decision closer than float rounding to its
+  // threshold may legitimately differ between two correct FP32 evaluations)
+  std::vector<R> margin;
   uint64_t decision_sig = 0;  // hash of every discrete decision (FD masking)
   // backward
   std::vector<R> dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors;  // [P,3], [P,4], [P], [P,3]
@@ -383,6 +387,7 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
   st.final_T.assign(N, 0);
   st.n_contrib.assign(N, 0);
   st.out_color.assign(kChannels * N, 0);
+  st.margin.assign(N, R(1));
   uint64_t sig = 1469598103934665603ULL;
   sig_mix(sig, st.R_);
   for (int py = 0; py < H; ++py)
@@ -392,6 +397,7 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
       const R fx = (R)px, fy = (R)py;
       R T = 1, C[kChannels] = {0, 0, 0};
       uint32_t contributor = 0, last = 0;
+      R mg = 1;
       for (uint32_t k = b; k < e; ++k) {
         ++contributor;
         const uint32_t g = st.point_list[k];
@@ -400,8 +406,10 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
         const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
         if (power > R(0)) { sig_mix(sig, 1); continue; }
         const R alpha = std::min(R(0.99), co[3] * std::exp(power));
+        mg = std::min(mg, std::abs(alpha * R(255) - R(1)));
         if (alpha < R(1) / R(255)) { sig_mix(sig, 2); continue; }
         const R test_T = T * (R(1) - alpha);
+        mg = std::min(mg, std::abs(test_T * R(10000) - R(1)));
         if (test_T < R(0.0001)) { sig_mix(sig, 3); break; }
         for (int ch = 0; ch < kChannels; ++ch) C[ch] += feat[kChannels * g + ch] * alpha * T;
         T = test_T;
@@ -411,6 +419,7 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
       const size_t pid = (size_t)py * W + px;
       st.final_T[pid] = T;
       st.n_contrib[pid] = last;
+      st.margin[pid] = mg;
       for (int ch = 0; ch < kChannels; ++ch) st.out_color[ch * N + pid] = C[ch] + T * in.bg[ch];
     }
   for (int i = 0; i < in.P; ++i) sig_mix(sig, (uint64_t)st.radii[i] * 131 + st.tiles_touched[i]);

@@ -5,6 +5,7 @@
 // can feed the GPU's outputs of stage k into the oracle's stage k+1
 // ("identical preprocess outputs" parity) via oracle_set().
 #include <chrono>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -76,6 +77,7 @@ template <class R> bool lookup(State<R>& st, const std::string& name, void*& ptr
   OMNIGS_ARR("ranges", ranges)
   OMNIGS_ARR("final_T", final_T)
   OMNIGS_ARR("n_contrib", n_contrib)
+  OMNIGS_ARR("margin", margin)
   OMNIGS_ARR("out_color", out_color)
   OMNIGS_ARR("dL_dmean2D", dL_dmean2D)
   OMNIGS_ARR("dL_dconic", dL_dconic)
