@@ -94,16 +94,17 @@ def build_torch_ext(jobs: int = 4) -> str | None:
     return out
 
 
-def clean() -> None:
+def clean(hip_only: bool = False) -> None:
+    """Removes the objects and the libraries that the next build rewrites."""
     shutil.rmtree(os.path.join(ROOT, "_build"), ignore_errors=True)
     if os.path.isdir(LIBDIR):
         for f in os.listdir(LIBDIR):
-            if f.endswith(".so"):
+            if f.endswith(".so") and (not hip_only or f == os.path.basename(HIP_LIB)):
                 os.remove(os.path.join(LIBDIR, f))
 
 
 def build_all(hip_only: bool = False) -> None:
-    clean()
+    clean(hip_only)
     build_hip()
     if not hip_only:
         build_torch_ext()
