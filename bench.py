@@ -58,6 +58,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--gaussians", type=int, default=1_500_000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -71,6 +73,8 @@ def main() -> int:
     from omnigs_amd import capi, scenes
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        return copy_calibration(dev, args.steps)
     sc = scenes.room_scene(P=args.gaussians)
     P, W, H = sc["means3D"].shape[0], sc["width"], sc["height"]
     N = W * H
@@ -152,6 +156,21 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out))
+    return 0
+
+
+def copy_calibration(dev, steps: int) -> int:
+    """A plain device-to-device copy of 1 GiB: 1 GiB read + 1 GiB written per
+    launch, the yardstick that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 1 << 28  # float32 elements = 1 GiB
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(max(1, steps)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                      "launches": max(1, steps)}))
     return 0
 
 
