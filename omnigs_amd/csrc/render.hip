@@ -214,9 +214,6 @@ __device__ __forceinline__ int wave_max_int(int v) {
 }
 
 constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
-// A wave with at most this many contributing lanes adds their terms to LDS
-// directly instead of running the cross-lane reduction.
-constexpr int kSparseLanes = 6;
 
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
@@ -263,11 +260,12 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     dp1 = dL_dpix[N + pid];
     dp2 = dL_dpix[2 * N + pid];
   }
-  const float bg_dot = bg[0] * dp0 + bg[1] * dp1 + bg[2] * dp2;
+  const float ddelx_dx = 0.5f * (float)W, ddely_dy = 0.5f * (float)H;  // NDC scale of dL/dmean2D
+  // -T_final (bg . dL/dpix): the background term of dL/dalpha is this times 1/(1 - alpha)
+  const float neg_tb = -T_final * (bg[0] * dp0 + bg[1] * dp1 + bg[2] * dp2);
   float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-  float lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
   float last_alpha = 0.f;
-  const float ddelx_dx = 0.5f * (float)W, ddely_dy = 0.5f * (float)H;
+  float lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
 
   for (int base = 0; base < n; base += kBatch) {
     __syncthreads();
@@ -280,14 +278,14 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       s_gid[tid] = g;
       s_xy[tid] = m;
       s_co[tid] = co;
-      s_q[tid] = make_float4(q.x, q.y, q.z, 0.f);
+      s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
       s_rgb[0][tid] = colors[3 * g];
       s_rgb[1][tid] = colors[3 * g + 1];
       s_rgb[2][tid] = colors[3 * g + 2];
       s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
     }
 #pragma unroll
-    for (int q = 0; q < kGradTerms; ++q) s_acc[q][tid] = 0.f;
+    for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
     __syncthreads();
 
     const int cnt = min(kBatch, n - base);
@@ -298,85 +296,76 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const int j = s_list[w][ii];
       // 0-based position of this instance in the tile's front-to-back list
       const int pos = n - 1 - (base + j);
-      bool ok = inside && pos < last;
-      float g_c0 = 0.f, g_c1 = 0.f, g_c2 = 0.f, g_mx = 0.f, g_my = 0.f, g_a = 0.f, g_b = 0.f, g_cc = 0.f,
-            g_o = 0.f;
+      const float2 m = s_xy[j];
+      const float4 q = s_q[j];
+      const float dx = m.x - fx, dy = m.y - fy;
+      const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
+      const float G = exp2_hw(power2);
+      const float alpha = fminf(0.99f, q.w * G);
+      const bool ok = inside && pos < last && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
+      // per-pixel moments: w = G dL/dalpha, v = alpha T
+      float m_wdx = 0.f, m_wdy = 0.f, m_wdx2 = 0.f, m_wdxdy = 0.f, m_wdy2 = 0.f, m_w = 0.f;
+      float m_v0 = 0.f, m_v1 = 0.f, m_v2 = 0.f;
       if (ok) {
-        const float2 m = s_xy[j];
-        const float4 co = s_co[j];
-        const float4 q = s_q[j];
-        const float dx = m.x - fx, dy = m.y - fy;
-        const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
-        const float G = exp2_hw(power2);
-        const float alpha = fminf(0.99f, co.w * G);
-        ok = power2 <= 0.0f && alpha >= 1.0f / 255.0f;
-        if (ok) {
-          // T and 1/(1 - alpha) only scale gradients here (no decision depends
-          // on them), so the hardware reciprocal replaces two IEEE divisions.
-          const float inv_one_minus = __builtin_amdgcn_rcpf(1.f - alpha);
-          T = T * inv_one_minus;
-          const float dchannel_dcolor = alpha * T;
-          const float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
-          acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-          acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-          acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-          lc0 = c0;
-          lc1 = c1;
-          lc2 = c2;
-          float dL_dalpha = (c0 - acc0) * dp0 + (c1 - acc1) * dp1 + (c2 - acc2) * dp2;
-          g_c0 = dchannel_dcolor * dp0;
-          g_c1 = dchannel_dcolor * dp1;
-          g_c2 = dchannel_dcolor * dp2;
-          dL_dalpha *= T;
-          last_alpha = alpha;
-          dL_dalpha += (-T_final * inv_one_minus) * bg_dot;
-          const float dL_dG = co.w * dL_dalpha;
-          const float gdx = G * dx, gdy = G * dy;
-          const float dG_ddelx = -gdx * co.x - gdy * co.y;
-          const float dG_ddely = -gdy * co.z - gdx * co.y;
-          g_mx = dL_dG * dG_ddelx * ddelx_dx;
-          g_my = dL_dG * dG_ddely * ddely_dy;
-          g_a = -0.5f * gdx * dx * dL_dG;
-          g_b = -0.5f * gdx * dy * dL_dG;
-          g_cc = -0.5f * gdy * dy * dL_dG;
-          g_o = G * dL_dalpha;
-        }
+        // T and 1/(1 - alpha) only scale gradients here (no decision depends
+        // on them), so the hardware reciprocal replaces two IEEE divisions.
+        const float r = __builtin_amdgcn_rcpf(1.f - alpha);
+        T = T * r;
+        const float v = alpha * T;
+        const float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
+        const float one_minus_la = 1.f - last_alpha;
+        acc0 = last_alpha * lc0 + one_minus_la * acc0;
+        acc1 = last_alpha * lc1 + one_minus_la * acc1;
+        acc2 = last_alpha * lc2 + one_minus_la * acc2;
+        const float dL_dalpha = ((c0 - acc0) * dp0 + (c1 - acc1) * dp1 + (c2 - acc2) * dp2) * T + neg_tb * r;
+        last_alpha = alpha;
+        lc0 = c0;
+        lc1 = c1;
+        lc2 = c2;
+        const float wgt = G * dL_dalpha;
+        m_wdx = wgt * dx;
+        m_wdy = wgt * dy;
+        m_wdx2 = m_wdx * dx;
+        m_wdxdy = m_wdx * dy;
+        m_wdy2 = m_wdy * dy;
+        m_w = wgt;
+        m_v0 = v * dp0;
+        m_v1 = v * dp1;
+        m_v2 = v * dp2;
       }
-      const uint64_t bal = __ballot(ok);
-      if (bal == 0ull) continue;  // wave-uniform
-      if (__popcll(bal) <= kSparseLanes) {
-        // few pixels hit: their terms go straight into LDS
-        if (ok) {
-          atomicAdd(&s_acc[0][j], g_c0);
-          atomicAdd(&s_acc[1][j], g_c1);
-          atomicAdd(&s_acc[2][j], g_c2);
-          atomicAdd(&s_acc[3][j], g_mx);
-          atomicAdd(&s_acc[4][j], g_my);
-          atomicAdd(&s_acc[5][j], g_a);
-          atomicAdd(&s_acc[6][j], g_b);
-          atomicAdd(&s_acc[7][j], g_cc);
-          atomicAdd(&s_acc[8][j], g_o);
-        }
-      } else {
-        const float t[8] = {g_c0, g_c1, g_c2, g_mx, g_my, g_a, g_b, g_cc};
-        const float grp = wave_reduce8(t, lane);
-        const float op = wave_reduce1(g_o);
-        if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][j], grp);
-        if (lane == 0) atomicAdd(&s_acc[8][j], op);
-      }
+      if (__ballot(ok) == 0ull) continue;  // wave-uniform
+      const float t[8] = {m_v0, m_v1, m_v2, m_wdx, m_wdy, m_wdx2, m_wdxdy, m_wdy2};
+      const float grp = wave_reduce8(t, lane);
+      const float sw = wave_reduce1(m_w);
+      if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][j], grp);
+      if (lane == 0) atomicAdd(&s_acc[8][j], sw);
     }
     __syncthreads();
     if (tid < cnt) {
-      float v[kGradTerms];
-#pragma unroll
-      for (int q = 0; q < kGradTerms; ++q) v[q] = s_acc[q][tid];
+      // moments -> the reference's per-pair terms, once per (tile, Gaussian):
+      //   dL/dmean2D = -o (0.5W (A Swdx + B Swdy), 0.5H (C Swdy + B Swdx))
+      //   dL/dconic  = -0.5 o (Swdx2, Swdxdy, Swdy2),  dL/dopacity = Sw,  dL/dcolor = Sv dL/dpix
+      const float4 co = s_co[tid];
+      const float sv0 = s_acc[0][tid], sv1 = s_acc[1][tid], sv2 = s_acc[2][tid];
+      const float swdx = s_acc[3][tid], swdy = s_acc[4][tid];
+      const float swdx2 = s_acc[5][tid], swdxdy = s_acc[6][tid], swdy2 = s_acc[7][tid];
+      const float sw = s_acc[8][tid];
+      const float v[kGradTerms] = {sv0,
+                                   sv1,
+                                   sv2,
+                                   -co.w * ddelx_dx * (co.x * swdx + co.y * swdy),
+                                   -co.w * ddely_dy * (co.z * swdy + co.y * swdx),
+                                   -0.5f * co.w * swdx2,
+                                   -0.5f * co.w * swdxdy,
+                                   -0.5f * co.w * swdy2,
+                                   sw};
       const uint32_t g = s_gid[tid];
       float* dst[kGradTerms] = {&dL_dcolors[3 * g],     &dL_dcolors[3 * g + 1], &dL_dcolors[3 * g + 2],
                                 &dL_dmean2D[3 * g],     &dL_dmean2D[3 * g + 1], &dL_dconic[4 * g],
                                 &dL_dconic[4 * g + 1],  &dL_dconic[4 * g + 3],  &dL_dopacity[g]};
 #pragma unroll
-      for (int q = 0; q < kGradTerms; ++q)
-        if (v[q] != 0.f) unsafeAtomicAdd(dst[q], v[q]);
+      for (int qq = 0; qq < kGradTerms; ++qq)
+        if (v[qq] != 0.f) unsafeAtomicAdd(dst[qq], v[qq]);
     }
   }
 }
