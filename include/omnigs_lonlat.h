@@ -149,9 +149,7 @@ typedef struct omnigs_lonlat_state_view {
   /* geometry buffer, per Gaussian */
   const uint32_t* depth_key;     /* float bits of |t|; 0xFFFFFFFF when culled */
   const int32_t* radii;          /* internal radii (written when the forward got radii == NULL) */
-  const float* means2D;          /* [P,2] pixel coordinates */
-  const float* conic_opacity;    /* [P,4] */
-  const float* rgb;              /* [P,3] */
+  const float* render_records;   /* [P,12]: x, y, conic A, B, C, opacity, r, g, b, 0, 0, 0 */
   const uint8_t* clamped;        /* [P] bit c = channel c clamped */
   const uint32_t* tiles_touched; /* [P] */
   const uint32_t* depth_order;   /* [P] Gaussian ids by (depth bits, id) */

@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
 )
 NUM_STAGES = 10
 
-_STATE_FIELDS = ("depth_key", "radii", "means2D", "conic_opacity", "rgb", "clamped", "tiles_touched",
+_STATE_FIELDS = ("depth_key", "radii", "render_records", "clamped", "tiles_touched",
                  "depth_order", "offsets", "point_list", "tile_keys", "final_T", "n_contrib", "ranges")
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -233,8 +233,8 @@ def state(buffers, P: int, R: int, width: int, height: int) -> dict:
     tiles = ((width + 15) // 16) * ((height + 15) // 16)
     N = width * height
     spec = {  # field: (buffer, dtype, count)
-        "depth_key": (geom, torch.int32, P), "radii": (geom, torch.int32, P), "means2D": (geom, torch.float32, 2 * P),
-        "conic_opacity": (geom, torch.float32, 4 * P), "rgb": (geom, torch.float32, 3 * P),
+        "depth_key": (geom, torch.int32, P), "radii": (geom, torch.int32, P),
+        "render_records": (geom, torch.float32, 12 * P),
         "clamped": (geom, torch.uint8, P), "tiles_touched": (geom, torch.int32, P),
         "depth_order": (geom, torch.int32, P), "offsets": (geom, torch.int32, P),
         "point_list": (binning, torch.int32, R), "tile_keys": (binning, torch.int32, R),
@@ -249,6 +249,11 @@ def state(buffers, P: int, R: int, width: int, height: int) -> dict:
         nbytes = n * torch.empty((), dtype=dt).element_size()
         assert 0 <= off and off + nbytes <= buf.tensor.numel(), name
         out[name] = buf.tensor[off:off + nbytes].view(dt)
+    if "render_records" in out:
+        rec = out["render_records"].view(P, 12)
+        out["means2D"] = rec[:, 0:2].reshape(-1)
+        out["conic_opacity"] = rec[:, 2:6].reshape(-1)
+        out["rgb"] = rec[:, 6:9].reshape(-1)
     return out
 
 

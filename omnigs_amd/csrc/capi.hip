@@ -141,9 +141,7 @@ int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, con
     const GeomState g = carve_geom(const_cast<char*>(geom_buffer), P);
     out->depth_key = g.depth_key;
     out->radii = g.radii;
-    out->means2D = reinterpret_cast<const float*>(g.xy);
-    out->conic_opacity = reinterpret_cast<const float*>(g.conic_opacity);
-    out->rgb = g.rgb;
+    out->render_records = reinterpret_cast<const float*>(g.rec);
     out->clamped = g.clamped;
     out->tiles_touched = g.tiles_touched;
     out->depth_order = g.order;
@@ -240,7 +238,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   const BinState b = carve_bin(bbuf, R);
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
-    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.xy, radii, grid, b.tile_key, b.gid, s), "duplicate");
+    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rec, radii, grid, b.tile_key, b.gid, s), "duplicate");
   }
   SortResult ts{b.tile_key, b.gid};
   if (R > 0) {
@@ -256,11 +254,10 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   }
   *num_rendered = (int)R;
   if (!render) return OMNIGS_OK;
-  const float* colors = a->colors_precomp ? a->colors_precomp : g.rgb;
   {
     StageTimer tm(OMNIGS_STAGE_RENDER, s);
-    OMNIGS_TRY(launch_render_forward(grid, img.ranges, ts.vals, W, H, g.xy, colors, g.conic_opacity, img.final_T,
-                                     img.n_contrib, a->background, a->out_color, s),
+    OMNIGS_TRY(launch_render_forward(grid, img.ranges, ts.vals, W, H, g.rec, img.final_T, img.n_contrib,
+                                     a->background, a->out_color, s),
                "render");
   }
   return OMNIGS_OK;
@@ -317,11 +314,10 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
     OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dcolor), 3 * (size_t)P, 0u, s), "clear");
   }
 
-  const float* colors = a->colors_precomp ? a->colors_precomp : g.rgb;
   if (R > 0) {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
-    OMNIGS_TRY(launch_render_backward(grid, img.ranges, point_list, W, H, a->background, g.xy, g.conic_opacity,
-                                      colors, img.final_T, img.n_contrib, a->dL_dpix, a->dL_dmean2D, a->dL_dconic,
+    OMNIGS_TRY(launch_render_backward(grid, img.ranges, point_list, W, H, a->background, g.rec, img.final_T,
+                                      img.n_contrib, a->dL_dpix, a->dL_dmean2D, a->dL_dconic,
                                       a->dL_dopacity, a->dL_dcolor, s),
                "render backward");
   }

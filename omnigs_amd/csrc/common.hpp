@@ -102,9 +102,10 @@ inline size_t scan_blocks(size_t n) { return (n + kScanTile - 1) / kScanTile; }
 struct GeomState {
   uint32_t* depth_key;     // float bits of the camera distance (sort key); ~0 if culled
   int32_t* radii;          // internal radii (used when the caller passes none)
-  float2* xy;              // pixel-space mean
-  float4* conic_opacity;   // (A, B, C, opacity)
-  float* rgb;              // [P,3] SH colour
+  // Render record, 48 B per Gaussian so one gather fetches everything the
+  // tile kernels need: {x, y, A, B}, {C, opacity, r, g}, {b, 0, 0, 0}
+  // (pixel-space mean, conic, opacity, colour -- SH-derived or precomputed).
+  float4* rec;
   uint8_t* clamped;        // bit c set when channel c was clamped at 0
   uint32_t* tiles_touched; // rect area
   uint32_t* order;         // Gaussians sorted by (depth bits, index)
@@ -121,9 +122,7 @@ inline GeomState carve_geom(char* base, size_t P) {
   Carver<char> c(base);
   g.depth_key = c.take<uint32_t>(P);
   g.radii = c.take<int32_t>(P);
-  g.xy = c.take<float2>(P);
-  g.conic_opacity = c.take<float4>(P);
-  g.rgb = c.take<float>(3 * P);
+  g.rec = c.take<float4>(3 * P);
   g.clamped = c.take<uint8_t>(P);
   g.tiles_touched = c.take<uint32_t>(P);
   g.order = c.take<uint32_t>(P);
@@ -146,6 +145,21 @@ struct BinState {
   uint32_t* scan_partials;
   size_t bytes;
 };
+
+struct RenderRec {
+  float2 xy;
+  float4 co;  // A, B, C, opacity
+  float3 rgb;
+};
+__device__ __forceinline__ RenderRec load_rec(const float4* __restrict__ rec, uint32_t g) {
+  const float4 a = rec[3 * g], b = rec[3 * g + 1], c = rec[3 * g + 2];
+  return {make_float2(a.x, a.y), make_float4(a.z, a.w, b.x, b.y), make_float3(b.z, b.w, c.x)};
+}
+__device__ __forceinline__ void store_rec(float4* __restrict__ rec, uint32_t g, float2 xy, float4 co, float3 rgb) {
+  rec[3 * g] = make_float4(xy.x, xy.y, co.x, co.y);
+  rec[3 * g + 1] = make_float4(co.z, co.w, rgb.x, rgb.y);
+  rec[3 * g + 2] = make_float4(rgb.z, 0.f, 0.f, 0.f);
+}
 
 inline BinState carve_bin(char* base, size_t R) {
   BinState b;

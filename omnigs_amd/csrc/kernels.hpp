@@ -63,7 +63,7 @@ struct SortResult {
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
                              hipStream_t s, SortResult* out);
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float2* xy,
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s);
@@ -71,12 +71,10 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // render.hip
 hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
-                                 const float2* xy, const float* colors, const float4* conic_opacity,
-                                 float* final_T, uint32_t* n_contrib, const float* bg, float* out_color,
-                                 hipStream_t s);
+                                 const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
+                                 float* out_color, hipStream_t s);
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
-                                  const float* bg, const float2* xy, const float4* conic_opacity,
-                                  const float* colors, const float* final_T, const uint32_t* n_contrib,
+                                  const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
                                   float* dL_dcolors, hipStream_t s);
 

@@ -69,7 +69,10 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
   const int area = (rc.x1 - rc.x0) * (rc.y1 - rc.y0);
   if (area == 0) return;
 
-  if (!a.colors_precomp) {
+  float3 col;
+  if (a.colors_precomp) {
+    col = make_float3(a.colors_precomp[3 * idx], a.colors_precomp[3 * idx + 1], a.colors_precomp[3 * idx + 2]);
+  } else {
     const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
     float3 d = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
     const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
@@ -92,15 +95,12 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
     rgb.y += 0.5f;
     rgb.z += 0.5f;
     g.clamped[idx] = (uint8_t)((rgb.x < 0.f) | ((rgb.y < 0.f) << 1) | ((rgb.z < 0.f) << 2));
-    g.rgb[3 * idx] = fmaxf(rgb.x, 0.f);
-    g.rgb[3 * idx + 1] = fmaxf(rgb.y, 0.f);
-    g.rgb[3 * idx + 2] = fmaxf(rgb.z, 0.f);
+    col = make_float3(fmaxf(rgb.x, 0.f), fmaxf(rgb.y, 0.f), fmaxf(rgb.z, 0.f));
   }
 
   g.depth_key[idx] = __float_as_uint(r);
   radii_out[idx] = radius;
-  g.xy[idx] = pix;
-  g.conic_opacity[idx] = conic;
+  store_rec(g.rec, idx, pix, conic, col);
   g.tiles_touched[idx] = (uint32_t)area;
 }
 

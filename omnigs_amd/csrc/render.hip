@@ -84,9 +84,8 @@ __device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* l
 
 __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
-    const float2* __restrict__ xy, const float* __restrict__ colors, const float4* __restrict__ conic_opacity,
-    float* __restrict__ final_T, uint32_t* __restrict__ n_contrib, const float* __restrict__ bg,
-    float* __restrict__ out_color) {
+    const float4* __restrict__ rec, float* __restrict__ final_T, uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ bg, float* __restrict__ out_color) {
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_q[kBatch];  // prescaled conic (a, b, c) and opacity
   __shared__ float s_rgb[kChannels][kBatch];
@@ -113,14 +112,15 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     const int k = base + tid;
     if (k < n) {
       const uint32_t g = point_list[range.x + k];
-      const float2 m = xy[g];
-      const float4 co = conic_opacity[g];
+      const RenderRec rr = load_rec(rec, g);
+      const float2 m = rr.xy;
+      const float4 co = rr.co;
       const float3 q = prescale_conic(co);
       s_xy[tid] = m;
       s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
-      s_rgb[0][tid] = colors[3 * g];
-      s_rgb[1][tid] = colors[3 * g + 1];
-      s_rgb[2][tid] = colors[3 * g + 2];
+      s_rgb[0][tid] = rr.rgb.x;
+      s_rgb[1][tid] = rr.rgb.y;
+      s_rgb[2][tid] = rr.rgb.z;
       s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
     }
     __syncthreads();
@@ -217,8 +217,8 @@ constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
 
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
-    const float* __restrict__ bg, const float2* __restrict__ xy, const float4* __restrict__ conic_opacity,
-    const float* __restrict__ colors, const float* __restrict__ final_Ts, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
+    const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dL_dpix, float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
     float* __restrict__ dL_dopacity, float* __restrict__ dL_dcolors) {
   __shared__ uint32_t s_gid[kBatch];
@@ -272,16 +272,17 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     const int k = base + tid;
     if (k < n) {
       const uint32_t g = point_list[range.x + (n - 1 - k)];
-      const float2 m = xy[g];
-      const float4 co = conic_opacity[g];
+      const RenderRec rr = load_rec(rec, g);
+      const float2 m = rr.xy;
+      const float4 co = rr.co;
       const float3 q = prescale_conic(co);
       s_gid[tid] = g;
       s_xy[tid] = m;
       s_co[tid] = co;
       s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
-      s_rgb[0][tid] = colors[3 * g];
-      s_rgb[1][tid] = colors[3 * g + 1];
-      s_rgb[2][tid] = colors[3 * g + 2];
+      s_rgb[0][tid] = rr.rgb.x;
+      s_rgb[1][tid] = rr.rgb.y;
+      s_rgb[2][tid] = rr.rgb.z;
       s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
     }
 #pragma unroll
@@ -371,25 +372,23 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
 }
 
 hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
-                                 const float2* xy, const float* colors, const float4* conic_opacity,
-                                 float* final_T, uint32_t* n_contrib, const float* bg, float* out_color,
-                                 hipStream_t s) {
+                                 const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
+                                 float* out_color, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(render_forward_kernel, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list, W, H,
-                     xy, colors, conic_opacity, final_T, n_contrib, bg, out_color);
+                     rec, final_T, n_contrib, bg, out_color);
   return hipGetLastError();
 }
 
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
-                                  const float* bg, const float2* xy, const float4* conic_opacity,
-                                  const float* colors, const float* final_T, const uint32_t* n_contrib,
+                                  const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
                                   float* dL_dcolors, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(render_backward_kernel, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list, W,
-                     H, bg, xy, conic_opacity, colors, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic,
+                     H, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic,
                      dL_dopacity, dL_dcolors);
   return hipGetLastError();
 }

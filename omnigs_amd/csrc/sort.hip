@@ -317,7 +317,7 @@ __device__ __forceinline__ uint32_t first_greater(const uint32_t* __restrict__ a
 
 __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t R, const uint32_t* __restrict__ order,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                const float2* __restrict__ xy,
+                                                                const float4* __restrict__ rec,
                                                                 const int* __restrict__ radii, TileGrid grid,
                                                                 uint32_t* __restrict__ tile_key,
                                                                 uint32_t* __restrict__ gid) {
@@ -342,7 +342,8 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     const uint32_t gg = order[s];
     s_end[j] = offsets[s];
     s_gid[j] = gg;
-    const Rect rc = tile_rect(xy[gg], radii[gg], grid);
+    const float4 r0 = rec[3 * gg];
+    const Rect rc = tile_rect(make_float2(r0.x, r0.y), radii[gg], grid);
     const int w = max(1, rc.x1 - rc.x0);
     s_x0[j] = (uint16_t)rc.x0;
     s_y0[j] = (uint16_t)rc.y0;
@@ -367,11 +368,11 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   }
 }
 
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float2* xy,
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
-  hipLaunchKernelGGL(duplicate_kernel, dim3(blocks), dim3(kDupThreads), 0, s, P, R, order, offsets, xy, radii, grid,
+  hipLaunchKernelGGL(duplicate_kernel, dim3(blocks), dim3(kDupThreads), 0, s, P, R, order, offsets, rec, radii, grid,
                      tile_key, gid);
   return hipGetLastError();
 }

@@ -83,7 +83,7 @@ def test_state_view_offsets_are_inside_buffers(lib):
     gsz = lib.omnigs_lonlat_geometry_bytes(P)
     bsz = lib.omnigs_lonlat_binning_bytes(R)
     isz = lib.omnigs_lonlat_image_bytes(W, H)
-    for f in ("depth_key", "radii", "means2D", "conic_opacity", "rgb", "clamped", "tiles_touched",
+    for f in ("depth_key", "radii", "render_records", "clamped", "tiles_touched",
               "depth_order", "offsets"):
         v = getattr(sv, f)
         assert base_g <= v < base_g + gsz and v % 256 == 0, f
