@@ -75,6 +75,8 @@ typedef struct omnigs_lonlat_forward_args {
   int prefiltered;             /* accepted and ignored, as in the reference's lonlat path */
   float* out_color;            /* [3,H,W] output image (planar CHW) */
   int* radii;                  /* [P] output radii in pixels, or NULL (kept internally) */
+  int seam_wrap;               /* 0: clamp tile columns at the image edge (the reference's behaviour);
+                                  1: wrap them around the longitude seam (extension, no reference counterpart) */
 } omnigs_lonlat_forward_args;
 
 typedef struct omnigs_lonlat_backward_args {
@@ -109,6 +111,7 @@ typedef struct omnigs_lonlat_backward_args {
   float* dL_drot;              /* [P,4] or NULL when scales is NULL */
   float* dpx_dt;               /* [P,3] optional: d(px)/d(t) rows of the lon/lat Jacobian */
   float* dpy_dt;               /* [P,3] optional */
+  int seam_wrap;               /* must equal the forward's */
 } omnigs_lonlat_backward_args;
 
 /* Forward: preprocess, binning, render.  On success *num_rendered = R. */

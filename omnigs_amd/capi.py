@@ -49,7 +49,7 @@ class ForwardArgs(ctypes.Structure):
         ("colors_precomp", ctypes.c_void_p), ("opacities", ctypes.c_void_p), ("scales", ctypes.c_void_p),
         ("scale_modifier", ctypes.c_float), ("rotations", ctypes.c_void_p), ("cov3D_precomp", ctypes.c_void_p),
         ("viewmatrix", ctypes.c_void_p), ("campos", ctypes.c_void_p), ("prefiltered", ctypes.c_int),
-        ("out_color", ctypes.c_void_p), ("radii", ctypes.c_void_p),
+        ("out_color", ctypes.c_void_p), ("radii", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
     ]
 
 
@@ -66,7 +66,7 @@ class BackwardArgs(ctypes.Structure):
         ("dL_dmean2D", ctypes.c_void_p), ("dL_dconic", ctypes.c_void_p), ("dL_dopacity", ctypes.c_void_p),
         ("dL_dcolor", ctypes.c_void_p), ("dL_dmean3D", ctypes.c_void_p), ("dL_dcov3D", ctypes.c_void_p),
         ("dL_dsh", ctypes.c_void_p), ("dL_dscale", ctypes.c_void_p), ("dL_drot", ctypes.c_void_p),
-        ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p),
+        ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
     ]
 
 
@@ -159,7 +159,7 @@ def stream_handle(device: torch.device | None = None) -> int:
 
 def forward(*, background, means3D, opacities, viewmatrix, campos, width, height, shs=None, colors_precomp=None,
             scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0, prefiltered=False,
-            buffers=None):
+            buffers=None, seam_wrap=False):
     """Runs omnigs_lonlat_forward.  Returns (R, out_color [3,H,W], radii [P], (geom, binning, image))."""
     lib = load()
     dev = means3D.device
@@ -172,7 +172,7 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
     bufs = buffers or (TorchBuffer(dev), TorchBuffer(dev), TorchBuffer(dev))
     a = ForwardArgs(P, degree, M, width, height, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
                     _ptr(opacities), _ptr(scales), float(scale_modifier), _ptr(rotations), _ptr(cov3D_precomp),
-                    _ptr(viewmatrix), _ptr(campos), int(prefiltered), _ptr(out), _ptr(radii))
+                    _ptr(viewmatrix), _ptr(campos), int(prefiltered), _ptr(out), _ptr(radii), int(bool(seam_wrap)))
     R = ctypes.c_int(0)
     st = lib.omnigs_lonlat_forward(ctypes.byref(a), bufs[0].c, bufs[1].c, bufs[2].c, stream_handle(dev),
                                    ctypes.byref(R))
@@ -183,7 +183,7 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
 
 def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buffers, shs=None,
              colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0,
-             want_jacobian=False):
+             want_jacobian=False, seam_wrap=False):
     """Runs omnigs_lonlat_backward; returns a dict of gradient tensors."""
     lib = load()
     dev = means3D.device
@@ -213,7 +213,7 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
                      _ptr(g["dL_dsh"]) if shs is not None else None,
                      _ptr(g["dL_dscales"]) if scales is not None else None,
                      _ptr(g["dL_drotations"]) if scales is not None else None,
-                     _ptr(g.get("dpx_dt")), _ptr(g.get("dpy_dt")))
+                     _ptr(g.get("dpx_dt")), _ptr(g.get("dpy_dt")), int(bool(seam_wrap)))
     st = lib.omnigs_lonlat_backward(ctypes.byref(a), stream_handle(dev))
     if st != OMNIGS_OK:
         raise OmnigsError(st, "omnigs_lonlat_backward")

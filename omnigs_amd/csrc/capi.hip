@@ -190,7 +190,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     return fail_arg("SH degree / coefficient count");
   if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
   hipStream_t s = (hipStream_t)stream_;
-  const TileGrid grid = tile_grid(W, H);
+  const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
 
   char* gbuf = grow(geometry_buffer, carve_geom(nullptr, P).bytes);
   char* ibuf = render ? grow(image_buffer, carve_img(nullptr, W, H).bytes) : kept_image;
@@ -203,7 +203,8 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   int* radii = a->radii ? a->radii : g.radii;
 
   PreprocessArgs pa{P, a->D, a->M, W, H, a->means3D, a->shs, a->colors_precomp, a->opacities, a->scales,
-                    a->scale_modifier, a->rotations, a->cov3D_precomp, a->viewmatrix, a->campos};
+                    a->scale_modifier, a->rotations, a->cov3D_precomp, a->viewmatrix, a->campos,
+                    a->seam_wrap ? 1 : 0};
   {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS, s);
     OMNIGS_TRY(launch_preprocess(pa, g, radii, s), "preprocess");
@@ -293,7 +294,7 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
   if (a->scales && (!a->dL_dscale || !a->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
   hipStream_t s = (hipStream_t)stream_;
-  const TileGrid grid = tile_grid(W, H);
+  const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
   const GeomState g = carve_geom(a->geom_buffer, P);
   const ImgState img = carve_img(a->image_buffer, W, H);
   const int* radii = a->radii ? a->radii : g.radii;

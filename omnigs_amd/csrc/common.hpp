@@ -36,9 +36,12 @@ constexpr float kTwoInvPi = 0.636619772367581343075535053490057448f;
 
 struct TileGrid {
   int gx, gy;
+  int wrap;  // 1: tile columns wrap around the longitude seam (extension; 0 = the reference's clamp)
   __host__ __device__ int tiles() const { return gx * gy; }
 };
-inline TileGrid tile_grid(int W, int H) { return {(W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY}; }
+inline TileGrid tile_grid(int W, int H, int wrap = 0) {
+  return {(W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, wrap};
+}
 
 // Tile rectangle [x0,x1) x [y0,y1) of a Gaussian at pixel (px,py) with integer
 // radius r.  The float evaluation order is the reference's getRect
@@ -55,10 +58,24 @@ __device__ __forceinline__ int rect_hi(float p, int r, int tile, int g) {
   int v = (int)((((p + (float)r) + (float)tile) - 1.0f) / (float)tile);
   return min(g, max(0, v));
 }
+// Seam-wrap mode (an extension; the reference clamps at the image edge,
+// auxiliary.h:56-66): the column range is floored instead of truncated, may
+// leave [0, gx) and is read modulo gx; a range wider than the ring is the ring.
 __device__ __forceinline__ Rect tile_rect(float2 p, int r, TileGrid g) {
+  if (g.wrap) {
+    int x0 = (int)floorf((p.x - (float)r) / (float)kTileX);
+    int x1 = (int)floorf((((p.x + (float)r) + (float)kTileX) - 1.0f) / (float)kTileX);
+    if (x1 - x0 >= g.gx) {
+      x0 = 0;
+      x1 = g.gx;
+    }
+    return {x0, rect_lo(p.y, r, kTileY, g.gy), x1, rect_hi(p.y, r, kTileY, g.gy)};
+  }
   return {rect_lo(p.x, r, kTileX, g.gx), rect_lo(p.y, r, kTileY, g.gy), rect_hi(p.x, r, kTileX, g.gx),
           rect_hi(p.y, r, kTileY, g.gy)};
 }
+// Tile column of rect column x (identity in clamp mode).
+__device__ __forceinline__ int wrap_col(int x, TileGrid g) { return g.wrap ? ((x % g.gx) + g.gx) % g.gx : x; }
 
 // ---------------------------------------------------------------------------
 // Buffer layouts.  The three byte buffers are private to this library; the

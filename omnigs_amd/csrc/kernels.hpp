@@ -17,6 +17,7 @@ struct PreprocessArgs {
   const float* cov3D_precomp;
   const float* viewmatrix;
   const float* campos;
+  int seam_wrap;
 };
 
 struct PreprocessBackwardArgs {

@@ -111,7 +111,7 @@ __global__ void mark_visible_kernel(int P, uint8_t* __restrict__ present) {
 
 hipError_t launch_preprocess(const PreprocessArgs& a, const GeomState& g, int* radii, hipStream_t s) {
   if (a.P == 0) return hipSuccess;
-  const TileGrid grid = tile_grid(a.W, a.H);
+  const TileGrid grid = tile_grid(a.W, a.H, a.seam_wrap);
   hipLaunchKernelGGL(preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, g, radii, grid);
   return hipGetLastError();
 }

@@ -39,12 +39,20 @@ __device__ __forceinline__ float power2_of(float3 q, float dx, float dy) {
 }
 __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
 
+// Horizontal offset to the nearest copy of the Gaussian across the longitude
+// seam (seam-wrap mode only).
+template <bool kWrap>
+__device__ __forceinline__ float seam_dx(float dx, float W, float invW) {
+  if (kWrap) dx -= W * rintf(dx * invW);
+  return dx;
+}
+
 // Conservative 4-bit mask of the tile's 8x8 wave quadrants that Gaussian
 // (m, co) can reach with alpha >= 1/255: outside the ellipse
 // 0.5 d^T Q d <= ln(255 o) the alpha test fails for every pixel, so skipping
 // those (wave, instance) pairs changes no result.  The box is padded by
 // 1% + 0.05 px against rounding; degenerate conics get the full mask.
-__device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile, int gx) {
+__device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile, int gx, float wrapW) {
   const float det = co.x * co.z - co.y * co.y;
   const float o255 = 255.0f * co.w;
   if (!(o255 > 0.9999f)) return 0u;  // alpha < 1/255 everywhere
@@ -53,6 +61,7 @@ __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile,
   const float ex = sqrtf(tau2 * co.z / det) * 1.01f + 0.05f;
   const float ey = sqrtf(tau2 * co.x / det) * 1.01f + 0.05f;
   const float x0 = (float)((tile % gx) * kTileX), y0 = (float)((tile / gx) * kTileY);
+  if (wrapW > 0.f) m.x -= wrapW * rintf((m.x - (x0 + 0.5f * kTileX)) / wrapW);  // nearest copy across the seam
   uint32_t mk = 0;
 #pragma unroll
   for (int q = 0; q < kWavesPerTile; ++q) {
@@ -82,10 +91,12 @@ __device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* l
   return count;
 }
 
+template <bool kWrap>
 __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
     const float4* __restrict__ rec, float* __restrict__ final_T, uint32_t* __restrict__ n_contrib,
     const float* __restrict__ bg, float* __restrict__ out_color) {
+  const float Wf = (float)W, invW = 1.0f / (float)W;
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_q[kBatch];  // prescaled conic (a, b, c) and opacity
   __shared__ float s_rgb[kChannels][kBatch];
@@ -121,7 +132,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       s_rgb[0][tid] = rr.rgb.x;
       s_rgb[1][tid] = rr.rgb.y;
       s_rgb[2][tid] = rr.rgb.z;
-      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
+      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
     __syncthreads();
     const int cnt = min(kBatch, n - base);
@@ -130,7 +141,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const int j = s_list[w][ii];
       const float2 m = s_xy[j];
       const float4 q = s_q[j];
-      const float dx = m.x - fx, dy = m.y - fy;
+      const float dx = seam_dx<kWrap>(m.x - fx, Wf, invW), dy = m.y - fy;
       const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
       if (power2 > 0.0f) continue;
       const float alpha = fminf(0.99f, q.w * exp2_hw(power2));
@@ -215,12 +226,14 @@ __device__ __forceinline__ int wave_max_int(int v) {
 
 constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
 
+template <bool kWrap>
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
     const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
     const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dL_dpix, float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
     float* __restrict__ dL_dopacity, float* __restrict__ dL_dcolors) {
+  const float Wf = (float)W, invW = 1.0f / (float)W;
   __shared__ uint32_t s_gid[kBatch];
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_co[kBatch];
@@ -283,7 +296,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       s_rgb[0][tid] = rr.rgb.x;
       s_rgb[1][tid] = rr.rgb.y;
       s_rgb[2][tid] = rr.rgb.z;
-      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx);
+      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
 #pragma unroll
     for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
@@ -299,7 +312,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const int pos = n - 1 - (base + j);
       const float2 m = s_xy[j];
       const float4 q = s_q[j];
-      const float dx = m.x - fx, dy = m.y - fy;
+      const float dx = seam_dx<kWrap>(m.x - fx, Wf, invW), dy = m.y - fy;
       const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
       const float G = exp2_hw(power2);
       const float alpha = fminf(0.99f, q.w * G);
@@ -376,8 +389,12 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
                                  float* out_color, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(render_forward_kernel, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list, W, H,
-                     rec, final_T, n_contrib, bg, out_color);
+  if (grid.wrap)
+    hipLaunchKernelGGL(render_forward_kernel<true>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list,
+                       W, H, rec, final_T, n_contrib, bg, out_color);
+  else
+    hipLaunchKernelGGL(render_forward_kernel<false>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges,
+                       point_list, W, H, rec, final_T, n_contrib, bg, out_color);
   return hipGetLastError();
 }
 
@@ -387,9 +404,14 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
                                   float* dL_dcolors, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(render_backward_kernel, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list, W,
-                     H, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic,
-                     dL_dopacity, dL_dcolors);
+  if (grid.wrap)
+    hipLaunchKernelGGL(render_backward_kernel<true>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges,
+                       point_list, W, H, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity,
+                       dL_dcolors);
+  else
+    hipLaunchKernelGGL(render_backward_kernel<false>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges,
+                       point_list, W, H, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity,
+                       dL_dcolors);
   return hipGetLastError();
 }
 

@@ -323,7 +323,8 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ gid) {
   __shared__ uint32_t s_end[kDupChunk];
   __shared__ uint32_t s_gid[kDupChunk];
-  __shared__ uint16_t s_x0[kDupChunk], s_y0[kDupChunk], s_w[kDupChunk];
+  __shared__ int s_x0[kDupChunk];
+  __shared__ uint16_t s_y0[kDupChunk], s_w[kDupChunk];
   __shared__ float s_inv_w[kDupChunk];
   __shared__ uint32_t s_span[2];
   const int tid = threadIdx.x;
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     const float4 r0 = rec[3 * gg];
     const Rect rc = tile_rect(make_float2(r0.x, r0.y), radii[gg], grid);
     const int w = max(1, rc.x1 - rc.x0);
-    s_x0[j] = (uint16_t)rc.x0;
+    s_x0[j] = rc.x0;
     s_y0[j] = (uint16_t)rc.y0;
     s_w[j] = (uint16_t)w;
     s_inv_w[j] = 1.0f / (float)w;
@@ -362,7 +363,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     if (r < 0) { --q; r += (int)w; }
     else if (r >= (int)w) { ++q; r -= (int)w; }
     const uint32_t y = (uint32_t)s_y0[j] + q;
-    const uint32_t x = (uint32_t)s_x0[j] + (uint32_t)r;
+    const uint32_t x = (uint32_t)wrap_col(s_x0[j] + r, grid);
     tile_key[k] = y * (uint32_t)grid.gx + x;
     gid[k] = s_gid[j];
   }

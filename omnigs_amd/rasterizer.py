@@ -70,6 +70,7 @@ class GaussianRasterizationSettings:
     tanfovy: float = 0.0
     prefiltered: bool = False
     camera_type: int = LONLAT
+    seam_wrap: bool = False  # extension: wrap tile columns around the longitude seam
 
 
 def _empty_like(ref: torch.Tensor) -> torch.Tensor:
@@ -87,7 +88,7 @@ def rasterize_gaussians(means3D, means2D, opacities, settings: GaussianRasteriza
         scales if scales is not None else e, rotations if rotations is not None else e,
         cov3D_precomp if cov3D_precomp is not None else e, settings.image_height, settings.image_width, settings.bg,
         float(settings.scale_modifier), settings.viewmatrix, pm, int(settings.sh_degree), settings.campos,
-        bool(settings.prefiltered), int(settings.camera_type), bool(checkpoint))
+        bool(settings.prefiltered), int(settings.camera_type), bool(checkpoint), bool(settings.seam_wrap))
 
 
 class GaussianRasterizer(torch.nn.Module):
@@ -117,7 +118,7 @@ class GaussianRasterizer(torch.nn.Module):
 
 def render_lonlat(params: dict, viewmatrix: torch.Tensor, campos: torch.Tensor, image_height: int,
                   image_width: int, bg_color: torch.Tensor, active_sh_degree: int = 3, scaling_modifier: float = 1.0,
-                  checkpoint: bool = False):
+                  checkpoint: bool = False, seam_wrap: bool = False):
     """The caller contract of GaussianRenderer::renderLonlat
     (src/gaussian_renderer.cpp:166-291) over raw parameters
     {xyz, features_dc [P,1,3], features_rest [P,15,3], scaling (log), rotation
@@ -132,7 +133,8 @@ def render_lonlat(params: dict, viewmatrix: torch.Tensor, campos: torch.Tensor, 
         pass
     settings = GaussianRasterizationSettings(image_height=image_height, image_width=image_width, bg=bg_color,
                                              viewmatrix=viewmatrix, campos=campos, sh_degree=active_sh_degree,
-                                             scale_modifier=scaling_modifier, camera_type=LONLAT)
+                                             scale_modifier=scaling_modifier, camera_type=LONLAT,
+                                             seam_wrap=seam_wrap)
     opacity = torch.sigmoid(params["opacity"])
     scales = torch.exp(params["scaling"])
     rotations = torch.nn.functional.normalize(params["rotation"])

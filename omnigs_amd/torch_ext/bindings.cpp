@@ -12,10 +12,10 @@ std::tuple<torch::Tensor, torch::Tensor> rasterize_gaussians(
     torch::Tensor colors_precomp, torch::Tensor scales, torch::Tensor rotations, torch::Tensor cov3D_precomp,
     int image_height, int image_width, torch::Tensor bg, float scale_modifier, torch::Tensor viewmatrix,
     torch::Tensor projmatrix, int sh_degree, torch::Tensor campos, bool prefiltered, int camera_type,
-    bool checkpoint) {
+    bool checkpoint, bool seam_wrap) {
   omnigs_torch::GaussianRasterizationSettings rs(image_height, image_width, 0.f, 0.f, bg, scale_modifier, viewmatrix,
                                                  projmatrix, torch::Tensor(), sh_degree, campos, prefiltered,
-                                                 (omnigs_torch::CameraModelType)camera_type, false);
+                                                 (omnigs_torch::CameraModelType)camera_type, false, seam_wrap);
   omnigs_torch::GaussianRasterizer rasterizer(rs, checkpoint);
   const bool has_shs = shs.defined() && shs.numel() > 0;
   const bool has_cols = colors_precomp.defined() && colors_precomp.numel() > 0;
@@ -36,18 +36,20 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("colors"), py::arg("opacity"), py::arg("scales"), py::arg("rotations"), py::arg("scale_modifier"),
         py::arg("cov3D_precomp"), py::arg("viewmatrix"), py::arg("projmatrix"), py::arg("tan_fovx"),
         py::arg("tan_fovy"), py::arg("image_height"), py::arg("image_width"), py::arg("sh"), py::arg("degree"),
-        py::arg("campos"), py::arg("prefiltered"), py::arg("camera_type") = 1, py::arg("render_depth") = false);
+        py::arg("campos"), py::arg("prefiltered"), py::arg("camera_type") = 1, py::arg("render_depth") = false,
+        py::arg("seam_wrap") = false);
   m.def("RasterizeGaussiansBackwardCUDA", &RasterizeGaussiansBackwardCUDA, py::arg("background"),
         py::arg("means3D"), py::arg("radii"), py::arg("colors"), py::arg("scales"), py::arg("rotations"),
         py::arg("scale_modifier"), py::arg("cov3D_precomp"), py::arg("viewmatrix"), py::arg("projmatrix"),
         py::arg("tan_fovx"), py::arg("tan_fovy"), py::arg("dL_dout_color"), py::arg("sh"), py::arg("degree"),
         py::arg("campos"), py::arg("geomBuffer"), py::arg("R"), py::arg("binningBuffer"), py::arg("imageBuffer"),
-        py::arg("camera_type") = 1);
+        py::arg("camera_type") = 1, py::arg("seam_wrap") = false);
   m.def("markVisible", &markVisible, py::arg("means3D"), py::arg("viewmatrix"), py::arg("projmatrix"),
         py::arg("camera_type") = 1);
   m.def("rasterize_gaussians", &rasterize_gaussians, py::arg("means3D"), py::arg("means2D"), py::arg("opacities"),
         py::arg("shs"), py::arg("colors_precomp"), py::arg("scales"), py::arg("rotations"), py::arg("cov3D_precomp"),
         py::arg("image_height"), py::arg("image_width"), py::arg("bg"), py::arg("scale_modifier"),
         py::arg("viewmatrix"), py::arg("projmatrix"), py::arg("sh_degree"), py::arg("campos"),
-        py::arg("prefiltered") = false, py::arg("camera_type") = 3, py::arg("checkpoint") = false);
+        py::arg("prefiltered") = false, py::arg("camera_type") = 3, py::arg("checkpoint") = false,
+        py::arg("seam_wrap") = false);
 }

@@ -32,6 +32,7 @@ void save_settings(torch::autograd::AutogradContext* ctx, const GaussianRasteriz
   ctx->saved_data["image_height"] = rs.image_height_;
   ctx->saved_data["image_width"] = rs.image_width_;
   ctx->saved_data["prefiltered"] = rs.prefiltered_;
+  ctx->saved_data["seam_wrap"] = rs.seam_wrap_;
 }
 
 char* resize_tensor(void* ctx, size_t nbytes) {
@@ -55,7 +56,7 @@ torch::autograd::tensor_list GaussianRasterizerFunction::forward(
   auto res = RasterizeGaussiansCUDA(rs.bg_, means3D, colors_precomp, opacities, scales, rotations,
                                     rs.scale_modifier_, cov3Ds_precomp, rs.viewmatrix_, rs.projmatrix_, rs.tanfovx_,
                                     rs.tanfovy_, rs.image_height_, rs.image_width_, sh, rs.sh_degree_, rs.campos_,
-                                    rs.prefiltered_, (int)rs.camera_type_, rs.render_depth_);
+                                    rs.prefiltered_, (int)rs.camera_type_, rs.render_depth_, rs.seam_wrap_);
   const int R = std::get<0>(res);
   auto color = std::get<1>(res);
   auto radii = std::get<2>(res);
@@ -78,7 +79,8 @@ torch::autograd::tensor_list GaussianRasterizerFunction::backward(torch::autogra
   auto s = ctx->get_saved_variables();
   auto r = RasterizeGaussiansBackwardCUDA(s[0], s[5], s[9], s[4], s[6], s[7], scale_modifier, s[8], s[1], s[2],
                                           tanfovx, tanfovy, grad_outputs[0].contiguous(), s[10], sh_degree, s[3],
-                                          s[11], R, s[12], s[13], camera_type);
+                                          s[11], R, s[12], s[13], camera_type,
+                                          ctx->saved_data["seam_wrap"].toBool());
   return grads_in_reference_order(r);
 }
 
@@ -91,7 +93,7 @@ torch::autograd::tensor_list GaussianRasterizerCheckpointFunction::forward(
   auto res = RasterizeGaussiansCUDA(rs.bg_, means3D, colors_precomp, opacities, scales, rotations,
                                     rs.scale_modifier_, cov3Ds_precomp, rs.viewmatrix_, rs.projmatrix_, rs.tanfovx_,
                                     rs.tanfovy_, rs.image_height_, rs.image_width_, sh, rs.sh_degree_, rs.campos_,
-                                    rs.prefiltered_, (int)rs.camera_type_, rs.render_depth_);
+                                    rs.prefiltered_, (int)rs.camera_type_, rs.render_depth_, rs.seam_wrap_);
   const int R = std::get<0>(res);
   auto color = std::get<1>(res);
   auto radii = std::get<2>(res);
@@ -142,6 +144,7 @@ torch::autograd::tensor_list GaussianRasterizerCheckpointFunction::backward(
     a.viewmatrix = fptr(vm);
     a.campos = fptr(cp);
     a.prefiltered = prefiltered ? 1 : 0;
+    a.seam_wrap = ctx->saved_data["seam_wrap"].toBool() ? 1 : 0;
     omnigs_buffer gb{&resize_tensor, &geom}, bb{&resize_tensor, &binning};
     int R2 = 0;
     void* stream = reinterpret_cast<void*>(c10::hip::getCurrentHIPStream(means3D.device().index()).stream());
@@ -153,7 +156,7 @@ torch::autograd::tensor_list GaussianRasterizerCheckpointFunction::backward(
   }
   auto r = RasterizeGaussiansBackwardCUDA(bg, means3D, radii, cols, scales, rots, scale_modifier, cov, vm, s[2],
                                           tanfovx, tanfovy, grad_outputs[0].contiguous(), sh, sh_degree, cp, geom, R,
-                                          binning, img, camera_type);
+                                          binning, img, camera_type, ctx->saved_data["seam_wrap"].toBool());
   return grads_in_reference_order(r);
 }
 

@@ -23,11 +23,12 @@ struct GaussianRasterizationSettings {
   GaussianRasterizationSettings(int image_height, int image_width, float tanfovx, float tanfovy, torch::Tensor bg,
                                 float scale_modifier, torch::Tensor viewmatrix, torch::Tensor projmatrix,
                                 torch::Tensor RwcT, int sh_degree, torch::Tensor campos, bool prefiltered,
-                                CameraModelType camera_type = LONLAT, bool render_depth = false)
+                                CameraModelType camera_type = LONLAT, bool render_depth = false,
+                                bool seam_wrap = false)
       : image_height_(image_height), image_width_(image_width), tanfovx_(tanfovx), tanfovy_(tanfovy), bg_(bg),
         scale_modifier_(scale_modifier), viewmatrix_(viewmatrix), projmatrix_(projmatrix), RwcT_(RwcT),
         sh_degree_(sh_degree), campos_(campos), prefiltered_(prefiltered), camera_type_(camera_type),
-        render_depth_(render_depth) {}
+        render_depth_(render_depth), seam_wrap_(seam_wrap) {}
 
   int image_height_;
   int image_width_;
@@ -43,6 +44,7 @@ struct GaussianRasterizationSettings {
   bool prefiltered_;
   CameraModelType camera_type_;
   bool render_depth_;
+  bool seam_wrap_;  // extension: wrap tile columns around the longitude seam
 };
 
 class GaussianRasterizerFunction : public torch::autograd::Function<GaussianRasterizerFunction> {

@@ -64,7 +64,7 @@ std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torc
     const float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
     const torch::Tensor& /*projmatrix*/, const float /*tan_fovx*/, const float /*tan_fovy*/, const int image_height,
     const int image_width, const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
-    const bool prefiltered, const int camera_type, const bool /*render_depth*/) {
+    const bool prefiltered, const int camera_type, const bool /*render_depth*/, const bool seam_wrap) {
   if (means3D.ndimension() != 2 || means3D.size(1) != 3) {
     AT_ERROR("means3D must have dimensions (num_points, 3)");
   }
@@ -106,6 +106,7 @@ std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torc
     a.prefiltered = prefiltered ? 1 : 0;
     a.out_color = out_color.data_ptr<float>();
     a.radii = radii.data_ptr<int>();
+    a.seam_wrap = seam_wrap ? 1 : 0;
     omnigs_buffer gb{&resize_tensor, &geomBuffer}, bb{&resize_tensor, &binningBuffer}, ib{&resize_tensor, &imgBuffer};
     check(omnigs_lonlat_forward(&a, gb, bb, ib, current_stream(means3D), &rendered), "RasterizeGaussiansCUDA");
   }
@@ -122,7 +123,7 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
                                const float /*tan_fovy*/, const torch::Tensor& dL_dout_color, const torch::Tensor& sh,
                                const int degree, const torch::Tensor& campos, const torch::Tensor& geomBuffer,
                                const int R, const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer,
-                               const int camera_type) {
+                               const int camera_type, const bool seam_wrap) {
   const int P = (int)means3D.size(0);
   const int H = (int)dL_dout_color.size(1);
   const int W = (int)dL_dout_color.size(2);
@@ -176,6 +177,7 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
     a.dL_dsh = M ? dL_dsh.data_ptr<float>() : nullptr;
     a.dL_dscale = dL_dscales.data_ptr<float>();
     a.dL_drot = dL_drotations.data_ptr<float>();
+    a.seam_wrap = seam_wrap ? 1 : 0;
     check(omnigs_lonlat_backward(&a, current_stream(means3D)), "RasterizeGaussiansBackwardCUDA");
   }
   return std::make_tuple(dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales,

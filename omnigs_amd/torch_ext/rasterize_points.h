@@ -4,7 +4,9 @@
 // caller built against them (GaussianRasterizerFunction, GaussianRenderer)
 // links against this library unchanged.  Only camera_type 3 (LONLAT) is
 // served; camera_type 1 (pinhole) is outside this build's scope and raises,
-// as any other invalid value does in the reference.
+// as any other invalid value does in the reference.  The trailing seam_wrap
+// parameter (default false = the reference's clamp at the image edge) is an
+// extension: tile columns wrap around the longitude seam.
 #pragma once
 
 #include <torch/torch.h>
@@ -17,7 +19,7 @@ std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torc
     const float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
     const torch::Tensor& projmatrix, const float tan_fovx, const float tan_fovy, const int image_height,
     const int image_width, const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
-    const bool prefiltered, const int camera_type = 1, const bool render_depth = false);
+    const bool prefiltered, const int camera_type = 1, const bool render_depth = false, const bool seam_wrap = false);
 
 std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,
            torch::Tensor>
@@ -29,7 +31,7 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
                                const torch::Tensor& dL_dout_color, const torch::Tensor& sh, const int degree,
                                const torch::Tensor& campos, const torch::Tensor& geomBuffer, const int R,
                                const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer,
-                               const int camera_type = 1);
+                               const int camera_type = 1, const bool seam_wrap = false);
 
 torch::Tensor markVisible(torch::Tensor& means3D, torch::Tensor& viewmatrix, torch::Tensor& projmatrix,
                           const int camera_type = 1);

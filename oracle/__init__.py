@@ -60,6 +60,7 @@ def _load():
     lib.oracle_set.restype = ctypes.c_int
     lib.oracle_higher_msb.argtypes = [ctypes.c_uint]
     lib.oracle_higher_msb.restype = ctypes.c_int
+    lib.oracle_set_seam_wrap.argtypes = [vp, ctypes.c_int]
     lib.oracle_time_fwd_bwd.argtypes = [vp, vp]
     lib.oracle_time_fwd_bwd.restype = ctypes.c_double
     _lib = lib
@@ -98,6 +99,8 @@ class Oracle:
                                    arr("opacities"), arr("scales"), float(scene.get("scale_modifier", 1.0)),
                                    arr("rotations"), arr("cov3D_precomp"), arr("viewmatrix"), arr("campos"))
         self._keep = []
+        if scene.get("seam_wrap"):
+            lib.oracle_set_seam_wrap(self.h, 1)
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -114,6 +114,25 @@ inline Rect tile_rect(double px, double py, int radius, int gx, int gy, int bx, 
   return tile_rect((float)px, (float)py, radius, gx, gy, bx, by);
 }
 
+// Seam-wrap extension (no reference counterpart; the reference clamps): the
+// column range uses floor, may leave [0, gx) and is read modulo gx; a range
+// wider than the ring becomes the ring.  Columns are returned as signed ints.
+struct WRect {
+  int x0, y0, x1, y1;
+};
+inline WRect tile_rect_wrap(float px, float py, int radius, int gx, int gy, int bx, int by) {
+  Rect c = tile_rect(px, py, radius, gx, gy, bx, by);
+  int x0 = (int)std::floor((px - (float)radius) / (float)bx);
+  int x1 = (int)std::floor((((px + (float)radius) + (float)bx) - 1.0f) / (float)bx);
+  if (x1 - x0 >= gx) {
+    x0 = 0;
+    x1 = gx;
+  }
+  return {x0, (int)c.y0, x1, (int)c.y1};
+}
+inline uint32_t wrap_col(int x, int gx) { return (uint32_t)(((x % gx) + gx) % gx); }
+template <class R> inline R seam_dx(R dx, int W) { return dx - R(W) * std::nearbyint(dx / R(W)); }
+
 // getHigherMsb, rasterizer_impl.cu:47-62 (bit count of n: highest set bit + 1).
 inline uint32_t higher_msb(uint32_t n) {
   uint32_t msb = sizeof(n) * 4, step = msb;
@@ -128,6 +147,7 @@ inline uint32_t higher_msb(uint32_t n) {
 template <class R> struct Inputs {
   int P = 0, D = 0, M = 0, W = 0, H = 0;
   int block_x = 16, block_y = 16;  // config.h:26-27
+  int seam_wrap = 0;                // extension: 1 = wrap tile columns and dx around the seam
   const R* bg = nullptr;            // [3]
   const R* means = nullptr;         // [P,3]
   const R* shs = nullptr;           // [P,M,3] or null
@@ -300,6 +320,10 @@ template <class R> void preprocess(const Inputs<R>& in, State<R>& st) {
     int radius = (int)radius_f;
     R px = ndc2pix(sx, in.W), py = ndc2pix(sy, in.H);
     Rect rc = tile_rect(px, py, radius, gx, gy, in.block_x, in.block_y);
+    if (in.seam_wrap) {
+      WRect w = tile_rect_wrap((float)px, (float)py, radius, gx, gy, in.block_x, in.block_y);
+      rc = {0, (uint32_t)w.y0, (uint32_t)(w.x1 - w.x0), (uint32_t)w.y1};  // width only
+    }
     if ((rc.x1 - rc.x0) * (rc.y1 - rc.y0) == 0) continue;
     if (!in.colors) {
       V3<R> c = sh_to_rgb(in.D, in.M, in.shs + (size_t)3 * in.M * i, p, campos, &st.clamped[3 * i]);
@@ -340,10 +364,17 @@ template <class R> void binning(const Inputs<R>& in, State<R>& st) {
   for (int i = 0; i < P; ++i) {
     if (st.radii[i] <= 0) continue;
     uint32_t off = i == 0 ? 0 : st.offsets[i - 1];
-    Rect rc = tile_rect(st.xy[2 * i], st.xy[2 * i + 1], st.radii[i], gx, gy, in.block_x, in.block_y);
-    for (uint32_t y = rc.y0; y < rc.y1; ++y)
-      for (uint32_t x = rc.x0; x < rc.x1; ++x) {
-        keys[off] = ((uint64_t)(y * gx + x) << 32) | depth_bits(st.depth[i]);
+    WRect rc;
+    if (in.seam_wrap) {
+      rc = tile_rect_wrap((float)st.xy[2 * i], (float)st.xy[2 * i + 1], st.radii[i], gx, gy, in.block_x, in.block_y);
+    } else {
+      Rect c = tile_rect(st.xy[2 * i], st.xy[2 * i + 1], st.radii[i], gx, gy, in.block_x, in.block_y);
+      rc = {(int)c.x0, (int)c.y0, (int)c.x1, (int)c.y1};
+    }
+    for (int y = rc.y0; y < rc.y1; ++y)
+      for (int x = rc.x0; x < rc.x1; ++x) {
+        const uint32_t col = in.seam_wrap ? wrap_col(x, gx) : (uint32_t)x;
+        keys[off] = ((uint64_t)((uint32_t)y * gx + col) << 32) | depth_bits(st.depth[i]);
         vals[off] = (uint32_t)i;
         ++off;
       }
@@ -401,7 +432,9 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
       for (uint32_t k = b; k < e; ++k) {
         ++contributor;
         const uint32_t g = st.point_list[k];
-        const R dx = st.xy[2 * g] - fx, dy = st.xy[2 * g + 1] - fy;
+        R dx = st.xy[2 * g] - fx;
+        const R dy = st.xy[2 * g + 1] - fy;
+        if (in.seam_wrap) dx = seam_dx(dx, W);
         const R* co = &st.conic[4 * g];
         const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
         if (power > R(0)) { sig_mix(sig, 1); continue; }
@@ -465,7 +498,9 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
         const uint32_t k = kk - 1;
         if (k - b >= last) continue;
         const uint32_t g = st.point_list[k];
-        const R dx = st.xy[2 * g] - fx, dy = st.xy[2 * g + 1] - fy;
+        R dx = st.xy[2 * g] - fx;
+        const R dy = st.xy[2 * g + 1] - fy;
+        if (in.seam_wrap) dx = seam_dx(dx, W);
         const R* co = &st.conic[4 * g];
         const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
         if (power > R(0)) continue;

@@ -23,14 +23,14 @@ struct HandleBase {
 
 template <class R> struct Handle : HandleBase {
   // owned copies of every input (so the caller's buffers may go away)
-  int P = 0, D = 0, M = 0, W = 0, H = 0, bx = 16, by = 16;
+  int P = 0, D = 0, M = 0, W = 0, H = 0, bx = 16, by = 16, seam_wrap = 0;
   std::vector<R> bg, means, shs, colors, opac, scales, rots, cov3d_pre, view, campos;
   bool has_shs = false, has_colors = false, has_scales = false, has_rots = false, has_cov = false;
   R mod = 1;
   State<R> st;
   Inputs<R> inputs() const {
     Inputs<R> in;
-    in.P = P; in.D = D; in.M = M; in.W = W; in.H = H; in.block_x = bx; in.block_y = by;
+    in.P = P; in.D = D; in.M = M; in.W = W; in.H = H; in.block_x = bx; in.block_y = by; in.seam_wrap = seam_wrap;
     in.bg = bg.data(); in.means = means.data(); in.opac = opac.data();
     in.shs = has_shs ? shs.data() : nullptr;
     in.colors = has_colors ? colors.data() : nullptr;
@@ -139,6 +139,11 @@ void* oracle_create(int precision, int P, int D, int M, int W, int H, int block_
 }
 
 void oracle_destroy(void* h) { delete static_cast<HandleBase*>(h); }
+
+// Extension switch: 1 = wrap tile columns and dx around the longitude seam.
+void oracle_set_seam_wrap(void* h, int on) {
+  dispatch(h, [&](auto& hd) { hd.seam_wrap = on ? 1 : 0; return 0; });
+}
 
 // stage: 0 preprocess, 1 binning, 2 render, 3 render_backward (needs dL_dpix),
 // 4 cov2d_backward, 5 preprocess_backward.  Returns 0 on success.
