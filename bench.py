@@ -3,9 +3,10 @@
 
 Workload: the 1.5M-Gaussian room scene at 1920x960 (FP32, SH degree 3, seeded;
 omnigs_amd.scenes.room_scene).  One step = forward (preprocess, binning,
-render) + the L1 upstream gradient sign(img - target)/(3HW) + backward
-(render backward, preprocess backward), all through the C ABI with inputs
-resident in HBM.  Prints ONE JSON line.
+render) + the L1 upstream gradient sign(img - target)/(3HW) from the fused
+loss kernel (omnigs_photometric_loss with lambda 0) + backward (render
+backward, preprocess backward), all through the C ABI with inputs resident in
+HBM.  Prints ONE JSON line.
 
 The roofline object prices the dominant stage (by measured time) against
 HBM: achieved = its algorithmic bytes per launch / its average launch time
@@ -83,11 +84,12 @@ def main() -> int:
     ds["opacities"] = ds["opacities"].reshape(-1).contiguous()
     target = ds["target"]
     bufs = (capi.TorchBuffer(dev), capi.TorchBuffer(dev), capi.TorchBuffer(dev))
-    inv = 1.0 / (3.0 * H * W)
+    l1 = capi.PhotometricLoss(lambda_dssim=0.0)
+    dL = torch.empty((3, H, W), dtype=torch.float32, device=dev)
 
     def step():
         R, img, radii, _ = capi.forward_scene(ds, buffers=bufs)
-        dL = torch.sign(img - target).mul_(inv)
+        l1(img, target, dL_dimage=dL)
         g = capi.backward_scene(ds, R, radii, bufs, dL)
         return R, radii, g
 

@@ -138,6 +138,24 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* args, void* stream
 /* markVisible for camera_type 3: every Gaussian is visible (present[i] = 1). */
 int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream);
 
+/* Fused training loss and its image gradient, the mapper's per-iteration
+ * loss (reference src/gaussian_mapper.cpp:384-407 with loss_utils.h:31-129):
+ *   masked = image * mask                      (mask [H,W], NULL = ones)
+ *   loss   = (1 - lambda) mean|masked - gt| + lambda (1 - mean SSIM(masked, gt))
+ * over the top height - skip_bottom rows (skip_bottom = round(H *
+ * skip_bottom_ratio) in the reference; 0 keeps every row).  SSIM is
+ * loss_utils::ssim: 11x11 Gaussian window with sigma 1.5, zero padding of 5,
+ * C1 = 0.01^2, C2 = 0.03^2.  lambda = 0 computes the L1 term alone.
+ * image, gt_image, dL_dimage: [channels,height,width] FP32 device arrays;
+ * dL_dimage is d loss / d image over every row (zero in the dropped rows).
+ * loss_out: 3 device floats {loss, L1, SSIM (0 when lambda = 0)}.
+ * workspace: device memory of omnigs_loss_workspace_bytes() bytes.  Replaces
+ * the torch ops plus loss.backward() the reference runs for this. */
+size_t omnigs_loss_workspace_bytes(int channels, int height, int width, int skip_bottom);
+int omnigs_photometric_loss(const float* image, const float* gt_image, const float* mask, int channels, int height,
+                            int width, int skip_bottom, float lambda_dssim, void* workspace, size_t workspace_bytes,
+                            float* dL_dimage, float* loss_out, void* stream);
+
 /* Byte sizes the forward requests from each buffer callback. */
 size_t omnigs_lonlat_geometry_bytes(int P);
 size_t omnigs_lonlat_binning_bytes(int R);

@@ -25,7 +25,8 @@ EXPORTED_SYMBOLS = (
     "omnigs_lonlat_forward", "omnigs_lonlat_recompute", "omnigs_lonlat_backward", "omnigs_lonlat_mark_visible",
     "omnigs_lonlat_geometry_bytes", "omnigs_lonlat_binning_bytes", "omnigs_lonlat_image_bytes",
     "omnigs_lonlat_state", "omnigs_profile_enable", "omnigs_profile_read", "omnigs_stage_name",
-    "omnigs_status_string", "omnigs_last_error", "omnigs_version",
+    "omnigs_status_string", "omnigs_last_error", "omnigs_version", "omnigs_loss_workspace_bytes",
+    "omnigs_photometric_loss",
 )
 NUM_STAGES = 10
 
@@ -113,6 +114,11 @@ def load() -> ctypes.CDLL:
     lib.omnigs_status_string.restype = ctypes.c_char_p
     lib.omnigs_last_error.restype = ctypes.c_char_p
     lib.omnigs_version.restype = ctypes.c_char_p
+    lib.omnigs_loss_workspace_bytes.argtypes = [ctypes.c_int] * 4
+    lib.omnigs_loss_workspace_bytes.restype = ctypes.c_size_t
+    lib.omnigs_photometric_loss.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [
+        ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.omnigs_photometric_loss.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -264,6 +270,51 @@ def mark_visible(P: int, device) -> torch.Tensor:
     if st != OMNIGS_OK:
         raise OmnigsError(st, "omnigs_lonlat_mark_visible")
     return present
+
+
+def skip_bottom_rows(height: int, skip_bottom_ratio: float) -> int:
+    """Rows the mapper drops from the bottom: round(H * ratio)
+    (reference src/gaussian_mapper.cpp:389-391; 0 when the ratio is 0)."""
+    return int(round(height * skip_bottom_ratio)) if skip_bottom_ratio > 0.0 else 0
+
+
+class PhotometricLoss:
+    """Fused (1 - lambda) L1 + lambda (1 - SSIM) and dL/dimage over the C ABI
+    (omnigs_photometric_loss), with the workspace kept between calls."""
+
+    def __init__(self, lambda_dssim: float = 0.2, skip_bottom_ratio: float = 0.0):
+        self.lambda_dssim = float(lambda_dssim)
+        self.skip_bottom_ratio = float(skip_bottom_ratio)
+        self._ws = None
+
+    def __call__(self, image: torch.Tensor, gt_image: torch.Tensor, mask: torch.Tensor | None = None,
+                 dL_dimage: torch.Tensor | None = None):
+        """Returns (values [3] = {loss, L1, SSIM}, dL_dimage [C,H,W]), both on
+        the image's device, queued on the current stream."""
+        lib = load()
+        if image.dim() != 3 or gt_image.shape != image.shape:
+            raise ValueError(f"image {tuple(image.shape)} and gt_image {tuple(gt_image.shape)} must be equal [C,H,W]")
+        if image.dtype != torch.float32 or gt_image.dtype != torch.float32:
+            raise TypeError("image and gt_image must be float32")
+        C, H, W = image.shape
+        if mask is not None:
+            mask = mask.reshape(H, W).to(torch.float32).contiguous()
+        skip = skip_bottom_rows(H, self.skip_bottom_ratio)
+        nbytes = lib.omnigs_loss_workspace_bytes(C, H, W, skip)
+        if nbytes == 0:
+            raise ValueError(f"no rows left: height {H}, skip_bottom {skip}")
+        if self._ws is None or self._ws.numel() < nbytes or self._ws.device != image.device:
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=image.device)
+        if dL_dimage is None:
+            dL_dimage = torch.empty_like(image)
+        out = torch.empty(3, dtype=torch.float32, device=image.device)
+        image, gt_image = image.contiguous(), gt_image.contiguous()
+        st = lib.omnigs_photometric_loss(_ptr(image), _ptr(gt_image), _ptr(mask), C, H, W, skip, self.lambda_dssim,
+                                         self._ws.data_ptr(), self._ws.numel(), _ptr(dL_dimage), out.data_ptr(),
+                                         stream_handle(image.device))
+        if st != OMNIGS_OK:
+            raise OmnigsError(st, "omnigs_photometric_loss")
+        return out, dL_dimage
 
 
 def scene_to_device(scene: dict, device="cuda:0") -> dict:

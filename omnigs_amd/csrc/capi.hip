@@ -162,6 +162,26 @@ int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, con
   return OMNIGS_OK;
 }
 
+size_t omnigs_loss_workspace_bytes(int channels, int height, int width, int skip_bottom) {
+  if (channels <= 0 || height <= 0 || width <= 0 || skip_bottom < 0 || skip_bottom >= height) return 0;
+  return loss_workspace_floats(channels, height, width, skip_bottom) * sizeof(float);
+}
+
+int omnigs_photometric_loss(const float* image, const float* gt_image, const float* mask, int channels, int height,
+                            int width, int skip_bottom, float lambda_dssim, void* workspace, size_t workspace_bytes,
+                            float* dL_dimage, float* loss_out, void* stream) {
+  if (channels <= 0 || height <= 0 || width <= 0) return fail_arg("empty image");
+  if (skip_bottom < 0 || skip_bottom >= height) return fail_arg("skip_bottom outside [0, height)");
+  if (!(lambda_dssim >= 0.f && lambda_dssim <= 1.f)) return fail_arg("lambda_dssim outside [0, 1]");
+  if (!image || !gt_image || !workspace || !dL_dimage || !loss_out) return fail_arg("missing required pointer");
+  if (workspace_bytes < omnigs_loss_workspace_bytes(channels, height, width, skip_bottom))
+    return fail_arg("workspace smaller than omnigs_loss_workspace_bytes()");
+  OMNIGS_TRY(launch_loss(image, gt_image, mask, channels, height, width, skip_bottom, lambda_dssim,
+                         static_cast<float*>(workspace), dL_dimage, loss_out, (hipStream_t)stream),
+             "photometric loss");
+  return OMNIGS_OK;
+}
+
 int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream) {
   if (P < 0) return fail_arg("P < 0");
   if (P > 0 && !present) return fail_arg("present is NULL");

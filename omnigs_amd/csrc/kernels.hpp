@@ -82,4 +82,11 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
 // preprocess_backward.hip
 hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s);
 
+// loss.hip
+// Floats of workspace the fused loss needs for a [C,H,W] image with `skip`
+// bottom rows dropped (0 when nothing is kept).
+size_t loss_workspace_floats(int C, int H, int W, int skip);
+hipError_t launch_loss(const float* x, const float* y, const float* mask, int C, int H, int W, int skip, float lambda,
+                       float* workspace, float* dL_dx, float* out, hipStream_t s);
+
 }  // namespace omnigs
