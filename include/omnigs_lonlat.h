@@ -77,6 +77,11 @@ typedef struct omnigs_lonlat_forward_args {
   int* radii;                  /* [P] output radii in pixels, or NULL (kept internally) */
   int seam_wrap;               /* 0: clamp tile columns at the image edge (the reference's behaviour);
                                   1: wrap them around the longitude seam (extension, no reference counterpart) */
+  /* Colour decoder (extension, no reference counterpart): when both are set
+   * the colours come from the decoder (omnigs_mlp_*) evaluated inside
+   * preprocess, and shs / colors_precomp are ignored. */
+  const void* mlp_packed;      /* omnigs_mlp_packed_bytes() bytes from omnigs_mlp_pack */
+  const uint16_t* latents;     /* [P,16] BF16 per-Gaussian latents */
 } omnigs_lonlat_forward_args;
 
 typedef struct omnigs_lonlat_backward_args {
@@ -112,6 +117,14 @@ typedef struct omnigs_lonlat_backward_args {
   float* dpx_dt;               /* [P,3] optional: d(px)/d(t) rows of the lon/lat Jacobian */
   float* dpy_dt;               /* [P,3] optional */
   int seam_wrap;               /* must equal the forward's */
+  /* Colour decoder (as in the forward).  When set, dL_dsh is unused and the
+   * decoder's backward adds the view-direction term to dL_dmean3D. */
+  const void* mlp_packed;
+  const uint16_t* latents;     /* [P,16] BF16 */
+  float* dL_dlatent;           /* [P,16] output */
+  float* dL_dmlp;              /* [omnigs_mlp_param_count()] output: gradient of the FP32 parameter vector */
+  void* mlp_workspace;         /* device scratch of omnigs_mlp_workspace_bytes(P) bytes */
+  size_t mlp_workspace_bytes;
 } omnigs_lonlat_backward_args;
 
 /* Forward: preprocess, binning, render.  On success *num_rendered = R. */
@@ -155,6 +168,69 @@ size_t omnigs_loss_workspace_bytes(int channels, int height, int width, int skip
 int omnigs_photometric_loss(const float* image, const float* gt_image, const float* mask, int channels, int height,
                             int width, int skip_bottom, float lambda_dssim, void* workspace, size_t workspace_bytes,
                             float* dL_dimage, float* loss_out, void* stream);
+
+/* Colour decoder (extension; no counterpart in the reference, whose colours
+ * come from computeColorFromSH, cuda_rasterizer/forward.cu:30-83).
+ * rgb = sigmoid(W3 relu(W2 relu(W1 [latent(16), Y2(dir)(9)] + b1) + b2) + b3)
+ * with Y2 the degree-2 real SH basis of the view direction, BF16 operands and
+ * FP32 accumulation on the MFMA unit.  The FP32 parameter vector (theta,
+ * omnigs_mlp_param_count() floats) holds, in order: W1 [64][25], b1 [64],
+ * W2 [64][64], b2 [64], W3 [3][64], b3 [3] (weights row-major [out][in]).
+ * omnigs_mlp_pack converts it into the kernels' BF16 operand image. */
+int omnigs_mlp_param_count(void);
+size_t omnigs_mlp_packed_bytes(void);
+int omnigs_mlp_pack(const float* theta, void* packed, void* stream);
+/* rgb[i] = decoder(latents[i], dirs[i]) for N unit directions [N,3]. */
+int omnigs_mlp_eval(int N, const uint16_t* latents, const float* dirs, const void* packed, float* rgb, void* stream);
+/* Device scratch of one decoder training pass over P Gaussians. */
+size_t omnigs_mlp_workspace_bytes(int P);
+
+/* One distillation pass: every Gaussian takes one of its num_dirs seeded
+ * view directions (chosen by seed and step), the teacher colour is its SH
+ * colour there (degree 3, + 0.5, clamped at 0, as computeColorFromSH), and
+ * the loss is the mean over P x 3 of (decoder - teacher)^2.  Writes the
+ * gradients of the FP32 latents and of theta, and the loss. */
+typedef struct omnigs_mlp_distill_args {
+  int P, M;                /* shs is [P,M,3], M >= 16 */
+  const float* shs;
+  const float* latents;    /* [P,16] FP32 master latents (rounded to BF16 for the MFMA) */
+  const void* mlp_packed;
+  unsigned int seed;
+  int step;
+  int num_dirs;            /* directions per Gaussian (>= 1) */
+  float* dL_dlatent;       /* [P,16] */
+  float* dL_dmlp;          /* [param_count] */
+  float* loss;             /* 1 float */
+  void* workspace;         /* omnigs_mlp_workspace_bytes(P) */
+  size_t workspace_bytes;
+} omnigs_mlp_distill_args;
+int omnigs_mlp_distill_step(const omnigs_mlp_distill_args* args, void* stream);
+
+/* Seeded direction k of Gaussian g used by the distillation (unit vector). */
+void omnigs_mlp_seeded_direction(unsigned int seed, unsigned int g, unsigned int k, float out[3]);
+
+/* Fused Adam step over up to 8 parameter groups in one launch, the update of
+ * torch::optim::Adam as the reference's GaussianModel uses it (betas (0.9,
+ * 0.999) by default, eps 1e-15, no weight decay; gaussian_model.cpp:485-518):
+ *   m = b1 m + (1 - b1) g;  v = b2 v + (1 - b2) g^2;
+ *   p -= lr / (1 - b1^step) * m / (sqrt(v) / sqrt(1 - b2^step) + eps).
+ * step is the 1-based step count of this update (torch increments it first).
+ * param_bf16, when set, receives a BF16 copy of the updated parameter. */
+typedef struct omnigs_adam_group {
+  float* param;
+  const float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  uint16_t* param_bf16;
+  size_t numel;
+  float lr;
+  float eps;
+  long long step;
+} omnigs_adam_group;
+int omnigs_adam_step(const omnigs_adam_group* groups, int n_groups, float beta1, float beta2, void* stream);
+
+/* FP32 -> BF16 (round to nearest even). */
+int omnigs_f32_to_bf16(const float* in, uint16_t* out, size_t n, void* stream);
 
 /* Byte sizes the forward requests from each buffer callback. */
 size_t omnigs_lonlat_geometry_bytes(int P);
@@ -203,7 +279,8 @@ enum omnigs_stage {
   OMNIGS_STAGE_CLEAR_GRADS = 7,
   OMNIGS_STAGE_RENDER_BACKWARD = 8,
   OMNIGS_STAGE_PREPROCESS_BACKWARD = 9,
-  OMNIGS_NUM_STAGES = 10
+  OMNIGS_STAGE_DECODER_BACKWARD = 10,
+  OMNIGS_NUM_STAGES = 11
 };
 int omnigs_profile_enable(int on);
 int omnigs_profile_read(double* ms, int* count);

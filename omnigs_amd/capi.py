@@ -26,9 +26,11 @@ EXPORTED_SYMBOLS = (
     "omnigs_lonlat_geometry_bytes", "omnigs_lonlat_binning_bytes", "omnigs_lonlat_image_bytes",
     "omnigs_lonlat_state", "omnigs_profile_enable", "omnigs_profile_read", "omnigs_stage_name",
     "omnigs_status_string", "omnigs_last_error", "omnigs_version", "omnigs_loss_workspace_bytes",
-    "omnigs_photometric_loss",
+    "omnigs_photometric_loss", "omnigs_mlp_param_count", "omnigs_mlp_packed_bytes", "omnigs_mlp_pack",
+    "omnigs_mlp_eval", "omnigs_mlp_workspace_bytes", "omnigs_mlp_distill_step", "omnigs_mlp_seeded_direction",
+    "omnigs_adam_step", "omnigs_f32_to_bf16",
 )
-NUM_STAGES = 10
+NUM_STAGES = 11
 
 _STATE_FIELDS = ("depth_key", "radii", "render_records", "clamped", "tiles_touched",
                  "depth_order", "offsets", "point_list", "tile_keys", "final_T", "n_contrib", "ranges")
@@ -51,6 +53,7 @@ class ForwardArgs(ctypes.Structure):
         ("scale_modifier", ctypes.c_float), ("rotations", ctypes.c_void_p), ("cov3D_precomp", ctypes.c_void_p),
         ("viewmatrix", ctypes.c_void_p), ("campos", ctypes.c_void_p), ("prefiltered", ctypes.c_int),
         ("out_color", ctypes.c_void_p), ("radii", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
+        ("mlp_packed", ctypes.c_void_p), ("latents", ctypes.c_void_p),
     ]
 
 
@@ -68,6 +71,25 @@ class BackwardArgs(ctypes.Structure):
         ("dL_dcolor", ctypes.c_void_p), ("dL_dmean3D", ctypes.c_void_p), ("dL_dcov3D", ctypes.c_void_p),
         ("dL_dsh", ctypes.c_void_p), ("dL_dscale", ctypes.c_void_p), ("dL_drot", ctypes.c_void_p),
         ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
+        ("mlp_packed", ctypes.c_void_p), ("latents", ctypes.c_void_p), ("dL_dlatent", ctypes.c_void_p),
+        ("dL_dmlp", ctypes.c_void_p), ("mlp_workspace", ctypes.c_void_p), ("mlp_workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+class DistillArgs(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int), ("M", ctypes.c_int), ("shs", ctypes.c_void_p), ("latents", ctypes.c_void_p),
+        ("mlp_packed", ctypes.c_void_p), ("seed", ctypes.c_uint), ("step", ctypes.c_int), ("num_dirs", ctypes.c_int),
+        ("dL_dlatent", ctypes.c_void_p), ("dL_dmlp", ctypes.c_void_p), ("loss", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+class AdamGroup(ctypes.Structure):
+    _fields_ = [
+        ("param", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
+        ("exp_avg_sq", ctypes.c_void_p), ("param_bf16", ctypes.c_void_p), ("numel", ctypes.c_size_t),
+        ("lr", ctypes.c_float), ("eps", ctypes.c_float), ("step", ctypes.c_longlong),
     ]
 
 
@@ -119,6 +141,24 @@ def load() -> ctypes.CDLL:
     lib.omnigs_photometric_loss.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [
         ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.omnigs_photometric_loss.restype = ctypes.c_int
+    lib.omnigs_mlp_param_count.restype = ctypes.c_int
+    lib.omnigs_mlp_packed_bytes.restype = ctypes.c_size_t
+    lib.omnigs_mlp_pack.argtypes = [ctypes.c_void_p] * 3
+    lib.omnigs_mlp_pack.restype = ctypes.c_int
+    lib.omnigs_mlp_eval.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5
+    lib.omnigs_mlp_eval.restype = ctypes.c_int
+    lib.omnigs_mlp_workspace_bytes.argtypes = [ctypes.c_int]
+    lib.omnigs_mlp_workspace_bytes.restype = ctypes.c_size_t
+    lib.omnigs_mlp_distill_step.argtypes = [ctypes.POINTER(DistillArgs), ctypes.c_void_p]
+    lib.omnigs_mlp_distill_step.restype = ctypes.c_int
+    lib.omnigs_mlp_seeded_direction.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                                ctypes.POINTER(ctypes.c_float)]
+    lib.omnigs_mlp_seeded_direction.restype = None
+    lib.omnigs_adam_step.argtypes = [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_void_p]
+    lib.omnigs_adam_step.restype = ctypes.c_int
+    lib.omnigs_f32_to_bf16.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    lib.omnigs_f32_to_bf16.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -165,8 +205,9 @@ def stream_handle(device: torch.device | None = None) -> int:
 
 def forward(*, background, means3D, opacities, viewmatrix, campos, width, height, shs=None, colors_precomp=None,
             scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0, prefiltered=False,
-            buffers=None, seam_wrap=False):
-    """Runs omnigs_lonlat_forward.  Returns (R, out_color [3,H,W], radii [P], (geom, binning, image))."""
+            buffers=None, seam_wrap=False, mlp=None):
+    """Runs omnigs_lonlat_forward.  Returns (R, out_color [3,H,W], radii [P], (geom, binning, image)).
+    mlp: (packed image, BF16 latents [P,16] as int16/uint16 tensor) to take colours from the decoder."""
     lib = load()
     dev = means3D.device
     P = means3D.shape[0]
@@ -178,7 +219,8 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
     bufs = buffers or (TorchBuffer(dev), TorchBuffer(dev), TorchBuffer(dev))
     a = ForwardArgs(P, degree, M, width, height, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
                     _ptr(opacities), _ptr(scales), float(scale_modifier), _ptr(rotations), _ptr(cov3D_precomp),
-                    _ptr(viewmatrix), _ptr(campos), int(prefiltered), _ptr(out), _ptr(radii), int(bool(seam_wrap)))
+                    _ptr(viewmatrix), _ptr(campos), int(prefiltered), _ptr(out), _ptr(radii), int(bool(seam_wrap)),
+                    _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None)
     R = ctypes.c_int(0)
     st = lib.omnigs_lonlat_forward(ctypes.byref(a), bufs[0].c, bufs[1].c, bufs[2].c, stream_handle(dev),
                                    ctypes.byref(R))
@@ -189,8 +231,9 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
 
 def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buffers, shs=None,
              colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0,
-             want_jacobian=False, seam_wrap=False):
-    """Runs omnigs_lonlat_backward; returns a dict of gradient tensors."""
+             want_jacobian=False, seam_wrap=False, mlp=None, mlp_workspace=None):
+    """Runs omnigs_lonlat_backward; returns a dict of gradient tensors (plus
+    dL_dlatent [P,16] and dL_dmlp [param_count] when mlp = (packed, latents))."""
     lib = load()
     dev = means3D.device
     P = means3D.shape[0]
@@ -209,6 +252,12 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
     if want_jacobian:
         g["dpx_dt"] = torch.empty((P, 3), **f)
         g["dpy_dt"] = torch.empty((P, 3), **f)
+    if mlp:
+        g["dL_dlatent"] = torch.empty((P, 16), **f)
+        g["dL_dmlp"] = torch.empty((lib.omnigs_mlp_param_count(),), **f)
+        nbytes = lib.omnigs_mlp_workspace_bytes(P)
+        if mlp_workspace is None or mlp_workspace.numel() < nbytes:
+            mlp_workspace = torch.empty((nbytes,), dtype=torch.uint8, device=dev)
     geom, binning, image = buffers
     a = BackwardArgs(P, degree, M, R, W, H, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
                      _ptr(scales), float(scale_modifier), _ptr(rotations), _ptr(cov3D_precomp), _ptr(viewmatrix),
@@ -219,7 +268,10 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
                      _ptr(g["dL_dsh"]) if shs is not None else None,
                      _ptr(g["dL_dscales"]) if scales is not None else None,
                      _ptr(g["dL_drotations"]) if scales is not None else None,
-                     _ptr(g.get("dpx_dt")), _ptr(g.get("dpy_dt")), int(bool(seam_wrap)))
+                     _ptr(g.get("dpx_dt")), _ptr(g.get("dpy_dt")), int(bool(seam_wrap)),
+                     _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None, _ptr(g.get("dL_dlatent")),
+                     _ptr(g.get("dL_dmlp")), mlp_workspace.data_ptr() if mlp else None,
+                     mlp_workspace.numel() if mlp else 0)
     st = lib.omnigs_lonlat_backward(ctypes.byref(a), stream_handle(dev))
     if st != OMNIGS_OK:
         raise OmnigsError(st, "omnigs_lonlat_backward")
@@ -270,6 +322,72 @@ def mark_visible(P: int, device) -> torch.Tensor:
     if st != OMNIGS_OK:
         raise OmnigsError(st, "omnigs_lonlat_mark_visible")
     return present
+
+
+def _check(st: int, where: str) -> None:
+    if st != OMNIGS_OK:
+        raise OmnigsError(st, where)
+
+
+def mlp_pack(theta: torch.Tensor, packed: torch.Tensor | None = None) -> torch.Tensor:
+    """FP32 decoder parameters -> the kernels' BF16 operand image (uint8 tensor)."""
+    lib = load()
+    assert theta.dtype == torch.float32 and theta.numel() == lib.omnigs_mlp_param_count()
+    if packed is None:
+        packed = torch.empty((lib.omnigs_mlp_packed_bytes(),), dtype=torch.uint8, device=theta.device)
+    _check(lib.omnigs_mlp_pack(_ptr(theta.contiguous()), packed.data_ptr(), stream_handle(theta.device)),
+           "omnigs_mlp_pack")
+    return packed
+
+
+def mlp_eval(latents_bf16: torch.Tensor, dirs: torch.Tensor, packed: torch.Tensor) -> torch.Tensor:
+    lib = load()
+    N = dirs.shape[0]
+    rgb = torch.empty((N, 3), dtype=torch.float32, device=dirs.device)
+    _check(lib.omnigs_mlp_eval(N, _ptr(latents_bf16), _ptr(dirs.contiguous()), packed.data_ptr(), _ptr(rgb),
+                               stream_handle(dirs.device)), "omnigs_mlp_eval")
+    return rgb
+
+
+def mlp_distill_step(shs, latents, packed, seed, step, num_dirs, dL_dlatent, dL_dmlp, loss, workspace) -> None:
+    lib = load()
+    P, M = shs.shape[0], shs.shape[1]
+    a = DistillArgs(P, M, _ptr(shs), _ptr(latents), packed.data_ptr(), int(seed) & 0xFFFFFFFF, int(step),
+                    int(num_dirs), _ptr(dL_dlatent), _ptr(dL_dmlp), _ptr(loss), workspace.data_ptr(),
+                    workspace.numel())
+    _check(lib.omnigs_mlp_distill_step(ctypes.byref(a), stream_handle(shs.device)), "omnigs_mlp_distill_step")
+
+
+def mlp_workspace(P: int, device) -> torch.Tensor:
+    return torch.empty((load().omnigs_mlp_workspace_bytes(P),), dtype=torch.uint8, device=device)
+
+
+def seeded_direction(seed: int, g: int, k: int):
+    out = (ctypes.c_float * 3)()
+    load().omnigs_mlp_seeded_direction(seed & 0xFFFFFFFF, g, k, out)
+    return tuple(out)
+
+
+def adam_step(groups, beta1: float = 0.9, beta2: float = 0.999, device=None) -> None:
+    """groups: dicts {param, grad, exp_avg, exp_avg_sq, lr, eps, step[, param_bf16]} of FP32 tensors."""
+    lib = load()
+    arr = (AdamGroup * len(groups))()
+    for i, g in enumerate(groups):
+        arr[i] = AdamGroup(_ptr(g["param"]), _ptr(g["grad"]), _ptr(g["exp_avg"]), _ptr(g["exp_avg_sq"]),
+                           _ptr(g.get("param_bf16")), g["param"].numel(), float(g["lr"]), float(g.get("eps", 1e-15)),
+                           int(g["step"]))
+    dev = device or groups[0]["param"].device
+    _check(lib.omnigs_adam_step(arr, len(groups), float(beta1), float(beta2), stream_handle(dev)), "omnigs_adam_step")
+
+
+def f32_to_bf16(x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """BF16 bits (int16 tensor) of an FP32 tensor, rounded to nearest even on the GPU."""
+    lib = load()
+    if out is None:
+        out = torch.empty(x.shape, dtype=torch.int16, device=x.device)
+    _check(lib.omnigs_f32_to_bf16(_ptr(x.contiguous()), _ptr(out), x.numel(), stream_handle(x.device)),
+           "omnigs_f32_to_bf16")
+    return out
 
 
 def skip_bottom_rows(height: int, skip_bottom_ratio: float) -> int:

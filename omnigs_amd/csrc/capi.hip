@@ -10,6 +10,7 @@
 
 #include "../../include/omnigs_lonlat.h"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 namespace {
 
@@ -129,7 +130,7 @@ int omnigs_profile_read(double* ms, int* count) {
 const char* omnigs_stage_name(int stage) {
   static const char* names[OMNIGS_NUM_STAGES] = {"preprocess", "depth_sort", "scan", "duplicate", "tile_sort",
                                                  "ranges", "render", "clear_grads", "render_backward",
-                                                 "preprocess_backward"};
+                                                 "preprocess_backward", "decoder_backward"};
   return (stage >= 0 && stage < OMNIGS_NUM_STAGES) ? names[stage] : "unknown";
 }
 
@@ -205,8 +206,10 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || (render && !a->out_color))
     return fail_arg("missing required pointer");
   if (!render && !kept_image) return fail_arg("recompute needs the forward's image buffer");
-  if (!a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
-  if (!a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
+  const bool mlp = a->mlp_packed && a->latents;
+  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
+  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  if (!mlp && !a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
     return fail_arg("SH degree / coefficient count");
   if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
   hipStream_t s = (hipStream_t)stream_;
@@ -224,7 +227,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
 
   PreprocessArgs pa{P, a->D, a->M, W, H, a->means3D, a->shs, a->colors_precomp, a->opacities, a->scales,
                     a->scale_modifier, a->rotations, a->cov3D_precomp, a->viewmatrix, a->campos,
-                    a->seam_wrap ? 1 : 0};
+                    a->seam_wrap ? 1 : 0, mlp ? a->mlp_packed : nullptr, mlp ? a->latents : nullptr};
   {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS, s);
     OMNIGS_TRY(launch_preprocess(pa, g, radii, s), "preprocess");
@@ -309,8 +312,12 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
     return fail_arg("missing required pointer");
   if (!a->dL_dmean2D || !a->dL_dconic || !a->dL_dopacity || !a->dL_dcolor || !a->dL_dmean3D || !a->dL_dcov3D)
     return fail_arg("missing gradient output");
-  if (a->shs && !a->dL_dsh) return fail_arg("dL_dsh is NULL while shs is given");
-  if (!a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  const bool mlp = a->mlp_packed && a->latents;
+  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
+  if (mlp && (!a->dL_dlatent || !a->dL_dmlp || !a->mlp_workspace)) return fail_arg("decoder gradient output missing");
+  if (mlp && a->mlp_workspace_bytes < omnigs_mlp_workspace_bytes(P)) return fail_arg("mlp_workspace too small");
+  if (!mlp && a->shs && !a->dL_dsh) return fail_arg("dL_dsh is NULL while shs is given");
+  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
   if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
   if (a->scales && (!a->dL_dscale || !a->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
   hipStream_t s = (hipStream_t)stream_;
@@ -342,7 +349,8 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
                                       a->dL_dopacity, a->dL_dcolor, s),
                "render backward");
   }
-  PreprocessBackwardArgs pb{P, a->D, a->M, W, H, a->means3D, a->shs, a->scales, a->scale_modifier, a->rotations,
+  PreprocessBackwardArgs pb{P, a->D, a->M, W, H, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,
+                            a->rotations,
                             a->cov3D_precomp, a->viewmatrix, a->campos, radii, g.clamped, a->dL_dmean2D,
                             a->dL_dconic, a->dL_dcolor, a->dL_dmean3D, a->dL_dcov3D, a->dL_dsh,
                             a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt};
@@ -350,6 +358,80 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
     StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
     OMNIGS_TRY(launch_preprocess_backward(pb, s), "preprocess backward");
   }
+  if (mlp) {
+    StageTimer tm(OMNIGS_STAGE_DECODER_BACKWARD, s);
+    OMNIGS_TRY(launch_mlp_backward(P, a->means3D, a->campos, radii, a->latents, a->mlp_packed, a->dL_dcolor,
+                                   a->dL_dmean3D, a->dL_dlatent, a->dL_dmlp, static_cast<float*>(a->mlp_workspace), s),
+               "decoder backward");
+  }
+  return OMNIGS_OK;
+}
+
+int omnigs_mlp_param_count(void) { return kThetaSize; }
+size_t omnigs_mlp_packed_bytes(void) { return kMlpPackedBytes; }
+size_t omnigs_mlp_workspace_bytes(int P) { return P < 0 ? 0 : mlp_workspace_floats(P) * sizeof(float); }
+
+int omnigs_mlp_pack(const float* theta, void* packed, void* stream) {
+  if (!theta || !packed) return fail_arg("NULL theta or packed");
+  OMNIGS_TRY(launch_mlp_pack(theta, packed, (hipStream_t)stream), "mlp pack");
+  return OMNIGS_OK;
+}
+
+int omnigs_mlp_eval(int N, const uint16_t* latents, const float* dirs, const void* packed, float* rgb, void* stream) {
+  if (N < 0) return fail_arg("N < 0");
+  if (N > 0 && (!latents || !dirs || !packed || !rgb)) return fail_arg("missing pointer");
+  OMNIGS_TRY(launch_mlp_eval(N, latents, dirs, packed, rgb, (hipStream_t)stream), "mlp eval");
+  return OMNIGS_OK;
+}
+
+int omnigs_mlp_distill_step(const omnigs_mlp_distill_args* a, void* stream) {
+  if (!a) return fail_arg("NULL args");
+  if (a->P <= 0 || a->M < 16 || a->num_dirs < 1) return fail_arg("P <= 0, M < 16 or num_dirs < 1");
+  if (!a->shs || !a->latents || !a->mlp_packed || !a->dL_dlatent || !a->dL_dmlp || !a->loss || !a->workspace)
+    return fail_arg("missing pointer");
+  if (a->workspace_bytes < omnigs_mlp_workspace_bytes(a->P)) return fail_arg("workspace too small");
+  OMNIGS_TRY(launch_mlp_distill(a->P, a->M, a->shs, a->latents, a->mlp_packed, a->seed, a->step, a->num_dirs,
+                                a->dL_dlatent, a->dL_dmlp, a->loss, static_cast<float*>(a->workspace),
+                                (hipStream_t)stream),
+             "mlp distill");
+  return OMNIGS_OK;
+}
+
+void omnigs_mlp_seeded_direction(unsigned int seed, unsigned int g, unsigned int k, float out[3]) {
+  const float3 d = seeded_direction(seed, g, k);
+  out[0] = d.x;
+  out[1] = d.y;
+  out[2] = d.z;
+}
+
+int omnigs_adam_step(const omnigs_adam_group* groups, int n, float beta1, float beta2, void* stream) {
+  if (n < 0 || n > kAdamMaxGroups || (n > 0 && !groups)) return fail_arg("group count outside [0, 8]");
+  if (!(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f)) return fail_arg("betas outside [0, 1)");
+  AdamGroupTable t{};
+  t.n = n;
+  t.one_minus_beta1 = (float)(1.0 - (double)beta1);
+  t.one_minus_beta2 = (float)(1.0 - (double)beta2);
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    const omnigs_adam_group& g = groups[i];
+    if (g.numel > 0 && (!g.param || !g.grad || !g.exp_avg || !g.exp_avg_sq)) return fail_arg("missing group pointer");
+    if (g.step < 1) return fail_arg("step < 1");
+    // scalars as torch::optim::Adam forms them (double), used in FP32
+    const double bc1 = 1.0 - std::pow((double)beta1, (double)g.step);
+    const double bc2 = 1.0 - std::pow((double)beta2, (double)g.step);
+    t.g[i] = AdamGroupDev{g.param, g.grad, g.exp_avg, g.exp_avg_sq, g.param_bf16, (float)((double)g.lr / bc1),
+                          (float)std::sqrt(bc2), g.eps};
+    t.offset[i] = off;
+    off += g.numel;
+  }
+  t.offset[n] = off;
+  OMNIGS_TRY(launch_adam(t, beta1, beta2, (hipStream_t)stream), "adam");
+  return OMNIGS_OK;
+}
+
+int omnigs_f32_to_bf16(const float* in, uint16_t* out, size_t n, void* stream) {
+  if (n > 0 && (!in || !out)) return fail_arg("missing pointer");
+  OMNIGS_TRY(launch_to_bf16(in, out, n, (hipStream_t)stream), "to bf16");
   return OMNIGS_OK;
 }
 

@@ -18,6 +18,9 @@ struct PreprocessArgs {
   const float* viewmatrix;
   const float* campos;
   int seam_wrap;
+  // colour decoder (colours from the decoder when both are set)
+  const void* mlp_packed;
+  const uint16_t* latents;  // BF16 [P,16]
 };
 
 struct PreprocessBackwardArgs {
@@ -81,6 +84,40 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
 
 // preprocess_backward.hip
 hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s);
+
+// mlp.hip
+constexpr int kMlpTrainBlocks = 512;  // persistent grid of the decoder training pass
+constexpr int kThetaReduceRows = 32;  // rows per stage-1 block of the weight-gradient reduction
+int mlp_train_blocks(int P);
+size_t mlp_workspace_floats(int P);
+hipError_t launch_mlp_pack(const float* theta, void* packed, hipStream_t s);
+hipError_t launch_mlp_eval(int N, const uint16_t* latents, const float* dirs, const void* packed, float* rgb,
+                           hipStream_t s);
+hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* latents, const void* packed, uint32_t seed,
+                              int step, int num_dirs, float* dL_dlatent, float* dL_dtheta, float* loss, float* ws,
+                              hipStream_t s);
+hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos, const int* radii,
+                               const uint16_t* latents, const void* packed, const float* dL_dcolor, float* dL_dmean3D,
+                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s);
+struct AdamGroupDev {
+  float* param;
+  const float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  uint16_t* param_bf16;  // optional BF16 copy of the updated parameter
+  float step_size;       // lr / (1 - beta1^t)
+  float bc2_sqrt;        // sqrt(1 - beta2^t)
+  float eps;
+};
+constexpr int kAdamMaxGroups = 8;
+struct AdamGroupTable {
+  AdamGroupDev g[kAdamMaxGroups];
+  size_t offset[kAdamMaxGroups + 1];  // element offsets; offset[n] = total
+  int n;
+  float one_minus_beta1, one_minus_beta2;
+};
+hipError_t launch_adam(const AdamGroupTable& tab, float beta1, float beta2, hipStream_t s);
+hipError_t launch_to_bf16(const float* in, uint16_t* out, size_t n, hipStream_t s);
 
 // loss.hip
 // Floats of workspace the fused loss needs for a [C,H,W] image with `skip`

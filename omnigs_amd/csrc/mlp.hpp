@@ -1,0 +1,252 @@
+// Learned colour decoder: per-Gaussian 16-dim latent plus the 9 degree-2 SH
+// basis values of the view direction -> 64 -> 64 -> 3 (ReLU, ReLU, sigmoid),
+// BF16 operands on the gfx950 MFMA unit (v_mfma_f32_16x16x32_bf16) with FP32
+// accumulation.  It stands in for the degree-3 SH evaluation of the
+// reference's computeColorFromSH (cuda_rasterizer/forward.cu:30-83) through
+// the colour slot of the render record; it has no counterpart in the reference.
+//
+// Everything runs in transposed form, H = W X, with one matrix column per
+// Gaussian, so that a wave evaluates 64 Gaussians as four 16-column blocks.
+// For the 16x16x32 MFMA, lane l = c + 16 q (c < 16, q < 4) holds
+//   A[row c][k = 8q + e]  and  B[k = 8q + e][col c]  (e < 8, BF16),
+//   D[row 4q + e][col c]                               (e < 4, FP32).
+// A D tile therefore feeds the next layer's B operand without moving data if
+// the K index of that layer is permuted: B slot (q, e) of K block kb holds
+// hidden unit pi(kb, q, e) = 32 kb + 4 q + e (e < 4) or 32 kb + 16 + 4 q + e - 4
+// (e >= 4), the units lane (c, q) holds in D tiles 2 kb and 2 kb + 1.  The
+// packed weight image stores every A operand in lane order with that
+// permutation applied (mlp.hip, pack kernel).
+#pragma once
+
+#include "common.hpp"
+
+namespace omnigs {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLatent = 16, kHidden = 64, kMlpIn = 25, kMlpOut = 3;
+
+// FP32 master parameter vector (theta), row-major weights [out][in]:
+constexpr int kThW1 = 0;                            // W1 [64][25]
+constexpr int kThB1 = kThW1 + kHidden * kMlpIn;     // b1 [64]
+constexpr int kThW2 = kThB1 + kHidden;              // W2 [64][64]
+constexpr int kThB2 = kThW2 + kHidden * kHidden;    // b2 [64]
+constexpr int kThW3 = kThB2 + kHidden;              // W3 [3][64]
+constexpr int kThB3 = kThW3 + kMlpOut * kHidden;    // b3 [3]
+constexpr int kThetaSize = kThB3 + kMlpOut;         // 6019
+constexpr int kThetaStride = 6144;                  // per-wave partial row (padded)
+
+// Packed image (bytes).  Forward operands first so a forward-only kernel
+// stages only kMlpFwdBytes into LDS.
+constexpr int kPkW1A = 0;                   // [4 rt][64 lanes] bf16x8
+constexpr int kPkW2A = kPkW1A + 4 * 64 * 16;  // [4 rt][2 kb][64] bf16x8
+constexpr int kPkW3A = kPkW2A + 8 * 64 * 16;  // [2 kb][64] bf16x8
+constexpr int kPkB1 = kPkW3A + 2 * 64 * 16;   // f32 [64]
+constexpr int kPkB2 = kPkB1 + 64 * 4;         // f32 [64]
+constexpr int kPkB3 = kPkB2 + 64 * 4;         // f32 [4] (b3, 0)
+constexpr int kMlpFwdBytes = kPkB3 + 16;      // 14864
+constexpr int kPkW3T = 15104;                 // [4 rt][64] bf16x8: W3^T (K = 3 outputs)
+constexpr int kPkW2T = kPkW3T + 4 * 64 * 16;  // [4 rt][2 kb][64] bf16x8: W2^T, K permuted
+constexpr int kPkW1T = kPkW2T + 8 * 64 * 16;  // [2 rt][2 kb][64] bf16x8: W1^T (rows = inputs)
+constexpr int kMlpPackedBytes = kPkW1T + 4 * 64 * 16;  // 31488
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 relu4(f32x4 v) {
+  f32x4 r;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) r[e] = fmaxf(v[e], 0.f);
+  return r;
+}
+
+__device__ __forceinline__ bf16x8 pack_bf16(f32x4 lo, f32x4 hi) {
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    r[e] = (__bf16)lo[e];
+    r[e + 4] = (__bf16)hi[e];
+  }
+  return r;
+}
+
+__device__ __forceinline__ bf16x8 zero_bf16x8() {
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) r[e] = (__bf16)0.f;
+  return r;
+}
+
+// The 9 degree-2 real SH basis values the decoder sees (same constants and
+// signs as the colour path's basis, auxiliary.h:32-49).
+__device__ __forceinline__ void sh2_basis(float x, float y, float z, float b[9]) {
+  b[0] = 0.28209479177387814f;
+  b[1] = -0.4886025119029199f * y;
+  b[2] = 0.4886025119029199f * z;
+  b[3] = -0.4886025119029199f * x;
+  b[4] = 1.0925484305920792f * x * y;
+  b[5] = -1.0925484305920792f * y * z;
+  b[6] = 0.31539156525252005f * (2.f * z * z - x * x - y * y);
+  b[7] = -1.0925484305920792f * x * z;
+  b[8] = 0.5462742152960396f * (x * x - y * y);
+}
+
+// d(sum_k g_k b_k)/d(x, y, z) for the basis above.
+__device__ __forceinline__ float3 sh2_basis_vjp(float x, float y, float z, const float g[9]) {
+  const float c1 = 0.4886025119029199f, c20 = 1.0925484305920792f, c22 = 0.31539156525252005f,
+              c24 = 0.5462742152960396f;
+  float3 r;
+  r.x = -c1 * g[3] + c20 * y * g[4] - c20 * z * g[7] - 2.f * c22 * x * g[6] + 2.f * c24 * x * g[8];
+  r.y = -c1 * g[1] + c20 * x * g[4] - c20 * z * g[5] - 2.f * c22 * y * g[6] - 2.f * c24 * y * g[8];
+  r.z = c1 * g[2] - c20 * y * g[5] - c20 * x * g[7] + 4.f * c22 * z * g[6];
+  return r;
+}
+
+// B operand of layer 1 for lane (c, q): inputs 8q..8q+7 of its Gaussian:
+// q = 0, 1: latent; q = 2: basis 0..7; q = 3: basis 8, then zeros.
+__device__ __forceinline__ bf16x8 input_operand(int q, const bf16x8& latent_half, float x, float y, float z) {
+  if (q < 2) return latent_half;
+  float b[9];
+  sh2_basis(x, y, z, b);
+  bf16x8 r = zero_bf16x8();
+  if (q == 2) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)b[e];
+  } else {
+    r[0] = (__bf16)b[8];
+  }
+  return r;
+}
+
+// Activations of one 16-Gaussian column block.
+struct MlpAct {
+  f32x4 h1[4];  // post-ReLU layer 1, D tiles rt = 0..3
+  f32x4 h2[4];  // post-ReLU layer 2
+  f32x4 o;      // output logits: rows 0..2 valid in lanes q = 0
+};
+
+// Forward of one column block.  w: packed image (LDS or global), lane: 0..63.
+__device__ __forceinline__ void mlp_forward_block(const char* w, int lane, const bf16x8& xin, MlpAct& a) {
+  const bf16x8* W1A = reinterpret_cast<const bf16x8*>(w + kPkW1A);
+  const bf16x8* W2A = reinterpret_cast<const bf16x8*>(w + kPkW2A);
+  const bf16x8* W3A = reinterpret_cast<const bf16x8*>(w + kPkW3A);
+  const f32x4* B1 = reinterpret_cast<const f32x4*>(w + kPkB1);
+  const f32x4* B2 = reinterpret_cast<const f32x4*>(w + kPkB2);
+  const f32x4 b3 = *reinterpret_cast<const f32x4*>(w + kPkB3);
+  const int q = lane >> 4;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) a.h1[rt] = relu4(mfma16(W1A[rt * 64 + lane], xin, B1[rt * 4 + q]));
+  const bf16x8 x2[2] = {pack_bf16(a.h1[0], a.h1[1]), pack_bf16(a.h1[2], a.h1[3])};
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f32x4 acc = B2[rt * 4 + q];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) acc = mfma16(W2A[(rt * 2 + kb) * 64 + lane], x2[kb], acc);
+    a.h2[rt] = relu4(acc);
+  }
+  const bf16x8 x3[2] = {pack_bf16(a.h2[0], a.h2[1]), pack_bf16(a.h2[2], a.h2[3])};
+  f32x4 acc = q == 0 ? b3 : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) acc = mfma16(W3A[kb * 64 + lane], x3[kb], acc);
+  a.o = acc;
+}
+
+// Backward of one column block from dO (rows 0..2 in lanes q = 0, zero
+// elsewhere): hidden-unit adjoints and the input adjoint dx (D tiles: dx[0]
+// rows = latent dims, dx[1] rows 16 + 4q + e = basis index 4q + e).
+struct MlpGrad {
+  f32x4 dh1[4], dh2[4];
+  f32x4 dx[2];
+};
+
+__device__ __forceinline__ void mlp_backward_block(const char* w, int lane, const MlpAct& a, f32x4 dO, MlpGrad& g) {
+  const bf16x8* W3T = reinterpret_cast<const bf16x8*>(w + kPkW3T);
+  const bf16x8* W2T = reinterpret_cast<const bf16x8*>(w + kPkW2T);
+  const bf16x8* W1T = reinterpret_cast<const bf16x8*>(w + kPkW1T);
+  const int q = lane >> 4;
+  const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+  const bf16x8 bo = q == 0 ? pack_bf16(f32x4{dO[0], dO[1], dO[2], 0.f}, z4) : zero_bf16x8();
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const f32x4 t = mfma16(W3T[rt * 64 + lane], bo, z4);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) g.dh2[rt][e] = a.h2[rt][e] > 0.f ? t[e] : 0.f;
+  }
+  const bf16x8 b2[2] = {pack_bf16(g.dh2[0], g.dh2[1]), pack_bf16(g.dh2[2], g.dh2[3])};
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f32x4 t = z4;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) t = mfma16(W2T[(rt * 2 + kb) * 64 + lane], b2[kb], t);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) g.dh1[rt][e] = a.h1[rt][e] > 0.f ? t[e] : 0.f;
+  }
+  const bf16x8 b1[2] = {pack_bf16(g.dh1[0], g.dh1[1]), pack_bf16(g.dh1[2], g.dh1[3])};
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt) {
+    f32x4 t = z4;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) t = mfma16(W1T[(rt * 2 + kb) * 64 + lane], b1[kb], t);
+    g.dx[rt] = t;
+  }
+}
+
+// Seeded direction k of Gaussian g for distillation: uniform on the sphere
+// from a counter-based integer hash (same integers on host and device).
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+__host__ __device__ __forceinline__ float3 seeded_direction(uint32_t seed, uint32_t g, uint32_t k) {
+  const uint32_t h1 = mix32(seed ^ mix32(g * 0x9E3779B9U + k * 0x85EBCA6BU + 0x27d4eb2fU));
+  const uint32_t h2 = mix32(h1 ^ 0x165667b1U);
+  const float u = (float)(h1 >> 8) * (1.0f / 16777216.0f), v = (float)(h2 >> 8) * (1.0f / 16777216.0f);
+  const float z = 1.f - 2.f * u;
+  const float s = sqrtf(fmaxf(0.f, 1.f - z * z));
+  const float ph = 6.283185307179586f * v;
+  return make_float3(s * cosf(ph), s * sinf(ph), z);
+}
+
+// Direction index a distillation step gives Gaussian g.
+__host__ __device__ __forceinline__ uint32_t distill_dir_index(uint32_t seed, int step, uint32_t g, int num_dirs) {
+  return mix32(seed * 0x632BE5ABU + (uint32_t)step * 0x9E3779B9U + g) % (uint32_t)num_dirs;
+}
+
+__device__ __forceinline__ float sigmoidf(float v) { return 1.f / (1.f + __expf(-v)); }
+
+// Latent inputs 8h..8h+7 of Gaussian G (BF16 [P,16]); zeros when !ok.
+__device__ __forceinline__ bf16x8 load_latent_half(const uint16_t* __restrict__ lat, size_t G, int q, bool ok) {
+  if (!ok) return zero_bf16x8();
+  return *reinterpret_cast<const bf16x8*>(lat + G * kLatent + 8 * q);
+}
+
+// Decoder colours of the wave's 64 Gaussians (wave_base + lane), evaluated
+// inside the forward preprocess.  dir: this lane's own view direction.  All
+// 64 lanes must call it.  w: forward part of the packed image in LDS.
+__device__ __forceinline__ float3 mlp_colour_wave(const char* w, const uint16_t* __restrict__ latents, int P, size_t wave_base,
+                                  float3 dir) {
+  const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
+  float3 mine = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const size_t G = wave_base + 16 * cb + c;
+    const bool ok = G < (size_t)P;
+    const float x = __shfl(dir.x, 16 * cb + c), y = __shfl(dir.y, 16 * cb + c), z = __shfl(dir.z, 16 * cb + c);
+    const bf16x8 xin = input_operand(q, load_latent_half(latents, G, q & 1, ok && q < 2), x, y, z);
+    MlpAct a;
+    mlp_forward_block(w, lane, xin, a);
+    const float o0 = __shfl(a.o[0], c), o1 = __shfl(a.o[1], c), o2 = __shfl(a.o[2], c);
+    if (q == cb) mine = make_float3(o0, o1, o2);
+  }
+  return make_float3(sigmoidf(mine.x), sigmoidf(mine.y), sigmoidf(mine.z));
+}
+
+
+}  // namespace omnigs

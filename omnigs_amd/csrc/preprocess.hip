@@ -8,26 +8,33 @@
 // written here is the float bit pattern of the camera distance |t|.
 //
 // HBM-bound: ~236 B read (means, scales, rotations, opacity, 48 SH floats)
-// and ~45 B written per Gaussian.
+// and ~45 B written per Gaussian.  With the colour decoder the 192 B of SH
+// become a 32 B BF16 latent and the colour comes from a wave-wide MFMA
+// evaluation (mlp.hpp).
 #include "kernels.hpp"
 #include "lonlat_math.hpp"
+#include "mlp.hpp"
 
 namespace omnigs {
 
-__global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomState g, int* __restrict__ radii_out,
-                                                         TileGrid grid) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.P) return;
+// Projected geometry of one Gaussian.
+struct GeomOut {
+  float3 p;  // world mean
+  float r;   // camera distance |t|
+  float2 pix;
+  float4 conic;
+  int radius, area;
+};
 
-  radii_out[idx] = 0;
-  g.tiles_touched[idx] = 0;
-  g.depth_key[idx] = 0xFFFFFFFFu;  // culled Gaussians sort last and emit nothing
-
+// Geometry part of preprocessLonlatCUDA; false when the Gaussian is culled
+// (too close, singular 2D covariance or empty tile rect).
+__device__ __forceinline__ bool preprocess_geometry(const PreprocessArgs& a, int idx, TileGrid grid, GeomOut& o) {
   const View v = load_view(a.viewmatrix);
   const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+  o.p = p;
   const float3 t = to_camera(v, p);
   const float rr = t.x * t.x + t.y * t.y + t.z * t.z;
-  if (rr <= 0.04f) return;
+  if (rr <= 0.04f) return false;
   const float r = sqrtf(rr);
   const float inv_r = 1.0f / (r + 1e-7f);
   const float lon = atan2f(t.x, t.z);
@@ -52,56 +59,94 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
   const float cyy = quad_form(S, A.ay, A.ay) + 0.3f;
 
   const float det = cxx * cyy - cxy * cxy;
-  if (det == 0.0f) return;
+  if (det == 0.0f) return false;
   const float det_inv = 1.f / det;
-  const float4 conic = make_float4(cyy * det_inv, -cxy * det_inv, cxx * det_inv, a.opacities[idx]);
+  o.conic = make_float4(cyy * det_inv, -cxy * det_inv, cxx * det_inv, a.opacities[idx]);
 
   const float mid = 0.5f * (cxx + cyy);
   const float disc = sqrtf(fmaxf(0.1f, mid * mid - det));
   const float lambda = fmaxf(mid + disc, mid - disc);
-  const int radius = (int)ceilf(3.f * sqrtf(lambda));
+  o.radius = (int)ceilf(3.f * sqrtf(lambda));
 
   // ndc -> pixel in FP64, rounded to float (reference ndc2Pix evaluates in double).
   const float px = (float)((((double)sx + 1.0) * (double)a.W - 1.0) * 0.5);
   const float py = (float)((((double)sy + 1.0) * (double)a.H - 1.0) * 0.5);
-  const float2 pix = make_float2(px, py);
-  const Rect rc = tile_rect(pix, radius, grid);
-  const int area = (rc.x1 - rc.x0) * (rc.y1 - rc.y0);
-  if (area == 0) return;
+  o.pix = make_float2(px, py);
+  const Rect rc = tile_rect(o.pix, o.radius, grid);
+  o.area = (rc.x1 - rc.x0) * (rc.y1 - rc.y0);
+  o.r = r;
+  return o.area != 0;
+}
+
+// kMlp: colours from the decoder (every lane of a wave that holds at least one
+// Gaussian runs the wave-wide MFMA evaluation, culled or not).
+template <bool kMlp>
+__global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomState g, int* __restrict__ radii_out,
+                                                         TileGrid grid) {
+  __shared__ __attribute__((aligned(16))) char s_w[kMlp ? kMlpFwdBytes : 16];
+  if constexpr (kMlp) {
+    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_packed);
+    for (int i = threadIdx.x; i < kMlpFwdBytes / 16; i += blockDim.x) reinterpret_cast<uint4*>(s_w)[i] = src[i];
+    __syncthreads();
+  }
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((kMlp ? (idx & ~63) : idx) >= a.P) return;
+  const bool in_range = idx < a.P;
+
+  if (in_range) {
+    radii_out[idx] = 0;
+    g.tiles_touched[idx] = 0;
+    g.depth_key[idx] = 0xFFFFFFFFu;  // culled Gaussians sort last and emit nothing
+  }
+  GeomOut o;
+  const bool live = in_range && preprocess_geometry(a, idx, grid, o);
 
   float3 col;
-  if (a.colors_precomp) {
-    col = make_float3(a.colors_precomp[3 * idx], a.colors_precomp[3 * idx + 1], a.colors_precomp[3 * idx + 2]);
-  } else {
-    const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
-    float3 d = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
-    const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
-    d = make_float3(d.x / len, d.y / len, d.z / len);
-    float b[16];
-    sh_basis(a.D, d.x, d.y, d.z, b);
-    const int nc = min(a.M, sh_coeffs(a.D));
-    float3 c[16];
-    load_sh(a.shs, a.M, idx, nc, c);
-    float3 rgb = make_float3(0.f, 0.f, 0.f);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      if (i < nc) {
-        rgb.x += b[i] * c[i].x;
-        rgb.y += b[i] * c[i].y;
-        rgb.z += b[i] * c[i].z;
-      }
+  if constexpr (kMlp) {
+    float3 dir = make_float3(0.f, 0.f, 1.f);
+    if (in_range) {
+      const float3 d = make_float3(o.p.x - a.campos[0], o.p.y - a.campos[1], o.p.z - a.campos[2]);
+      const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+      if (len > 0.f) dir = make_float3(d.x / len, d.y / len, d.z / len);
     }
-    rgb.x += 0.5f;
-    rgb.y += 0.5f;
-    rgb.z += 0.5f;
-    g.clamped[idx] = (uint8_t)((rgb.x < 0.f) | ((rgb.y < 0.f) << 1) | ((rgb.z < 0.f) << 2));
-    col = make_float3(fmaxf(rgb.x, 0.f), fmaxf(rgb.y, 0.f), fmaxf(rgb.z, 0.f));
+    col = mlp_colour_wave(s_w, a.latents, a.P, (size_t)(idx & ~63), dir);
+    if (!live) return;
+    g.clamped[idx] = 0;
+  } else {
+    if (!live) return;
+    if (a.colors_precomp) {
+      col = make_float3(a.colors_precomp[3 * idx], a.colors_precomp[3 * idx + 1], a.colors_precomp[3 * idx + 2]);
+    } else {
+      const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
+      float3 d = make_float3(o.p.x - campos.x, o.p.y - campos.y, o.p.z - campos.z);
+      const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+      d = make_float3(d.x / len, d.y / len, d.z / len);
+      float b[16];
+      sh_basis(a.D, d.x, d.y, d.z, b);
+      const int nc = min(a.M, sh_coeffs(a.D));
+      float3 c[16];
+      load_sh(a.shs, a.M, idx, nc, c);
+      float3 rgb = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        if (i < nc) {
+          rgb.x += b[i] * c[i].x;
+          rgb.y += b[i] * c[i].y;
+          rgb.z += b[i] * c[i].z;
+        }
+      }
+      rgb.x += 0.5f;
+      rgb.y += 0.5f;
+      rgb.z += 0.5f;
+      g.clamped[idx] = (uint8_t)((rgb.x < 0.f) | ((rgb.y < 0.f) << 1) | ((rgb.z < 0.f) << 2));
+      col = make_float3(fmaxf(rgb.x, 0.f), fmaxf(rgb.y, 0.f), fmaxf(rgb.z, 0.f));
+    }
   }
 
-  g.depth_key[idx] = __float_as_uint(r);
-  radii_out[idx] = radius;
-  store_rec(g.rec, idx, pix, conic, col);
-  g.tiles_touched[idx] = (uint32_t)area;
+  g.depth_key[idx] = __float_as_uint(o.r);
+  radii_out[idx] = o.radius;
+  store_rec(g.rec, idx, o.pix, o.conic, col);
+  g.tiles_touched[idx] = (uint32_t)o.area;
 }
 
 __global__ void mark_visible_kernel(int P, uint8_t* __restrict__ present) {
@@ -112,7 +157,10 @@ __global__ void mark_visible_kernel(int P, uint8_t* __restrict__ present) {
 hipError_t launch_preprocess(const PreprocessArgs& a, const GeomState& g, int* radii, hipStream_t s) {
   if (a.P == 0) return hipSuccess;
   const TileGrid grid = tile_grid(a.W, a.H, a.seam_wrap);
-  hipLaunchKernelGGL(preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, g, radii, grid);
+  if (a.mlp_packed && a.latents)
+    hipLaunchKernelGGL(preprocess_kernel<true>, dim3((a.P + 255) / 256), dim3(256), 0, s, a, g, radii, grid);
+  else
+    hipLaunchKernelGGL(preprocess_kernel<false>, dim3((a.P + 255) / 256), dim3(256), 0, s, a, g, radii, grid);
   return hipGetLastError();
 }
 

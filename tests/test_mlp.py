@@ -1,0 +1,275 @@
+"""Colour decoder (extension): oracle pins on the CPU, kernel parity on the GPU.
+
+CPU pins of oracle/mlp.py:
+  * BF16 rounding against torch's float -> bfloat16 conversion (nearest even);
+  * the seeded distillation directions against the library's host function,
+    unit length and uniformity on the sphere;
+  * the SH teacher colour against the C++ oracle's preprocess colours (which
+    restate computeColorFromSH);
+  * every backward formula against central finite differences of the FP64
+    function (BF16 rounding off).
+GPU parity (BF16 rounding on in the oracle; sums differ in order, so a hidden
+activation can land on the other side of a BF16 rounding boundary): decoder
+colours within 2e-3 absolute; distillation loss within 1e-3 relative and its
+gradients within 1e-2 of the reference norm; Adam within 1e-6 relative of
+torch.optim.Adam; the decoder path of the rasterizer against the same pieces
+composed by hand.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(0, 3, 20000), rng.uniform(-1e-3, 1e-3, 5000), [0.0, -0.0, 1.0, 65504.0, 1e-30],
+                        # exact ties: mantissa low half = 0x8000 with even / odd upper halves
+                        np.array([0x3F808000, 0x3F818000, 0xBF808000], dtype=np.uint32).view(np.float32)])
+    x = x.astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(om.bf16(x), ref)
+
+
+def test_seeded_directions_match_library_and_are_uniform():
+    from omnigs_amd import capi
+    capi.load()
+    g = np.arange(200)
+    k = (g * 7) % 64
+    d = om.seeded_directions(1234, g, k)
+    lib = np.array([capi.seeded_direction(1234, int(gi), int(ki)) for gi, ki in zip(g, k)])
+    np.testing.assert_allclose(d, lib, atol=2e-6)
+    D = om.seeded_directions(7, np.repeat(np.arange(4000), 16), np.tile(np.arange(16), 4000))
+    np.testing.assert_allclose(np.linalg.norm(D, axis=1), 1.0, atol=1e-6)
+    assert np.all(np.abs(D.mean(0)) < 0.01)                 # no preferred direction
+    np.testing.assert_allclose((D ** 2).mean(0), 1 / 3, atol=0.01)  # isotropic second moment
+    idx = om.distill_dir_index(3, 5, np.arange(100000), 64)
+    counts = np.bincount(idx.astype(np.int64), minlength=64)
+    assert counts.min() > 0.9 * 100000 / 64 and counts.max() < 1.1 * 100000 / 64
+
+
+def test_teacher_colour_matches_oracle_preprocess(oracle_lib):
+    from oracle import Oracle
+    from omnigs_amd import scenes
+    sc = scenes.small_scene(P=500, width=128, height=64, seed=3)
+    o = Oracle(sc, 64).run("preprocess")
+    rgb = o.get("rgb").reshape(-1, 3)
+    radii = o.get("radii")
+    v = sc["means3D"].astype(np.float64) - sc["campos"].astype(np.float64)
+    dirs = v / np.linalg.norm(v, axis=1, keepdims=True)
+    ref = om.sh3_colour(sc["shs"], dirs)
+    ok = radii > 0
+    np.testing.assert_allclose(rgb[ok], ref[ok], atol=1e-6)
+
+
+def _fd(f, x, i, h):
+    xp, xm = x.copy(), x.copy()
+    xp.flat[i] += h
+    xm.flat[i] -= h
+    return (f(xp) - f(xm)) / (2 * h)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    N = 12
+    theta = om.init_theta(2).astype(np.float64)
+    lat = rng.normal(0, 0.8, size=(N, 16))
+    v = rng.normal(size=(N, 3))
+    w = rng.normal(size=(N, 3))
+
+    def loss(th, la, vv):
+        d = vv / np.linalg.norm(vv, axis=1, keepdims=True)
+        rgb, _ = om.forward(th, la, d, bf16_ops=False)
+        return float(np.sum(w * rgb))
+
+    d = v / np.linalg.norm(v, axis=1, keepdims=True)
+    _, cache = om.forward(theta, lat, d, bf16_ops=False)
+    dlat, dbasis, dth = om.backward(cache, w)
+    dmean = om.direction_to_mean_vjp(v, om.sh2_basis_vjp(d, dbasis))
+    h = 1e-6
+    for i in rng.choice(theta.size, 60, replace=False):
+        fd = _fd(lambda t: loss(t, lat, v), theta, i, h)
+        assert abs(fd - dth[i]) <= 1e-6 + 1e-5 * abs(fd), ("theta", i, fd, dth[i])
+    for i in rng.choice(lat.size, 30, replace=False):
+        fd = _fd(lambda t: loss(theta, t, v), lat, i, h)
+        assert abs(fd - dlat.flat[i]) <= 1e-6 + 1e-5 * abs(fd), ("latent", i, fd, dlat.flat[i])
+    for i in range(v.size):
+        fd = _fd(lambda t: loss(theta, lat, t), v, i, h)
+        assert abs(fd - dmean.flat[i]) <= 1e-6 + 1e-5 * abs(fd), ("mean", i, fd, dmean.flat[i])
+
+
+def test_distill_step_gradient_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    P = 40
+    from omnigs_amd import scenes
+    shs = scenes.small_scene(P=P, width=64, height=32, seed=4)["shs"]
+    theta = om.init_theta(5).astype(np.float64)
+    lat = rng.normal(0, 0.5, size=(P, 16))
+    g = np.arange(P)
+    dirs = om.seeded_directions(9, g, om.distill_dir_index(9, 2, g, 64)).astype(np.float64)
+    target = om.sh3_colour(shs, dirs)
+
+    def loss(th, la):
+        rgb, _ = om.forward(th, la, dirs, bf16_ops=False)
+        return float(np.mean((rgb - target) ** 2))
+
+    rgb, cache = om.forward(theta, lat, dirs, bf16_ops=False)
+    dlat, _, dth = om.backward(cache, 2.0 * (rgb - target) / (3.0 * P))
+    for i in rng.choice(theta.size, 40, replace=False):
+        fd = _fd(lambda t: loss(t, lat), theta, i, 1e-6)
+        assert abs(fd - dth[i]) <= 1e-8 + 1e-5 * abs(fd)
+    for i in rng.choice(lat.size, 20, replace=False):
+        fd = _fd(lambda t: loss(theta, t), lat, i, 1e-6)
+        assert abs(fd - dlat.flat[i]) <= 1e-8 + 1e-5 * abs(fd)
+
+
+def test_capi_decoder_entries_without_gpu():
+    from omnigs_amd import capi
+    lib = capi.load()
+    assert lib.omnigs_mlp_param_count() == om.PARAM_COUNT == 6019
+    assert lib.omnigs_mlp_packed_bytes() > 0 and lib.omnigs_mlp_packed_bytes() % 16 == 0
+    assert lib.omnigs_mlp_workspace_bytes(1_500_000) < 64 << 20
+    fake = ctypes.c_void_p(16)
+    assert lib.omnigs_mlp_pack(None, fake, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    assert lib.omnigs_mlp_eval(-1, fake, fake, fake, fake, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    a = capi.DistillArgs(10, 8, 16, 16, 16, 0, 0, 64, 16, 16, 16, 16, 1 << 30)
+    assert lib.omnigs_mlp_distill_step(ctypes.byref(a), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT  # M < 16
+    grp = (capi.AdamGroup * 1)(capi.AdamGroup(16, 16, 16, 16, None, 4, 1e-3, 1e-15, 0))
+    assert lib.omnigs_adam_step(grp, 1, 0.9, 0.999, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT  # step < 1
+    assert lib.omnigs_adam_step(grp, 9, 0.9, 0.999, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT  # > 8 groups
+
+
+# ----------------------------------------------------------------------------- GPU
+
+def _bf16_bits(x: np.ndarray) -> np.ndarray:
+    return om.bf16(x).view(np.uint32).__rshift__(16).astype(np.uint16).view(np.int16)
+
+
+@pytest.mark.gpu
+def test_gpu_decoder_eval_matches_oracle(gpu):
+    import torch
+    from omnigs_amd import capi
+    from tests.gpu_util import announce
+    announce("decoder eval vs oracle, N=5000")
+    rng = np.random.default_rng(11)
+    N = 5000
+    theta = om.init_theta(3)
+    lat = rng.normal(0, 0.7, size=(N, 16)).astype(np.float32)
+    d = rng.normal(size=(N, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    packed = capi.mlp_pack(torch.from_numpy(theta).to(gpu))
+    rgb = capi.mlp_eval(torch.from_numpy(_bf16_bits(lat)).to(gpu), torch.from_numpy(d).to(gpu), packed)
+    torch.cuda.synchronize()
+    ref, _ = om.forward(theta, lat, d)
+    err = np.abs(rgb.cpu().numpy() - ref)
+    assert err.max() < 2e-3 and err.mean() < 1e-4, (err.max(), err.mean())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("P", [3000, 64])
+def test_gpu_distill_step_matches_oracle(gpu, P):
+    import torch
+    from omnigs_amd import capi, scenes
+    from tests.gpu_util import announce
+    announce(f"decoder distill step vs oracle, P={P}")
+    rng = np.random.default_rng(P)
+    shs = scenes.small_scene(P=P, width=64, height=32, seed=5)["shs"].astype(np.float32)
+    theta = om.init_theta(6)
+    lat = rng.normal(0, 0.5, size=(P, 16)).astype(np.float32)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)  # noqa: E731
+    packed = capi.mlp_pack(dev(theta))
+    dlat = torch.empty((P, 16), dtype=torch.float32, device=gpu)
+    dth = torch.empty((om.PARAM_COUNT,), dtype=torch.float32, device=gpu)
+    loss = torch.empty((1,), dtype=torch.float32, device=gpu)
+    ws = capi.mlp_workspace(P, gpu)
+    for step in (0, 7):
+        capi.mlp_distill_step(dev(shs), dev(lat), packed, 99, step, 64, dlat, dth, loss, ws)
+        torch.cuda.synchronize()
+        rl, rdlat, rdth = om.distill_step(theta, lat, shs, 99, step, 64)
+        assert abs(float(loss) - rl) <= 1e-3 * rl, (float(loss), rl)
+        for name, got, ref in (("latent", dlat.cpu().numpy(), rdlat), ("theta", dth.cpu().numpy(), rdth)):
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            assert rel < 1e-2, (name, rel)
+
+
+@pytest.mark.gpu
+def test_gpu_adam_matches_torch(gpu):
+    import torch
+    from omnigs_amd import capi
+    from tests.gpu_util import announce
+    announce("fused Adam vs torch.optim.Adam")
+    g = torch.Generator().manual_seed(0)
+    shapes = [(1000,), (37, 16)]
+    ps = [torch.randn(s, generator=g) for s in shapes]
+    mine = [p.clone().to(gpu) for p in ps]
+    ref = [torch.nn.Parameter(p.clone().double()) for p in ps]
+    opt = torch.optim.Adam([{"params": [ref[0]], "lr": 1e-3}, {"params": [ref[1]], "lr": 5e-3}], eps=1e-15)
+    st = [(torch.zeros_like(m), torch.zeros_like(m)) for m in mine]
+    bf = torch.empty(shapes[1], dtype=torch.int16, device=gpu)
+    for step in range(1, 5):
+        grads = [torch.randn(s, generator=g) for s in shapes]
+        for r, gr in zip(ref, grads):
+            r.grad = gr.double()
+        opt.step()
+        capi.adam_step([
+            dict(param=mine[0], grad=grads[0].to(gpu), exp_avg=st[0][0], exp_avg_sq=st[0][1], lr=1e-3, eps=1e-15,
+                 step=step),
+            dict(param=mine[1], grad=grads[1].to(gpu), exp_avg=st[1][0], exp_avg_sq=st[1][1], lr=5e-3, eps=1e-15,
+                 step=step, param_bf16=bf)])
+        torch.cuda.synchronize()
+        for m, r in zip(mine, ref):
+            np.testing.assert_allclose(m.cpu().numpy(), r.detach().numpy(), rtol=1e-6, atol=1e-7)
+    np.testing.assert_array_equal(bf.cpu().numpy(), _bf16_bits(mine[1].cpu().numpy()))
+
+
+@pytest.mark.gpu
+def test_gpu_decoder_render_path_matches_composition(gpu):
+    """The rasterizer's decoder mode equals decoder colours fed through
+    colors_precomp (forward), and its backward equals the precomputed-colour
+    backward plus the oracle decoder backward of the same dL/dcolors."""
+    import torch
+    from omnigs_amd import capi, scenes
+    from tests.gpu_util import announce, np_, to_dev
+    announce("decoder render path vs composition, 20k @ 512x256")
+    sc = scenes.small_scene(P=20000, seed=2)
+    ds = to_dev(sc)
+    P = 20000
+    rng = np.random.default_rng(2)
+    theta = om.init_theta(8)
+    lat = rng.normal(0, 0.7, size=(P, 16)).astype(np.float32)
+    lat_bits = torch.from_numpy(_bf16_bits(lat)).to(gpu)
+    packed = capi.mlp_pack(torch.from_numpy(theta).to(gpu))
+    v = ds["means3D"] - ds["campos"]
+    dirs = (v / v.norm(dim=1, keepdim=True)).contiguous()
+    cols = capi.mlp_eval(lat_bits, dirs, packed)
+    common = dict(background=ds["background"], means3D=ds["means3D"], opacities=ds["opacities"].reshape(-1),
+                  viewmatrix=ds["viewmatrix"], campos=ds["campos"], width=512, height=256, scales=ds["scales"],
+                  rotations=ds["rotations"])
+    R1, img1, radii1, b1 = capi.forward(mlp=(packed, lat_bits), **common)
+    R2, img2, radii2, b2 = capi.forward(colors_precomp=cols, **common)
+    torch.cuda.synchronize()
+    assert R1 == R2 and torch.equal(radii1, radii2)
+    assert float((img1 - img2).abs().max()) < 1e-5
+    dL = torch.from_numpy(sc["dL_dpix"]).to(gpu)
+    bc = dict(background=ds["background"], means3D=ds["means3D"], viewmatrix=ds["viewmatrix"], campos=ds["campos"],
+              dL_dpix=dL, scales=ds["scales"], rotations=ds["rotations"])
+    g1 = capi.backward(R=R1, radii=radii1, buffers=b1, mlp=(packed, lat_bits), **bc)
+    g2 = capi.backward(R=R2, radii=radii2, buffers=b2, colors_precomp=cols, **bc)
+    torch.cuda.synchronize()
+    dc = np_(g2["dL_dcolors"])
+    np.testing.assert_allclose(np_(g1["dL_dcolors"]), dc, rtol=1e-5, atol=1e-9)
+    d = np_(dirs)
+    _, cache = om.forward(theta, lat, d)
+    rdlat, rdbasis, rdth = om.backward(cache, dc)
+    vis = np_(radii1) > 0
+    rdmean = np.where(vis[:, None], om.direction_to_mean_vjp(np_(v).astype(np.float64),
+                                                             om.sh2_basis_vjp(d.astype(np.float64), rdbasis)), 0.0)
+    for name, got, ref in (("dL_dlatent", np_(g1["dL_dlatent"]), rdlat), ("dL_dmlp", np_(g1["dL_dmlp"]), rdth),
+                           ("direction term", np_(g1["dL_dmeans3D"]) - np_(g2["dL_dmeans3D"]), rdmean)):
+        rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert rel < 1e-2, (name, rel)
