@@ -59,6 +59,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--gaussians", type=int, default=1_500_000)
+    ap.add_argument("--no-decoder", action="store_true", help="skip the learned-colour leg")
+    ap.add_argument("--distill-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
     args = ap.parse_args()
@@ -93,33 +95,15 @@ def main() -> int:
         g = capi.backward_scene(ds, R, radii, bufs, dL)
         return R, radii, g
 
-    for _ in range(args.warmup):
-        R, radii, _ = step()
-    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    median_ms, ms_per_step, stages, (R, radii, g) = time_steps(step, args.warmup, args.steps, dev)
+    peak_hbm = torch.cuda.max_memory_allocated(dev)
     Pv = int((radii > 0).sum())
-
-    capi.profile_read()  # drop warm-up events
-    capi.profile_enable(True)
-    stream = torch.cuda.current_stream(dev)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        evs[i][0].record(stream)
-        R, radii, g = step()
-        evs[i][1].record(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    capi.profile_enable(False)
-    stages = capi.profile_read()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    step_ms = sorted(a.elapsed_time(b) for a, b in evs)
-    median_ms = statistics.median(step_ms)
 
     depth_passes = 4
     tile_passes = (int(tiles).bit_length() + 7) // 8
     alg = algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes)
-    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items()}
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items() if v[1]}
     dominant = max(stage_ms, key=lambda k: stage_ms[k])
     dom_ms = stage_ms[dominant]
     achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
@@ -145,7 +129,7 @@ def main() -> int:
         "data": "synthetic (seeded room scene; random-init Gaussians)",
         "config": {"workload": "plain SH fwd+bwd, 1.5M Gaussians, 1920x960 equirectangular, tile 16x16",
                    "gaussians": P, "width": W, "height": H, "num_rendered": R, "visible": Pv,
-                   "sh_degree": 3},
+                   "sh_degree": 3, "peak_hbm_bytes": peak_hbm},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": alg[dominant], "avg_launch_ms": round(dom_ms, 5)},
@@ -155,10 +139,133 @@ def main() -> int:
         "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
     }
 
+    if not args.no_decoder:
+        out["mlp_colour"] = decoder_leg(sc, ds, bufs, target, l1, dL, args, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out))
     return 0
+
+
+def time_steps(step, warmup: int, steps: int, dev):
+    """W untimed steps, then K timed ones bracketed by synchronisation, each
+    also bracketed by events on the launch stream; stage timers on.  Returns
+    (median event ms, wall ms per step, stage totals, last step's result)."""
+    import torch
+
+    from omnigs_amd import capi
+    res = None
+    for _ in range(warmup):
+        res = step()
+    torch.cuda.synchronize()
+    capi.profile_read()  # drop warm-up events
+    capi.profile_enable(True)
+    stream = torch.cuda.current_stream(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        evs[i][0].record(stream)
+        res = step()
+        evs[i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    capi.profile_enable(False)
+    stages = capi.profile_read()
+    return (statistics.median(a.elapsed_time(b) for a, b in evs), (t1 - t0) * 1000.0 / steps, stages, res)
+
+
+# PSNR of the reference on 360Roam (supp.pdf p.5, Table 3): the quality level
+# at which the decoder's extra error is priced as a PSNR loss.
+REFERENCE_PSNR_DB = 25.464
+
+
+def random_rotation(rng):
+    import numpy as np
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
+    """The learned-colour form: distils the decoder from the scene's own SH
+    colours (2000 passes, 64 seeded directions per Gaussian, Adam with network
+    lr 1e-3 and latent lr 1e-2), judges it against the SH render on 16
+    held-out random poses, then times fwd+bwd with decoder colours and the
+    decoder's backward (latent and weight gradients) on the timed workload."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from omnigs_amd import capi, scenes
+    from omnigs_amd.decoder import ColourDecoder
+
+    P, W, H = sc["means3D"].shape[0], sc["width"], sc["height"]
+    shs = ds["shs"].contiguous()
+    dec = ColourDecoder(P, dev, seed=0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    losses = dec.distill(shs, steps=args.distill_steps, lr=1e-3, num_dirs=64, seed=1)
+    e1.record()
+    torch.cuda.synchronize()
+    distill_ms = e0.elapsed_time(e1)
+    L = losses.cpu().numpy()
+
+    rng = np.random.default_rng(2024)
+    mses, maxerr = [], 0.0
+    for _ in range(16):
+        R_ = random_rotation(rng)
+        c = rng.uniform(-0.5, 0.5, size=3)
+        vm = torch.from_numpy(scenes.viewmatrix_from_pose(R_, -R_ @ c).astype(np.float32)).to(dev)
+        cp = torch.from_numpy(c.astype(np.float32)).to(dev)
+        common = dict(background=ds["background"], means3D=ds["means3D"], opacities=ds["opacities"],
+                      viewmatrix=vm, campos=cp, width=W, height=H, scales=ds["scales"], rotations=ds["rotations"])
+        _, ish, _, _ = capi.forward(shs=shs, **common)
+        _, imlp, _, _ = capi.forward(mlp=dec.mlp_args(), **common)
+        d = imlp - ish
+        mses.append(float((d * d).mean()))
+        maxerr = max(maxerr, float(d.abs().max()))
+    mse = float(np.mean(mses))
+    psnr_vs_sh = 10 * math.log10(1.0 / mse)
+    mse_ref = 10 ** (-REFERENCE_PSNR_DB / 10)
+    loss_db = 10 * math.log10(1.0 + mse / mse_ref)
+
+    mlp = dec.mlp_args()
+    ws = capi.mlp_workspace(P, dev)
+
+    def step():
+        R, img, radii, _ = capi.forward_scene(ds, buffers=bufs, mlp=mlp)
+        l1(img, target, dL_dimage=dL)
+        g = capi.backward_scene(ds, R, radii, bufs, dL, mlp=mlp, mlp_workspace=ws)
+        return R, radii, g
+
+    median_ms, ms_per_step, stages, _ = time_steps(step, args.warmup, args.steps, dev)
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items() if v[1]}
+    # training pass FLOPs per Gaussian: forward, backward chain and weight gradients, 5888 MACs each
+    flop_per_pass = 3 * 2 * 5888 * P
+    dec_ms = stage_ms.get("decoder_backward", 0.0)
+    return {
+        "ms_per_iter": round(median_ms, 4),
+        "ms_per_step_wall": round(ms_per_step, 4),
+        "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "distill": {"passes": args.distill_steps, "ms_total": round(distill_ms, 1),
+                    "ms_per_pass": round(distill_ms / max(1, args.distill_steps), 4),
+                    "loss_first": float(L[:10].mean()), "loss_last": float(L[-50:].mean())},
+        "heldout_poses": 16,
+        "psnr_vs_sh_db": round(psnr_vs_sh, 3),
+        "max_channel_err": round(maxerr, 4),
+        "psnr_loss_db_at_reference_quality": round(loss_db, 4),
+        "gate_db": 0.1,
+        "gate_passed": bool(loss_db <= 0.1),
+        "colour_path_used": "decoder" if loss_db <= 0.1 else "SH (fallback: gate failed)",
+        "decoder_backward_mfma": {"flop": flop_per_pass, "ms": round(dec_ms, 5),
+                                  "TFLOPs": round(flop_per_pass / (dec_ms * 1e-3) / 1e12, 2) if dec_ms else None,
+                                  "peak_TFLOPs_bf16": 2500.0},
+    }
 
 
 def copy_calibration(dev, steps: int) -> int:

@@ -211,6 +211,8 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
     lib = load()
     dev = means3D.device
     P = means3D.shape[0]
+    if mlp:
+        shs = colors_precomp = None  # colours come from the decoder
     M = shs.shape[1] if (shs is not None and shs.numel()) else 0
     # every pixel and every radius is written when P > 0; P == 0 leaves zeros (reference behaviour)
     alloc = torch.empty if P > 0 else torch.zeros
@@ -237,6 +239,8 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
     lib = load()
     dev = means3D.device
     P = means3D.shape[0]
+    if mlp:
+        shs = colors_precomp = None  # colours come from the decoder
     H, W = dL_dpix.shape[1], dL_dpix.shape[2]
     M = shs.shape[1] if (shs is not None and shs.numel()) else 0
     f = dict(dtype=torch.float32, device=dev)

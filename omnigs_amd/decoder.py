@@ -84,9 +84,11 @@ class ColourDecoder:
         capi.mlp_pack(self.theta, self.packed)
 
     def distill(self, shs: torch.Tensor, steps: int = 2000, lr: float = 1e-3, num_dirs: int = 64, seed: int = 1,
-                lr_latent: float | None = None, log_every: int = 0):
+                lr_latent: float | None = 1e-2, log_every: int = 0):
         """Distills the decoder from SH colours (shs [P,16,3]); returns the
-        per-step losses as a tensor (kept on the device until the end)."""
+        per-step losses as a tensor (kept on the device until the end).
+        lr is the network's Adam rate; the latents default to 1e-2 because
+        each one sees a single sample per pass."""
         assert shs.shape[0] == self.P and shs.dtype == torch.float32
         shs = shs.contiguous()
         st = self._adam_state()

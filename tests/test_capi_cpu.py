@@ -91,3 +91,36 @@ def test_state_view_offsets_are_inside_buffers(lib):
         assert base_b <= getattr(sv, f) < base_b + bsz, f
     for f in ("final_T", "n_contrib", "ranges"):
         assert base_i <= getattr(sv, f) < base_i + isz, f
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """The ctypes mirrors in omnigs_amd.capi have the C header's sizes and
+    field offsets (compiled with the host C compiler)."""
+    import shutil
+    import subprocess
+
+    from omnigs_amd import capi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    structs = {"omnigs_lonlat_forward_args": capi.ForwardArgs, "omnigs_lonlat_backward_args": capi.BackwardArgs,
+               "omnigs_mlp_distill_args": capi.DistillArgs, "omnigs_adam_group": capi.AdamGroup,
+               "omnigs_lonlat_state_view": capi.StateView}
+    lines = []
+    for cname, ct in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in ct._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    src = tmp_path / "abi.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"omnigs_lonlat.h\"\nint main(void) {\n"
+                   + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "abi"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        name, field, val = line.split()
+        got[(name, field)] = int(val)
+    for cname, ct in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(ct), cname
+        for fname, _ in ct._fields_:
+            assert got[(cname, fname)] == getattr(ct, fname).offset, (cname, fname)
