@@ -60,6 +60,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--gaussians", type=int, default=1_500_000)
     ap.add_argument("--no-decoder", action="store_true", help="skip the learned-colour leg")
+    ap.add_argument("--memory-compare", action="store_true",
+                    help="only the checkpointed-vs-stored backward comparison (3M Gaussians at 4096x2048)")
     ap.add_argument("--distill-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
@@ -78,6 +80,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     if args.copy_calibration:
         return copy_calibration(dev, args.steps)
+    if args.memory_compare:
+        return memory_compare(dev, args)
     sc = scenes.room_scene(P=args.gaussians)
     P, W, H = sc["means3D"].shape[0], sc["width"], sc["height"]
     N = W * H
@@ -142,6 +146,7 @@ def main() -> int:
     if not args.no_decoder:
         out["mlp_colour"] = decoder_leg(sc, ds, bufs, target, l1, dL, args, dev)
     if not args.no_cpu:
+        target = sc["target"]
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out))
     return 0
@@ -268,6 +273,65 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
     }
 
 
+def memory_compare(dev, args) -> int:
+    """Checkpointed vs stored backward through the LibTorch autograd Function
+    (caller contract renderLonlat with activations), 3M Gaussians at
+    4096x2048: ms per fwd+bwd iteration, peak HBM over the iteration and the
+    bytes the render holds between its forward and its backward."""
+    import numpy as np
+    import torch
+
+    from omnigs_amd import rasterizer, scenes
+
+    P = args.gaussians if args.gaussians != 1_500_000 else 3_000_000
+    W, H = 4096, 2048
+    sc = scenes.room_scene(P=P, width=W, height=H)
+    rng = np.random.default_rng(5)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)  # noqa: E731
+    params = {
+        "xyz": t(sc["means3D"]).requires_grad_(),
+        "features_dc": t(sc["shs"][:, :1, :]).requires_grad_(),
+        "features_rest": t(sc["shs"][:, 1:, :]).requires_grad_(),
+        "scaling": t(np.log(sc["scales"])).requires_grad_(),
+        "rotation": t(sc["rotations"] * rng.uniform(0.5, 2.0, size=(P, 1))).requires_grad_(),
+        "opacity": t(np.log(sc["opacities"] / (1.0 - sc["opacities"]))).reshape(P, 1).requires_grad_(),
+    }
+    vm, cp, bg, target = t(sc["viewmatrix"]), t(sc["campos"]), t(sc["background"]), t(sc["target"])
+    res = {}
+    for mode in ("stored", "checkpointed"):
+        ck = mode == "checkpointed"
+
+        def it():
+            for p in params.values():
+                p.grad = None
+            base = torch.cuda.memory_allocated(dev)
+            img, _, _, _ = rasterizer.render_lonlat(params, vm, cp, H, W, bg, checkpoint=ck)
+            held = torch.cuda.memory_allocated(dev) - base
+            (img - target).abs().mean().backward()
+            return held
+
+        for _ in range(2):
+            it()
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        e = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        held = 0
+        for i in range(args.steps):
+            e[i][0].record()
+            held = it()
+            e[i][1].record()
+        torch.cuda.synchronize()
+        res[mode] = {"ms_per_iter": round(statistics.median(a.elapsed_time(b) for a, b in e), 3),
+                     "peak_hbm_bytes_over_params": torch.cuda.max_memory_allocated(dev) - base,
+                     "held_fwd_to_bwd_bytes": held}
+    print(json.dumps({"metric": "checkpointed vs stored backward, fwd+bwd ms/iter and HBM",
+                      "config": {"workload": "autograd renderLonlat (activations included) + L1",
+                                 "gaussians": P, "width": W, "height": H},
+                      "steps": args.steps, **res}))
+    return 0
+
+
 def copy_calibration(dev, steps: int) -> int:
     """A plain device-to-device copy of 1 GiB: 1 GiB read + 1 GiB written per
     launch, the yardstick that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
@@ -284,22 +348,35 @@ def copy_calibration(dev, steps: int) -> int:
 
 
 def cpu_baseline(sc) -> dict:
-    """The FP32 oracle on one host thread over one full fwd+bwd iteration of
-    the same workload (bounded: ~30 s on a 2.7 GHz Xeon)."""
+    """The FP32 oracle timed on the host over one full fwd+bwd iteration of the
+    same workload: the OpenMP tile-parallel variant on the host's cores (the
+    reported baseline) and the single-thread line-for-line port beside it
+    (bounded: about 15 s + 2 s)."""
     import numpy as np
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import Oracle
 
-    o = Oracle({k: v for k, v in sc.items() if k != "target"}, 32)
-    t = time.perf_counter()
-    o.forward()
-    img = o.get("out_color").reshape(3, sc["height"], sc["width"])
-    dL = (np.sign(img - sc["target"]) / (3.0 * sc["height"] * sc["width"])).astype(np.float32)
-    o.backward(dL)
-    dt = time.perf_counter() - t
-    return {"value": round(dt * 1000.0, 1), "unit": "ms/iter", "cores": 1, "kind": "port",
-            "sample": "one full fwd+bwd iteration (1.5M Gaussians, 1920x960) of the FP32 oracle, single thread"}
+    target = sc["target"]
+    sc = {k: v for k, v in sc.items() if k != "target"}
+    want = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, len(os.sched_getaffinity(0)))
+
+    def one(threads):
+        o = Oracle(sc, 32)
+        used = o.set_threads(threads)
+        t = time.perf_counter()
+        o.forward()
+        img = o.get("out_color").reshape(3, sc["height"], sc["width"])
+        dL = (np.sign(img - target) / (3.0 * sc["height"] * sc["width"])).astype(np.float32)
+        o.backward(dL)
+        return time.perf_counter() - t, used
+
+    dt_par, used = one(want)
+    dt_1, _ = one(1)
+    return {"value": round(dt_par * 1000.0, 1), "unit": "ms/iter", "cores": used, "kind": "port",
+            "sample": f"one full fwd+bwd iteration (1.5M Gaussians, 1920x960) of the FP32 oracle, OpenMP "
+                      f"tile-parallel with deterministic reductions on {used} host threads",
+            "single_thread": {"value": round(dt_1 * 1000.0, 1), "unit": "ms/iter", "cores": 1}}
 
 
 if __name__ == "__main__":

@@ -169,6 +169,68 @@ int omnigs_photometric_loss(const float* image, const float* gt_image, const flo
                             int width, int skip_bottom, float lambda_dssim, void* workspace, size_t workspace_bytes,
                             float* dL_dimage, float* loss_out, void* stream);
 
+/* Batched multi-view forward / backward (extension; the reference renders one
+ * view per RasterizeGaussiansCUDA call, rasterize_points.cu:49-164).  V <= 32
+ * equirectangular views of one Gaussian set, each with its own size and pose,
+ * in one launch per stage: preprocess reads each Gaussian once for all views,
+ * binning runs over (view, Gaussian) items with view-major tile ids, the
+ * render kernels cover every tile of every view, and the backward sums each
+ * Gaussian's gradients over the views in registers.  Per-view images match
+ * V single-view forwards.  Tile shape 16x16 (the reference's), 32x8 or 8x32.
+ * No host synchronisation and no allocation happen inside (the instance
+ * count stays on the device, bounded by the workspace's capacity), so a
+ * caller may capture forward + backward into one HIP graph.  status (device,
+ * 2 uints) receives {instances needed, 1 if that exceeded capacity}; after
+ * an overflow the images are incomplete and the caller re-runs with a larger
+ * capacity.  Pixel arrays (out_color, dL_dpix) hold view v's [3, H_v, W_v]
+ * block at float offset 3 * sum_{u<v} W_u H_u. */
+#define OMNIGS_MAX_VIEWS 32
+typedef struct omnigs_lonlat_batch_args {
+  int P, D, M, V;
+  int tile_x, tile_y;
+  int seam_wrap;
+  const int* widths;           /* host [V] */
+  const int* heights;          /* host [V] */
+  const float* viewmatrices;   /* device [V,16] */
+  const float* campos;         /* device [V,3] */
+  const float* background;     /* device [3] */
+  const float* means3D;        /* [P,3] */
+  const float* shs;            /* [P,M,3] or NULL */
+  const float* colors_precomp; /* [P,3] or NULL */
+  const float* opacities;      /* [P] */
+  const float* scales;         /* [P,3] or NULL */
+  float scale_modifier;
+  const float* rotations;      /* [P,4] or NULL */
+  const float* cov3D_precomp;  /* [P,6] or NULL */
+  const void* mlp_packed;      /* colour decoder (with latents) or NULL */
+  const uint16_t* latents;
+  unsigned int capacity;       /* instance capacity of the workspace, all views together */
+  void* workspace;             /* omnigs_lonlat_batch_workspace_bytes() bytes, kept between forward and backward */
+  size_t workspace_bytes;
+  float* out_color;            /* device, all views */
+  unsigned int* status;        /* device [2] */
+} omnigs_lonlat_batch_args;
+
+typedef struct omnigs_lonlat_batch_grads {
+  const float* dL_dpix;        /* device, layout of out_color */
+  float* dL_dmean3D;           /* [P,3] */
+  float* dL_dopacity;          /* [P] */
+  float* dL_dcov3D;            /* [P,6] */
+  float* dL_dsh;               /* [P,M,3] when shs is given */
+  float* dL_dscale;            /* [P,3] when scales is given */
+  float* dL_drot;              /* [P,4] when scales is given */
+  float* dL_dlatent;           /* [P,16] with the decoder */
+  float* dL_dmlp;              /* [omnigs_mlp_param_count()] with the decoder */
+  void* mlp_workspace;         /* omnigs_mlp_workspace_bytes(P) with the decoder */
+  size_t mlp_workspace_bytes;
+} omnigs_lonlat_batch_grads;
+
+size_t omnigs_lonlat_batch_workspace_bytes(int P, int V, const int* widths, const int* heights, int tile_x,
+                                           int tile_y, unsigned int capacity);
+int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* args, void* stream);
+int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* args, const omnigs_lonlat_batch_grads* grads,
+                                 void* stream);
+
 /* Colour decoder (extension; no counterpart in the reference, whose colours
  * come from computeColorFromSH, cuda_rasterizer/forward.cu:30-83).
  * rgb = sigmoid(W3 relu(W2 relu(W1 [latent(16), Y2(dir)(9)] + b1) + b2) + b3)

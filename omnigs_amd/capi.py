@@ -28,7 +28,8 @@ EXPORTED_SYMBOLS = (
     "omnigs_status_string", "omnigs_last_error", "omnigs_version", "omnigs_loss_workspace_bytes",
     "omnigs_photometric_loss", "omnigs_mlp_param_count", "omnigs_mlp_packed_bytes", "omnigs_mlp_pack",
     "omnigs_mlp_eval", "omnigs_mlp_workspace_bytes", "omnigs_mlp_distill_step", "omnigs_mlp_seeded_direction",
-    "omnigs_adam_step", "omnigs_f32_to_bf16",
+    "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
+    "omnigs_lonlat_batch_backward",
 )
 NUM_STAGES = 11
 
@@ -73,6 +74,29 @@ class BackwardArgs(ctypes.Structure):
         ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
         ("mlp_packed", ctypes.c_void_p), ("latents", ctypes.c_void_p), ("dL_dlatent", ctypes.c_void_p),
         ("dL_dmlp", ctypes.c_void_p), ("mlp_workspace", ctypes.c_void_p), ("mlp_workspace_bytes", ctypes.c_size_t),
+    ]
+
+
+class BatchArgs(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int), ("D", ctypes.c_int), ("M", ctypes.c_int), ("V", ctypes.c_int),
+        ("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("seam_wrap", ctypes.c_int),
+        ("widths", ctypes.c_void_p), ("heights", ctypes.c_void_p), ("viewmatrices", ctypes.c_void_p),
+        ("campos", ctypes.c_void_p), ("background", ctypes.c_void_p), ("means3D", ctypes.c_void_p),
+        ("shs", ctypes.c_void_p), ("colors_precomp", ctypes.c_void_p), ("opacities", ctypes.c_void_p),
+        ("scales", ctypes.c_void_p), ("scale_modifier", ctypes.c_float), ("rotations", ctypes.c_void_p),
+        ("cov3D_precomp", ctypes.c_void_p), ("mlp_packed", ctypes.c_void_p), ("latents", ctypes.c_void_p),
+        ("capacity", ctypes.c_uint), ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+        ("out_color", ctypes.c_void_p), ("status", ctypes.c_void_p),
+    ]
+
+
+class BatchGrads(ctypes.Structure):
+    _fields_ = [
+        ("dL_dpix", ctypes.c_void_p), ("dL_dmean3D", ctypes.c_void_p), ("dL_dopacity", ctypes.c_void_p),
+        ("dL_dcov3D", ctypes.c_void_p), ("dL_dsh", ctypes.c_void_p), ("dL_dscale", ctypes.c_void_p),
+        ("dL_drot", ctypes.c_void_p), ("dL_dlatent", ctypes.c_void_p), ("dL_dmlp", ctypes.c_void_p),
+        ("mlp_workspace", ctypes.c_void_p), ("mlp_workspace_bytes", ctypes.c_size_t),
     ]
 
 
@@ -159,6 +183,14 @@ def load() -> ctypes.CDLL:
     lib.omnigs_adam_step.restype = ctypes.c_int
     lib.omnigs_f32_to_bf16.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     lib.omnigs_f32_to_bf16.restype = ctypes.c_int
+    lib.omnigs_lonlat_batch_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint]
+    lib.omnigs_lonlat_batch_workspace_bytes.restype = ctypes.c_size_t
+    lib.omnigs_lonlat_batch_forward.argtypes = [ctypes.POINTER(BatchArgs), ctypes.c_void_p]
+    lib.omnigs_lonlat_batch_forward.restype = ctypes.c_int
+    lib.omnigs_lonlat_batch_backward.argtypes = [ctypes.POINTER(BatchArgs), ctypes.POINTER(BatchGrads),
+                                                 ctypes.c_void_p]
+    lib.omnigs_lonlat_batch_backward.restype = ctypes.c_int
     _lib = lib
     return lib
 

@@ -367,6 +367,215 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   return OMNIGS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched multi-view path.
+namespace {
+
+struct BatchState {
+  GeomState g;  // per item (V P)
+  BinState b;   // per instance (capacity)
+  float* final_T;
+  uint32_t* n_contrib;
+  uint2* ranges;
+  float* dmean2D;  // per item [3]
+  float* dconic;   // per item [4]
+  float* dopacity; // per item
+  float* dcolor;   // per item [3]
+  size_t bytes;
+};
+
+BatchState carve_batch(char* base, size_t items, size_t pixels, size_t tiles, size_t capacity) {
+  BatchState st;
+  size_t off = 0;
+  auto at = [&](size_t n) { return base ? base + off : nullptr; };
+  st.g = carve_geom(at(0), items);
+  off += st.g.bytes;
+  st.b = carve_bin(at(0), capacity);
+  off += st.b.bytes;
+  Carver<char> c(at(0));
+  st.final_T = c.take<float>(pixels);
+  st.n_contrib = c.take<uint32_t>(pixels);
+  st.ranges = c.take<uint2>(tiles);
+  st.dmean2D = c.take<float>(3 * items);
+  st.dconic = c.take<float>(4 * items);
+  st.dopacity = c.take<float>(items);
+  st.dcolor = c.take<float>(3 * items);
+  off += align_up(c.off) + kAlign;
+  st.bytes = off;
+  return st;
+}
+
+int make_view_table(int P, int V, const int* widths, const int* heights, int bx, int by, int wrap, ViewTable& vt) {
+  if (V < 1 || V > kMaxViews) return fail_arg("V outside [1, 32]");
+  if (!widths || !heights) return fail_arg("widths / heights missing");
+  if (!tile_shape_supported(bx, by)) return fail_arg("tile shape must be 16x16, 32x8 or 8x32");
+  vt = ViewTable{};
+  vt.V = V;
+  vt.P = P;
+  vt.bx = bx;
+  vt.by = by;
+  vt.wrap = wrap ? 1 : 0;
+  uint64_t tiles = 0, pix = 0;
+  for (int v = 0; v < V; ++v) {
+    if (widths[v] <= 0 || heights[v] <= 0) return fail_arg("empty view");
+    const TileGrid g = tile_grid(widths[v], heights[v], wrap, bx, by);
+    vt.W[v] = widths[v];
+    vt.H[v] = heights[v];
+    vt.gx[v] = g.gx;
+    vt.gy[v] = g.gy;
+    vt.tile_base[v] = (uint32_t)tiles;
+    vt.pix_base[v] = (uint32_t)pix;
+    tiles += (uint64_t)g.tiles();
+    pix += (uint64_t)widths[v] * heights[v];
+  }
+  if (tiles > 0x7fffffffULL || pix > 0xffffffffULL) return fail_arg("batch too large");
+  if ((uint64_t)V * (uint64_t)P > 0x7fffffffULL) return fail_arg("V * P exceeds int range");
+  vt.tile_base[V] = (uint32_t)tiles;
+  vt.pix_base[V] = (uint32_t)pix;
+  return OMNIGS_OK;
+}
+
+}  // namespace
+
+size_t omnigs_lonlat_batch_workspace_bytes(int P, int V, const int* widths, const int* heights, int tile_x,
+                                           int tile_y, unsigned int capacity) {
+  ViewTable vt;
+  if (P < 0 || make_view_table(P, V, widths, heights, tile_x, tile_y, 0, vt) != OMNIGS_OK) return 0;
+  return carve_batch(nullptr, (size_t)V * P, vt.pix_base[V], vt.tile_base[V], capacity).bytes;
+}
+
+int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_) {
+  if (!a) return fail_arg("NULL args");
+  if (a->P < 0) return fail_arg("P < 0");
+  ViewTable vt;
+  const int st = make_view_table(a->P, a->V, a->widths, a->heights, a->tile_x, a->tile_y, a->seam_wrap, vt);
+  if (st != OMNIGS_OK) return st;
+  const int P = a->P;
+  const bool mlp = a->mlp_packed && a->latents;
+  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
+  if (!a->means3D || !a->opacities || !a->viewmatrices || !a->campos || !a->background || !a->out_color ||
+      !a->status || !a->workspace)
+    return fail_arg("missing required pointer");
+  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  if (!mlp && !a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
+    return fail_arg("SH degree / coefficient count");
+  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
+  if (a->capacity == 0) return fail_arg("capacity 0");
+  const size_t items = (size_t)a->V * P;
+  const BatchState bs = carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V],
+                                    vt.tile_base[vt.V], a->capacity);
+  if (a->workspace_bytes < bs.bytes) return fail_arg("workspace smaller than omnigs_lonlat_batch_workspace_bytes()");
+  hipStream_t s = (hipStream_t)stream_;
+  if (P == 0) {
+    OMNIGS_TRY(launch_fill_u32(a->status, 2, 0u, s), "status");
+    return OMNIGS_OK;
+  }
+  const GeomState& g = bs.g;
+  PreprocessArgs pa{P, a->D, a->M, 0, 0, a->means3D, a->shs, a->colors_precomp, a->opacities, a->scales,
+                    a->scale_modifier, a->rotations, a->cov3D_precomp, nullptr, nullptr, a->seam_wrap ? 1 : 0,
+                    mlp ? a->mlp_packed : nullptr, mlp ? a->latents : nullptr, a->tile_x, a->tile_y};
+  {
+    StageTimer tm(OMNIGS_STAGE_PREPROCESS, s);
+    OMNIGS_TRY(launch_preprocess_views(pa, a->viewmatrices, a->campos, vt, g, g.radii, s), "batch preprocess");
+  }
+  SortResult ds;
+  {
+    StageTimer tm(OMNIGS_STAGE_DEPTH_SORT, s);
+    OMNIGS_TRY(launch_radix_sort(g.depth_key, g.key_alt, nullptr, g.order, g.order_alt, items, 32, g.sort_counts,
+                                 g.scan_partials, s, &ds),
+               "batch depth sort");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_SCAN, s);
+    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, items, false, g.scan_partials, s, g.key_alt),
+               "batch scan");
+  }
+  const uint32_t* R_dev = g.offsets + (items - 1);
+  const BinState& b = bs.b;
+  {
+    StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
+    OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rec, g.radii,
+                                      vt, b.tile_key, b.gid, s),
+               "batch duplicate");
+  }
+  SortResult ts;
+  {
+    StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
+    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, a->capacity,
+                                 bit_width_u32(vt.tile_base[vt.V]), b.sort_counts, b.scan_partials, s, &ts, R_dev),
+               "batch tile sort");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_RANGES, s);
+    OMNIGS_TRY(launch_tile_ranges(ts.keys, a->capacity, bs.ranges, (int)vt.tile_base[vt.V], s, R_dev),
+               "batch ranges");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_RENDER, s);
+    OMNIGS_TRY(launch_render_forward_views(vt, bs.ranges, ts.vals, g.rec, bs.final_T, bs.n_contrib, a->background,
+                                           a->out_color, s),
+               "batch render");
+  }
+  return OMNIGS_OK;
+}
+
+int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs_lonlat_batch_grads* gr,
+                                 void* stream_) {
+  if (!a || !gr) return fail_arg("NULL args");
+  ViewTable vt;
+  const int st = make_view_table(a->P, a->V, a->widths, a->heights, a->tile_x, a->tile_y, a->seam_wrap, vt);
+  if (st != OMNIGS_OK) return st;
+  const int P = a->P;
+  if (P == 0) return OMNIGS_OK;
+  const bool mlp = a->mlp_packed && a->latents;
+  if (!gr->dL_dpix || !gr->dL_dmean3D || !gr->dL_dopacity || !gr->dL_dcov3D || !a->workspace)
+    return fail_arg("missing gradient pointer");
+  if (!mlp && a->shs && !gr->dL_dsh) return fail_arg("dL_dsh is NULL while shs is given");
+  if (a->scales && (!gr->dL_dscale || !gr->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
+  if (mlp && (!gr->dL_dlatent || !gr->dL_dmlp || !gr->mlp_workspace ||
+              gr->mlp_workspace_bytes < omnigs_mlp_workspace_bytes(P)))
+    return fail_arg("decoder gradient outputs / workspace missing");
+  const size_t items = (size_t)a->V * P;
+  const BatchState bs = carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V],
+                                    vt.tile_base[vt.V], a->capacity);
+  if (a->workspace_bytes < bs.bytes) return fail_arg("workspace smaller than omnigs_lonlat_batch_workspace_bytes()");
+  hipStream_t s = (hipStream_t)stream_;
+  const GeomState& g = bs.g;
+  const BinState& b = bs.b;
+  const uint32_t* point_list = (radix_passes(bit_width_u32(vt.tile_base[vt.V])) % 2) ? b.gid_alt : b.gid;
+  {
+    StageTimer tm(OMNIGS_STAGE_CLEAR_GRADS, s);
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dmean2D), 3 * items, 0u, s), "clear");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dconic), 4 * items, 0u, s), "clear");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dopacity), items, 0u, s), "clear");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dcolor), 3 * items, 0u, s), "clear");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
+    OMNIGS_TRY(launch_render_backward_views(vt, bs.ranges, point_list, a->background, g.rec, bs.final_T,
+                                            bs.n_contrib, gr->dL_dpix, bs.dmean2D, bs.dconic, bs.dopacity, bs.dcolor,
+                                            s),
+               "batch render backward");
+  }
+  PreprocessBackwardArgs pb{P, a->D, a->M, 0, 0, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,
+                            a->rotations, a->cov3D_precomp, nullptr, nullptr, g.radii, g.clamped, bs.dmean2D,
+                            bs.dconic, bs.dcolor, gr->dL_dmean3D, gr->dL_dcov3D, mlp ? nullptr : gr->dL_dsh,
+                            gr->dL_dscale, gr->dL_drot, nullptr, nullptr};
+  {
+    StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
+    OMNIGS_TRY(launch_preprocess_backward_views(pb, a->viewmatrices, a->campos, vt, bs.dopacity, gr->dL_dopacity, s),
+               "batch preprocess backward");
+  }
+  if (mlp) {
+    StageTimer tm(OMNIGS_STAGE_DECODER_BACKWARD, s);
+    OMNIGS_TRY(launch_mlp_backward(P, a->means3D, a->campos, g.radii, a->latents, a->mlp_packed, bs.dcolor,
+                                   gr->dL_dmean3D, gr->dL_dlatent, gr->dL_dmlp, static_cast<float*>(gr->mlp_workspace),
+                                   s, a->V),
+               "batch decoder backward");
+  }
+  return OMNIGS_OK;
+}
+
 int omnigs_mlp_param_count(void) { return kThetaSize; }
 size_t omnigs_mlp_packed_bytes(void) { return kMlpPackedBytes; }
 size_t omnigs_mlp_workspace_bytes(int P) { return P < 0 ? 0 : mlp_workspace_floats(P) * sizeof(float); }

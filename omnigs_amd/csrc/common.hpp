@@ -36,12 +36,32 @@ constexpr float kTwoInvPi = 0.636619772367581343075535053490057448f;
 
 struct TileGrid {
   int gx, gy;
-  int wrap;  // 1: tile columns wrap around the longitude seam (extension; 0 = the reference's clamp)
+  int wrap;    // 1: tile columns wrap around the longitude seam (extension; 0 = the reference's clamp)
+  int bx, by;  // tile size in pixels (16 x 16 is the reference's; 32 x 8 and 8 x 32 are extensions)
   __host__ __device__ int tiles() const { return gx * gy; }
 };
-inline TileGrid tile_grid(int W, int H, int wrap = 0) {
-  return {(W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, wrap};
+inline TileGrid tile_grid(int W, int H, int wrap = 0, int bx = kTileX, int by = kTileY) {
+  return {(W + bx - 1) / bx, (H + by - 1) / by, wrap, bx, by};
 }
+
+// Views of a batched launch (an extension: the reference renders one view per
+// call).  Item i of the batch is Gaussian i % P in view i / P; global tile ids
+// and pixel offsets run view after view.  Passed to kernels by value.
+constexpr int kMaxViews = 32;
+struct ViewTable {
+  int V, P;
+  int bx, by, wrap;
+  int W[kMaxViews], H[kMaxViews], gx[kMaxViews], gy[kMaxViews];
+  uint32_t tile_base[kMaxViews + 1];  // first global tile of view v; [V] = total tiles
+  uint32_t pix_base[kMaxViews + 1];   // first pixel of view v; [V] = total pixels
+  __host__ __device__ TileGrid grid(int v) const { return {gx[v], gy[v], wrap, bx, by}; }
+  // view owning global tile t (V is small: a scan over constant kernel arguments)
+  __device__ int view_of_tile(uint32_t t) const {
+    int v = 0;
+    while (v + 1 < V && t >= tile_base[v + 1]) ++v;
+    return v;
+  }
+};
 
 // Tile rectangle [x0,x1) x [y0,y1) of a Gaussian at pixel (px,py) with integer
 // radius r.  The float evaluation order is the reference's getRect
@@ -63,16 +83,16 @@ __device__ __forceinline__ int rect_hi(float p, int r, int tile, int g) {
 // leave [0, gx) and is read modulo gx; a range wider than the ring is the ring.
 __device__ __forceinline__ Rect tile_rect(float2 p, int r, TileGrid g) {
   if (g.wrap) {
-    int x0 = (int)floorf((p.x - (float)r) / (float)kTileX);
-    int x1 = (int)floorf((((p.x + (float)r) + (float)kTileX) - 1.0f) / (float)kTileX);
+    int x0 = (int)floorf((p.x - (float)r) / (float)g.bx);
+    int x1 = (int)floorf((((p.x + (float)r) + (float)g.bx) - 1.0f) / (float)g.bx);
     if (x1 - x0 >= g.gx) {
       x0 = 0;
       x1 = g.gx;
     }
-    return {x0, rect_lo(p.y, r, kTileY, g.gy), x1, rect_hi(p.y, r, kTileY, g.gy)};
+    return {x0, rect_lo(p.y, r, g.by, g.gy), x1, rect_hi(p.y, r, g.by, g.gy)};
   }
-  return {rect_lo(p.x, r, kTileX, g.gx), rect_lo(p.y, r, kTileY, g.gy), rect_hi(p.x, r, kTileX, g.gx),
-          rect_hi(p.y, r, kTileY, g.gy)};
+  return {rect_lo(p.x, r, g.bx, g.gx), rect_lo(p.y, r, g.by, g.gy), rect_hi(p.x, r, g.bx, g.gx),
+          rect_hi(p.y, r, g.by, g.gy)};
 }
 // Tile column of rect column x (identity in clamp mode).
 __device__ __forceinline__ int wrap_col(int x, TileGrid g) { return g.wrap ? ((x % g.gx) + g.gx) % g.gx : x; }

@@ -21,6 +21,7 @@ struct PreprocessArgs {
   // colour decoder (colours from the decoder when both are set)
   const void* mlp_packed;
   const uint16_t* latents;  // BF16 [P,16]
+  int tile_x = kTileX, tile_y = kTileY;
 };
 
 struct PreprocessBackwardArgs {
@@ -50,6 +51,9 @@ struct PreprocessBackwardArgs {
 // preprocess.hip
 hipError_t launch_preprocess(const PreprocessArgs& a, const GeomState& g, int* radii, hipStream_t s);
 hipError_t launch_mark_visible(int P, uint8_t* present, hipStream_t s);
+// Batched: items v P + g of the V views in vt (GeomState arrays sized V P).
+hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmats, const float* campos,
+                                   const ViewTable& vt, const GeomState& g, int* radii, hipStream_t s);
 
 // sort.hip
 // Inclusive (or exclusive) scan of n u32 values, optionally gathered through
@@ -64,13 +68,20 @@ struct SortResult {
   uint32_t* keys;
   uint32_t* vals;
 };
+// n_dev (optional): the element count on the device, n then being the capacity.
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
-                             hipStream_t s, SortResult* out);
+                             hipStream_t s, SortResult* out, const uint32_t* n_dev = nullptr);
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
-                              hipStream_t s);
+                              hipStream_t s, const uint32_t* R_dev = nullptr);
+// Batched expansion: items = V P, capacity instances at most; R_dev is the
+// device-side instance count (last inclusive offset); status [2] receives
+// {count needed, 1 if it exceeds the capacity}.
+hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
+                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // render.hip
@@ -82,8 +93,24 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
                                   const float* dL_dpix, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
                                   float* dL_dcolors, hipStream_t s);
 
+// Batched render over every tile of every view of vt; pixel arrays are the
+// views' slices laid out one after the other (colours [3, H_v, W_v] per view).
+hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
+                                       const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
+                                       float* out_color, hipStream_t s);
+hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
+                                        const float* bg, const float4* rec, const float* final_T,
+                                        const uint32_t* n_contrib, const float* dL_dpix, float* dL_dmean2D,
+                                        float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
+bool tile_shape_supported(int bx, int by);
+
 // preprocess_backward.hip
 hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s);
+// Batched: the per-item arrays of a (radii, clamped, dL_dmean2D, dL_dconic,
+// dL_dcolor) are [V P, ...]; parameter gradients are summed over views.
+hipError_t launch_preprocess_backward_views(const PreprocessBackwardArgs& a, const float* viewmats,
+                                            const float* campos, const ViewTable& vt, const float* dopacity_items,
+                                            float* dL_dopacity, hipStream_t s);
 
 // mlp.hip
 constexpr int kMlpTrainBlocks = 512;  // persistent grid of the decoder training pass
@@ -98,7 +125,7 @@ hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* laten
                               hipStream_t s);
 hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos, const int* radii,
                                const uint16_t* latents, const void* packed, const float* dL_dcolor, float* dL_dmean3D,
-                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s);
+                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V = 1);
 struct AdamGroupDev {
   float* param;
   const float* grad;

@@ -142,8 +142,9 @@ struct MlpTrainArgs {
   const float* means3D;
   const float* campos;
   const int* radii;
-  const float* dL_dcolor;  // [P,3]
-  float* dL_dmean3D;       // [P,3], += direction term for radii > 0
+  const float* dL_dcolor;  // [V P, 3]
+  float* dL_dmean3D;       // [P,3], += direction term of every view with radius > 0
+  int V;                   // views (items v P + g); campos is [V,3]
   // distillation
   const float* shs;  // [P,M,3] teacher, degree 3
   int M;
@@ -184,141 +185,191 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpTrainArgs a) {
   // dW1 row tile w (2), db1 / db2 row tile w, dW3 column tile w, db3 (wave 0)
   f32x4 gW2[4] = {z4, z4, z4, z4}, gW1[2] = {z4, z4}, gB1 = z4, gB2 = z4, gW3 = z4, gB3 = z4;
   float sq_err = 0.f;
-  const float3 campos = kDistill ? make_float3(0.f, 0.f, 0.f) : make_float3(a.campos[0], a.campos[1], a.campos[2]);
   const float inv3P = 1.f / (3.f * (float)a.P);
 
-  const int nchunks = (a.P + 63) / 64;
-  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+  // Per-lane inputs of one chunk, loaded a chunk ahead so that the HBM
+  // latency overlaps the previous chunk's MFMA work.
+  struct ChunkIn {
+    bf16x8 lat;    // latent inputs 8q..8q+7 (q < 2)
+    float3 m;      // mean (render backward)
+    float3 dc;     // dL/dcolor (render backward, q = 0)
+    float3 dm;     // dL/dmean3D so far (render backward, q = 0)
+    int rad;       // radius (render backward, q = 0)
+    float sh[12];  // SH coefficients 4q..4q+3 (distillation)
+  };
+  auto load_chunk = [&](int chunk, ChunkIn& in) {
     const size_t G = (size_t)chunk * 64 + 16 * w + c;
     const bool ok = G < (size_t)a.P;
-    // direction (every q lane of Gaussian c holds it)
-    float3 d = make_float3(0.f, 0.f, 1.f), v = make_float3(0.f, 0.f, 0.f);
-    float len = 1.f;
-    uint32_t k = 0;
-    if (ok) {
-      if (kDistill) {
-        k = distill_dir_index(a.seed, a.step, (uint32_t)G, a.num_dirs);
-        d = seeded_direction(a.seed, (uint32_t)G, k);
-      } else {
-        v = make_float3(a.means3D[3 * G] - campos.x, a.means3D[3 * G + 1] - campos.y, a.means3D[3 * G + 2] - campos.z);
-        d = unit_dir(v, &len);
-      }
-    }
-    const bf16x8 xin = kDistill ? input_operand(q, load_latent_half(a.latent_f32, G, q & 1, ok && q < 2), d.x, d.y, d.z)
-                                : input_operand(q, load_latent_half(a.latent_bf16, G, q & 1, ok && q < 2), d.x, d.y, d.z);
-    MlpAct act;
-    mlp_forward_block(s_w, lane, xin, act);
-
-    // output adjoint (lanes q = 0, rows 0..2)
-    f32x4 dO = z4;
-    if (kDistill) {
-      // teacher: degree-3 SH colour + 0.5, clamped at 0; lane q adds coefficients 4q..4q+3
-      float b[16];
-      sh_basis(3, d.x, d.y, d.z, b);
-      float3 t = make_float3(0.f, 0.f, 0.f);
-      if (ok) {
-        const float* sh = a.shs + (G * a.M + 4 * q) * 3;
+    if constexpr (kDistill) {
+      in.lat = load_latent_half(a.latent_f32, G, q & 1, ok && q < 2);
+      const float* sh = a.shs + (G * a.M + 4 * q) * 3;
 #pragma unroll
+      for (int i = 0; i < 12; ++i) in.sh[i] = (ok && 4 * q + i / 3 < a.M) ? sh[i] : 0.f;
+    } else {
+      in.lat = load_latent_half(a.latent_bf16, G, q & 1, ok && q < 2);
+      in.m = ok ? make_float3(a.means3D[3 * G], a.means3D[3 * G + 1], a.means3D[3 * G + 2])
+                : make_float3(0.f, 0.f, 0.f);
+      const bool own = ok && q == 0;
+      in.dc = own ? make_float3(a.dL_dcolor[3 * G], a.dL_dcolor[3 * G + 1], a.dL_dcolor[3 * G + 2])
+                  : make_float3(0.f, 0.f, 0.f);
+      in.dm = own ? make_float3(a.dL_dmean3D[3 * G], a.dL_dmean3D[3 * G + 1], a.dL_dmean3D[3 * G + 2])
+                  : make_float3(0.f, 0.f, 0.f);
+      in.rad = own ? a.radii[G] : 0;
+    }
+  };
+
+  const int nchunks = (a.P + 63) / 64;
+  const int nviews = kDistill ? 1 : a.V;
+  ChunkIn cur{}, nxt{};
+  if ((int)blockIdx.x < nchunks) load_chunk(blockIdx.x, cur);
+  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    if (chunk + (int)gridDim.x < nchunks) load_chunk(chunk + gridDim.x, nxt);
+    const size_t G = (size_t)chunk * 64 + 16 * w + c;
+    const bool ok = G < (size_t)a.P;
+    float4 dlat = make_float4(0.f, 0.f, 0.f, 0.f);
+    float3 dmv = make_float3(0.f, 0.f, 0.f);
+    bool any_dir = false;
+    for (int vi = 0; vi < nviews; ++vi) {
+      // view vi: items vi P + G (one view in distillation)
+      const size_t item = (size_t)vi * a.P + G;
+      const float3 dcv3 = vi == 0 ? cur.dc
+                                  : ((ok && q == 0) ? make_float3(a.dL_dcolor[3 * item], a.dL_dcolor[3 * item + 1],
+                                                                  a.dL_dcolor[3 * item + 2])
+                                                    : make_float3(0.f, 0.f, 0.f));
+      const int rad = vi == 0 ? cur.rad : ((ok && q == 0) ? a.radii[item] : 0);
+      // direction (every q lane of Gaussian c holds it)
+      float3 d = make_float3(0.f, 0.f, 1.f), v = make_float3(0.f, 0.f, 0.f);
+      float len = 1.f;
+      if (ok) {
+        if (kDistill) {
+          d = seeded_direction(a.seed, (uint32_t)G, distill_dir_index(a.seed, a.step, (uint32_t)G, a.num_dirs));
+        } else {
+          const float3 cp = make_float3(a.campos[3 * vi], a.campos[3 * vi + 1], a.campos[3 * vi + 2]);
+          v = make_float3(cur.m.x - cp.x, cur.m.y - cp.y, cur.m.z - cp.z);
+          d = unit_dir(v, &len);
+        }
+      }
+      const bf16x8 xin = input_operand(q, cur.lat, d.x, d.y, d.z);
+      MlpAct act;
+      mlp_forward_block(s_w, lane, xin, act);
+
+      // output adjoint (lanes q = 0, rows 0..2)
+      f32x4 dO = z4;
+      if (kDistill) {
+        // teacher: degree-3 SH colour + 0.5, clamped at 0; lane q adds coefficients 4q..4q+3
+        float b[16];
+        sh_basis(3, d.x, d.y, d.z, b);
+        float3 t = make_float3(0.f, 0.f, 0.f);
+  #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          if (4 * q + i < a.M) {
-            t.x += b[4 * q + i] * sh[3 * i];
-            t.y += b[4 * q + i] * sh[3 * i + 1];
-            t.z += b[4 * q + i] * sh[3 * i + 2];
+          t.x += b[4 * q + i] * cur.sh[3 * i];
+          t.y += b[4 * q + i] * cur.sh[3 * i + 1];
+          t.z += b[4 * q + i] * cur.sh[3 * i + 2];
+        }
+        t.x += __shfl_xor(t.x, 16);
+        t.y += __shfl_xor(t.y, 16);
+        t.z += __shfl_xor(t.z, 16);
+        t.x += __shfl_xor(t.x, 32);
+        t.y += __shfl_xor(t.y, 32);
+        t.z += __shfl_xor(t.z, 32);
+        t = make_float3(fmaxf(t.x + 0.5f, 0.f), fmaxf(t.y + 0.5f, 0.f), fmaxf(t.z + 0.5f, 0.f));
+        if (ok && q == 0) {
+          const float tt[3] = {t.x, t.y, t.z};
+  #pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            const float s = sigmoidf(act.o[e]);
+            const float err = s - tt[e];
+            sq_err += err * err;
+            dO[e] = 2.f * err * inv3P * s * (1.f - s);
           }
         }
-      }
-      t.x += __shfl_xor(t.x, 16);
-      t.y += __shfl_xor(t.y, 16);
-      t.z += __shfl_xor(t.z, 16);
-      t.x += __shfl_xor(t.x, 32);
-      t.y += __shfl_xor(t.y, 32);
-      t.z += __shfl_xor(t.z, 32);
-      t = make_float3(fmaxf(t.x + 0.5f, 0.f), fmaxf(t.y + 0.5f, 0.f), fmaxf(t.z + 0.5f, 0.f));
-      if (ok && q == 0) {
-        const float tt[3] = {t.x, t.y, t.z};
-#pragma unroll
+      } else if (ok && q == 0) {
+        const float dcv[3] = {dcv3.x, dcv3.y, dcv3.z};
+  #pragma unroll
         for (int e = 0; e < 3; ++e) {
           const float s = sigmoidf(act.o[e]);
-          const float err = s - tt[e];
-          sq_err += err * err;
-          dO[e] = 2.f * err * inv3P * s * (1.f - s);
+          dO[e] = dcv[e] * s * (1.f - s);
         }
       }
-    } else if (ok && q == 0) {
-#pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        const float s = sigmoidf(act.o[e]);
-        dO[e] = a.dL_dcolor[3 * G + e] * s * (1.f - s);
-      }
-    }
-    MlpGrad gr;
-    mlp_backward_block(s_w, lane, act, dO, gr);
+      MlpGrad gr;
+      mlp_backward_block(s_w, lane, act, dO, gr);
 
-    if (ok) *reinterpret_cast<float4*>(a.dL_dlatent + G * kLatent + 4 * q) =
-        make_float4(gr.dx[0][0], gr.dx[0][1], gr.dx[0][2], gr.dx[0][3]);
-    if (!kDistill) {
-      // basis adjoint 4q..4q+3 held here -> partial direction gradient, summed over q
-      float gb[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) gb[i] = 0.f;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (4 * q + e < 9) gb[4 * q + e] = gr.dx[1][e];
-      float3 gd = sh2_basis_vjp(d.x, d.y, d.z, gb);
-      gd.x += __shfl_xor(gd.x, 16);
-      gd.y += __shfl_xor(gd.y, 16);
-      gd.z += __shfl_xor(gd.z, 16);
-      gd.x += __shfl_xor(gd.x, 32);
-      gd.y += __shfl_xor(gd.y, 32);
-      gd.z += __shfl_xor(gd.z, 32);
-      if (ok && q == 0 && a.radii[G] > 0 && len > 0.f) {
-        // d(v/|v|)/dv applied to gd (dnormvdv, auxiliary.h:134-144)
-        const float dot = d.x * gd.x + d.y * gd.y + d.z * gd.z;
-        const float il = 1.f / len;
-        a.dL_dmean3D[3 * G] += (gd.x - d.x * dot) * il;
-        a.dL_dmean3D[3 * G + 1] += (gd.y - d.y * dot) * il;
-        a.dL_dmean3D[3 * G + 2] += (gd.z - d.z * dot) * il;
+      dlat.x += gr.dx[0][0];
+      dlat.y += gr.dx[0][1];
+      dlat.z += gr.dx[0][2];
+      dlat.w += gr.dx[0][3];
+      if (!kDistill) {
+        // basis adjoint 4q..4q+3 held here -> partial direction gradient, summed over q
+        float gb[9];
+  #pragma unroll
+        for (int i = 0; i < 9; ++i) gb[i] = 0.f;
+  #pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (4 * q + e < 9) gb[4 * q + e] = gr.dx[1][e];
+        float3 gd = sh2_basis_vjp(d.x, d.y, d.z, gb);
+        gd.x += __shfl_xor(gd.x, 16);
+        gd.y += __shfl_xor(gd.y, 16);
+        gd.z += __shfl_xor(gd.z, 16);
+        gd.x += __shfl_xor(gd.x, 32);
+        gd.y += __shfl_xor(gd.y, 32);
+        gd.z += __shfl_xor(gd.z, 32);
+        if (ok && q == 0 && rad > 0 && len > 0.f) {
+          // d(v/|v|)/dv applied to gd (dnormvdv, auxiliary.h:134-144)
+          const float dot = d.x * gd.x + d.y * gd.y + d.z * gd.z;
+          const float il = 1.f / len;
+          dmv.x += (gd.x - d.x * dot) * il;
+          dmv.y += (gd.y - d.y * dot) * il;
+          dmv.z += (gd.z - d.z * dot) * il;
+          any_dir = true;
+        }
       }
-    }
 
-    // stage this column block (columns 16w + c) for the weight gradients
-    const int col = 16 * w + c;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) sX[8 * q + e][col] = xin[e];
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = 16 * rt + 4 * q + e;
-        sH1[r][col] = (__bf16)act.h1[rt][e];
-        sH2[r][col] = (__bf16)act.h2[rt][e];
-        sD1[r][col] = (__bf16)gr.dh1[rt][e];
-        sD2[r][col] = (__bf16)gr.dh2[rt][e];
+      // stage this column block (columns 16w + c) for the weight gradients
+      const int col = 16 * w + c;
+  #pragma unroll
+      for (int e = 0; e < 8; ++e) sX[8 * q + e][col] = xin[e];
+  #pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+  #pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = 16 * rt + 4 * q + e;
+          sH1[r][col] = (__bf16)act.h1[rt][e];
+          sH2[r][col] = (__bf16)act.h2[rt][e];
+          sD1[r][col] = (__bf16)gr.dh1[rt][e];
+          sD2[r][col] = (__bf16)gr.dh2[rt][e];
+        }
       }
+      if (q == 0) {
+  #pragma unroll
+        for (int e = 0; e < 3; ++e) sO[e][col] = (__bf16)dO[e];
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const int k0 = 32 * ks + 8 * q;
+        const bf16x8 aD2 = *reinterpret_cast<const bf16x8*>(&sD2[16 * w + c][k0]);
+        const bf16x8 aD1 = *reinterpret_cast<const bf16x8*>(&sD1[16 * w + c][k0]);
+  #pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+          gW2[ct] = mfma16(aD2, *reinterpret_cast<const bf16x8*>(&sH1[16 * ct + c][k0]), gW2[ct]);
+  #pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+          gW1[ct] = mfma16(aD1, *reinterpret_cast<const bf16x8*>(&sX[16 * ct + c][k0]), gW1[ct]);
+        gB1 = mfma16(aD1, ones, gB1);
+        gB2 = mfma16(aD2, ones, gB2);
+        const bf16x8 aO = *reinterpret_cast<const bf16x8*>(&sO[c][k0]);
+        gW3 = mfma16(aO, *reinterpret_cast<const bf16x8*>(&sH2[16 * w + c][k0]), gW3);
+        if (w == 0) gB3 = mfma16(aO, ones, gB3);
+      }
+      __syncthreads();
     }
-    if (q == 0) {
-#pragma unroll
-      for (int e = 0; e < 3; ++e) sO[e][col] = (__bf16)dO[e];
+    if (ok) *reinterpret_cast<float4*>(a.dL_dlatent + G * kLatent + 4 * q) = dlat;
+    if (!kDistill && ok && q == 0 && any_dir) {
+      a.dL_dmean3D[3 * G] = cur.dm.x + dmv.x;
+      a.dL_dmean3D[3 * G + 1] = cur.dm.y + dmv.y;
+      a.dL_dmean3D[3 * G + 2] = cur.dm.z + dmv.z;
     }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const int k0 = 32 * ks + 8 * q;
-      const bf16x8 aD2 = *reinterpret_cast<const bf16x8*>(&sD2[16 * w + c][k0]);
-      const bf16x8 aD1 = *reinterpret_cast<const bf16x8*>(&sD1[16 * w + c][k0]);
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-        gW2[ct] = mfma16(aD2, *reinterpret_cast<const bf16x8*>(&sH1[16 * ct + c][k0]), gW2[ct]);
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct)
-        gW1[ct] = mfma16(aD1, *reinterpret_cast<const bf16x8*>(&sX[16 * ct + c][k0]), gW1[ct]);
-      gB1 = mfma16(aD1, ones, gB1);
-      gB2 = mfma16(aD2, ones, gB2);
-      const bf16x8 aO = *reinterpret_cast<const bf16x8*>(&sO[c][k0]);
-      gW3 = mfma16(aO, *reinterpret_cast<const bf16x8*>(&sH2[16 * w + c][k0]), gW3);
-      if (w == 0) gB3 = mfma16(aO, ones, gB3);
-    }
-    __syncthreads();
+    cur = nxt;
   }
 
   // block partial row (each wave writes its own tiles)
@@ -446,6 +497,7 @@ hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* laten
   a.seed = seed;
   a.step = step;
   a.num_dirs = num_dirs;
+  a.V = 1;
   a.dL_dlatent = dL_dlatent;
   a.theta_partial = ws;
   const int nb = mlp_train_blocks(P);
@@ -455,10 +507,11 @@ hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* laten
 
 hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos, const int* radii,
                                const uint16_t* latents, const void* packed, const float* dL_dcolor, float* dL_dmean3D,
-                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s) {
+                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V) {
   if (P == 0) return hipSuccess;
   MlpTrainArgs a{};
   a.P = P;
+  a.V = V;
   a.packed = (const char*)packed;
   a.latent_bf16 = latents;
   a.means3D = means3D;

@@ -230,18 +230,24 @@ __device__ __forceinline__ bf16x8 load_latent_half(const uint16_t* __restrict__ 
 // Decoder colours of the wave's 64 Gaussians (wave_base + lane), evaluated
 // inside the forward preprocess.  dir: this lane's own view direction.  All
 // 64 lanes must call it.  w: forward part of the packed image in LDS.
-__device__ __forceinline__ float3 mlp_colour_wave(const char* w, const uint16_t* __restrict__ latents, int P, size_t wave_base,
-                                  float3 dir) {
+__device__ __forceinline__ float3 mlp_colour_wave(const char* w, const uint16_t* __restrict__ latents, int P,
+                                                  size_t wave_base, float3 dir) {
   const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
-  float3 mine = make_float3(0.f, 0.f, 0.f);
+  // all four column blocks' latent loads first: one HBM latency per wave
+  bf16x8 lat[4];
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
     const size_t G = wave_base + 16 * cb + c;
-    const bool ok = G < (size_t)P;
+    lat[cb] = load_latent_half(latents, G, q & 1, G < (size_t)P && q < 2);
+  }
+  float3 mine = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
     const float x = __shfl(dir.x, 16 * cb + c), y = __shfl(dir.y, 16 * cb + c), z = __shfl(dir.z, 16 * cb + c);
-    const bf16x8 xin = input_operand(q, load_latent_half(latents, G, q & 1, ok && q < 2), x, y, z);
+    const bf16x8 xin = input_operand(q, lat[cb], x, y, z);
     MlpAct a;
     mlp_forward_block(w, lane, xin, a);
+    // lanes (c, 0) hold the logits of Gaussian 16cb + c; lane 16cb + c takes them
     const float o0 = __shfl(a.o[0], c), o1 = __shfl(a.o[1], c), o2 = __shfl(a.o[2], c);
     if (q == cb) mine = make_float3(o0, o1, o2);
   }

@@ -26,11 +26,34 @@ struct GeomOut {
   int radius, area;
 };
 
-// Geometry part of preprocessLonlatCUDA; false when the Gaussian is culled
-// (too close, singular 2D covariance or empty tile rect).
-__device__ __forceinline__ bool preprocess_geometry(const PreprocessArgs& a, int idx, TileGrid grid, GeomOut& o) {
-  const View v = load_view(a.viewmatrix);
-  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+// Per-Gaussian inputs of the projection (read once per Gaussian).
+struct GaussGeo {
+  float3 p;
+  Sym3 S;
+  float opacity;
+};
+
+__device__ __forceinline__ GaussGeo load_gauss(const PreprocessArgs& a, int idx) {
+  GaussGeo gg;
+  gg.p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+  if (a.cov3D_precomp) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) gg.S.m[k] = a.cov3D_precomp[6 * idx + k];
+  } else {
+    const float3 s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
+    const float4 q = reinterpret_cast<const float4*>(a.rotations)[idx];
+    gg.S = cov3d_from(s, a.scale_modifier, q);
+  }
+  gg.opacity = a.opacities[idx];
+  return gg;
+}
+
+// Geometry part of preprocessLonlatCUDA for one view; false when the
+// Gaussian is culled (too close, singular 2D covariance or empty tile rect).
+__device__ __forceinline__ bool project_view(const GaussGeo& gg, const float* viewmatrix, int W, int H, TileGrid grid,
+                                             GeomOut& o) {
+  const View v = load_view(viewmatrix);
+  const float3 p = gg.p;
   o.p = p;
   const float3 t = to_camera(v, p);
   const float rr = t.x * t.x + t.y * t.y + t.z * t.z;
@@ -42,17 +65,8 @@ __device__ __forceinline__ bool preprocess_geometry(const PreprocessArgs& a, int
   const float sx = lon * kInvPi;
   const float sy = lat * kTwoInvPi;
 
-  Sym3 S;
-  if (a.cov3D_precomp) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
-  } else {
-    const float3 s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
-    const float4 q = reinterpret_cast<const float4*>(a.rotations)[idx];
-    S = cov3d_from(s, a.scale_modifier, q);
-  }
-
-  const Jac j = lonlat_jacobian(t, a.W, a.H);
+  const Sym3& S = gg.S;
+  const Jac j = lonlat_jacobian(t, W, H);
   const Proj2 A = project_rows(j, v);
   const float cxx = quad_form(S, A.ax, A.ax) + 0.3f;
   const float cxy = quad_form(S, A.ax, A.ay);
@@ -61,7 +75,7 @@ __device__ __forceinline__ bool preprocess_geometry(const PreprocessArgs& a, int
   const float det = cxx * cyy - cxy * cxy;
   if (det == 0.0f) return false;
   const float det_inv = 1.f / det;
-  o.conic = make_float4(cyy * det_inv, -cxy * det_inv, cxx * det_inv, a.opacities[idx]);
+  o.conic = make_float4(cyy * det_inv, -cxy * det_inv, cxx * det_inv, gg.opacity);
 
   const float mid = 0.5f * (cxx + cyy);
   const float disc = sqrtf(fmaxf(0.1f, mid * mid - det));
@@ -69,13 +83,42 @@ __device__ __forceinline__ bool preprocess_geometry(const PreprocessArgs& a, int
   o.radius = (int)ceilf(3.f * sqrtf(lambda));
 
   // ndc -> pixel in FP64, rounded to float (reference ndc2Pix evaluates in double).
-  const float px = (float)((((double)sx + 1.0) * (double)a.W - 1.0) * 0.5);
-  const float py = (float)((((double)sy + 1.0) * (double)a.H - 1.0) * 0.5);
+  const float px = (float)((((double)sx + 1.0) * (double)W - 1.0) * 0.5);
+  const float py = (float)((((double)sy + 1.0) * (double)H - 1.0) * 0.5);
   o.pix = make_float2(px, py);
   const Rect rc = tile_rect(o.pix, o.radius, grid);
   o.area = (rc.x1 - rc.x0) * (rc.y1 - rc.y0);
   o.r = r;
   return o.area != 0;
+}
+
+__device__ __forceinline__ bool preprocess_geometry(const PreprocessArgs& a, int idx, TileGrid grid, GeomOut& o) {
+  return project_view(load_gauss(a, idx), a.viewmatrix, a.W, a.H, grid, o);
+}
+
+// SH colour (computeColorFromSH, forward.cu:30-83) from coefficients already
+// in registers; returns the clamp-at-zero flags.
+__device__ __forceinline__ float3 sh_colour(int D, int nc, const float3 c[16], float3 p, float3 campos,
+                                            uint8_t* clamped) {
+  float3 d = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
+  const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+  d = make_float3(d.x / len, d.y / len, d.z / len);
+  float b[16];
+  sh_basis(D, d.x, d.y, d.z, b);
+  float3 rgb = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (i < nc) {
+      rgb.x += b[i] * c[i].x;
+      rgb.y += b[i] * c[i].y;
+      rgb.z += b[i] * c[i].z;
+    }
+  }
+  rgb.x += 0.5f;
+  rgb.y += 0.5f;
+  rgb.z += 0.5f;
+  *clamped = (uint8_t)((rgb.x < 0.f) | ((rgb.y < 0.f) << 1) | ((rgb.z < 0.f) << 2));
+  return make_float3(fmaxf(rgb.x, 0.f), fmaxf(rgb.y, 0.f), fmaxf(rgb.z, 0.f));
 }
 
 // kMlp: colours from the decoder (every lane of a wave that holds at least one
@@ -117,29 +160,10 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
     if (a.colors_precomp) {
       col = make_float3(a.colors_precomp[3 * idx], a.colors_precomp[3 * idx + 1], a.colors_precomp[3 * idx + 2]);
     } else {
-      const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
-      float3 d = make_float3(o.p.x - campos.x, o.p.y - campos.y, o.p.z - campos.z);
-      const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
-      d = make_float3(d.x / len, d.y / len, d.z / len);
-      float b[16];
-      sh_basis(a.D, d.x, d.y, d.z, b);
       const int nc = min(a.M, sh_coeffs(a.D));
       float3 c[16];
       load_sh(a.shs, a.M, idx, nc, c);
-      float3 rgb = make_float3(0.f, 0.f, 0.f);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        if (i < nc) {
-          rgb.x += b[i] * c[i].x;
-          rgb.y += b[i] * c[i].y;
-          rgb.z += b[i] * c[i].z;
-        }
-      }
-      rgb.x += 0.5f;
-      rgb.y += 0.5f;
-      rgb.z += 0.5f;
-      g.clamped[idx] = (uint8_t)((rgb.x < 0.f) | ((rgb.y < 0.f) << 1) | ((rgb.z < 0.f) << 2));
-      col = make_float3(fmaxf(rgb.x, 0.f), fmaxf(rgb.y, 0.f), fmaxf(rgb.z, 0.f));
+      col = sh_colour(a.D, nc, c, o.p, make_float3(a.campos[0], a.campos[1], a.campos[2]), &g.clamped[idx]);
     }
   }
 
@@ -149,6 +173,77 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
   g.tiles_touched[idx] = (uint32_t)o.area;
 }
 
+// Batched preprocess: one thread per Gaussian reads its parameters (and its
+// SH coefficients or decoder latents) once and writes item v P + g of every
+// view v.  viewmats [V,16] and campos [V,3] are device arrays.
+template <bool kMlp>
+__global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a, const float* __restrict__ viewmats,
+                                                               const float* __restrict__ camposv, ViewTable vt,
+                                                               GeomState g, int* __restrict__ radii_out) {
+  __shared__ __attribute__((aligned(16))) char s_w[kMlp ? kMlpFwdBytes : 16];
+  if constexpr (kMlp) {
+    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_packed);
+    for (int i = threadIdx.x; i < kMlpFwdBytes / 16; i += blockDim.x) reinterpret_cast<uint4*>(s_w)[i] = src[i];
+    __syncthreads();
+  }
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((kMlp ? (idx & ~63) : idx) >= a.P) return;
+  const bool in_range = idx < a.P;
+  GaussGeo gg{};
+  float3 c[16];
+  const int nc = (kMlp || a.colors_precomp) ? 0 : min(a.M, sh_coeffs(a.D));
+  if (in_range) {
+    gg = load_gauss(a, idx);
+    if (!kMlp && !a.colors_precomp) load_sh(a.shs, a.M, idx, nc, c);
+  }
+  for (int v = 0; v < vt.V; ++v) {
+    const size_t item = (size_t)v * a.P + idx;
+    if (in_range) {
+      radii_out[item] = 0;
+      g.tiles_touched[item] = 0;
+      g.depth_key[item] = 0xFFFFFFFFu;
+    }
+    GeomOut o;
+    const bool live = in_range && project_view(gg, viewmats + 16 * v, vt.W[v], vt.H[v], vt.grid(v), o);
+    const float3 cp = make_float3(camposv[3 * v], camposv[3 * v + 1], camposv[3 * v + 2]);
+    float3 col;
+    if constexpr (kMlp) {
+      float3 dir = make_float3(0.f, 0.f, 1.f);
+      if (in_range) {
+        const float3 d = make_float3(gg.p.x - cp.x, gg.p.y - cp.y, gg.p.z - cp.z);
+        const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+        if (len > 0.f) dir = make_float3(d.x / len, d.y / len, d.z / len);
+      }
+      col = mlp_colour_wave(s_w, a.latents, a.P, (size_t)(idx & ~63), dir);
+      if (!live) continue;
+      g.clamped[item] = 0;
+    } else {
+      if (!live) continue;
+      if (a.colors_precomp) {
+        col = make_float3(a.colors_precomp[3 * idx], a.colors_precomp[3 * idx + 1], a.colors_precomp[3 * idx + 2]);
+      } else {
+        col = sh_colour(a.D, nc, c, gg.p, cp, &g.clamped[item]);
+      }
+    }
+    g.depth_key[item] = __float_as_uint(o.r);
+    radii_out[item] = o.radius;
+    store_rec(g.rec, item, o.pix, o.conic, col);
+    g.tiles_touched[item] = (uint32_t)o.area;
+  }
+}
+
+hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmats, const float* campos,
+                                   const ViewTable& vt, const GeomState& g, int* radii, hipStream_t s) {
+  if (a.P == 0 || vt.V == 0) return hipSuccess;
+  if (a.mlp_packed && a.latents)
+    hipLaunchKernelGGL(preprocess_views_kernel<true>, dim3((a.P + 255) / 256), dim3(256), 0, s, a, viewmats, campos,
+                       vt, g, radii);
+  else
+    hipLaunchKernelGGL(preprocess_views_kernel<false>, dim3((a.P + 255) / 256), dim3(256), 0, s, a, viewmats,
+                       campos, vt, g, radii);
+  return hipGetLastError();
+}
+
 __global__ void mark_visible_kernel(int P, uint8_t* __restrict__ present) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < P) present[idx] = 1;
@@ -156,7 +251,7 @@ __global__ void mark_visible_kernel(int P, uint8_t* __restrict__ present) {
 
 hipError_t launch_preprocess(const PreprocessArgs& a, const GeomState& g, int* radii, hipStream_t s) {
   if (a.P == 0) return hipSuccess;
-  const TileGrid grid = tile_grid(a.W, a.H, a.seam_wrap);
+  const TileGrid grid = tile_grid(a.W, a.H, a.seam_wrap, a.tile_x, a.tile_y);
   if (a.mlp_packed && a.latents)
     hipLaunchKernelGGL(preprocess_kernel<true>, dim3((a.P + 255) / 256), dim3(256), 0, s, a, g, radii, grid);
   else

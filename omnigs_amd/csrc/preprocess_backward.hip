@@ -45,75 +45,24 @@ __device__ __forceinline__ void sh_basis_grad(int deg, float x, float y, float z
   }
 }
 
-__global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBackwardArgs a) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.P) return;
-  const int M = a.M;
-
-  if (!(a.radii[idx] > 0)) {
-    a.dL_dmean3D[3 * idx] = 0.f;
-    a.dL_dmean3D[3 * idx + 1] = 0.f;
-    a.dL_dmean3D[3 * idx + 2] = 0.f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = 0.f;
-    if (a.dL_dsh)
-      for (int k = 0; k < 3 * M; ++k) a.dL_dsh[(size_t)3 * M * idx + k] = 0.f;
-    if (a.dL_dscale) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) a.dL_dscale[3 * idx + k] = 0.f;
-    }
-    if (a.dL_drot) reinterpret_cast<float4*>(a.dL_drot)[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (a.dpx_dt) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) a.dpx_dt[3 * idx + k] = 0.f;
-    }
-    if (a.dpy_dt) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) a.dpy_dt[3 * idx + k] = 0.f;
-    }
-    return;
-  }
-
-  const View v = load_view(a.viewmatrix);
-  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+// Covariance and 2D-mean terms of one view (computeCov2DLonLatCUDA, and the
+// 2D-mean part of preprocessLonLatCUDA): dL/dconic -> dcov[6] and the world
+// mean gradient; j receives the lon/lat Jacobian.
+__device__ __forceinline__ void cov_mean_backward(const View& v, float3 p, const Sym3& S, int W, int H, float4 dcon4,
+                                                  float dmx, float dmy, float dcov[6], float3& dmean, Jac& j) {
   const float3 t = to_camera(v, p);
-
-  // ---- 3D covariance (recomputed; identical to the forward's) ----
-  Sym3 S;
-  float3 s = make_float3(0.f, 0.f, 0.f);
-  float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
-  if (a.cov3D_precomp) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
-  } else {
-    s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
-    q = reinterpret_cast<const float4*>(a.rotations)[idx];
-    S = cov3d_from(s, a.scale_modifier, q);
-  }
-
-  // ---- covariance term (computeCov2DLonLatCUDA) ----
-  const Jac j = lonlat_jacobian(t, a.W, a.H);
-  if (a.dpx_dt) {
-    a.dpx_dt[3 * idx] = j.px_x;
-    a.dpx_dt[3 * idx + 1] = 0.f;
-    a.dpx_dt[3 * idx + 2] = j.px_z;
-  }
-  if (a.dpy_dt) {
-    a.dpy_dt[3 * idx] = j.py_x;
-    a.dpy_dt[3 * idx + 1] = j.py_y;
-    a.dpy_dt[3 * idx + 2] = j.py_z;
-  }
+  j = lonlat_jacobian(t, W, H);
   const Proj2 A = project_rows(j, v);
   const float ca = quad_form(S, A.ax, A.ax) + 0.3f;
   const float cb = quad_form(S, A.ax, A.ay);
   const float cc = quad_form(S, A.ay, A.ay) + 0.3f;
 
-  const float4 dcon4 = reinterpret_cast<const float4*>(a.dL_dconic)[idx];
   const float dcx = dcon4.x, dcy = dcon4.y, dcz = dcon4.w;
   const float denom = ca * cc - cb * cb;
   const float d2inv = 1.0f / ((denom * denom) + 1e-7f);
   float dL_da = 0.f, dL_db = 0.f, dL_dc = 0.f;
-  float dcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dcov[k] = 0.f;
   const float* ax = A.ax;
   const float* ay = A.ay;
   if (d2inv != 0.f) {
@@ -127,8 +76,6 @@ __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBack
     dcov[2] = 2.f * ax[0] * ax[2] * dL_da + (ax[0] * ay[2] + ax[2] * ay[0]) * dL_db + 2.f * ay[0] * ay[2] * dL_dc;
     dcov[4] = 2.f * ax[2] * ax[1] * dL_da + (ax[1] * ay[2] + ax[2] * ay[1]) * dL_db + 2.f * ay[1] * ay[2] * dL_dc;
   }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = dcov[k];
 
   // dL/dA (rows of J R), then dL/dJ = dL/dA R^T
   float dAx[3], dAy[3];
@@ -168,123 +115,302 @@ __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBack
   const float dty = dJ10 * t.x * temp1 - dJ11 * Hp * 2.f * rxz * t.y * inv_r4 + dJ12 * t.z * temp1;
   const float dtz = dJ00 * temp3 + dJ02 * temp4 + dJ10 * temp2 + dJ11 * t.z * temp1 +
                     dJ12 * temp5 * (2.f * tztz * rxz2 - txtx * r2);
-  float3 dmean = to_world_dir(v, make_float3(dtx, dty, dtz));
+  dmean = to_world_dir(v, make_float3(dtx, dty, dtz));
 
   // ---- 2D-mean term (NDC -> pixel: x 2/W, 2/H) ----
-  {
-    const float dL_dpx = a.dL_dmean2D[3 * idx] * (2.0f / (float)a.W);
-    const float dL_dpy = a.dL_dmean2D[3 * idx + 1] * (2.0f / (float)a.H);
-    const float3 dt2 = make_float3(dL_dpx * j.px_x + dL_dpy * j.py_x, dL_dpy * j.py_y,
-                                   dL_dpx * j.px_z + dL_dpy * j.py_z);
-    const float3 dm2 = to_world_dir(v, dt2);
-    dmean.x += dm2.x;
-    dmean.y += dm2.y;
-    dmean.z += dm2.z;
+  const float dL_dpx = dmx * (2.0f / (float)W);
+  const float dL_dpy = dmy * (2.0f / (float)H);
+  const float3 dt2 = make_float3(dL_dpx * j.px_x + dL_dpy * j.py_x, dL_dpy * j.py_y, dL_dpx * j.px_z + dL_dpy * j.py_z);
+  const float3 dm2 = to_world_dir(v, dt2);
+  dmean.x += dm2.x;
+  dmean.y += dm2.y;
+  dmean.z += dm2.z;
+}
+
+// SH term of one view (computeColorFromSH backward, backward.cu:30-151):
+// f[3i + ch] = basis_i dL/dcolor_ch (clamped channels give 0); returns the
+// view-direction gradient on the mean.  c: the nc coefficients in registers.
+__device__ __forceinline__ float3 sh_backward(int D, int nc, const float3 c[16], float3 p, float3 campos, uint8_t cm,
+                                              float3 dcol, float f[48]) {
+  const float3 d0 = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
+  const float len = sqrtf(d0.x * d0.x + d0.y * d0.y + d0.z * d0.z);
+  const float x = d0.x / len, y = d0.y / len, z = d0.z / len;
+  const float gx = (cm & 1) ? 0.f : dcol.x;
+  const float gy = (cm & 2) ? 0.f : dcol.y;
+  const float gz = (cm & 4) ? 0.f : dcol.z;
+  float b[16];
+  sh_basis(D, x, y, z, b);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const bool on = i < nc;
+    f[3 * i] = on ? b[i] * gx : 0.f;
+    f[3 * i + 1] = on ? b[i] * gy : 0.f;
+    f[3 * i + 2] = on ? b[i] * gz : 0.f;
   }
+  float3 dm = make_float3(0.f, 0.f, 0.f);
+  if (D > 0) {
+    float bx[16], by[16], bz[16];
+    sh_basis_grad(D, x, y, z, bx, by, bz);
+    float ddx = 0.f, ddy = 0.f, ddz = 0.f;
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+      if (i < nc) {
+        const float sg = c[i].x * gx + c[i].y * gy + c[i].z * gz;
+        ddx += bx[i] * sg;
+        ddy += by[i] * sg;
+        ddz += bz[i] * sg;
+      }
+    }
+    // derivative of the normalisation d0/|d0| (dnormvdv, auxiliary.h:134-144)
+    const float sum2 = d0.x * d0.x + d0.y * d0.y + d0.z * d0.z;
+    const float invsum32 = 1.0f / sqrtf(sum2 * sum2 * sum2);
+    dm.x = ((sum2 - d0.x * d0.x) * ddx - d0.y * d0.x * ddy - d0.z * d0.x * ddz) * invsum32;
+    dm.y = (-d0.x * d0.y * ddx + (sum2 - d0.y * d0.y) * ddy - d0.z * d0.y * ddz) * invsum32;
+    dm.z = (-d0.x * d0.z * ddx - d0.y * d0.z * ddy + (sum2 - d0.z * d0.z) * ddz) * invsum32;
+  }
+  return dm;
+}
+
+// computeCov3D backward (backward.cu:489-552): dL/dscale (w.r.t. s, ignoring
+// the modifier) and dL/dq (w.r.t. the quaternion as given).
+__device__ __forceinline__ void scale_rot_backward(const float dcov[6], float3 s, float4 q, float mod, float dsc[3],
+                                                   float4& dq) {
+  float R[3][3];
+  quat_to_rot(q, R);
+  const float sv[3] = {mod * s.x, mod * s.y, mod * s.z};
+  // dL/dSigma with halved off-diagonals
+  float D[3][3];
+  D[0][0] = dcov[0];
+  D[1][1] = dcov[3];
+  D[2][2] = dcov[5];
+  D[0][1] = D[1][0] = 0.5f * dcov[1];
+  D[0][2] = D[2][0] = 0.5f * dcov[2];
+  D[1][2] = D[2][1] = 0.5f * dcov[4];
+  // M[k][i] = s_k R[i][k];  dM = 2 M D;  dscale_k = sum_i R[i][k] dM[k][i]
+  float E[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float dMki = 2.f * (sv[k] * R[0][k] * D[0][i] + sv[k] * R[1][k] * D[1][i] + sv[k] * R[2][k] * D[2][i]);
+      acc += R[i][k] * dMki;
+      E[k][i] = sv[k] * dMki;
+    }
+    dsc[k] = acc;
+  }
+  const float w = q.x, x = q.y, y = q.z, z = q.w;
+  dq.x = 2.f * z * (E[0][1] - E[1][0]) + 2.f * y * (E[2][0] - E[0][2]) + 2.f * x * (E[1][2] - E[2][1]);
+  dq.y = 2.f * y * (E[1][0] + E[0][1]) + 2.f * z * (E[2][0] + E[0][2]) + 2.f * w * (E[1][2] - E[2][1]) -
+         4.f * x * (E[2][2] + E[1][1]);
+  dq.z = 2.f * x * (E[1][0] + E[0][1]) + 2.f * w * (E[2][0] - E[0][2]) + 2.f * z * (E[1][2] + E[2][1]) -
+         4.f * y * (E[2][2] + E[0][0]);
+  dq.w = 2.f * w * (E[0][1] - E[1][0]) + 2.f * x * (E[2][0] + E[0][2]) + 2.f * y * (E[1][2] + E[2][1]) -
+         4.f * z * (E[1][1] + E[0][0]);
+}
+
+// Writes one Gaussian's SH gradient row (M coefficients, zero past 16).
+__device__ __forceinline__ void store_dsh(float* dsh, int M, const float f[48]) {
+  if (M == 16) {
+    float4* d4 = reinterpret_cast<float4*>(dsh);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) d4[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i < M) {
+        dsh[3 * i] = f[3 * i];
+        dsh[3 * i + 1] = f[3 * i + 1];
+        dsh[3 * i + 2] = f[3 * i + 2];
+      }
+    }
+    for (int k = 48; k < 3 * M; ++k) dsh[k] = 0.f;
+  }
+}
+
+__global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBackwardArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+  const int M = a.M;
+
+  if (!(a.radii[idx] > 0)) {
+    a.dL_dmean3D[3 * idx] = 0.f;
+    a.dL_dmean3D[3 * idx + 1] = 0.f;
+    a.dL_dmean3D[3 * idx + 2] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = 0.f;
+    if (a.dL_dsh)
+      for (int k = 0; k < 3 * M; ++k) a.dL_dsh[(size_t)3 * M * idx + k] = 0.f;
+    if (a.dL_dscale) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.dL_dscale[3 * idx + k] = 0.f;
+    }
+    if (a.dL_drot) reinterpret_cast<float4*>(a.dL_drot)[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a.dpx_dt) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.dpx_dt[3 * idx + k] = 0.f;
+    }
+    if (a.dpy_dt) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.dpy_dt[3 * idx + k] = 0.f;
+    }
+    return;
+  }
+
+  const View v = load_view(a.viewmatrix);
+  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+
+  // ---- 3D covariance (recomputed; identical to the forward's) ----
+  Sym3 S;
+  float3 s = make_float3(0.f, 0.f, 0.f);
+  float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
+  if (a.cov3D_precomp) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
+  } else {
+    s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
+    q = reinterpret_cast<const float4*>(a.rotations)[idx];
+    S = cov3d_from(s, a.scale_modifier, q);
+  }
+
+  float dcov[6];
+  float3 dmean;
+  Jac j;
+  cov_mean_backward(v, p, S, a.W, a.H, reinterpret_cast<const float4*>(a.dL_dconic)[idx], a.dL_dmean2D[3 * idx],
+                    a.dL_dmean2D[3 * idx + 1], dcov, dmean, j);
+  if (a.dpx_dt) {
+    a.dpx_dt[3 * idx] = j.px_x;
+    a.dpx_dt[3 * idx + 1] = 0.f;
+    a.dpx_dt[3 * idx + 2] = j.px_z;
+  }
+  if (a.dpy_dt) {
+    a.dpy_dt[3 * idx] = j.py_x;
+    a.dpy_dt[3 * idx + 1] = j.py_y;
+    a.dpy_dt[3 * idx + 2] = j.py_z;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = dcov[k];
 
   // ---- SH term ----
   if (a.shs) {
-    const float3 campos = make_float3(a.campos[0], a.campos[1], a.campos[2]);
-    const float3 d0 = make_float3(p.x - campos.x, p.y - campos.y, p.z - campos.z);
-    const float len = sqrtf(d0.x * d0.x + d0.y * d0.y + d0.z * d0.z);
-    const float x = d0.x / len, y = d0.y / len, z = d0.z / len;
-    const uint8_t cm = a.clamped[idx];
-    const float gx = (cm & 1) ? 0.f : a.dL_dcolor[3 * idx];
-    const float gy = (cm & 2) ? 0.f : a.dL_dcolor[3 * idx + 1];
-    const float gz = (cm & 4) ? 0.f : a.dL_dcolor[3 * idx + 2];
-    float b[16];
-    sh_basis(a.D, x, y, z, b);
     const int nc = min(M, sh_coeffs(a.D));
-    float* dsh = a.dL_dsh + (size_t)3 * M * idx;
-    if (M == 16) {
-      float f[48];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const bool on = i < nc;
-        f[3 * i] = on ? b[i] * gx : 0.f;
-        f[3 * i + 1] = on ? b[i] * gy : 0.f;
-        f[3 * i + 2] = on ? b[i] * gz : 0.f;
-      }
-      float4* d4 = reinterpret_cast<float4*>(dsh);
-#pragma unroll
-      for (int i = 0; i < 12; ++i) d4[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
-    } else {
-      for (int i = 0; i < M; ++i) {
-        const bool on = i < nc;
-        dsh[3 * i] = on ? b[i] * gx : 0.f;
-        dsh[3 * i + 1] = on ? b[i] * gy : 0.f;
-        dsh[3 * i + 2] = on ? b[i] * gz : 0.f;
-      }
-    }
+    float3 c[16];
+    load_sh(a.shs, M, idx, nc, c);
+    float f[48];
+    const float3 dm = sh_backward(a.D, nc, c, p, make_float3(a.campos[0], a.campos[1], a.campos[2]), a.clamped[idx],
+                                  make_float3(a.dL_dcolor[3 * idx], a.dL_dcolor[3 * idx + 1], a.dL_dcolor[3 * idx + 2]),
+                                  f);
+    store_dsh(a.dL_dsh + (size_t)3 * M * idx, M, f);
     if (a.D > 0) {
-      float3 c[16];
-      load_sh(a.shs, M, idx, nc, c);
-      float bx[16], by[16], bz[16];
-      sh_basis_grad(a.D, x, y, z, bx, by, bz);
-      float ddx = 0.f, ddy = 0.f, ddz = 0.f;
-#pragma unroll
-      for (int i = 1; i < 16; ++i) {
-        if (i < nc) {
-          const float sg = c[i].x * gx + c[i].y * gy + c[i].z * gz;
-          ddx += bx[i] * sg;
-          ddy += by[i] * sg;
-          ddz += bz[i] * sg;
-        }
-      }
-      // derivative of the normalisation d0/|d0| (dnormvdv, auxiliary.h:134-144)
-      const float sum2 = d0.x * d0.x + d0.y * d0.y + d0.z * d0.z;
-      const float invsum32 = 1.0f / sqrtf(sum2 * sum2 * sum2);
-      dmean.x += ((sum2 - d0.x * d0.x) * ddx - d0.y * d0.x * ddy - d0.z * d0.x * ddz) * invsum32;
-      dmean.y += (-d0.x * d0.y * ddx + (sum2 - d0.y * d0.y) * ddy - d0.z * d0.y * ddz) * invsum32;
-      dmean.z += (-d0.x * d0.z * ddx - d0.y * d0.z * ddy + (sum2 - d0.z * d0.z) * ddz) * invsum32;
+      dmean.x += dm.x;
+      dmean.y += dm.y;
+      dmean.z += dm.z;
     }
   }
   a.dL_dmean3D[3 * idx] = dmean.x;
   a.dL_dmean3D[3 * idx + 1] = dmean.y;
   a.dL_dmean3D[3 * idx + 2] = dmean.z;
 
-  // ---- scale / rotation term (computeCov3D backward, backward.cu:489-552) ----
+  // ---- scale / rotation term ----
   if (a.scales && a.dL_dscale && a.dL_drot) {
-    float R[3][3];
-    quat_to_rot(q, R);
-    const float sv[3] = {a.scale_modifier * s.x, a.scale_modifier * s.y, a.scale_modifier * s.z};
-    // dL/dSigma with halved off-diagonals
-    float D[3][3];
-    D[0][0] = dcov[0];
-    D[1][1] = dcov[3];
-    D[2][2] = dcov[5];
-    D[0][1] = D[1][0] = 0.5f * dcov[1];
-    D[0][2] = D[2][0] = 0.5f * dcov[2];
-    D[1][2] = D[2][1] = 0.5f * dcov[4];
-    // M[k][i] = s_k R[i][k];  dM = 2 M D;  dscale_k = sum_i R[i][k] dM[k][i]
-    float E[3][3];
     float dsc[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float acc = 0.f;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const float dMki =
-            2.f * (sv[k] * R[0][k] * D[0][i] + sv[k] * R[1][k] * D[1][i] + sv[k] * R[2][k] * D[2][i]);
-        acc += R[i][k] * dMki;
-        E[k][i] = sv[k] * dMki;
-      }
-      dsc[k] = acc;
-    }
+    float4 dq;
+    scale_rot_backward(dcov, s, q, a.scale_modifier, dsc, dq);
     a.dL_dscale[3 * idx] = dsc[0];
     a.dL_dscale[3 * idx + 1] = dsc[1];
     a.dL_dscale[3 * idx + 2] = dsc[2];
-    const float w = q.x, x = q.y, y = q.z, z = q.w;
-    float4 dq;
-    dq.x = 2.f * z * (E[0][1] - E[1][0]) + 2.f * y * (E[2][0] - E[0][2]) + 2.f * x * (E[1][2] - E[2][1]);
-    dq.y = 2.f * y * (E[1][0] + E[0][1]) + 2.f * z * (E[2][0] + E[0][2]) + 2.f * w * (E[1][2] - E[2][1]) -
-           4.f * x * (E[2][2] + E[1][1]);
-    dq.z = 2.f * x * (E[1][0] + E[0][1]) + 2.f * w * (E[2][0] - E[0][2]) + 2.f * z * (E[1][2] + E[2][1]) -
-           4.f * y * (E[2][2] + E[0][0]);
-    dq.w = 2.f * w * (E[0][1] - E[1][0]) + 2.f * x * (E[2][0] + E[0][2]) + 2.f * y * (E[1][2] + E[2][1]) -
-           4.f * z * (E[1][1] + E[0][0]);
     reinterpret_cast<float4*>(a.dL_drot)[idx] = dq;
   }
+}
+
+// Batched backward: one thread per Gaussian sums every view's terms (items
+// v P + g) in registers and writes each parameter gradient once, including
+// dL/dopacity (the render backward leaves it per item).  With the colour
+// decoder (shs NULL) the colour term is the decoder backward's.
+__global__ __launch_bounds__(256) void preprocess_backward_views_kernel(PreprocessBackwardArgs a,
+                                                                        const float* __restrict__ viewmats,
+                                                                        const float* __restrict__ camposv,
+                                                                        ViewTable vt,
+                                                                        const float* __restrict__ dopacity_items,
+                                                                        float* __restrict__ dL_dopacity) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+  const int M = a.M;
+  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+  Sym3 S;
+  float3 s = make_float3(0.f, 0.f, 0.f);
+  float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
+  if (a.cov3D_precomp) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
+  } else {
+    s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
+    q = reinterpret_cast<const float4*>(a.rotations)[idx];
+    S = cov3d_from(s, a.scale_modifier, q);
+  }
+  const int nc = a.shs ? min(M, sh_coeffs(a.D)) : 0;
+  float3 c[16];
+  if (a.shs) load_sh(a.shs, M, idx, nc, c);
+  float dcov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float3 dmean = make_float3(0.f, 0.f, 0.f);
+  float fsum[48];
+#pragma unroll
+  for (int k = 0; k < 48; ++k) fsum[k] = 0.f;
+  float dop = 0.f;
+  for (int vi = 0; vi < vt.V; ++vi) {
+    const size_t item = (size_t)vi * a.P + idx;
+    dop += dopacity_items[item];
+    if (!(a.radii[item] > 0)) continue;
+    const View v = load_view(viewmats + 16 * vi);
+    float dc[6];
+    float3 dm;
+    Jac j;
+    cov_mean_backward(v, p, S, vt.W[vi], vt.H[vi], reinterpret_cast<const float4*>(a.dL_dconic)[item],
+                      a.dL_dmean2D[3 * item], a.dL_dmean2D[3 * item + 1], dc, dm, j);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dcov[k] += dc[k];
+    dmean.x += dm.x;
+    dmean.y += dm.y;
+    dmean.z += dm.z;
+    if (a.shs) {
+      float f[48];
+      const float3 dsv = sh_backward(
+          a.D, nc, c, p, make_float3(camposv[3 * vi], camposv[3 * vi + 1], camposv[3 * vi + 2]), a.clamped[item],
+          make_float3(a.dL_dcolor[3 * item], a.dL_dcolor[3 * item + 1], a.dL_dcolor[3 * item + 2]), f);
+#pragma unroll
+      for (int k = 0; k < 48; ++k) fsum[k] += f[k];
+      if (a.D > 0) {
+        dmean.x += dsv.x;
+        dmean.y += dsv.y;
+        dmean.z += dsv.z;
+      }
+    }
+  }
+  a.dL_dmean3D[3 * idx] = dmean.x;
+  a.dL_dmean3D[3 * idx + 1] = dmean.y;
+  a.dL_dmean3D[3 * idx + 2] = dmean.z;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = dcov[k];
+  if (a.shs && a.dL_dsh) store_dsh(a.dL_dsh + (size_t)3 * M * idx, M, fsum);
+  dL_dopacity[idx] = dop;
+  if (a.scales && a.dL_dscale && a.dL_drot) {
+    float dsc[3];
+    float4 dq;
+    scale_rot_backward(dcov, s, q, a.scale_modifier, dsc, dq);
+    a.dL_dscale[3 * idx] = dsc[0];
+    a.dL_dscale[3 * idx + 1] = dsc[1];
+    a.dL_dscale[3 * idx + 2] = dsc[2];
+    reinterpret_cast<float4*>(a.dL_drot)[idx] = dq;
+  }
+}
+
+hipError_t launch_preprocess_backward_views(const PreprocessBackwardArgs& a, const float* viewmats,
+                                            const float* campos, const ViewTable& vt, const float* dopacity_items,
+                                            float* dL_dopacity, hipStream_t s) {
+  if (a.P == 0) return hipSuccess;
+  hipLaunchKernelGGL(preprocess_backward_views_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, viewmats, campos,
+                     vt, dopacity_items, dL_dopacity);
+  return hipGetLastError();
 }
 
 hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s) {

@@ -1,6 +1,9 @@
-// Tile render, forward and backward.  One 256-thread workgroup per 16x16
-// tile; each 64-lane wavefront owns an 8x8 quadrant of the tile so that a
-// wave's pixels are spatially compact (more waves skip a small Gaussian).
+// Tile render, forward and backward.  One 256-thread workgroup per tile of
+// 256 pixels (16x16 as in the reference; 32x8 and 8x32 as extensions); each
+// 64-lane wavefront owns an 8x8 quadrant of the tile so that a wave's pixels
+// are spatially compact (more waves skip a small Gaussian).  Batched launches
+// (kMulti) cover every tile of every view: a block finds its view in the
+// ViewTable and renders into that view's slice of the pixel arrays.
 //
 // Per-pixel semantics follow the reference's renderCUDA forward
 // (forward.cu:346-467) and backward (backward.cu:672-843):
@@ -19,11 +22,35 @@ namespace omnigs {
 constexpr int kBatch = kTilePixels;  // instances staged per LDS round
 constexpr int kWavesPerTile = kTilePixels / kWave;
 
+// Quadrant layout of a TX x TY tile: QX x QY quadrants of 8x8 pixels.
+template <int TX, int TY>
+struct Quads {
+  static constexpr int QX = TX / 8, QY = TY / 8;
+  static_assert(TX * TY == kTilePixels && QX * QY == kWavesPerTile, "256-pixel tiles of 8x8 quadrants");
+};
+
+template <int TX, int TY>
 __device__ __forceinline__ void pixel_of(int tile, int gx, int tid, int& px, int& py) {
   const int lane = tid & (kWave - 1), w = tid / kWave;
   const int tx = tile % gx, ty = tile / gx;
-  px = tx * kTileX + (w & 1) * 8 + (lane & 7);
-  py = ty * kTileY + (w >> 1) * 8 + (lane >> 3);
+  px = tx * TX + (w % Quads<TX, TY>::QX) * 8 + (lane & 7);
+  py = ty * TY + (w / Quads<TX, TY>::QX) * 8 + (lane >> 3);
+}
+
+// Where this block renders: view size, tile grid width, tile within the view
+// and the view's first pixel in the (batched) pixel arrays.
+struct TileLoc {
+  int W, H, gx, tile;
+  size_t pix;
+};
+template <bool kMulti>
+__device__ __forceinline__ TileLoc locate(int W, int H, int gx, const ViewTable& vt) {
+  if constexpr (kMulti) {
+    const int v = vt.view_of_tile(blockIdx.x);
+    return {vt.W[v], vt.H[v], vt.gx[v], (int)(blockIdx.x - vt.tile_base[v]), (size_t)vt.pix_base[v]};
+  } else {
+    return {W, H, gx, (int)blockIdx.x, 0};
+  }
 }
 
 // exp(power) is evaluated as exp2 of a power prescaled by log2(e) at staging:
@@ -52,7 +79,9 @@ __device__ __forceinline__ float seam_dx(float dx, float W, float invW) {
 // 0.5 d^T Q d <= ln(255 o) the alpha test fails for every pixel, so skipping
 // those (wave, instance) pairs changes no result.  The box is padded by
 // 1% + 0.05 px against rounding; degenerate conics get the full mask.
+template <int TX, int TY>
 __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile, int gx, float wrapW) {
+  constexpr int QX = Quads<TX, TY>::QX;
   const float det = co.x * co.z - co.y * co.y;
   const float o255 = 255.0f * co.w;
   if (!(o255 > 0.9999f)) return 0u;  // alpha < 1/255 everywhere
@@ -60,12 +89,12 @@ __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile,
   const float tau2 = fmaxf(2.0f * __logf(o255) * 1.0001f, 0.f) + 1e-3f;
   const float ex = sqrtf(tau2 * co.z / det) * 1.01f + 0.05f;
   const float ey = sqrtf(tau2 * co.x / det) * 1.01f + 0.05f;
-  const float x0 = (float)((tile % gx) * kTileX), y0 = (float)((tile / gx) * kTileY);
-  if (wrapW > 0.f) m.x -= wrapW * rintf((m.x - (x0 + 0.5f * kTileX)) / wrapW);  // nearest copy across the seam
+  const float x0 = (float)((tile % gx) * TX), y0 = (float)((tile / gx) * TY);
+  if (wrapW > 0.f) m.x -= wrapW * rintf((m.x - (x0 + 0.5f * TX)) / wrapW);  // nearest copy across the seam
   uint32_t mk = 0;
 #pragma unroll
   for (int q = 0; q < kWavesPerTile; ++q) {
-    const float qx = x0 + (float)((q & 1) * 8), qy = y0 + (float)((q >> 1) * 8);
+    const float qx = x0 + (float)((q % QX) * 8), qy = y0 + (float)((q / QX) * 8);
     const bool hit = (m.x + ex >= qx) && (m.x - ex <= qx + 7.f) && (m.y + ey >= qy) && (m.y - ey <= qy + 7.f);
     mk |= hit ? (1u << q) : 0u;
   }
@@ -91,11 +120,13 @@ __device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* l
   return count;
 }
 
-template <bool kWrap>
+template <int TX, int TY, bool kWrap, bool kMulti>
 __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
-    int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
-    const float4* __restrict__ rec, float* __restrict__ final_T, uint32_t* __restrict__ n_contrib,
+    int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
+    ViewTable vt, const float4* __restrict__ rec, float* __restrict__ final_T, uint32_t* __restrict__ n_contrib,
     const float* __restrict__ bg, float* __restrict__ out_color) {
+  const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
+  const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_q[kBatch];  // prescaled conic (a, b, c) and opacity
@@ -105,12 +136,12 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), w = tid / kWave;
-  const int tile = blockIdx.x;
+  const int tile = loc.tile;
   int px, py;
-  pixel_of(tile, gx, tid, px, py);
+  pixel_of<TX, TY>(tile, gx, tid, px, py);
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
-  const uint2 range = ranges[tile];
+  const uint2 range = ranges[blockIdx.x];
   const int n = (int)(range.y - range.x);
 
   bool done = !inside;
@@ -132,7 +163,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       s_rgb[0][tid] = rr.rgb.x;
       s_rgb[1][tid] = rr.rgb.y;
       s_rgb[2][tid] = rr.rgb.z;
-      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx, kWrap ? Wf : 0.f);
+      s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
     __syncthreads();
     const int cnt = min(kBatch, n - base);
@@ -163,11 +194,12 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   if (inside) {
     const size_t pid = (size_t)py * W + px;
     const size_t N = (size_t)W * H;
-    final_T[pid] = T;
-    n_contrib[pid] = last;
-    out_color[pid] = C0 + T * bg[0];
-    out_color[N + pid] = C1 + T * bg[1];
-    out_color[2 * N + pid] = C2 + T * bg[2];
+    final_T[loc.pix + pid] = T;
+    n_contrib[loc.pix + pid] = last;
+    float* out = out_color + 3 * loc.pix;
+    out[pid] = C0 + T * bg[0];
+    out[N + pid] = C1 + T * bg[1];
+    out[2 * N + pid] = C2 + T * bg[2];
   }
 }
 
@@ -226,13 +258,15 @@ __device__ __forceinline__ int wave_max_int(int v) {
 
 constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
 
-template <bool kWrap>
+template <int TX, int TY, bool kWrap, bool kMulti>
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
-    int gx, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W, int H,
-    const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
+    int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
+    ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
     const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dL_dpix, float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
     float* __restrict__ dL_dopacity, float* __restrict__ dL_dcolors) {
+  const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
+  const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
   __shared__ uint32_t s_gid[kBatch];
   __shared__ float2 s_xy[kBatch];
@@ -246,14 +280,17 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), w = tid / kWave;
-  const int tile = blockIdx.x;
+  const int tile = loc.tile;
   int px, py;
-  pixel_of(tile, gx, tid, px, py);
+  pixel_of<TX, TY>(tile, gx, tid, px, py);
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
-  const uint2 range = ranges[tile];
+  const uint2 range = ranges[blockIdx.x];
   const size_t N = (size_t)W * H;
   const size_t pid = (size_t)py * W + px;
+  final_Ts += loc.pix;
+  n_contrib += loc.pix;
+  dL_dpix += 3 * loc.pix;
 
   const float T_final = inside ? final_Ts[pid] : 0.f;
   float T = T_final;
@@ -296,7 +333,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       s_rgb[0][tid] = rr.rgb.x;
       s_rgb[1][tid] = rr.rgb.y;
       s_rgb[2][tid] = rr.rgb.z;
-      s_mask[tid] = (uint8_t)quadrant_mask(m, co, tile, gx, kWrap ? Wf : 0.f);
+      s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
 #pragma unroll
     for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
@@ -384,18 +421,49 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   }
 }
 
+namespace {
+
+// Dispatches a kernel template over (tile shape, wrap).
+template <bool kMulti, template <int, int, bool, bool> class K, class... Args>
+hipError_t dispatch_shape(int bx, int by, bool wrap, unsigned blocks, hipStream_t s, Args... args) {
+#define OMNIGS_SHAPE(TX, TY)                                                                                \
+  if (bx == TX && by == TY) {                                                                               \
+    if (wrap)                                                                                               \
+      hipLaunchKernelGGL((K<TX, TY, true, kMulti>::fn), dim3(blocks), dim3(kTilePixels), 0, s, args...);   \
+    else                                                                                                    \
+      hipLaunchKernelGGL((K<TX, TY, false, kMulti>::fn), dim3(blocks), dim3(kTilePixels), 0, s, args...);  \
+    return hipGetLastError();                                                                               \
+  }
+  OMNIGS_SHAPE(16, 16)
+  OMNIGS_SHAPE(32, 8)
+  OMNIGS_SHAPE(8, 32)
+#undef OMNIGS_SHAPE
+  return hipErrorInvalidValue;
+}
+
+template <int TX, int TY, bool kWrap, bool kMulti>
+struct FwdK {
+  static constexpr auto fn = render_forward_kernel<TX, TY, kWrap, kMulti>;
+};
+template <int TX, int TY, bool kWrap, bool kMulti>
+struct BwdK {
+  static constexpr auto fn = render_backward_kernel<TX, TY, kWrap, kMulti>;
+};
+
+}  // namespace
+
+bool tile_shape_supported(int bx, int by) {
+  return (bx == 16 && by == 16) || (bx == 32 && by == 8) || (bx == 8 && by == 32);
+}
+
 hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                  const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
                                  float* out_color, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
-  if (grid.wrap)
-    hipLaunchKernelGGL(render_forward_kernel<true>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges, point_list,
-                       W, H, rec, final_T, n_contrib, bg, out_color);
-  else
-    hipLaunchKernelGGL(render_forward_kernel<false>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges,
-                       point_list, W, H, rec, final_T, n_contrib, bg, out_color);
-  return hipGetLastError();
+  ViewTable none{};
+  return dispatch_shape<false, FwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
+                                     point_list, W, H, none, rec, final_T, n_contrib, bg, out_color);
 }
 
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
@@ -404,15 +472,29 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
                                   float* dL_dcolors, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
-  if (grid.wrap)
-    hipLaunchKernelGGL(render_backward_kernel<true>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges,
-                       point_list, W, H, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity,
-                       dL_dcolors);
-  else
-    hipLaunchKernelGGL(render_backward_kernel<false>, dim3(tiles), dim3(kTilePixels), 0, s, grid.gx, ranges,
-                       point_list, W, H, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity,
-                       dL_dcolors);
-  return hipGetLastError();
+  ViewTable none{};
+  return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
+                                     point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D,
+                                     dL_dconic, dL_dopacity, dL_dcolors);
+}
+
+hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
+                                       const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
+                                       float* out_color, hipStream_t s) {
+  const uint32_t tiles = vt.tile_base[vt.V];
+  if (tiles == 0) return hipSuccess;
+  return dispatch_shape<true, FwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, rec,
+                                    final_T, n_contrib, bg, out_color);
+}
+
+hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
+                                        const float* bg, const float4* rec, const float* final_T,
+                                        const uint32_t* n_contrib, const float* dL_dpix, float* dL_dmean2D,
+                                        float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
+  const uint32_t tiles = vt.tile_base[vt.V];
+  if (tiles == 0) return hipSuccess;
+  return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
+                                    final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
 }
 
 }  // namespace omnigs

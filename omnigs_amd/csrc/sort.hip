@@ -138,10 +138,18 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, s
 //              and writes each to base[d] + row prefix + rank (stable).
 // counts holds kRadix * blocks row entries followed by kRadix digit totals.
 // ---------------------------------------------------------------------------
+// n: element count, or the capacity when n_dev holds the count on the device
+// (min(*n_dev, n) elements are sorted; blocks past it count nothing).
+__device__ __forceinline__ size_t live_count(size_t n, const uint32_t* n_dev) {
+  return n_dev ? min((size_t)*n_dev, n) : n;
+}
+
 __global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const uint32_t* __restrict__ keys, size_t n,
-                                                                   int shift, uint32_t mask,
-                                                                   uint32_t* __restrict__ counts, uint32_t nblocks) {
+                                                                   const uint32_t* __restrict__ n_dev, int shift,
+                                                                   uint32_t mask, uint32_t* __restrict__ counts,
+                                                                   uint32_t nblocks) {
   __shared__ uint32_t hist[kRadix];
+  n = live_count(n, n_dev);
   for (int d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d] = 0;
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * kSortTile;
@@ -172,8 +180,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_rowscan_kernel(uint32_t* _
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, size_t n, int shift, int bits, const uint32_t* __restrict__ counts,
-    uint32_t nblocks) {
+    uint32_t* __restrict__ vout, size_t n, const uint32_t* __restrict__ n_dev, int shift, int bits,
+    const uint32_t* __restrict__ counts, uint32_t nblocks) {
   constexpr int kWaves = kSortThreads / kWave;
   static_assert(kRadix == kSortThreads, "one digit per thread in the base scans");
   __shared__ uint32_t s_key[kSortTile];
@@ -185,7 +193,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
   const size_t base = (size_t)blockIdx.x * kSortTile;
-  const uint32_t nloc = (uint32_t)min((size_t)kSortTile, n - base);
+  n = live_count(n, n_dev);
+  const uint32_t nloc = base >= n ? 0u : (uint32_t)min((size_t)kSortTile, n - base);
 
   // keys and values of this block, kept in registers for the ranking rounds
   uint32_t keys[kSortItems], vals[kSortItems];
@@ -256,7 +265,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* /*partials*/,
-                             hipStream_t s, SortResult* out) {
+                             hipStream_t s, SortResult* out, const uint32_t* n_dev) {
   // Pass p reads (src) and writes (dst); the first pass reads vals_in (or
   // indices) and writes vals_alt, so an odd pass count ends in the *_alt arrays.
   uint32_t* ksrc = keys;
@@ -274,11 +283,11 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
   for (int shift = 0; shift < bits; shift += kRadixBits) {
     const int pb = bits - shift < kRadixBits ? bits - shift : kRadixBits;
     const uint32_t mask = (1u << pb) - 1u;
-    hipLaunchKernelGGL(radix_count_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, n, shift,
-                       mask, counts, nb);
+    hipLaunchKernelGGL(radix_count_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, n, n_dev,
+                       shift, mask, counts, nb);
     hipLaunchKernelGGL(radix_rowscan_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts, nb);
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, vsrc, kdst,
-                       vdst, n, shift, pb, (const uint32_t*)counts, nb);
+                       vdst, n, n_dev, shift, pb, (const uint32_t*)counts, nb);
     uint32_t* kt = ksrc;
     ksrc = kdst;
     kdst = kt;
@@ -315,20 +324,38 @@ __device__ __forceinline__ uint32_t first_greater(const uint32_t* __restrict__ a
   return lo;
 }
 
-__global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t R, const uint32_t* __restrict__ order,
+// kMulti: items of a batch (item i = Gaussian i % vt.P in view i / vt.P), each
+// keyed by its view's global tile id.  R_dev: instance count on the device
+// (clamped to R, the capacity); status[0] receives the count needed and
+// status[1] = 1 when it does not fit.
+template <bool kMulti>
+__global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t R, const uint32_t* __restrict__ R_dev,
+                                                                uint32_t* __restrict__ status,
+                                                                const uint32_t* __restrict__ order,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 const float4* __restrict__ rec,
                                                                 const int* __restrict__ radii, TileGrid grid,
-                                                                uint32_t* __restrict__ tile_key,
+                                                                ViewTable vt, uint32_t* __restrict__ tile_key,
                                                                 uint32_t* __restrict__ gid) {
+  constexpr int kViewSlots = kMulti ? kDupChunk : 1;
   __shared__ uint32_t s_end[kDupChunk];
   __shared__ uint32_t s_gid[kDupChunk];
   __shared__ int s_x0[kDupChunk];
   __shared__ uint16_t s_y0[kDupChunk], s_w[kDupChunk];
   __shared__ float s_inv_w[kDupChunk];
+  __shared__ uint8_t s_view[kViewSlots];
   __shared__ uint32_t s_span[2];
   const int tid = threadIdx.x;
+  if (R_dev) {
+    const uint32_t want = *R_dev;
+    if (status && blockIdx.x == 0 && tid == 0) {
+      status[0] = want;
+      status[1] = want > R ? 1u : 0u;
+    }
+    R = min(R, want);
+  }
   const uint32_t k0 = blockIdx.x * (uint32_t)kDupChunk;
+  if (k0 >= R) return;  // past the live instances (capacity-sized grids)
   const uint32_t k1 = min(R, k0 + (uint32_t)kDupChunk);
   if (tid == 0) {
     s_span[0] = first_greater(offsets, (uint32_t)P, k0);
@@ -344,7 +371,13 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     s_end[j] = offsets[s];
     s_gid[j] = gg;
     const float4 r0 = rec[3 * gg];
-    const Rect rc = tile_rect(make_float2(r0.x, r0.y), radii[gg], grid);
+    TileGrid gv = grid;
+    if constexpr (kMulti) {
+      const int v = (int)(gg / (uint32_t)vt.P);
+      s_view[j] = (uint8_t)v;
+      gv = vt.grid(v);
+    }
+    const Rect rc = tile_rect(make_float2(r0.x, r0.y), radii[gg], gv);
     const int w = max(1, rc.x1 - rc.x0);
     s_x0[j] = rc.x0;
     s_y0[j] = (uint16_t)rc.y0;
@@ -363,8 +396,13 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     if (r < 0) { --q; r += (int)w; }
     else if (r >= (int)w) { ++q; r -= (int)w; }
     const uint32_t y = (uint32_t)s_y0[j] + q;
-    const uint32_t x = (uint32_t)wrap_col(s_x0[j] + r, grid);
-    tile_key[k] = y * (uint32_t)grid.gx + x;
+    if constexpr (kMulti) {
+      const int v = s_view[j];
+      const TileGrid gv = vt.grid(v);
+      tile_key[k] = vt.tile_base[v] + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + r, gv);
+    } else {
+      tile_key[k] = y * (uint32_t)grid.gx + (uint32_t)wrap_col(s_x0[j] + r, grid);
+    }
     gid[k] = s_gid[j];
   }
 }
@@ -373,8 +411,19 @@ hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
-  hipLaunchKernelGGL(duplicate_kernel, dim3(blocks), dim3(kDupThreads), 0, s, P, R, order, offsets, rec, radii, grid,
-                     tile_key, gid);
+  ViewTable none{};
+  hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
+                     (uint32_t*)nullptr, order, offsets, rec, radii, grid, none, tile_key, gid);
+  return hipGetLastError();
+}
+
+hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
+                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
+  if (items == 0 || capacity == 0) return hipSuccess;
+  const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;
+  hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
+                     order, offsets, rec, radii, vt.grid(0), vt, tile_key, gid);
   return hipGetLastError();
 }
 
@@ -393,8 +442,10 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t R, uint2* __restrict__ ranges) {
+__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t R, const uint32_t* __restrict__ R_dev,
+                                   uint2* __restrict__ ranges) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (R_dev) R = min(R, *R_dev);
   if (k >= R) return;
   const uint32_t t = keys[k];
   if (k == 0) {
@@ -410,11 +461,11 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t R
 }
 
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
-                              hipStream_t s) {
+                              hipStream_t s, const uint32_t* R_dev) {
   hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(ranges), 2 * (size_t)tiles, 0u, s);
   if (e != hipSuccess || R == 0) return e;
   hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, sorted_tile_keys,
-                     (uint32_t)R, ranges);
+                     (uint32_t)R, R_dev, ranges);
   return hipGetLastError();
 }
 

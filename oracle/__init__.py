@@ -61,6 +61,8 @@ def _load():
     lib.oracle_higher_msb.argtypes = [ctypes.c_uint]
     lib.oracle_higher_msb.restype = ctypes.c_int
     lib.oracle_set_seam_wrap.argtypes = [vp, ctypes.c_int]
+    lib.oracle_set_threads.argtypes = [vp, ctypes.c_int]
+    lib.oracle_set_threads.restype = ctypes.c_int
     lib.oracle_time_fwd_bwd.argtypes = [vp, vp]
     lib.oracle_time_fwd_bwd.restype = ctypes.c_double
     _lib = lib
@@ -152,6 +154,10 @@ class Oracle:
         rc = _load().oracle_set(self.h, name.encode(), a.ctypes.data, a.size)
         if rc:
             raise RuntimeError(f"oracle_set({name}) -> {rc}")
+
+    def set_threads(self, n: int) -> int:
+        """Threads of the OpenMP CPU baseline (1 = serial oracle); returns the count in effect."""
+        return int(_load().oracle_set_threads(self.h, int(n)))
 
     def time_fwd_bwd(self, dL_dpix) -> float:
         d = np.ascontiguousarray(np.asarray(dL_dpix, dtype=self.ftype))

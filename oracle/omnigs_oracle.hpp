@@ -26,6 +26,10 @@
 #include <cstring>
 #include <numeric>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+#endif
 
 namespace omnigs_oracle {
 
@@ -159,6 +163,12 @@ template <class R> struct Inputs {
   const R* cov3d_pre = nullptr;     // [P,6] or null
   const R* view = nullptr;          // [16], Tcw^T column-major
   const R* campos = nullptr;        // [3]
+  // > 1: the OpenMP CPU baseline -- Gaussians, pixel rows and tiles in
+  // parallel, a parallel stable sort (same result), and the render backward
+  // reduced per Gaussian in sorted-instance order (the same sums for any
+  // thread count, rounded in another order than the serial oracle).  No
+  // decision signature is recorded in this mode.
+  int threads = 1;
 };
 
 template <class R> struct State {
@@ -291,6 +301,7 @@ template <class R> void preprocess(const Inputs<R>& in, State<R>& st) {
   st.clamped.assign(3 * P, 0);
   st.tiles_touched.assign(P, 0);
   V3<R> campos{in.campos[0], in.campos[1], in.campos[2]};
+#pragma omp parallel for schedule(static) if (in.threads > 1) num_threads(in.threads)
   for (int i = 0; i < P; ++i) {
     V3<R> p{in.means[3 * i], in.means[3 * i + 1], in.means[3 * i + 2]};
     V3<R> t = xform_point(p, in.view);
@@ -361,6 +372,7 @@ template <class R> void binning(const Inputs<R>& in, State<R>& st) {
   st.R_ = P ? st.offsets[P - 1] : 0;
   std::vector<uint64_t> keys(st.R_);
   std::vector<uint32_t> vals(st.R_);
+#pragma omp parallel for schedule(dynamic, 1024) if (in.threads > 1) num_threads(in.threads)
   for (int i = 0; i < P; ++i) {
     if (st.radii[i] <= 0) continue;
     uint32_t off = i == 0 ? 0 : st.offsets[i - 1];
@@ -383,8 +395,14 @@ template <class R> void binning(const Inputs<R>& in, State<R>& st) {
   const uint64_t mask = end_bit >= 64 ? ~0ULL : ((1ULL << end_bit) - 1);
   std::vector<uint32_t> perm(st.R_);
   std::iota(perm.begin(), perm.end(), 0u);
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](uint32_t a, uint32_t b) { return (keys[a] & mask) < (keys[b] & mask); });
+  auto by_key = [&](uint32_t a, uint32_t b) { return (keys[a] & mask) < (keys[b] & mask); };
+#ifdef _OPENMP
+  if (in.threads > 1) {
+    omp_set_num_threads(in.threads);
+    __gnu_parallel::stable_sort(perm.begin(), perm.end(), by_key);
+  } else
+#endif
+    std::stable_sort(perm.begin(), perm.end(), by_key);
   st.keys.resize(st.R_);
   st.point_list.resize(st.R_);
   for (uint32_t k = 0; k < st.R_; ++k) {
@@ -420,9 +438,14 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
   st.out_color.assign(kChannels * N, 0);
   st.margin.assign(N, R(1));
   uint64_t sig = 1469598103934665603ULL;
+  const bool par = in.threads > 1;
   sig_mix(sig, st.R_);
+#pragma omp parallel for schedule(dynamic, 1) if (par) num_threads(in.threads)
   for (int py = 0; py < H; ++py)
     for (int px = 0; px < W; ++px) {
+      auto mix = [&](uint64_t v) {
+        if (!par) sig_mix(sig, v);
+      };
       const int tile = (py / by) * gx + (px / bx);
       const uint32_t b = st.ranges[2 * tile], e = st.ranges[2 * tile + 1];
       const R fx = (R)px, fy = (R)py;
@@ -437,17 +460,17 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
         if (in.seam_wrap) dx = seam_dx(dx, W);
         const R* co = &st.conic[4 * g];
         const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
-        if (power > R(0)) { sig_mix(sig, 1); continue; }
+        if (power > R(0)) { mix(1); continue; }
         const R alpha = std::min(R(0.99), co[3] * std::exp(power));
         mg = std::min(mg, std::abs(alpha * R(255) - R(1)));
-        if (alpha < R(1) / R(255)) { sig_mix(sig, 2); continue; }
+        if (alpha < R(1) / R(255)) { mix(2); continue; }
         const R test_T = T * (R(1) - alpha);
         mg = std::min(mg, std::abs(test_T * R(10000) - R(1)));
-        if (test_T < R(0.0001)) { sig_mix(sig, 3); break; }
+        if (test_T < R(0.0001)) { mix(3); break; }
         for (int ch = 0; ch < kChannels; ++ch) C[ch] += feat[kChannels * g + ch] * alpha * T;
         T = test_T;
         last = contributor;
-        sig_mix(sig, 4 + k);
+        mix(4 + k);
       }
       const size_t pid = (size_t)py * W + px;
       st.final_T[pid] = T;
@@ -456,7 +479,7 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
       for (int ch = 0; ch < kChannels; ++ch) st.out_color[ch * N + pid] = C[ch] + T * in.bg[ch];
     }
   for (int i = 0; i < in.P; ++i) sig_mix(sig, (uint64_t)st.radii[i] * 131 + st.tiles_touched[i]);
-  st.decision_sig = sig;
+  st.decision_sig = par ? 0 : sig;
 }
 
 template <class R> void forward(const Inputs<R>& in, State<R>& st) {
@@ -479,59 +502,101 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
   st.dL_dopacity.assign(P, 0);
   st.dL_dcolors.assign(3 * (size_t)P, 0);
   const R ddelx_dx = R(0.5 * W), ddely_dy = R(0.5 * H);
-  for (int py = 0; py < H; ++py)
-    for (int px = 0; px < W; ++px) {
-      const int tile = (py / by) * gx + (px / bx);
-      const uint32_t b = st.ranges[2 * tile], e = st.ranges[2 * tile + 1];
-      const size_t pid = (size_t)py * W + px;
-      const R fx = (R)px, fy = (R)py;
-      const R T_final = st.final_T[pid];
-      R T = T_final;
-      const uint32_t last = st.n_contrib[pid];
-      R dpix[kChannels], accum[kChannels] = {0, 0, 0}, last_color[kChannels] = {0, 0, 0};
-      for (int ch = 0; ch < kChannels; ++ch) dpix[ch] = dL_dpix[ch * N + pid];
-      R last_alpha = 0;
-      R bg_dot = 0;
-      for (int ch = 0; ch < kChannels; ++ch) bg_dot += in.bg[ch] * dpix[ch];
-      // contributor index k - b (0-based); only those before last are blended.
-      for (uint32_t kk = e; kk > b; --kk) {
-        const uint32_t k = kk - 1;
-        if (k - b >= last) continue;
-        const uint32_t g = st.point_list[k];
-        R dx = st.xy[2 * g] - fx;
-        const R dy = st.xy[2 * g + 1] - fy;
-        if (in.seam_wrap) dx = seam_dx(dx, W);
-        const R* co = &st.conic[4 * g];
-        const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
-        if (power > R(0)) continue;
-        const R G = std::exp(power);
-        const R alpha = std::min(R(0.99), co[3] * G);
-        if (alpha < R(1) / R(255)) continue;
-        T = T / (R(1) - alpha);
-        const R dch_dcolor = alpha * T;
-        R dL_dalpha = 0;
-        for (int ch = 0; ch < kChannels; ++ch) {
-          const R c = col[kChannels * g + ch];
-          accum[ch] = last_alpha * last_color[ch] + (R(1) - last_alpha) * accum[ch];
-          last_color[ch] = c;
-          dL_dalpha += (c - accum[ch]) * dpix[ch];
-          st.dL_dcolors[3 * g + ch] += dch_dcolor * dpix[ch];
-        }
-        dL_dalpha *= T;
-        last_alpha = alpha;
-        dL_dalpha += (-T_final / (R(1) - alpha)) * bg_dot;
-        const R dL_dG = co[3] * dL_dalpha;
-        const R gdx = G * dx, gdy = G * dy;
-        const R dG_ddelx = -gdx * co[0] - gdy * co[1];
-        const R dG_ddely = -gdy * co[2] - gdx * co[1];
-        st.dL_dmean2D[3 * g] += dL_dG * dG_ddelx * ddelx_dx;
-        st.dL_dmean2D[3 * g + 1] += dL_dG * dG_ddely * ddely_dy;
-        st.dL_dconic[4 * g] += R(-0.5) * gdx * dx * dL_dG;
-        st.dL_dconic[4 * g + 1] += R(-0.5) * gdx * dy * dL_dG;
-        st.dL_dconic[4 * g + 3] += R(-0.5) * gdy * dy * dL_dG;
-        st.dL_dopacity[g] += G * dL_dalpha;
+  // One pixel's reverse traversal; sink(k, g, vals[9]) receives the
+  // contributions of sorted instance k (Gaussian g): dcolor x3, dmean2D x2,
+  // dconic (0, 1, 3) x3, dopacity.
+  auto pixel = [&](int px, int py, auto&& sink) {
+    const int tile = (py / by) * gx + (px / bx);
+    const uint32_t b = st.ranges[2 * tile], e = st.ranges[2 * tile + 1];
+    const size_t pid = (size_t)py * W + px;
+    const R fx = (R)px, fy = (R)py;
+    const R T_final = st.final_T[pid];
+    R T = T_final;
+    const uint32_t last = st.n_contrib[pid];
+    R dpix[kChannels], accum[kChannels] = {0, 0, 0}, last_color[kChannels] = {0, 0, 0};
+    for (int ch = 0; ch < kChannels; ++ch) dpix[ch] = dL_dpix[ch * N + pid];
+    R last_alpha = 0;
+    R bg_dot = 0;
+    for (int ch = 0; ch < kChannels; ++ch) bg_dot += in.bg[ch] * dpix[ch];
+    // contributor index k - b (0-based); only those before last are blended.
+    for (uint32_t kk = e; kk > b; --kk) {
+      const uint32_t k = kk - 1;
+      if (k - b >= last) continue;
+      const uint32_t g = st.point_list[k];
+      R dx = st.xy[2 * g] - fx;
+      const R dy = st.xy[2 * g + 1] - fy;
+      if (in.seam_wrap) dx = seam_dx(dx, W);
+      const R* co = &st.conic[4 * g];
+      const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
+      if (power > R(0)) continue;
+      const R G = std::exp(power);
+      const R alpha = std::min(R(0.99), co[3] * G);
+      if (alpha < R(1) / R(255)) continue;
+      T = T / (R(1) - alpha);
+      const R dch_dcolor = alpha * T;
+      R dL_dalpha = 0;
+      R v[9];
+      for (int ch = 0; ch < kChannels; ++ch) {
+        const R c = col[kChannels * g + ch];
+        accum[ch] = last_alpha * last_color[ch] + (R(1) - last_alpha) * accum[ch];
+        last_color[ch] = c;
+        dL_dalpha += (c - accum[ch]) * dpix[ch];
+        v[ch] = dch_dcolor * dpix[ch];
       }
+      dL_dalpha *= T;
+      last_alpha = alpha;
+      dL_dalpha += (-T_final / (R(1) - alpha)) * bg_dot;
+      const R dL_dG = co[3] * dL_dalpha;
+      const R gdx = G * dx, gdy = G * dy;
+      const R dG_ddelx = -gdx * co[0] - gdy * co[1];
+      const R dG_ddely = -gdy * co[2] - gdx * co[1];
+      v[3] = dL_dG * dG_ddelx * ddelx_dx;
+      v[4] = dL_dG * dG_ddely * ddely_dy;
+      v[5] = R(-0.5) * gdx * dx * dL_dG;
+      v[6] = R(-0.5) * gdx * dy * dL_dG;
+      v[7] = R(-0.5) * gdy * dy * dL_dG;
+      v[8] = G * dL_dalpha;
+      sink(k, g, v);
     }
+  };
+  auto add_to_gaussian = [&](uint32_t g, const R* v) {
+    for (int ch = 0; ch < kChannels; ++ch) st.dL_dcolors[3 * g + ch] += v[ch];
+    st.dL_dmean2D[3 * g] += v[3];
+    st.dL_dmean2D[3 * g + 1] += v[4];
+    st.dL_dconic[4 * g] += v[5];
+    st.dL_dconic[4 * g + 1] += v[6];
+    st.dL_dconic[4 * g + 3] += v[7];
+    st.dL_dopacity[g] += v[8];
+  };
+  if (in.threads <= 1) {
+    for (int py = 0; py < H; ++py)
+      for (int px = 0; px < W; ++px) pixel(px, py, [&](uint32_t, uint32_t g, const R* v) { add_to_gaussian(g, v); });
+    return;
+  }
+  // Tile-parallel: every sorted instance belongs to one tile, so per-instance
+  // partial sums have a single writer; then each Gaussian adds its instances
+  // in sorted order.
+  const int gy = (H + by - 1) / by;
+  std::vector<R> part((size_t)st.R_ * 9, R(0));
+#pragma omp parallel for schedule(dynamic, 1) num_threads(in.threads)
+  for (int t = 0; t < gx * gy; ++t) {
+    const int tx = t % gx, ty = t / gx;
+    for (int py = ty * by; py < std::min(H, (ty + 1) * by); ++py)
+      for (int px = tx * bx; px < std::min(W, (tx + 1) * bx); ++px)
+        pixel(px, py, [&](uint32_t k, uint32_t, const R* v) {
+          for (int j = 0; j < 9; ++j) part[(size_t)k * 9 + j] += v[j];
+        });
+  }
+  std::vector<uint32_t> first(P + 1, 0), idx(st.R_);
+  for (uint32_t k = 0; k < st.R_; ++k) ++first[st.point_list[k] + 1];
+  for (int g = 0; g < P; ++g) first[g + 1] += first[g];
+  {
+    std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+    for (uint32_t k = 0; k < st.R_; ++k) idx[fill[st.point_list[k]]++] = k;
+  }
+#pragma omp parallel for schedule(static) num_threads(in.threads)
+  for (int g = 0; g < P; ++g)
+    for (uint32_t j = first[g]; j < first[g + 1]; ++j) add_to_gaussian((uint32_t)g, &part[(size_t)idx[j] * 9]);
 }
 
 // computeCov2DLonLatCUDA, backward.cu:297-485: dL/dconic -> dL/dcov3D and the
@@ -543,6 +608,7 @@ template <class R> void cov2d_backward(const Inputs<R>& in, State<R>& st) {
   st.dpx_dt.assign(3 * (size_t)P, 0);
   st.dpy_dt.assign(3 * (size_t)P, 0);
   const R* view = in.view;
+#pragma omp parallel for schedule(static) if (in.threads > 1) num_threads(in.threads)
   for (int i = 0; i < P; ++i) {
     if (!(st.radii[i] > 0)) continue;
     const R* c3 = in.cov3d_pre ? in.cov3d_pre + 6 * i : &st.cov3d[6 * i];
@@ -721,6 +787,7 @@ template <class R> void preprocess_backward(const Inputs<R>& in, State<R>& st) {
   st.dL_drot.assign(4 * (size_t)P, 0);
   V3<R> campos{in.campos[0], in.campos[1], in.campos[2]};
   const R dsx_dpx = R(2) / (R)in.W, dsy_dpy = R(2) / (R)in.H;
+#pragma omp parallel for schedule(static) if (in.threads > 1) num_threads(in.threads)
   for (int i = 0; i < P; ++i) {
     if (!(st.radii[i] > 0)) continue;
     const R dL_dpx = st.dL_dmean2D[3 * i] * dsx_dpx;

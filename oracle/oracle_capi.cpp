@@ -23,7 +23,7 @@ struct HandleBase {
 
 template <class R> struct Handle : HandleBase {
   // owned copies of every input (so the caller's buffers may go away)
-  int P = 0, D = 0, M = 0, W = 0, H = 0, bx = 16, by = 16, seam_wrap = 0;
+  int P = 0, D = 0, M = 0, W = 0, H = 0, bx = 16, by = 16, seam_wrap = 0, threads = 1;
   std::vector<R> bg, means, shs, colors, opac, scales, rots, cov3d_pre, view, campos;
   bool has_shs = false, has_colors = false, has_scales = false, has_rots = false, has_cov = false;
   R mod = 1;
@@ -31,6 +31,7 @@ template <class R> struct Handle : HandleBase {
   Inputs<R> inputs() const {
     Inputs<R> in;
     in.P = P; in.D = D; in.M = M; in.W = W; in.H = H; in.block_x = bx; in.block_y = by; in.seam_wrap = seam_wrap;
+    in.threads = threads;
     in.bg = bg.data(); in.means = means.data(); in.opac = opac.data();
     in.shs = has_shs ? shs.data() : nullptr;
     in.colors = has_colors ? colors.data() : nullptr;
@@ -143,6 +144,19 @@ void oracle_destroy(void* h) { delete static_cast<HandleBase*>(h); }
 // Extension switch: 1 = wrap tile columns and dx around the longitude seam.
 void oracle_set_seam_wrap(void* h, int on) {
   dispatch(h, [&](auto& hd) { hd.seam_wrap = on ? 1 : 0; return 0; });
+}
+
+// Threads of the OpenMP CPU baseline (1 = the serial oracle).  Returns the
+// count in effect (1 when built without OpenMP).
+int oracle_set_threads(void* h, int n) {
+#ifdef _OPENMP
+  const int t = n < 1 ? 1 : n;
+#else
+  const int t = 1;
+  (void)n;
+#endif
+  dispatch(h, [&](auto& hd) { hd.threads = t; return 0; });
+  return t;
 }
 
 // stage: 0 preprocess, 1 binning, 2 render, 3 render_backward (needs dL_dpix),

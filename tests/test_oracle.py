@@ -338,3 +338,29 @@ def test_fd_gradients_precomputed_inputs(oracle_lib, which):
         assert abs(fd - an) <= 2e-4 * abs(an) + 1e-6 * np.abs(g).max() + 1e-9, (which, k, fd, an)
         checked += 1
     assert checked >= 0.7 * len(idx)
+
+
+def test_openmp_baseline_matches_serial_and_is_thread_count_independent(oracle_lib):
+    """The multi-threaded CPU baseline computes the serial oracle's results:
+    forward bit-identical, backward equal up to summation order, and the same
+    bits for any thread count."""
+    from oracle import Oracle
+    sc = scenes.small_scene(P=3000, width=256, height=128, seed=6)
+    sc = {k: (v.astype(np.float64) if isinstance(v, np.ndarray) and v.dtype == np.float32 else v)
+          for k, v in sc.items()}
+    dL = sc["dL_dpix"]
+    ser = Oracle(sc, 64).forward().backward(dL)
+    outs = []
+    for t in (2, 4):
+        o = Oracle(sc, 64)
+        if o.set_threads(t) < 2:
+            pytest.skip("oracle built without OpenMP")
+        o.forward().backward(dL)
+        outs.append(o)
+    for name in ("out_color", "final_T", "n_contrib", "point_list", "radii"):
+        np.testing.assert_array_equal(outs[0].get(name), ser.get(name), err_msg=name)
+    for name in ("dL_dmean2D", "dL_dconic", "dL_dopacity", "dL_dcolors", "dL_dmeans3D", "dL_dsh", "dL_dscales",
+                 "dL_drot"):
+        a, b = outs[0].get(name), ser.get(name)
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * max(np.abs(b).max(), 1e-300), err_msg=name)
+        np.testing.assert_array_equal(outs[0].get(name), outs[1].get(name), err_msg=name)
