@@ -60,6 +60,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--gaussians", type=int, default=1_500_000)
     ap.add_argument("--no-decoder", action="store_true", help="skip the learned-colour leg")
+    ap.add_argument("--batch", action="store_true",
+                    help="only the batched multi-view workload: 1.5M Gaussians, 8/16/32 views of uneven size, "
+                         "autotuned tile shape and views per launch (views/sec)")
     ap.add_argument("--memory-compare", action="store_true",
                     help="only the checkpointed-vs-stored backward comparison (3M Gaussians at 4096x2048)")
     ap.add_argument("--distill-steps", type=int, default=2000)
@@ -82,6 +85,8 @@ def main() -> int:
         return copy_calibration(dev, args.steps)
     if args.memory_compare:
         return memory_compare(dev, args)
+    if args.batch:
+        return batch_workload(dev, args)
     sc = scenes.room_scene(P=args.gaussians)
     P, W, H = sc["means3D"].shape[0], sc["width"], sc["height"]
     N = W * H
@@ -144,7 +149,8 @@ def main() -> int:
     }
 
     if not args.no_decoder:
-        out["mlp_colour"] = decoder_leg(sc, ds, bufs, target, l1, dL, args, dev)
+        out["mlp_colour"], dec = decoder_leg(sc, ds, bufs, target, l1, dL, args, dev)
+        out["batched_mlp_colour"] = batched_decoder_leg(sc, ds, dec, args, dev)
     if not args.no_cpu:
         target = sc["target"]
         out["cpu_baseline"] = cpu_baseline(sc)
@@ -270,7 +276,95 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
         "decoder_backward_mfma": {"flop": flop_per_pass, "ms": round(dec_ms, 5),
                                   "TFLOPs": round(flop_per_pass / (dec_ms * 1e-3) / 1e12, 2) if dec_ms else None,
                                   "peak_TFLOPs_bf16": 2500.0},
-    }
+    }, dec
+
+
+def random_views(rng, V, sizes_choice, cube: float):
+    """V seeded equirectangular views: size drawn from sizes_choice, random
+    rotation, centre uniform in a cube of side `cube` m around the origin."""
+    import numpy as np
+
+    from omnigs_amd import scenes
+    sizes, vms, cps = [], [], []
+    for _ in range(V):
+        sizes.append(tuple(sizes_choice[rng.integers(len(sizes_choice))]))
+        R = random_rotation(rng)
+        c = rng.uniform(-cube / 2, cube / 2, size=3)
+        vms.append(scenes.viewmatrix_from_pose(R, -R @ c))
+        cps.append(c.astype(np.float32))
+    return sizes, np.stack(vms).astype(np.float32), np.stack(cps).astype(np.float32)
+
+
+def batched_decoder_leg(sc, ds, dec, args, dev) -> dict:
+    """The batched + learned-colour form on the timed workload: 8 views of
+    1920x960 of the 1.5M-Gaussian scene per launch (poses in a 1 m cube), one
+    captured graph per step (forward, fused L1 gradient, backward with the
+    decoder backward), ms per view."""
+    import numpy as np
+    import torch
+
+    from omnigs_amd.batch import build_launches, capture, make_step, time_graph
+
+    P = sc["means3D"].shape[0]
+    V = 8
+    rng = np.random.default_rng(77)
+    sizes, vms, cps = random_views(rng, V, [(1920, 960)], 1.0)
+    inputs = dict(means3D=ds["means3D"], opacities=ds["opacities"], background=ds["background"], shs=ds["shs"],
+                  scales=ds["scales"], rotations=ds["rotations"])
+    target = torch.rand((3 * sum(w * h for w, h in sizes),), generator=torch.Generator(device=dev).manual_seed(3),
+                        device=dev)
+    out = {}
+    for name, mlp in (("sh", None), ("decoder", dec.mlp_args())):
+        rs, ts, lfs, dls = build_launches(P, sizes, (16, 16), V, inputs, torch.from_numpy(vms).to(dev),
+                                          torch.from_numpy(cps).to(dev), target, dev, mlp)
+        g = capture(make_step(rs, ts, lfs, dls), warmup=args.warmup)
+        ms = time_graph(g, args.steps)
+        assert not any(r.overflowed() for r in rs)
+        out[name] = {"ms_per_step": round(ms, 3), "ms_per_view": round(ms / V, 4),
+                     "views_per_sec": round(1000.0 * V / ms, 1), "instances": rs[0].needed()[0]}
+        del g, rs, ts, lfs, dls
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return {"workload": f"{V} views of 1920x960 per launch, 1.5M Gaussians, tile 16x16, hipGraph replay",
+            **out, "colour_path_used": "decoder only if its quality gate passed (see mlp_colour)"}
+
+
+def batch_workload(dev, args) -> int:
+    """Batched multi-view fwd+bwd: 1.5M Gaussians with the small-scene
+    distributions, V = 8, 16, 32 views with sizes drawn from {960x480,
+    1920x960, 2048x1024} and poses in a 2 m cube; for each V the autotuner
+    times every tile shape and views-per-launch as a captured graph and keeps
+    the fastest.  Prints one JSON line."""
+    import numpy as np
+    import torch
+
+    from omnigs_amd import scenes
+    from omnigs_amd.batch import autotune
+
+    P = args.gaussians
+    sc = scenes.small_scene(P=P, width=64, height=32, seed=0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    inputs = dict(means3D=t(sc["means3D"]), opacities=t(sc["opacities"].reshape(-1)),
+                  background=t(sc["background"]), shs=t(sc["shs"]), scales=t(sc["scales"]),
+                  rotations=t(sc["rotations"]))
+    choice = [(960, 480), (1920, 960), (2048, 1024)]
+    rows = {}
+    for V in (8, 16, 32):
+        rng = np.random.default_rng(100 + V)
+        sizes, vms, cps = random_views(rng, V, choice, 2.0)
+        target = torch.rand((3 * sum(w * h for w, h in sizes),), generator=torch.Generator(device=dev).manual_seed(V),
+                            device=dev)
+        res = autotune(P, sizes, inputs, t(vms), t(cps), target, device=dev, reps=max(3, args.steps // 4),
+                       views_per_launch=tuple(v for v in (1, 2, 4, 8, 16, 32) if v <= V),
+                       log=lambda r: print(json.dumps(r), file=sys.stderr, flush=True))
+        rows[str(V)] = {"best_tile": f"{res.tile[0]}x{res.tile[1]}", "best_views_per_launch": res.views_per_launch,
+                        "views_per_sec": round(res.views_per_sec, 1), "ms_per_view": round(res.ms_per_view, 4),
+                        "pixels": sum(w * h for w, h in sizes), "table": res.table}
+    print(json.dumps({"metric": "batched fwd+bwd views/sec (autotuned)", "unit": "views/s",
+                      "config": {"workload": "1.5M Gaussians, uneven equirectangular views, L1 step per launch",
+                                 "gaussians": P, "sizes_from": ["960x480", "1920x960", "2048x1024"]},
+                      "results": rows}))
+    return 0
 
 
 def memory_compare(dev, args) -> int:
