@@ -294,6 +294,23 @@ int omnigs_adam_step(const omnigs_adam_group* groups, int n_groups, float beta1,
 /* FP32 -> BF16 (round to nearest even). */
 int omnigs_f32_to_bf16(const float* in, uint16_t* out, size_t n, void* stream);
 
+/* 3DGS PLY files (host memory), the vertex layout the reference's
+ * GaussianModel::savePly / loadPly write and read through tinyply
+ * (src/gaussian_model.cpp:860-1069): x y z nx ny nz f_dc_0..2
+ * f_rest_0..(3K-1) opacity scale_0..2 rot_0..3, float32, normals zero,
+ * f_rest channel-major (f_rest_{c K + j} = features_rest[p][j][c],
+ * K = (sh_degree + 1)^2 - 1); raw (pre-activation) opacity, scale, rotation.
+ * features_dc is [P,1,3], features_rest [P,K,3], opacity [P], scaling [P,3],
+ * rotation [P,4].  The writer emits binary little-endian files with
+ * tinyply's header text; the reader takes any property order, scalar type
+ * and format (binary either byte order, or ASCII). */
+int omnigs_ply_write(const char* path, long long P, int sh_degree, const float* xyz, const float* features_dc,
+                     const float* features_rest, const float* opacity, const float* scaling, const float* rotation);
+int omnigs_ply_count(const char* path, long long* P);
+int omnigs_ply_read(const char* path, long long P, int sh_degree, float* xyz, float* features_dc,
+                    float* features_rest, float* opacity, float* scaling, float* rotation);
+const char* omnigs_ply_last_error(void);
+
 /* Byte sizes the forward requests from each buffer callback. */
 size_t omnigs_lonlat_geometry_bytes(int P);
 size_t omnigs_lonlat_binning_bytes(int R);

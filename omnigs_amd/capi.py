@@ -29,7 +29,8 @@ EXPORTED_SYMBOLS = (
     "omnigs_photometric_loss", "omnigs_mlp_param_count", "omnigs_mlp_packed_bytes", "omnigs_mlp_pack",
     "omnigs_mlp_eval", "omnigs_mlp_workspace_bytes", "omnigs_mlp_distill_step", "omnigs_mlp_seeded_direction",
     "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
-    "omnigs_lonlat_batch_backward",
+    "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
+    "omnigs_ply_last_error",
 )
 NUM_STAGES = 11
 

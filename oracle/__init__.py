@@ -28,11 +28,16 @@ _INT_ARRAYS = {"radii": np.int32, "clamped": np.uint8, "tiles_touched": np.uint3
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile (g++ only)."""
+    """Compile liboracle.so with the committed Makefile (g++ only), and the
+    reference-side tools under _ref/ where the reference sources exist."""
     if force and os.path.exists(_LIB_PATH):
         os.remove(_LIB_PATH)
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
     return _LIB_PATH
+
+
+PLY_REF = os.path.join(_HERE, "_ref", "ply_ref")
 
 
 def _load():
