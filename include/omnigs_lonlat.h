@@ -291,6 +291,22 @@ typedef struct omnigs_adam_group {
 } omnigs_adam_group;
 int omnigs_adam_step(const omnigs_adam_group* groups, int n_groups, float beta1, float beta2, void* stream);
 
+/* Densification statistics of one training step in one pass (the reference's
+ * max_radii2D update, gaussian_mapper.cpp:419-428, and
+ * GaussianModel::addDensificationStats, gaussian_model.cpp:839-853): for
+ * radii[i] > 0, max_radii2D[i] = max(max_radii2D[i], radii[i]),
+ * xyz_gradient_accum[i] += |dL_dmean2D[i].xy|, denom[i] += 1.  dL_dmean2D is
+ * the backward's [P,3] screen-space gradient; the other arrays are [P]. */
+int omnigs_densification_stats(int P, const int* radii, const float* dL_dmean2D, float* max_radii2D,
+                               float* xyz_gradient_accum, float* denom, void* stream);
+
+/* Position learning rate of GaussianModel::exponLrFunc
+ * (gaussian_model.cpp:1140-1153): log-linear from lr_init at step 0 to
+ * lr_final at max_steps, with the optional sine warm-up from lr_delay_mult
+ * over lr_delay_steps; 0 for step < 0 or lr_init = lr_final = 0. */
+float omnigs_expon_lr(int step, float lr_init, float lr_final, int lr_delay_steps, float lr_delay_mult,
+                      int max_steps);
+
 /* FP32 -> BF16 (round to nearest even). */
 int omnigs_f32_to_bf16(const float* in, uint16_t* out, size_t n, void* stream);
 

@@ -24,7 +24,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(ROOT, "_build", "obj")
 HIP_LIB = os.path.join(LIBDIR, "libomnigs_lonlat.so")
 
-HIP_SOURCES = ["preprocess.hip", "sort.hip", "render.hip", "preprocess_backward.hip", "loss.hip", "mlp.hip", "ply.hip", "capi.hip"]
+HIP_SOURCES = ["preprocess.hip", "sort.hip", "render.hip", "preprocess_backward.hip", "loss.hip", "mlp.hip", "ply.hip", "train.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [

@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "omnigs_mlp_eval", "omnigs_mlp_workspace_bytes", "omnigs_mlp_distill_step", "omnigs_mlp_seeded_direction",
     "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
     "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
-    "omnigs_ply_last_error",
+    "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr",
 )
 NUM_STAGES = 11
 
@@ -192,6 +192,11 @@ def load() -> ctypes.CDLL:
     lib.omnigs_lonlat_batch_backward.argtypes = [ctypes.POINTER(BatchArgs), ctypes.POINTER(BatchGrads),
                                                  ctypes.c_void_p]
     lib.omnigs_lonlat_batch_backward.restype = ctypes.c_int
+    lib.omnigs_densification_stats.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6
+    lib.omnigs_densification_stats.restype = ctypes.c_int
+    lib.omnigs_expon_lr.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_int]
+    lib.omnigs_expon_lr.restype = ctypes.c_float
     _lib = lib
     return lib
 

@@ -638,6 +638,21 @@ int omnigs_adam_step(const omnigs_adam_group* groups, int n, float beta1, float 
   return OMNIGS_OK;
 }
 
+int omnigs_densification_stats(int P, const int* radii, const float* dL_dmean2D, float* max_radii2D,
+                               float* xyz_gradient_accum, float* denom, void* stream) {
+  if (P < 0) return fail_arg("P < 0");
+  if (P > 0 && (!radii || !dL_dmean2D || !max_radii2D || !xyz_gradient_accum || !denom))
+    return fail_arg("missing pointer");
+  OMNIGS_TRY(launch_densify_stats(P, radii, dL_dmean2D, max_radii2D, xyz_gradient_accum, denom, (hipStream_t)stream),
+             "densification stats");
+  return OMNIGS_OK;
+}
+
+float omnigs_expon_lr(int step, float lr_init, float lr_final, int lr_delay_steps, float lr_delay_mult,
+                      int max_steps) {
+  return expon_lr(step, lr_init, lr_final, lr_delay_steps, lr_delay_mult, max_steps);
+}
+
 int omnigs_f32_to_bf16(const float* in, uint16_t* out, size_t n, void* stream) {
   if (n > 0 && (!in || !out)) return fail_arg("missing pointer");
   OMNIGS_TRY(launch_to_bf16(in, out, n, (hipStream_t)stream), "to bf16");

@@ -146,6 +146,11 @@ struct AdamGroupTable {
 hipError_t launch_adam(const AdamGroupTable& tab, float beta1, float beta2, hipStream_t s);
 hipError_t launch_to_bf16(const float* in, uint16_t* out, size_t n, hipStream_t s);
 
+// train.hip
+hipError_t launch_densify_stats(int P, const int* radii, const float* dmean2D, float* max_radii2D, float* accum,
+                                float* denom, hipStream_t s);
+float expon_lr(int step, float lr_init, float lr_final, int lr_delay_steps, float lr_delay_mult, int max_steps);
+
 // loss.hip
 // Floats of workspace the fused loss needs for a [C,H,W] image with `skip`
 // bottom rows dropped (0 when nothing is kept).
