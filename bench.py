@@ -43,9 +43,11 @@ def algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes):
         "ranges": 4 * R + 8 * tiles,
         # per instance: id 4 + xy 8 + conic 16 + rgb 12; per pixel 3 colours + T + n_contrib written
         "render": 40 * R + 8 * tiles + 20 * N,
-        "clear_grads": 44 * P,
         # per instance gather 40; per pixel T, n_contrib, dL/dpix read; 9 gradient floats per visible Gaussian
         "render_backward": 40 * R + 8 * tiles + 20 * N + 36 * Pv,
+        # per Gaussian: rec 32 + tiles_touched 4 read, dmean2D 12 + dconic 16 + dopacity 4 + dcolor 12 written
+        # (the per-instance records between the two stages are this design's own traffic, not counted)
+        "gather_grads": 80 * P,
         # read: means, scales, rot, SH, radii, clamped, dmean2D, dconic, dcolor (277 B);
         # write: dmeans, dcov3D, dSH, dscale, drot (256 B)
         "preprocess_backward": 533 * P,

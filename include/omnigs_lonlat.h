@@ -98,13 +98,19 @@ typedef struct omnigs_lonlat_backward_args {
   const float* viewmatrix;     /* [16] */
   const float* campos;         /* [3] */
   const int* radii;            /* [P] radii from the forward, or NULL (internal copy) */
-  char* geom_buffer;           /* the forward's three scratch buffers */
+  char* geom_buffer;           /* the forward's three scratch buffers; the backward also keeps its
+                                  per-instance gradient records in the binning buffer and a per-tile
+                                  word in the image buffer, so two backward calls on one forward's
+                                  buffers must not run concurrently */
   char* binning_buffer;
   char* image_buffer;
   const float* dL_dpix;        /* [3,H,W] upstream gradient of out_color */
-  /* Outputs.  The first four are accumulated by the render backward and are
-   * cleared by this call first; the rest are fully written (zero for
-   * Gaussians with radius 0). */
+  /* Outputs, every entry written (zero for Gaussians that touch no tile or
+   * have radius 0).  The first four are the per-Gaussian sums of the render
+   * backward's per-(tile, Gaussian) terms (gathered per Gaussian, without
+   * device-scope atomics; within a tile the waves combine their sums with
+   * LDS atomics, so the last bits can vary between runs, as with the
+   * reference's atomics). */
   float* dL_dmean2D;           /* [P,3] NDC units (x 0.5W, 0.5H), z unused */
   float* dL_dconic;            /* [P,4] (2x2 row-major; slots 0,1,3 used, .y holds half the off-diagonal) */
   float* dL_dopacity;          /* [P] */
@@ -371,7 +377,7 @@ enum omnigs_stage {
   OMNIGS_STAGE_TILE_SORT = 4,
   OMNIGS_STAGE_RANGES = 5,
   OMNIGS_STAGE_RENDER = 6,
-  OMNIGS_STAGE_CLEAR_GRADS = 7,
+  OMNIGS_STAGE_GATHER_GRADS = 7, /* per-Gaussian sum of the render backward's per-instance records */
   OMNIGS_STAGE_RENDER_BACKWARD = 8,
   OMNIGS_STAGE_PREPROCESS_BACKWARD = 9,
   OMNIGS_STAGE_DECODER_BACKWARD = 10,

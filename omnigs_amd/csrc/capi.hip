@@ -129,7 +129,7 @@ int omnigs_profile_read(double* ms, int* count) {
 
 const char* omnigs_stage_name(int stage) {
   static const char* names[OMNIGS_NUM_STAGES] = {"preprocess", "depth_sort", "scan", "duplicate", "tile_sort",
-                                                 "ranges", "render", "clear_grads", "render_backward",
+                                                 "ranges", "render", "gather_grads", "render_backward",
                                                  "preprocess_backward", "decoder_backward"};
   return (stage >= 0 && stage < OMNIGS_NUM_STAGES) ? names[stage] : "unknown";
 }
@@ -328,26 +328,29 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
 
   // The point list ends in gid (even pass count) or gid_alt (odd).
   const uint32_t* point_list = nullptr;
+  float4* grad_rec = nullptr;
+  float* grad_opacity = nullptr;
   if (R > 0) {
     const BinState b = carve_bin(a->binning_buffer, R);
     const int passes = radix_passes(bit_width_u32((uint32_t)grid.tiles()));
     point_list = (passes % 2) ? b.gid_alt : b.gid;
-  }
-
-  {
-    StageTimer tm(OMNIGS_STAGE_CLEAR_GRADS, s);
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dmean2D), 3 * (size_t)P, 0u, s), "clear");
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dconic), 4 * (size_t)P, 0u, s), "clear");
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dopacity), (size_t)P, 0u, s), "clear");
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dcolor), 3 * (size_t)P, 0u, s), "clear");
+    grad_rec = b.grad_rec;
+    grad_opacity = b.grad_opacity;
   }
 
   if (R > 0) {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward(grid, img.ranges, point_list, W, H, a->background, g.rec, img.final_T,
-                                      img.n_contrib, a->dL_dpix, a->dL_dmean2D, a->dL_dconic,
-                                      a->dL_dopacity, a->dL_dcolor, s),
+                                      img.n_contrib, a->dL_dpix, grad_rec, grad_opacity, img.live_first, s),
                "render backward");
+  }
+  {
+    // every entry of the four per-Gaussian arrays is written (zeros where no
+    // tile touched the Gaussian), so they need no clearing
+    StageTimer tm(OMNIGS_STAGE_GATHER_GRADS, s);
+    OMNIGS_TRY(launch_gather_grads(P, (uint32_t)R, g.rec, g.tiles_touched, grid, img.live_first, grad_rec,
+                                   grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor, s),
+               "gather grads");
   }
   PreprocessBackwardArgs pb{P, a->D, a->M, W, H, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,
                             a->rotations,
@@ -377,6 +380,7 @@ struct BatchState {
   float* final_T;
   uint32_t* n_contrib;
   uint2* ranges;
+  uint32_t* live_first;  // per global tile
   float* dmean2D;  // per item [3]
   float* dconic;   // per item [4]
   float* dopacity; // per item
@@ -396,6 +400,7 @@ BatchState carve_batch(char* base, size_t items, size_t pixels, size_t tiles, si
   st.final_T = c.take<float>(pixels);
   st.n_contrib = c.take<uint32_t>(pixels);
   st.ranges = c.take<uint2>(tiles);
+  st.live_first = c.take<uint32_t>(tiles);
   st.dmean2D = c.take<float>(3 * items);
   st.dconic = c.take<float>(4 * items);
   st.dopacity = c.take<float>(items);
@@ -544,18 +549,17 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
   const BinState& b = bs.b;
   const uint32_t* point_list = (radix_passes(bit_width_u32(vt.tile_base[vt.V])) % 2) ? b.gid_alt : b.gid;
   {
-    StageTimer tm(OMNIGS_STAGE_CLEAR_GRADS, s);
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dmean2D), 3 * items, 0u, s), "clear");
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dconic), 4 * items, 0u, s), "clear");
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dopacity), items, 0u, s), "clear");
-    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(bs.dcolor), 3 * items, 0u, s), "clear");
-  }
-  {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward_views(vt, bs.ranges, point_list, a->background, g.rec, bs.final_T,
-                                            bs.n_contrib, gr->dL_dpix, bs.dmean2D, bs.dconic, bs.dopacity, bs.dcolor,
-                                            s),
+                                            bs.n_contrib, gr->dL_dpix, b.grad_rec, b.grad_opacity, bs.live_first, s),
                "batch render backward");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_GATHER_GRADS, s);
+    OMNIGS_TRY(launch_gather_grads_views(vt, a->capacity, g.offsets + (items - 1), g.rec, g.tiles_touched,
+                                         bs.live_first, b.grad_rec, b.grad_opacity, bs.dmean2D, bs.dconic, bs.dopacity,
+                                         bs.dcolor, s),
+               "batch gather grads");
   }
   PreprocessBackwardArgs pb{P, a->D, a->M, 0, 0, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,
                             a->rotations, a->cov3D_precomp, nullptr, nullptr, g.radii, g.clamped, bs.dmean2D,

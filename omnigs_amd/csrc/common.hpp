@@ -97,6 +97,15 @@ __device__ __forceinline__ Rect tile_rect(float2 p, int r, TileGrid g) {
 // Tile column of rect column x (identity in clamp mode).
 __device__ __forceinline__ int wrap_col(int x, TileGrid g) { return g.wrap ? ((x % g.gx) + g.gx) % g.gx : x; }
 
+// Position of tile (tx, ty) in a Gaussian's rect enumeration -- rows
+// [y0, y1), columns x0 .. x0 + w - 1 read modulo gx in seam-wrap mode -- which
+// is the order the duplicate pass lays out the Gaussian's instances in.
+__device__ __forceinline__ uint32_t rect_local(const Rect& rc, int w, int tx, int ty, TileGrid g) {
+  int c = tx - rc.x0;
+  if (g.wrap) c = ((c % g.gx) + g.gx) % g.gx;
+  return (uint32_t)((ty - rc.y0) * w + c);
+}
+
 // ---------------------------------------------------------------------------
 // Buffer layouts.  The three byte buffers are private to this library; the
 // backward re-derives every array's offset from (P), (R, tiles) and (W, H),
@@ -140,8 +149,10 @@ struct GeomState {
   uint32_t* depth_key;     // float bits of the camera distance (sort key); ~0 if culled
   int32_t* radii;          // internal radii (used when the caller passes none)
   // Render record, 48 B per Gaussian so one gather fetches everything the
-  // tile kernels need: {x, y, A, B}, {C, opacity, r, g}, {b, 0, 0, 0}
-  // (pixel-space mean, conic, opacity, colour -- SH-derived or precomputed).
+  // tile kernels need: {x, y, A, B}, {C, opacity, r, g}, {b, radius, first, 0}
+  // (pixel-space mean, conic, opacity, colour -- SH-derived or precomputed --,
+  // the radius as int bits, and `first`, the index of the Gaussian's first
+  // instance in the depth-ordered expansion, written by the duplicate pass).
   float4* rec;
   uint8_t* clamped;        // bit c set when channel c was clamped at 0
   uint32_t* tiles_touched; // rect area
@@ -172,12 +183,17 @@ inline GeomState carve_geom(char* base, size_t P) {
   return g;
 }
 
-// Per-instance state: tile keys and Gaussian ids, ping-pong for the tile sort.
+// Per-instance state: tile keys and Gaussian ids, ping-pong for the tile sort;
+// the backward's per-instance gradient terms, indexed like the (unsorted)
+// depth-ordered expansion so that each Gaussian's terms are contiguous.
+constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
 struct BinState {
   uint32_t* tile_key;
   uint32_t* tile_key_alt;
   uint32_t* gid;
   uint32_t* gid_alt;
+  float4* grad_rec;       // [R][2]: dcolor x3, dmean2D x2, dconic x3 (32 B, one sector)
+  float* grad_opacity;    // [R]: dopacity
   uint32_t* sort_counts;  // [kRadix, blocks]
   uint32_t* scan_partials;
   size_t bytes;
@@ -192,11 +208,14 @@ __device__ __forceinline__ RenderRec load_rec(const float4* __restrict__ rec, ui
   const float4 a = rec[3 * g], b = rec[3 * g + 1], c = rec[3 * g + 2];
   return {make_float2(a.x, a.y), make_float4(a.z, a.w, b.x, b.y), make_float3(b.z, b.w, c.x)};
 }
-__device__ __forceinline__ void store_rec(float4* __restrict__ rec, uint32_t g, float2 xy, float4 co, float3 rgb) {
+__device__ __forceinline__ void store_rec(float4* __restrict__ rec, uint32_t g, float2 xy, float4 co, float3 rgb,
+                                          int radius) {
   rec[3 * g] = make_float4(xy.x, xy.y, co.x, co.y);
   rec[3 * g + 1] = make_float4(co.z, co.w, rgb.x, rgb.y);
-  rec[3 * g + 2] = make_float4(rgb.z, 0.f, 0.f, 0.f);
+  rec[3 * g + 2] = make_float4(rgb.z, __int_as_float(radius), 0.f, 0.f);
 }
+// Words 9 and 10 of a record (radius, first instance).
+constexpr int kRecRadiusWord = 9, kRecFirstWord = 10;
 
 inline BinState carve_bin(char* base, size_t R) {
   BinState b;
@@ -205,6 +224,8 @@ inline BinState carve_bin(char* base, size_t R) {
   b.tile_key_alt = c.take<uint32_t>(R);
   b.gid = c.take<uint32_t>(R);
   b.gid_alt = c.take<uint32_t>(R);
+  b.grad_rec = c.take<float4>(2 * (size_t)R);
+  b.grad_opacity = c.take<float>(R);
   b.sort_counts = c.take<uint32_t>((size_t)kRadix * (sort_blocks(R) + 1));
   b.scan_partials = c.take<uint32_t>(1);
   b.bytes = align_up(c.off) + kAlign;
@@ -216,6 +237,10 @@ struct ImgState {
   float* final_T;
   uint32_t* n_contrib;
   uint2* ranges;
+  // Per tile, written by the backward: 1 + `first` of the deepest instance it
+  // staged (the tile's lists run in depth order, so exactly the instances
+  // whose Gaussian's first < this value got a gradient record); 0: none.
+  uint32_t* live_first;
   size_t bytes;
 };
 
@@ -226,6 +251,7 @@ inline ImgState carve_img(char* base, int W, int H) {
   s.final_T = c.take<float>(N);
   s.n_contrib = c.take<uint32_t>(N);
   s.ranges = c.take<uint2>((size_t)tile_grid(W, H).tiles());
+  s.live_first = c.take<uint32_t>((size_t)tile_grid(W, H).tiles());
   s.bytes = align_up(c.off) + kAlign;
   return s;
 }

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
 
   g.depth_key[idx] = __float_as_uint(o.r);
   radii_out[idx] = o.radius;
-  store_rec(g.rec, idx, o.pix, o.conic, col);
+  store_rec(g.rec, idx, o.pix, o.conic, col, o.radius);
   g.tiles_touched[idx] = (uint32_t)o.area;
 }
 
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
     }
     g.depth_key[item] = __float_as_uint(o.r);
     radii_out[item] = o.radius;
-    store_rec(g.rec, item, o.pix, o.conic, col);
+    store_rec(g.rec, item, o.pix, o.conic, col, o.radius);
     g.tiles_touched[item] = (uint32_t)o.area;
   }
 }

@@ -12,9 +12,11 @@
 //   stop before blending once T (1 - alpha) < 1e-4
 //   out = sum c alpha T + T_final bg;  n_contrib = 1-based index of the last blended instance.
 // The backward walks each tile's list from the back, recovers T by division,
-// and reduces every Gaussian's nine gradient terms across the wave with DPP
-// adds, then across the four waves in LDS, before one global atomic per
-// (tile, Gaussian, term).
+// and reduces every Gaussian's nine gradient terms across the wave with
+// permlane/DPP adds (two instances per reduce-scatter), then across the four
+// waves in LDS.  Each (tile, Gaussian) instance's terms go to its own record
+// with plain stores, and a per-Gaussian pass (gather_grads) sums the records,
+// in place of device-scope atomics on the per-Gaussian arrays.
 #include "kernels.hpp"
 
 namespace omnigs {
@@ -250,30 +252,71 @@ __device__ __forceinline__ float wave_reduce1(float v) {
   return c;
 }
 
+// Reduce-scatter of two instances' eight terms (a[0..7] of instance A, b[0..7]
+// of instance B) in one pass: on return each 4-lane group holds one complete
+// sum, instance (lane >> 5) and term ((lane >> 2) & 7).  35 VALU operations
+// for 16 sums, against 18 for eight with wave_reduce8.
+__device__ __forceinline__ float wave_reduce16(const float a[8], const float b[8], int lane) {
+  // lanes 0-31 carry A's terms, lanes 32-63 B's
+  float s1[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s1[k] = fold32(a[k], b[k]);
+  // 16-lane rows: 0 -> A t[k], 1 -> A t[k+4], 2 -> B t[k], 3 -> B t[k+4]
+  float s2[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) s2[k] = fold16(s1[k], s1[k + 4]);
+  // half rows: lanes 0-7 keep t[k], lanes 8-15 keep t[k+2]
+  const bool h8 = (lane >> 3) & 1;
+  float s3[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const float keep = h8 ? s2[k + 2] : s2[k], give = h8 ? s2[k] : s2[k + 2];
+    s3[k] = keep + dpp<0x128>(give);  // row_ror:8
+  }
+  // quarter rows: lanes 0-3 of each 8 keep t[0 or 2 (+4)], lanes 4-7 t[1 or 3 (+4)]
+  const bool h4 = (lane >> 2) & 1;
+  const float keep = h4 ? s3[1] : s3[0], give = h4 ? s3[0] : s3[1];
+  float c = keep + dpp<0x141>(give);  // row_half_mirror: lane i <-> 7 - i
+  c += dpp<0x4e>(c);                  // quad_perm [2,3,0,1]
+  c += dpp<0xb1>(c);                  // quad_perm [1,0,3,2]
+  return c;
+}
+
+// Full wave sums of two terms: lanes 0-31 receive sum(x), lanes 32-63 sum(y).
+__device__ __forceinline__ float wave_reduce2(float x, float y) {
+  float c = fold32(x, y);
+  c = fold16(c, c);
+  c += dpp<0x128>(c);
+  c += dpp<0x141>(c);
+  c += dpp<0x4e>(c);
+  c += dpp<0xb1>(c);
+  return c;
+}
+
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) v = max(v, __shfl_xor(v, o, kWave));
   return v;
 }
 
-constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
+constexpr int kAccStride = kBatch + 1;  // padded so one instance's terms sit in distinct banks
 
 template <int TX, int TY, bool kWrap, bool kMulti>
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
     ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
-    const uint32_t* __restrict__ n_contrib,
-    const float* __restrict__ dL_dpix, float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
-    float* __restrict__ dL_dopacity, float* __restrict__ dL_dcolors) {
+    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, float4* __restrict__ grad_rec,
+    float* __restrict__ grad_opacity, uint32_t* __restrict__ live_first) {
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
-  __shared__ uint32_t s_gid[kBatch];
+  const TileGrid tg{gx, (H + TY - 1) / TY, kWrap ? 1 : 0, TX, TY};
+  __shared__ uint32_t s_inst[kBatch];  // index of the staged instance in the depth-ordered expansion
   __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_co[kBatch];
   __shared__ float4 s_q[kBatch];  // prescaled conic, as in the forward
   __shared__ float s_rgb[kChannels][kBatch];
-  __shared__ float s_acc[kGradTerms][kBatch];
+  __shared__ float s_acc[kGradTerms][kAccStride];
   __shared__ uint8_t s_mask[kBatch];
   __shared__ uint8_t s_list[kWavesPerTile][kBatch];
   __shared__ int s_nlive;
@@ -304,6 +347,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   if (lane == 0) atomicMax(&s_nlive, wlast);
   __syncthreads();
   const int n = s_nlive;
+  if (tid == 0 && n == 0) live_first[blockIdx.x] = 0u;
   float dp0 = 0.f, dp1 = 0.f, dp2 = 0.f;
   if (inside) {
     dp0 = dL_dpix[pid];
@@ -313,9 +357,56 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   const float ddelx_dx = 0.5f * (float)W, ddely_dy = 0.5f * (float)H;  // NDC scale of dL/dmean2D
   // -T_final (bg . dL/dpix): the background term of dL/dalpha is this times 1/(1 - alpha)
   const float neg_tb = -T_final * (bg[0] * dp0 + bg[1] * dp1 + bg[2] * dp2);
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-  float last_alpha = 0.f;
-  float lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
+  // The reference carries the colour accumulated behind each instance per
+  // channel, acc_c <- la lc_c + (1 - la) acc_c, and only ever uses
+  // sum_c (c_c - acc_c) dL/dpix_c; the same recurrence on the dot product
+  // s = acc . dL/dpix (with lcd = lc . dL/dpix) gives that sum directly.
+  float acc_dot = 0.f, last_alpha = 0.f, last_cd = 0.f;
+
+  // One (wave, instance) pair: updates the lanes' blending state and returns,
+  // wave-uniformly, whether any lane blends it.  If one does, (wgt, v) are
+  // G dL/dalpha and alpha T on blending lanes (zero elsewhere) and (dx, dy)
+  // the offsets; the nine moments follow from these four (moments()).
+  // `lim` = n - 1 - base - last: position n - 1 - (base + j) < last <=> j > lim.
+  auto eval = [&](int j, int lim, float& wgt, float& v, float& dx, float& dy) -> bool {
+    const float2 m = s_xy[j];
+    const float4 q = s_q[j];
+    dx = seam_dx<kWrap>(m.x - fx, Wf, invW);
+    dy = m.y - fy;
+    const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
+    const float G = exp2_hw(power2);
+    const float alpha = fminf(0.99f, q.w * G);
+    const bool ok = j > lim && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
+    if (__builtin_amdgcn_ballot_w64(ok) == 0ull) return false;
+    wgt = 0.f;
+    v = 0.f;
+    if (ok) {
+      // T and 1/(1 - alpha) only scale gradients here (no decision depends
+      // on them), so the hardware reciprocal replaces two IEEE divisions.
+      const float r = __builtin_amdgcn_rcpf(1.f - alpha);
+      T = T * r;
+      v = alpha * T;
+      const float cd = s_rgb[0][j] * dp0 + s_rgb[1][j] * dp1 + s_rgb[2][j] * dp2;
+      acc_dot = last_alpha * last_cd + (1.f - last_alpha) * acc_dot;
+      const float dL_dalpha = (cd - acc_dot) * T + neg_tb * r;
+      last_alpha = alpha;
+      last_cd = cd;
+      wgt = G * dL_dalpha;
+    }
+    return true;
+  };
+  // per-pixel moments: dL/dpix-weighted v for the colours, w dx, w dy, w dx^2, w dx dy, w dy^2
+  auto moments = [&](float wgt, float v, float dx, float dy, float t[8]) {
+    const float wdx = wgt * dx, wdy = wgt * dy;
+    t[0] = v * dp0;
+    t[1] = v * dp1;
+    t[2] = v * dp2;
+    t[3] = wdx;
+    t[4] = wdy;
+    t[5] = wdx * dx;
+    t[6] = wdx * dy;
+    t[7] = wdy * dy;
+  };
 
   for (int base = 0; base < n; base += kBatch) {
     __syncthreads();
@@ -326,7 +417,13 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const float2 m = rr.xy;
       const float4 co = rr.co;
       const float3 q = prescale_conic(co);
-      s_gid[tid] = g;
+      // this instance's slot in the expansion: the Gaussian's first instance
+      // plus the tile's position in its rect (the duplicate pass's order)
+      const float4 r2 = rec[3 * g + 2];
+      const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
+      const uint32_t first = __float_as_uint(r2.z);
+      s_inst[tid] = first + rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
+      if (k == 0) live_first[blockIdx.x] = first + 1u;  // position n - 1, the deepest staged
       s_xy[tid] = m;
       s_co[tid] = co;
       s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
@@ -343,53 +440,40 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     // slot j holds position n - 1 - (base + j); keep positions < wlast
     const int first_slot = n - base - wlast;
     const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane, first_slot);
-    for (int ii = 0; ii < len; ++ii) {
-      const int j = s_list[w][ii];
-      // 0-based position of this instance in the tile's front-to-back list
-      const int pos = n - 1 - (base + j);
-      const float2 m = s_xy[j];
-      const float4 q = s_q[j];
-      const float dx = seam_dx<kWrap>(m.x - fx, Wf, invW), dy = m.y - fy;
-      const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
-      const float G = exp2_hw(power2);
-      const float alpha = fminf(0.99f, q.w * G);
-      const bool ok = inside && pos < last && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
-      // per-pixel moments: w = G dL/dalpha, v = alpha T
-      float m_wdx = 0.f, m_wdy = 0.f, m_wdx2 = 0.f, m_wdxdy = 0.f, m_wdy2 = 0.f, m_w = 0.f;
-      float m_v0 = 0.f, m_v1 = 0.f, m_v2 = 0.f;
-      if (ok) {
-        // T and 1/(1 - alpha) only scale gradients here (no decision depends
-        // on them), so the hardware reciprocal replaces two IEEE divisions.
-        const float r = __builtin_amdgcn_rcpf(1.f - alpha);
-        T = T * r;
-        const float v = alpha * T;
-        const float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
-        const float one_minus_la = 1.f - last_alpha;
-        acc0 = last_alpha * lc0 + one_minus_la * acc0;
-        acc1 = last_alpha * lc1 + one_minus_la * acc1;
-        acc2 = last_alpha * lc2 + one_minus_la * acc2;
-        const float dL_dalpha = ((c0 - acc0) * dp0 + (c1 - acc1) * dp1 + (c2 - acc2) * dp2) * T + neg_tb * r;
-        last_alpha = alpha;
-        lc0 = c0;
-        lc1 = c1;
-        lc2 = c2;
-        const float wgt = G * dL_dalpha;
-        m_wdx = wgt * dx;
-        m_wdy = wgt * dy;
-        m_wdx2 = m_wdx * dx;
-        m_wdxdy = m_wdx * dy;
-        m_wdy2 = m_wdy * dy;
-        m_w = wgt;
-        m_v0 = v * dp0;
-        m_v1 = v * dp1;
-        m_v2 = v * dp2;
+    // Pairs that some lane blends are reduced two at a time (wave_reduce16);
+    // a last odd one alone.  The list order (back to front) is kept.
+    // lanes outside the image have last = 0, so no slot passes their limit
+    const int lim = n - 1 - base - last;
+    int ii = 0;
+    for (;;) {
+      float wa, va, xa, ya, wb, vb, xb, yb;
+      int ja = 0, jb = 0;
+      bool fa = false, fb = false;
+      while (ii < len && !fa) {
+        ja = s_list[w][ii++];
+        fa = eval(ja, lim, wa, va, xa, ya);
       }
-      if (__ballot(ok) == 0ull) continue;  // wave-uniform
-      const float t[8] = {m_v0, m_v1, m_v2, m_wdx, m_wdy, m_wdx2, m_wdxdy, m_wdy2};
-      const float grp = wave_reduce8(t, lane);
-      const float sw = wave_reduce1(m_w);
-      if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][j], grp);
-      if (lane == 0) atomicAdd(&s_acc[8][j], sw);
+      if (!fa) break;
+      while (ii < len && !fb) {
+        jb = s_list[w][ii++];
+        fb = eval(jb, lim, wb, vb, xb, yb);
+      }
+      float ta[8];
+      moments(wa, va, xa, ya, ta);
+      if (!fb) {
+        const float grp = wave_reduce8(ta, lane);
+        const float sw = wave_reduce1(wa);
+        if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][ja], grp);
+        if (lane == 0) atomicAdd(&s_acc[8][ja], sw);
+        break;
+      }
+      float tb[8];
+      moments(wb, vb, xb, yb, tb);
+      const float grp = wave_reduce16(ta, tb, lane);
+      const float sw = wave_reduce2(wa, wb);
+      const int jj = (lane & 32) ? jb : ja;
+      if ((lane & 3) == 0) atomicAdd(&s_acc[(lane >> 2) & 7][jj], grp);
+      if ((lane & 31) == 0) atomicAdd(&s_acc[8][jj], sw);
     }
     __syncthreads();
     if (tid < cnt) {
@@ -410,15 +494,176 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
                                    -0.5f * co.w * swdxdy,
                                    -0.5f * co.w * swdy2,
                                    sw};
-      const uint32_t g = s_gid[tid];
-      float* dst[kGradTerms] = {&dL_dcolors[3 * g],     &dL_dcolors[3 * g + 1], &dL_dcolors[3 * g + 2],
-                                &dL_dmean2D[3 * g],     &dL_dmean2D[3 * g + 1], &dL_dconic[4 * g],
-                                &dL_dconic[4 * g + 1],  &dL_dconic[4 * g + 3],  &dL_dopacity[g]};
-#pragma unroll
-      for (int qq = 0; qq < kGradTerms; ++qq)
-        if (v[qq] != 0.f) unsafeAtomicAdd(dst[qq], v[qq]);
+      // plain stores, zeros included: every staged instance owns its record
+      const size_t j = s_inst[tid];
+      grad_rec[2 * j] = make_float4(v[0], v[1], v[2], v[3]);
+      grad_rec[2 * j + 1] = make_float4(v[4], v[5], v[6], v[7]);
+      grad_opacity[j] = v[8];
     }
   }
+}
+
+// Per-Gaussian (per-item) sum of the backward's gradient records: the
+// instances of item i are the contiguous slots [first, first + tiles_touched)
+// of the expansion, one per tile of its rect in the duplicate pass's order; a
+// slot holds a record iff the tile's backward staged it (first < live_first of
+// the tile).  Items with at most kGatherSerial instances are summed by their
+// own lane; larger ones (a few Gaussians cover thousands of tiles) one at a
+// time by the whole wave, lanes striding over the instances, so no lane walks
+// a long list alone.  Writes every output entry, zeros included, in the
+// reference's layouts: dL_dmean2D [.,3] (z = 0), dL_dconic [.,2,2] ((1,0) = 0),
+// dL_dopacity [.], dL_dcolors [.,3].  R (with R_dev, the live count) bounds
+// the slots read.
+constexpr uint32_t kGatherSerial = 16;
+constexpr int kGatherGroup = 8;  // slots fetched per round trip
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+template <bool kMulti>
+__global__ __launch_bounds__(256) void gather_grads_kernel(int items, uint32_t R, const uint32_t* __restrict__ R_dev,
+                                                           const float4* __restrict__ rec,
+                                                           const uint32_t* __restrict__ tiles_touched, TileGrid grid,
+                                                           ViewTable vt, const uint32_t* __restrict__ live_first,
+                                                           const float4* __restrict__ grad_rec,
+                                                           const float* __restrict__ grad_opacity,
+                                                           float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
+                                                           float* __restrict__ dL_dopacity,
+                                                           float* __restrict__ dL_dcolors) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool valid = i < items;  // every lane stays to the end: the wave-wide path needs them all
+  if (R_dev) R = min(R, *R_dev);
+  const bool wrap = (kMulti ? vt.wrap : grid.wrap) != 0;
+  uint32_t cnt = valid ? tiles_touched[i] : 0u;
+  uint32_t first = 0, tile_base = 0;
+  int x0 = 0, y0 = 0, w = 1, gx = 1;
+  if (cnt > 0) {
+    TileGrid gv = grid;
+    if constexpr (kMulti) {
+      const int v = i / vt.P;
+      gv = vt.grid(v);
+      tile_base = vt.tile_base[v];
+    }
+    const float4 r0 = rec[3 * i], r2 = rec[3 * i + 2];
+    const Rect rc = tile_rect(make_float2(r0.x, r0.y), __float_as_int(r2.y), gv);
+    x0 = rc.x0;
+    y0 = rc.y0;
+    w = max(1, rc.x1 - rc.x0);
+    gx = gv.gx;
+    first = __float_as_uint(r2.z);
+  }
+  // Slot (first + l) sits at tile (y, x0 + c).  A group of slots is fetched
+  // in two rounds, each with all of its loads in flight: the tiles'
+  // live_first, then the records of the slots that hold one (first <
+  // live_first).  Tile 0 stands in for slots out of range.
+  auto tile_at = [&](int y, int c, int x0_, int gx_, uint32_t tb) {
+    int col = x0_ + c;
+    if (wrap) col = ((col % gx_) + gx_) % gx_;
+    return tb + (uint32_t)(y * gx_ + col);
+  };
+  auto add_group = [&](const bool* in, const uint32_t* tt, const uint32_t* jj, uint32_t fst, float* a) {
+    bool ok[kGatherGroup];
+#pragma unroll
+    for (int u = 0; u < kGatherGroup; ++u) ok[u] = in[u] && fst < live_first[tt[u]];
+    float4 r0[kGatherGroup], r1[kGatherGroup];
+    float ro[kGatherGroup];
+#pragma unroll
+    for (int u = 0; u < kGatherGroup; ++u) {
+      r0[u] = r1[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      ro[u] = 0.f;
+      if (ok[u]) {
+        r0[u] = grad_rec[2 * (size_t)jj[u]];
+        r1[u] = grad_rec[2 * (size_t)jj[u] + 1];
+        ro[u] = grad_opacity[jj[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherGroup; ++u) {
+      a[0] += r0[u].x; a[1] += r0[u].y; a[2] += r0[u].z; a[3] += r0[u].w;
+      a[4] += r1[u].x; a[5] += r1[u].y; a[6] += r1[u].z; a[7] += r1[u].w;
+      a[8] += ro[u];
+    }
+  };
+
+  float acc[kGradTerms];
+#pragma unroll
+  for (int q = 0; q < kGradTerms; ++q) acc[q] = 0.f;
+  if (cnt <= kGatherSerial) {
+    int y = y0, c = 0;
+    for (uint32_t l0 = 0; l0 < cnt; l0 += kGatherGroup) {
+      bool in[kGatherGroup];
+      uint32_t tt[kGatherGroup], jj[kGatherGroup];
+#pragma unroll
+      for (int u = 0; u < kGatherGroup; ++u) {
+        const uint32_t l = l0 + u, j = first + l;
+        in[u] = l < cnt && j < R;
+        tt[u] = in[u] ? tile_at(y, c, x0, gx, tile_base) : 0u;
+        jj[u] = in[u] ? j : 0u;
+        if (++c == w) {
+          c = 0;
+          ++y;
+        }
+      }
+      add_group(in, tt, jj, first, acc);
+    }
+  }
+  uint64_t big = __ballot(cnt > kGatherSerial);
+  while (big) {
+    const int src = __ffsll((unsigned long long)big) - 1;
+    big &= big - 1;
+    const uint32_t bcnt = readlane_u32(cnt, src), bfirst = readlane_u32(first, src);
+    const uint32_t btb = readlane_u32(tile_base, src);
+    const int bx0 = (int)readlane_u32((uint32_t)x0, src), by0 = (int)readlane_u32((uint32_t)y0, src);
+    const int bw = (int)readlane_u32((uint32_t)w, src), bgx = (int)readlane_u32((uint32_t)gx, src);
+    const float inv_w = 1.0f / (float)bw;
+    float part[kGradTerms];
+#pragma unroll
+    for (int q = 0; q < kGradTerms; ++q) part[q] = 0.f;
+    for (uint32_t l0 = lane; l0 < bcnt; l0 += kWave * kGatherGroup) {
+      bool in[kGatherGroup];
+      uint32_t tt[kGatherGroup], jj[kGatherGroup];
+#pragma unroll
+      for (int u = 0; u < kGatherGroup; ++u) {
+        const uint32_t l = l0 + (uint32_t)(u * kWave), j = bfirst + l;
+        // l / w via a float estimate and one integer correction (exact for l < 2^23)
+        int qy = (int)((float)l * inv_w);
+        int c = (int)l - qy * bw;
+        if (c < 0) {
+          --qy;
+          c += bw;
+        } else if (c >= bw) {
+          ++qy;
+          c -= bw;
+        }
+        in[u] = l < bcnt && j < R;
+        tt[u] = in[u] ? tile_at(by0 + qy, c, bx0, bgx, btb) : 0u;
+        jj[u] = in[u] ? j : 0u;
+      }
+      add_group(in, tt, jj, bfirst, part);
+    }
+    const float grp = wave_reduce8(part, lane);  // term q in lanes 8q .. 8q + 7
+    const float sw = wave_reduce1(part[8]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float tot = __uint_as_float(readlane_u32(__float_as_uint(grp), 8 * q));
+      if (lane == src) acc[q] = tot;
+    }
+    if (lane == src) acc[8] = sw;
+  }
+  if (!valid) return;
+  dL_dcolors[3 * (size_t)i] = acc[0];
+  dL_dcolors[3 * (size_t)i + 1] = acc[1];
+  dL_dcolors[3 * (size_t)i + 2] = acc[2];
+  dL_dmean2D[3 * (size_t)i] = acc[3];
+  dL_dmean2D[3 * (size_t)i + 1] = acc[4];
+  dL_dmean2D[3 * (size_t)i + 2] = 0.f;
+  dL_dconic[4 * (size_t)i] = acc[5];
+  dL_dconic[4 * (size_t)i + 1] = acc[6];
+  dL_dconic[4 * (size_t)i + 2] = 0.f;
+  dL_dconic[4 * (size_t)i + 3] = acc[7];
+  dL_dopacity[i] = acc[8];
 }
 
 namespace {
@@ -468,14 +713,26 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
 
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
-                                  const float* dL_dpix, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
-                                  float* dL_dcolors, hipStream_t s) {
+                                  const float* dL_dpix, float4* grad_rec, float* grad_opacity, uint32_t* live_first,
+                                  hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
   ViewTable none{};
   return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
-                                     point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, dL_dmean2D,
-                                     dL_dconic, dL_dopacity, dL_dcolors);
+                                     point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, grad_rec,
+                                     grad_opacity, live_first);
+}
+
+hipError_t launch_gather_grads(int P, uint32_t R, const float4* rec, const uint32_t* tiles_touched, TileGrid grid,
+                               const uint32_t* live_first, const float4* grad_rec, const float* grad_opacity,
+                               float* dL_dmean2D, float* dL_dconic, float* dL_dopacity, float* dL_dcolors,
+                               hipStream_t s) {
+  if (P == 0) return hipSuccess;
+  ViewTable none{};
+  hipLaunchKernelGGL(gather_grads_kernel<false>, dim3((P + 255) / 256), dim3(256), 0, s, P, R,
+                     (const uint32_t*)nullptr, rec, tiles_touched, grid, none, live_first, grad_rec, grad_opacity,
+                     dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+  return hipGetLastError();
 }
 
 hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
@@ -489,12 +746,24 @@ hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges,
 
 hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                         const float* bg, const float4* rec, const float* final_T,
-                                        const uint32_t* n_contrib, const float* dL_dpix, float* dL_dmean2D,
-                                        float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
+                                        const uint32_t* n_contrib, const float* dL_dpix, float4* grad_rec,
+                                        float* grad_opacity, uint32_t* live_first, hipStream_t s) {
   const uint32_t tiles = vt.tile_base[vt.V];
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
-                                    final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+                                    final_T, n_contrib, dL_dpix, grad_rec, grad_opacity, live_first);
+}
+
+hipError_t launch_gather_grads_views(const ViewTable& vt, uint32_t capacity, const uint32_t* R_dev,
+                                     const float4* rec, const uint32_t* tiles_touched, const uint32_t* live_first,
+                                     const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
+                                     float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
+  const int items = vt.V * vt.P;
+  if (items == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_grads_kernel<true>, dim3((items + 255) / 256), dim3(256), 0, s, items, capacity, R_dev,
+                     rec, tiles_touched, vt.grid(0), vt, live_first, grad_rec, grad_opacity, dL_dmean2D, dL_dconic,
+                     dL_dopacity, dL_dcolors);
+  return hipGetLastError();
 }
 
 }  // namespace omnigs

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ status,
                                                                 const uint32_t* __restrict__ order,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                const float4* __restrict__ rec,
+                                                                float4* __restrict__ rec,
                                                                 const int* __restrict__ radii, TileGrid grid,
                                                                 ViewTable vt, uint32_t* __restrict__ tile_key,
                                                                 uint32_t* __restrict__ gid) {
@@ -370,6 +370,8 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     const uint32_t gg = order[s];
     s_end[j] = offsets[s];
     s_gid[j] = gg;
+    // the Gaussian's first instance, kept in its record for the backward
+    reinterpret_cast<uint32_t*>(rec)[12 * (size_t)gg + kRecFirstWord] = s == 0 ? 0u : offsets[s - 1];
     const float4 r0 = rec[3 * gg];
     TileGrid gv = grid;
     if constexpr (kMulti) {
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   }
 }
 
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, float4* rec,
                             const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
@@ -418,7 +420,7 @@ hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint
 }
 
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
-                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const uint32_t* order, const uint32_t* offsets, float4* rec, const int* radii,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
   if (items == 0 || capacity == 0) return hipSuccess;
   const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;

@@ -229,6 +229,34 @@ def test_end_to_end_against_fp32_and_fp64(gpu, P, W, H, seed):
         assert f32 <= FLIP_GRAD_FRACTION, (mine, f32)
 
 
+def test_backward_repeatable_with_records_rewritten(gpu):
+    """Each backward writes every staged (tile, Gaussian) record before the
+    per-Gaussian gather reads them, so a backward with another dL/dpix in
+    between leaves nothing stale: repeated calls on one forward agree with
+    each other and with a backward on a fresh forward.  (The four waves of a
+    tile combine their sums with LDS atomics, so agreement is to float
+    rounding, not bitwise; a stale record would be off by a whole term.)"""
+    import torch
+    from tests.gpu_util import announce, run_backward
+    sc = scenes.small_scene(P=20000, width=512, height=256, seed=4)
+    ds, R, img, radii, bufs, st = _setup(sc)
+    dL1 = torch.from_numpy(sc["dL_dpix"]).to(gpu)
+    dL2 = torch.flip(dL1, dims=[2]).contiguous()
+    announce("repeatable backward: 20000 Gaussians 512x256 seed 4")
+    g1 = {k: v.clone() for k, v in run_backward(ds, R, radii, bufs, dL1).items()}
+    g2 = {k: v.clone() for k, v in run_backward(ds, R, radii, bufs, dL2).items()}
+    g1b = run_backward(ds, R, radii, bufs, dL1)
+    ds2, R2, _, radii2, bufs2, _ = _setup(sc)
+    g2b = run_backward(ds2, R2, radii2, bufs2, dL2)
+    assert R2 == R
+    for k in g1:
+        for x, y in ((g1[k], g1b[k]), (g2[k], g2b[k])):
+            scale = max(float(y.abs().max()), 1e-30)
+            assert float((x - y).abs().max()) <= 1e-5 * scale, k
+    d12 = float((g1["dL_dmeans2D"] - g2["dL_dmeans2D"]).abs().max())
+    assert d12 > 1e-3 * float(g1["dL_dmeans2D"].abs().max())
+
+
 def test_edge_cases_empty_and_all_culled(gpu):
     import torch
     from tests.gpu_util import np_, run_backward, run_forward, to_dev

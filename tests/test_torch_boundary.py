@@ -90,8 +90,10 @@ def test_boundary_matches_c_abi(gpu):
     for t, name, shp in zip(out, names, shapes):
         assert tuple(t.shape) == shp, name
         ref = g[name].reshape(shp)
-        # float atomics reorder between runs: equal to rounding
-        assert torch.allclose(t, ref, rtol=1e-5, atol=1e-6 * float(ref.abs().max()) + 1e-30), name
+        # the four waves of a tile add their sums into LDS in a run-dependent
+        # order, so two runs agree to rounding, amplified where an entry is
+        # a cancelling sum (rotation gradients): 1e-5 of the tensor's scale
+        assert torch.allclose(t, ref, rtol=1e-5, atol=1e-5 * float(ref.abs().max()) + 1e-30), name
     assert bool(markVisible(ds["means3D"], ds["viewmatrix"], ds["viewmatrix"], 3).all())
 
 
