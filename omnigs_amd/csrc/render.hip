@@ -170,26 +170,38 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     __syncthreads();
     const int cnt = min(kBatch, n - base);
     const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane);
-    for (int ii = 0; ii < len && !done; ++ii) {
-      const int j = s_list[w][ii];
-      const float2 m = s_xy[j];
-      const float4 q = s_q[j];
+    // Branch-free over the wave's list: every lane evaluates every pair and
+    // the reference's skip / stop decisions become selects, so the loop stays
+    // wave-uniform; it ends once every lane of the wave is done.  The next
+    // pair's operands are fetched from LDS while the current one computes.
+    int j = s_list[w][0];  // a stale slot when len == 0: never used
+    float2 m = s_xy[j];
+    float4 q = s_q[j];
+    float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
+    for (int ii = 0; ii < len; ++ii) {
+      const int jn = s_list[w][min(ii + 1, len - 1)];
       const float dx = seam_dx<kWrap>(m.x - fx, Wf, invW), dy = m.y - fy;
       const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
-      if (power2 > 0.0f) continue;
       const float alpha = fminf(0.99f, q.w * exp2_hw(power2));
-      if (alpha < 1.0f / 255.0f) continue;
       const float test_T = T * (1.0f - alpha);
-      if (test_T < 0.0001f) {
-        done = true;
-        continue;
-      }
-      const float wgt = alpha * T;
-      C0 += s_rgb[0][j] * wgt;
-      C1 += s_rgb[1][j] * wgt;
-      C2 += s_rgb[2][j] * wgt;
-      T = test_T;
-      last = (uint32_t)(base + j + 1);  // 1-based position in the tile's list
+      // power2 > 0 or alpha < 1/255: skipped; test_T < 1e-4: stop before blending
+      const bool ok = !done && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
+      const bool stop = ok && test_T < 0.0001f;
+      const bool blend = ok && !stop;
+      done = done || stop;
+      const float wgt = blend ? alpha * T : 0.f;
+      C0 += c0 * wgt;
+      C1 += c1 * wgt;
+      C2 += c2 * wgt;
+      T = blend ? test_T : T;
+      last = blend ? (uint32_t)(base + j + 1) : last;  // 1-based position in the tile's list
+      j = jn;
+      m = s_xy[j];
+      q = s_q[j];
+      c0 = s_rgb[0][j];
+      c1 = s_rgb[1][j];
+      c2 = s_rgb[2][j];
+      if (__ballot(!done) == 0ull) break;
     }
   }
 
