@@ -137,6 +137,9 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 inline size_t sort_blocks(size_t n) { return (n + kSortTile - 1) / kSortTile; }
+// Radix sort workspace (uint32 words): per-block digit counts [kRadix][blocks]
+// (scanned in place along each digit's row) and the kRadix digit totals.
+inline size_t sort_workspace_words(size_t n) { return (size_t)kRadix * (sort_blocks(n) + 1); }
 
 // Scan geometry: 256 threads x 16 items per block.
 constexpr int kScanThreads = 256;
@@ -161,7 +164,7 @@ struct GeomState {
   uint32_t* key_alt;       // ping-pong keys for the depth sort
   uint32_t* offsets;       // inclusive scan of tiles_touched in depth order
   uint32_t* scan_partials; // scan block sums
-  uint32_t* sort_counts;   // radix digit counts [kRadix, blocks] + [kRadix] totals
+  uint32_t* sort_counts;   // radix sort workspace (sort_workspace_words)
   size_t bytes;
 };
 
@@ -178,7 +181,7 @@ inline GeomState carve_geom(char* base, size_t P) {
   g.key_alt = c.take<uint32_t>(P);
   g.offsets = c.take<uint32_t>(P);
   g.scan_partials = c.take<uint32_t>(scan_blocks(P) + 1);
-  g.sort_counts = c.take<uint32_t>((size_t)kRadix * (sort_blocks(P) + 1));
+  g.sort_counts = c.take<uint32_t>(sort_workspace_words(P));
   g.bytes = align_up(c.off) + kAlign;
   return g;
 }
@@ -194,7 +197,7 @@ struct BinState {
   uint32_t* gid_alt;
   float4* grad_rec;       // [R][2]: dcolor x3, dmean2D x2, dconic x3 (32 B, one sector)
   float* grad_opacity;    // [R]: dopacity
-  uint32_t* sort_counts;  // [kRadix, blocks]
+  uint32_t* sort_counts;  // radix sort workspace (sort_workspace_words)
   uint32_t* scan_partials;
   size_t bytes;
 };
@@ -226,7 +229,7 @@ inline BinState carve_bin(char* base, size_t R) {
   b.gid_alt = c.take<uint32_t>(R);
   b.grad_rec = c.take<float4>(2 * (size_t)R);
   b.grad_opacity = c.take<float>(R);
-  b.sort_counts = c.take<uint32_t>((size_t)kRadix * (sort_blocks(R) + 1));
+  b.sort_counts = c.take<uint32_t>(sort_workspace_words(R));
   b.scan_partials = c.take<uint32_t>(1);
   b.bytes = align_up(c.off) + kAlign;
   return b;

@@ -63,7 +63,7 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, s
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned
-// pointers.  counts: kRadix*sort_blocks(n)+1 entries, partials: scan_blocks of that.
+// pointers.  counts: sort_workspace_words(n) entries (partials is unused).
 struct SortResult {
   uint32_t* keys;
   uint32_t* vals;

@@ -178,86 +178,88 @@ __global__ __launch_bounds__(kSortThreads) void radix_rowscan_kernel(uint32_t* _
   if (threadIdx.x == 0) counts[(size_t)kRadix * nblocks + d] = carry;
 }
 
+// Scatter of one pass.  Wave w of a block owns keys [512 w, 512 w + 512) of
+// the block's tile, lane l holding keys 512 w + 64 k + l; ranking item by item
+// with per-wave digit counters in LDS (no block barrier inside the ranking)
+// keeps the keys' input order inside each digit, which makes the sort stable.
+// The keys are then grouped by digit in LDS and written run by run
+// (consecutive threads, consecutive addresses).
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, size_t n, const uint32_t* __restrict__ n_dev, int shift, int bits,
     const uint32_t* __restrict__ counts, uint32_t nblocks) {
   constexpr int kWaves = kSortThreads / kWave;
-  static_assert(kRadix == kSortThreads, "one digit per thread in the base scans");
+  constexpr int kPerWave = kSortItems * kWave;
+  static_assert(kRadix == kSortThreads, "one digit per thread");
   __shared__ uint32_t s_key[kSortTile];
   __shared__ uint32_t s_val[kSortTile];
-  __shared__ uint32_t gbase[kRadix];   // global position of the block's first key of digit d
-  __shared__ uint32_t lstart[kRadix];  // first slot of digit d in the block's locally sorted tile
-  __shared__ uint32_t run[kRadix];     // keys of digit d placed so far
-  __shared__ uint32_t wcnt[kWaves][kRadix];
+  __shared__ uint32_t wcnt[kWaves][kRadix];  // per-wave digit counts, then per-wave offsets inside the digit
+  __shared__ uint32_t s_local[kRadix];       // first slot of digit d in the tile's digit-grouped order
+  __shared__ int32_t s_out[kRadix];          // global position of the key in slot i: s_out[d] + i
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
   const size_t base = (size_t)blockIdx.x * kSortTile;
   n = live_count(n, n_dev);
-  const uint32_t nloc = base >= n ? 0u : (uint32_t)min((size_t)kSortTile, n - base);
+  if (base >= n) return;  // uniform across the block
+  const uint32_t nloc = (uint32_t)min((size_t)kSortTile, n - base);
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][tid] = 0;
 
-  // keys and values of this block, kept in registers for the ranking rounds
-  uint32_t keys[kSortItems], vals[kSortItems];
+  uint32_t keys[kSortItems], vals[kSortItems], rank[kSortItems];
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    const uint32_t li = (uint32_t)(k * kSortThreads + tid);
+    const uint32_t li = (uint32_t)(w * kPerWave + k * kWave + lane);
     const size_t i = base + li;
     keys[k] = li < nloc ? kin[i] : 0u;
     vals[k] = li < nloc ? (vin ? vin[i] : (uint32_t)i) : 0u;
-  }
-  {
-    const uint32_t total = counts[(size_t)kRadix * nblocks + tid];
-    const uint32_t rowp = counts[(size_t)tid * nblocks + blockIdx.x];
-    const uint32_t rown = blockIdx.x + 1 < nblocks ? counts[(size_t)tid * nblocks + blockIdx.x + 1] : total;
-    uint32_t tot;
-    const uint32_t dbase = block_exclusive_scan(total, &tot);
-    gbase[tid] = dbase + rowp;
-    lstart[tid] = block_exclusive_scan(rown - rowp, &tot);
-    run[tid] = 0;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][tid] = 0;
   }
   __syncthreads();
   const uint64_t lanes_below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    if ((uint32_t)(k * kSortThreads) >= nloc) break;  // uniform across the block
-    const bool valid = (uint32_t)(k * kSortThreads + tid) < nloc;
+    const bool valid = (uint32_t)(w * kPerWave + k * kWave + lane) < nloc;
     const uint32_t d = (keys[k] >> shift) & mask;
-    // lanes of this wave holding the same digit
-    uint64_t same = __ballot(valid);
+    uint64_t same = __ballot(valid);  // lanes holding a key with the same digit
     for (int b = 0; b < bits; ++b) {
       const bool set = (d >> b) & 1u;
       const uint64_t bb = __ballot(set);
       same &= set ? bb : ~bb;
     }
-    const uint32_t rank = (uint32_t)__popcll(same & lanes_below);
-    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(same);
-    __syncthreads();
-    if (valid) {
-      uint32_t lp = lstart[d] + run[d] + rank;
-      for (int ww = 0; ww < w; ++ww) lp += wcnt[ww][d];
-      s_key[lp] = keys[k];
-      s_val[lp] = vals[k];
-    }
-    __syncthreads();
-    {
-      uint32_t add = 0;
-#pragma unroll
-      for (int ww = 0; ww < kWaves; ++ww) {
-        add += wcnt[ww][tid];
-        wcnt[ww][tid] = 0;
-      }
-      run[tid] += add;
-    }
-    __syncthreads();
+    const uint32_t before = wcnt[w][d];
+    const uint32_t r = (uint32_t)__popcll(same & lanes_below);
+    rank[k] = before + r;
+    if (valid && r == 0) wcnt[w][d] = before + (uint32_t)__popcll(same);
   }
-  // the block's keys are now grouped by digit in LDS: write each digit's run
-  // contiguously to its global slot (consecutive threads, consecutive addresses)
+  __syncthreads();
+  // thread d: the waves' offsets inside digit d, the digit's first slot in
+  // the tile, and where the digit's run of this tile goes globally
+  const uint32_t d = (uint32_t)tid;
+  uint32_t here = 0;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) {
+    const uint32_t c = wcnt[ww][d];
+    wcnt[ww][d] = here;
+    here += c;
+  }
+  uint32_t tot;
+  const uint32_t local = block_exclusive_scan(here, &tot);
+  const uint32_t digit_base = block_exclusive_scan(counts[(size_t)kRadix * nblocks + d], &tot);
+  s_local[d] = local;
+  s_out[d] = (int32_t)(digit_base + counts[(size_t)d * nblocks + blockIdx.x]) - (int32_t)local;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    if ((uint32_t)(w * kPerWave + k * kWave + lane) < nloc) {
+      const uint32_t dk = (keys[k] >> shift) & mask;
+      const uint32_t slot = s_local[dk] + wcnt[w][dk] + rank[k];
+      s_key[slot] = keys[k];
+      s_val[slot] = vals[k];
+    }
+  }
+  __syncthreads();
   for (uint32_t i = tid; i < nloc; i += kSortThreads) {
     const uint32_t key = s_key[i];
-    const uint32_t d = (key >> shift) & mask;
-    const uint32_t pos = gbase[d] + (i - lstart[d]);
+    const size_t pos = (size_t)((int64_t)s_out[(key >> shift) & mask] + (int64_t)i);
     kout[pos] = key;
     vout[pos] = s_val[i];
   }
