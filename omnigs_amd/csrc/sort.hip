@@ -282,8 +282,12 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
   }
   if (bits < 1) bits = 1;
   const uint32_t nb = (uint32_t)sort_blocks(n);
-  for (int shift = 0; shift < bits; shift += kRadixBits) {
-    const int pb = bits - shift < kRadixBits ? bits - shift : kRadixBits;
+  // the passes split the key bits evenly (13 bits: 7 + 6), so that the
+  // first pass's digit runs are longer than with 8 + 5
+  const int passes = radix_passes(bits);
+  const int width = (bits + passes - 1) / passes;
+  for (int shift = 0; shift < bits; shift += width) {
+    const int pb = bits - shift < width ? bits - shift : width;
     const uint32_t mask = (1u << pb) - 1u;
     hipLaunchKernelGGL(radix_count_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, n, n_dev,
                        shift, mask, counts, nb);
@@ -315,7 +319,8 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
 // reference's duplicateWithKeys does (rasterizer_impl.cu:127-138).
 // ---------------------------------------------------------------------------
 constexpr int kDupThreads = 256;
-constexpr int kDupChunk = 2048;
+constexpr int kDupRun = 8;  // consecutive instances per thread
+constexpr int kDupChunk = kDupThreads * kDupRun;
 
 __device__ __forceinline__ uint32_t first_greater(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
   uint32_t lo = 0, hi = n;  // first i with a[i] > v (a non-decreasing)
@@ -344,7 +349,6 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   __shared__ uint32_t s_gid[kDupChunk];
   __shared__ int s_x0[kDupChunk];
   __shared__ uint16_t s_y0[kDupChunk], s_w[kDupChunk];
-  __shared__ float s_inv_w[kDupChunk];
   __shared__ uint8_t s_view[kViewSlots];
   __shared__ uint32_t s_span[2];
   const int tid = threadIdx.x;
@@ -386,28 +390,58 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     s_x0[j] = rc.x0;
     s_y0[j] = (uint16_t)rc.y0;
     s_w[j] = (uint16_t)w;
-    s_inv_w[j] = 1.0f / (float)w;
   }
   __syncthreads();
-  for (uint32_t k = k0 + tid; k < k1; k += kDupThreads) {
-    const uint32_t j = first_greater(s_end, ns, k);
-    const uint32_t first = j == 0 ? begin0 : s_end[j - 1];
-    const uint32_t local = k - first;
-    const uint32_t w = s_w[j];
-    // local / w via a float estimate and one integer correction (exact for local < 2^23)
-    uint32_t q = (uint32_t)((float)local * s_inv_w[j]);
-    int r = (int)local - (int)(q * w);
-    if (r < 0) { --q; r += (int)w; }
-    else if (r >= (int)w) { ++q; r -= (int)w; }
+  // Each thread expands kDupRun consecutive instances: one search finds the
+  // owner of the first, then owners and rect positions (row q, column r)
+  // advance one instance at a time; a Gaussian's first instance is its rect's
+  // (0, 0), and owners with no instance are stepped over.
+  const uint32_t kb = k0 + (uint32_t)tid * kDupRun;
+  if (kb >= k1) return;
+  uint32_t j = first_greater(s_end, ns, kb);
+  uint32_t w = s_w[j];
+  const uint32_t local = kb - (j == 0 ? begin0 : s_end[j - 1]);
+  uint32_t q = local / w, r = local - q * w;
+  uint32_t keys[kDupRun], ids[kDupRun];
+#pragma unroll
+  for (int u = 0; u < kDupRun; ++u) {
+    const uint32_t k = kb + (uint32_t)u;
+    keys[u] = ids[u] = 0u;
+    if (k >= k1) continue;
+    while (k >= s_end[j]) {  // stops at the span's last owner: s_end[ns - 1] >= k1
+      ++j;
+      w = s_w[j];
+      q = r = 0;
+    }
     const uint32_t y = (uint32_t)s_y0[j] + q;
     if constexpr (kMulti) {
       const int v = s_view[j];
       const TileGrid gv = vt.grid(v);
-      tile_key[k] = vt.tile_base[v] + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + r, gv);
+      keys[u] = vt.tile_base[v] + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, gv);
     } else {
-      tile_key[k] = y * (uint32_t)grid.gx + (uint32_t)wrap_col(s_x0[j] + r, grid);
+      keys[u] = y * (uint32_t)grid.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, grid);
     }
-    gid[k] = s_gid[j];
+    ids[u] = s_gid[j];
+    if (++r == w) {
+      r = 0;
+      ++q;
+    }
+  }
+  if (kb + kDupRun <= k1) {  // whole run: 16-byte stores (kb is a multiple of kDupRun)
+    uint4* tk = reinterpret_cast<uint4*>(tile_key + kb);
+    uint4* gd = reinterpret_cast<uint4*>(gid + kb);
+    tk[0] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
+    tk[1] = make_uint4(keys[4], keys[5], keys[6], keys[7]);
+    gd[0] = make_uint4(ids[0], ids[1], ids[2], ids[3]);
+    gd[1] = make_uint4(ids[4], ids[5], ids[6], ids[7]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < kDupRun; ++u) {
+      if (kb + (uint32_t)u < k1) {
+        tile_key[kb + u] = keys[u];
+        gid[kb + u] = ids[u];
+      }
+    }
   }
 }
 
