@@ -78,9 +78,13 @@ __device__ __forceinline__ float seam_dx(float dx, float W, float invW) {
 
 // Conservative 4-bit mask of the tile's 8x8 wave quadrants that Gaussian
 // (m, co) can reach with alpha >= 1/255: outside the ellipse
-// 0.5 d^T Q d <= ln(255 o) the alpha test fails for every pixel, so skipping
-// those (wave, instance) pairs changes no result.  The box is padded by
-// 1% + 0.05 px against rounding; degenerate conics get the full mask.
+// d^T Q d <= tau2 = 2 ln(255 o) the alpha test fails for every pixel, so
+// skipping those (wave, instance) pairs changes no result.  A quadrant (the
+// square of its 64 pixel centres) is dropped when one of four axes separates
+// it from the ellipse: x and y (the ellipse's bounding box) and the ellipse's
+// two principal axes u_i, along which it reaches sqrt(tau2 / lambda_i) --
+// anisotropic Gaussians are much narrower than their box.  Extents are padded
+// by 1% + 0.05 px against rounding; degenerate conics get the full mask.
 template <int TX, int TY>
 __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile, int gx, float wrapW) {
   constexpr int QX = Quads<TX, TY>::QX;
@@ -91,13 +95,33 @@ __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile,
   const float tau2 = fmaxf(2.0f * __logf(o255) * 1.0001f, 0.f) + 1e-3f;
   const float ex = sqrtf(tau2 * co.z / det) * 1.01f + 0.05f;
   const float ey = sqrtf(tau2 * co.x / det) * 1.01f + 0.05f;
+  // principal axes: lambda_1,2 = mid +- disc; u1 along the larger eigenvalue
+  const float mid = 0.5f * (co.x + co.z);
+  const float disc = sqrtf(0.25f * (co.x - co.z) * (co.x - co.z) + co.y * co.y);
+  const float l1 = mid + disc, l2 = mid - disc;
+  float ux = co.y, uy = l1 - co.x;                     // (Q - l1 I) u = 0, first row
+  const float vx = l1 - co.z, vy = co.y;               // second row: better conditioned when A > C
+  if (vx * vx + vy * vy > ux * ux + uy * uy) { ux = vx; uy = vy; }
+  const float un = ux * ux + uy * uy;
+  const bool axes = un > 0.f && l2 > 0.f;              // else a circle or a degenerate axis: box test only
+  const float ui = axes ? rsqrtf(un) : 0.f;
+  ux *= ui;
+  uy *= ui;
+  // half extents of the ellipse and of the 7x7-pixel-centre square along u1, u2 = (-uy, ux)
+  const float sq = 3.5f * (fabsf(ux) + fabsf(uy));
+  const float e1 = axes ? sqrtf(tau2 / l1) * 1.01f + 0.05f + sq : 0.f;
+  const float e2 = axes ? sqrtf(tau2 / l2) * 1.01f + 0.05f + sq : 0.f;
   const float x0 = (float)((tile % gx) * TX), y0 = (float)((tile / gx) * TY);
   if (wrapW > 0.f) m.x -= wrapW * rintf((m.x - (x0 + 0.5f * TX)) / wrapW);  // nearest copy across the seam
   uint32_t mk = 0;
 #pragma unroll
   for (int q = 0; q < kWavesPerTile; ++q) {
     const float qx = x0 + (float)((q % QX) * 8), qy = y0 + (float)((q / QX) * 8);
-    const bool hit = (m.x + ex >= qx) && (m.x - ex <= qx + 7.f) && (m.y + ey >= qy) && (m.y - ey <= qy + 7.f);
+    bool hit = (m.x + ex >= qx) && (m.x - ex <= qx + 7.f) && (m.y + ey >= qy) && (m.y - ey <= qy + 7.f);
+    if (axes) {
+      const float cx = qx + 3.5f - m.x, cy = qy + 3.5f - m.y;  // square centre relative to the mean
+      hit = hit && fabsf(cx * ux + cy * uy) <= e1 && fabsf(cy * ux - cx * uy) <= e2;
+    }
     mk |= hit ? (1u << q) : 0u;
   }
   return mk;
