@@ -376,16 +376,20 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     const uint32_t gg = order[s];
     s_end[j] = offsets[s];
     s_gid[j] = gg;
-    // the Gaussian's first instance, kept in its record for the backward
-    reinterpret_cast<uint32_t*>(rec)[12 * (size_t)gg + kRecFirstWord] = s == 0 ? 0u : offsets[s - 1];
+    // the radius is read from the record (the preprocess stores it there, as
+    // it does in radii[]) so that the line this pass writes `first` into --
+    // the Gaussian's first instance, kept for the backward -- is the one it
+    // already reads
     const float4 r0 = rec[3 * gg];
+    const int radius = __float_as_int(rec[3 * gg + 2].y);
+    reinterpret_cast<uint32_t*>(rec)[12 * (size_t)gg + kRecFirstWord] = s == 0 ? 0u : offsets[s - 1];
     TileGrid gv = grid;
     if constexpr (kMulti) {
       const int v = (int)(gg / (uint32_t)vt.P);
       s_view[j] = (uint8_t)v;
       gv = vt.grid(v);
     }
-    const Rect rc = tile_rect(make_float2(r0.x, r0.y), radii[gg], gv);
+    const Rect rc = tile_rect(make_float2(r0.x, r0.y), radius, gv);
     const int w = max(1, rc.x1 - rc.x0);
     s_x0[j] = rc.x0;
     s_y0[j] = (uint16_t)rc.y0;
