@@ -160,10 +160,13 @@ def main() -> int:
     return 0
 
 
-def time_steps(step, warmup: int, steps: int, dev):
+def time_steps(step, warmup: int, steps: int, dev, stage_steps: int = 5):
     """W untimed steps, then K timed ones bracketed by synchronisation, each
-    also bracketed by events on the launch stream; stage timers on.  Returns
-    (median event ms, wall ms per step, stage totals, last step's result)."""
+    also bracketed by events on the launch stream, with the library's stage
+    timers off (their events cost the launch stream a few microseconds per
+    stage boundary); then `stage_steps` more with the stage timers on, for the
+    per-stage breakdown.  Returns (median event ms, wall ms per step, stage
+    totals, last step's result)."""
     import torch
 
     from omnigs_amd import capi
@@ -171,8 +174,8 @@ def time_steps(step, warmup: int, steps: int, dev):
     for _ in range(warmup):
         res = step()
     torch.cuda.synchronize()
-    capi.profile_read()  # drop warm-up events
-    capi.profile_enable(True)
+    capi.profile_enable(False)
+    capi.profile_read()  # drop anything recorded so far
     stream = torch.cuda.current_stream(dev)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
@@ -183,6 +186,10 @@ def time_steps(step, warmup: int, steps: int, dev):
         evs[i][1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    capi.profile_enable(True)
+    for _ in range(max(1, stage_steps)):
+        step()
+    torch.cuda.synchronize()
     capi.profile_enable(False)
     stages = capi.profile_read()
     return (statistics.median(a.elapsed_time(b) for a, b in evs), (t1 - t0) * 1000.0 / steps, stages, res)

@@ -194,6 +194,19 @@ int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream) {
 
 namespace {
 
+// One pinned host word per calling thread for the forward's read-back of the
+// instance count (allocated on first use, kept for the thread's lifetime).
+uint32_t* pinned_word() {
+  thread_local uint32_t* word = nullptr;
+  thread_local bool tried = false;
+  if (!tried) {
+    tried = true;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, 64, hipHostMallocDefault) == hipSuccess) word = static_cast<uint32_t*>(p);
+  }
+  return word;
+}
+
 // Shared by forward (render = true) and recompute (render = false, image
 // buffer given and kept).
 int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer, omnigs_buffer binning_buffer,
@@ -249,9 +262,22 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
                "scan");
   }
 
+  // The instance count sizes the binning buffer, so the host needs it before
+  // the rest of the forward can be queued.  It is copied into a pinned word
+  // and the stream is polled rather than waited on: the GPU idles until the
+  // next launch, and a blocking wait adds the thread's wake-up to that.
   uint32_t R = 0;
-  OMNIGS_TRY(hipMemcpyAsync(&R, g.offsets + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
-  OMNIGS_TRY(hipStreamSynchronize(s), "sync R");
+  if (uint32_t* pw = pinned_word()) {
+    OMNIGS_TRY(hipMemcpyAsync(pw, g.offsets + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
+    hipError_t q;
+    while ((q = hipStreamQuery(s)) == hipErrorNotReady) {
+    }
+    OMNIGS_TRY(q, "sync R");
+    R = *pw;
+  } else {
+    OMNIGS_TRY(hipMemcpyAsync(&R, g.offsets + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
+    OMNIGS_TRY(hipStreamSynchronize(s), "sync R");
+  }
   if (R > 0x7fffffffu) return fail_arg("instance count exceeds int range");
 
   char* bbuf = grow(binning_buffer, carve_bin(nullptr, R).bytes);
