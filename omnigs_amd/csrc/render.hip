@@ -154,9 +154,11 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
-  __shared__ float2 s_xy[kBatch];
-  __shared__ float4 s_q[kBatch];  // prescaled conic (a, b, c) and opacity
-  __shared__ float s_rgb[kChannels][kBatch];
+  // a pair's operands in two 16-byte reads at one address, plus blue:
+  // {x, y, a, b}, {c, opacity, r, g} with (a, b, c) the prescaled conic
+  __shared__ float4 s_pa[kBatch];
+  __shared__ float4 s_pb[kBatch];
+  __shared__ float s_blue[kBatch];
   __shared__ uint8_t s_mask[kBatch];
   __shared__ uint8_t s_list[kWavesPerTile][kBatch];
 
@@ -184,11 +186,9 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const float2 m = rr.xy;
       const float4 co = rr.co;
       const float3 q = prescale_conic(co);
-      s_xy[tid] = m;
-      s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
-      s_rgb[0][tid] = rr.rgb.x;
-      s_rgb[1][tid] = rr.rgb.y;
-      s_rgb[2][tid] = rr.rgb.z;
+      s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
+      s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
+      s_blue[tid] = rr.rgb.z;
       s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
     __syncthreads();
@@ -199,14 +199,13 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     // wave-uniform; it ends once every lane of the wave is done.  The next
     // pair's operands are fetched from LDS while the current one computes.
     int j = s_list[w][0];  // a stale slot when len == 0: never used
-    float2 m = s_xy[j];
-    float4 q = s_q[j];
-    float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
+    float4 pa = s_pa[j], pb = s_pb[j];
+    float c2 = s_blue[j];
     for (int ii = 0; ii < len; ++ii) {
       const int jn = s_list[w][min(ii + 1, len - 1)];
-      const float dx = seam_dx<kWrap>(m.x - fx, Wf, invW), dy = m.y - fy;
-      const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
-      const float alpha = fminf(0.99f, q.w * exp2_hw(power2));
+      const float dx = seam_dx<kWrap>(pa.x - fx, Wf, invW), dy = pa.y - fy;
+      const float power2 = power2_of(make_float3(pa.z, pa.w, pb.x), dx, dy);
+      const float alpha = fminf(0.99f, pb.y * exp2_hw(power2));
       const float test_T = T * (1.0f - alpha);
       // power2 > 0 or alpha < 1/255: skipped; test_T < 1e-4: stop before blending
       const bool ok = !done && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
@@ -214,17 +213,15 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const bool blend = ok && !stop;
       done = done || stop;
       const float wgt = blend ? alpha * T : 0.f;
-      C0 += c0 * wgt;
-      C1 += c1 * wgt;
+      C0 += pb.z * wgt;
+      C1 += pb.w * wgt;
       C2 += c2 * wgt;
       T = blend ? test_T : T;
       last = blend ? (uint32_t)(base + j + 1) : last;  // 1-based position in the tile's list
       j = jn;
-      m = s_xy[j];
-      q = s_q[j];
-      c0 = s_rgb[0][j];
-      c1 = s_rgb[1][j];
-      c2 = s_rgb[2][j];
+      pa = s_pa[j];
+      pb = s_pb[j];
+      c2 = s_blue[j];
       if (__ballot(!done) == 0ull) break;
     }
   }
@@ -348,10 +345,11 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   const float Wf = (float)W, invW = 1.0f / (float)W;
   const TileGrid tg{gx, (H + TY - 1) / TY, kWrap ? 1 : 0, TX, TY};
   __shared__ uint32_t s_inst[kBatch];  // index of the staged instance in the depth-ordered expansion
-  __shared__ float2 s_xy[kBatch];
   __shared__ float4 s_co[kBatch];
-  __shared__ float4 s_q[kBatch];  // prescaled conic, as in the forward
-  __shared__ float s_rgb[kChannels][kBatch];
+  // the forward's layout: {x, y, a, b}, {c, opacity, r, g}, blue
+  __shared__ float4 s_pa[kBatch];
+  __shared__ float4 s_pb[kBatch];
+  __shared__ float s_blue[kBatch];
   __shared__ float s_acc[kGradTerms][kAccStride];
   __shared__ uint8_t s_mask[kBatch];
   __shared__ uint8_t s_list[kWavesPerTile][kBatch];
@@ -405,13 +403,12 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   // the offsets; the nine moments follow from these four (moments()).
   // `lim` = n - 1 - base - last: position n - 1 - (base + j) < last <=> j > lim.
   auto eval = [&](int j, int lim, float& wgt, float& v, float& dx, float& dy) -> bool {
-    const float2 m = s_xy[j];
-    const float4 q = s_q[j];
-    dx = seam_dx<kWrap>(m.x - fx, Wf, invW);
-    dy = m.y - fy;
-    const float power2 = power2_of(make_float3(q.x, q.y, q.z), dx, dy);
+    const float4 pa = s_pa[j], pb = s_pb[j];
+    dx = seam_dx<kWrap>(pa.x - fx, Wf, invW);
+    dy = pa.y - fy;
+    const float power2 = power2_of(make_float3(pa.z, pa.w, pb.x), dx, dy);
     const float G = exp2_hw(power2);
-    const float alpha = fminf(0.99f, q.w * G);
+    const float alpha = fminf(0.99f, pb.y * G);
     const bool ok = j > lim && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
     if (__builtin_amdgcn_ballot_w64(ok) == 0ull) return false;
     wgt = 0.f;
@@ -422,8 +419,8 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const float r = __builtin_amdgcn_rcpf(1.f - alpha);
       T = T * r;
       v = alpha * T;
-      const float cd = s_rgb[0][j] * dp0 + s_rgb[1][j] * dp1 + s_rgb[2][j] * dp2;
-      acc_dot = last_alpha * last_cd + (1.f - last_alpha) * acc_dot;
+      const float cd = pb.z * dp0 + pb.w * dp1 + s_blue[j] * dp2;
+      acc_dot += last_alpha * (last_cd - acc_dot);  // = la lcd + (1 - la) acc
       const float dL_dalpha = (cd - acc_dot) * T + neg_tb * r;
       last_alpha = alpha;
       last_cd = cd;
@@ -460,12 +457,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const uint32_t first = __float_as_uint(r2.z);
       s_inst[tid] = first + rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
       if (k == 0) live_first[blockIdx.x] = first + 1u;  // position n - 1, the deepest staged
-      s_xy[tid] = m;
       s_co[tid] = co;
-      s_q[tid] = make_float4(q.x, q.y, q.z, co.w);
-      s_rgb[0][tid] = rr.rgb.x;
-      s_rgb[1][tid] = rr.rgb.y;
-      s_rgb[2][tid] = rr.rgb.z;
+      s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
+      s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
+      s_blue[tid] = rr.rgb.z;
       s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
 #pragma unroll
