@@ -257,8 +257,11 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   // Instance offsets in depth order.
   {
     StageTimer tm(OMNIGS_STAGE_SCAN, s);
-    // key_alt is free after the depth sort: it holds tiles_touched in depth order
-    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s, g.key_alt),
+    // key_alt is free after the depth sort: it holds tiles_touched in depth order.
+    // The same launches scan the preprocess blocks' slot counts into the first
+    // gradient slot of every block (the backward's records).
+    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s, g.key_alt,
+                           g.block_slots, slot_blocks(P), g.slot_base),
                "scan");
   }
 
@@ -288,7 +291,9 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   const BinState b = carve_bin(bbuf, R);
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
-    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rec, radii, grid, b.tile_key, b.gid, s), "duplicate");
+    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rec, radii, grid, b.tile_key, b.gid,
+                                reinterpret_cast<uint32_t*>(b.grad_opacity), s),
+               "duplicate");
   }
   SortResult ts{b.tile_key, b.gid};
   if (R > 0) {
@@ -367,14 +372,14 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   if (R > 0) {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward(grid, img.ranges, point_list, W, H, a->background, g.rec, img.final_T,
-                                      img.n_contrib, a->dL_dpix, grad_rec, grad_opacity, img.live_first, s),
+                                      img.n_contrib, a->dL_dpix, g.slot_base, (uint32_t)R, grad_rec, grad_opacity, s),
                "render backward");
   }
   {
     // every entry of the four per-Gaussian arrays is written (zeros where no
     // tile touched the Gaussian), so they need no clearing
     StageTimer tm(OMNIGS_STAGE_GATHER_GRADS, s);
-    OMNIGS_TRY(launch_gather_grads(P, (uint32_t)R, g.rec, g.tiles_touched, grid, img.live_first, grad_rec,
+    OMNIGS_TRY(launch_gather_grads(P, P, (uint32_t)R, nullptr, g.tiles_touched, g.slot_local, g.slot_base, grad_rec,
                                    grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor, s),
                "gather grads");
   }
@@ -406,7 +411,6 @@ struct BatchState {
   float* final_T;
   uint32_t* n_contrib;
   uint2* ranges;
-  uint32_t* live_first;  // per global tile
   float* dmean2D;  // per item [3]
   float* dconic;   // per item [4]
   float* dopacity; // per item
@@ -426,7 +430,6 @@ BatchState carve_batch(char* base, size_t items, size_t pixels, size_t tiles, si
   st.final_T = c.take<float>(pixels);
   st.n_contrib = c.take<uint32_t>(pixels);
   st.ranges = c.take<uint2>(tiles);
-  st.live_first = c.take<uint32_t>(tiles);
   st.dmean2D = c.take<float>(3 * items);
   st.dconic = c.take<float>(4 * items);
   st.dopacity = c.take<float>(items);
@@ -518,7 +521,8 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   }
   {
     StageTimer tm(OMNIGS_STAGE_SCAN, s);
-    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, items, false, g.scan_partials, s, g.key_alt),
+    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, items, false, g.scan_partials, s, g.key_alt,
+                           g.block_slots, slot_blocks(P), g.slot_base),
                "batch scan");
   }
   const uint32_t* R_dev = g.offsets + (items - 1);
@@ -526,7 +530,7 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rec, g.radii,
-                                      vt, b.tile_key, b.gid, s),
+                                      vt, b.tile_key, b.gid, reinterpret_cast<uint32_t*>(b.grad_opacity), s),
                "batch duplicate");
   }
   SortResult ts;
@@ -577,14 +581,15 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
   {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward_views(vt, bs.ranges, point_list, a->background, g.rec, bs.final_T,
-                                            bs.n_contrib, gr->dL_dpix, b.grad_rec, b.grad_opacity, bs.live_first, s),
+                                            bs.n_contrib, gr->dL_dpix, g.slot_base, a->capacity, b.grad_rec,
+                                            b.grad_opacity, s),
                "batch render backward");
   }
   {
     StageTimer tm(OMNIGS_STAGE_GATHER_GRADS, s);
-    OMNIGS_TRY(launch_gather_grads_views(vt, a->capacity, g.offsets + (items - 1), g.rec, g.tiles_touched,
-                                         bs.live_first, b.grad_rec, b.grad_opacity, bs.dmean2D, bs.dconic, bs.dopacity,
-                                         bs.dcolor, s),
+    OMNIGS_TRY(launch_gather_grads((int)items, P, a->capacity, g.offsets + (items - 1), g.tiles_touched,
+                                   g.slot_local, g.slot_base, b.grad_rec, b.grad_opacity, bs.dmean2D, bs.dconic,
+                                   bs.dopacity, bs.dcolor, s),
                "batch gather grads");
   }
   PreprocessBackwardArgs pb{P, a->D, a->M, 0, 0, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,

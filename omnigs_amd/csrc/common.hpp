@@ -147,18 +147,62 @@ constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 inline size_t scan_blocks(size_t n) { return (n + kScanTile - 1) / kScanTile; }
 
+// ---------------------------------------------------------------------------
+// Wave / block scans (64-lane wavefronts, 4 waves per 256-thread block).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t n = __shfl_up(v, o, kWave);
+    if (lane >= o) v += n;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread across a 256-thread block.
+// Returns the exclusive prefix; *total receives the block sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kScanThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint32_t inc = wave_inclusive_scan(v, lane);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanThreads / kWave; ++i) {
+    const uint32_t s = wsum[i];
+    woff += i < w ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + inc - v;
+}
+
+// Gradient slots are numbered per block of kSlotBlock Gaussians (the
+// preprocess's thread blocks): block b's slots start at slot_base[b].
+constexpr int kSlotBlock = 256;
+inline size_t slot_blocks(size_t n) { return (n + kSlotBlock - 1) / kSlotBlock; }
+// Opacity word of a gradient slot that holds no record (a NaN pattern the
+// hardware never produces: its NaNs are the canonical 0x7FC00000).
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
+
 // Per-Gaussian state written by the forward preprocess.
 struct GeomState {
   uint32_t* depth_key;     // float bits of the camera distance (sort key); ~0 if culled
   int32_t* radii;          // internal radii (used when the caller passes none)
   // Render record, 48 B per Gaussian so one gather fetches everything the
-  // tile kernels need: {x, y, A, B}, {C, opacity, r, g}, {b, radius, first, 0}
+  // tile kernels need: {x, y, A, B}, {C, opacity, r, g}, {b, radius, slot, 0}
   // (pixel-space mean, conic, opacity, colour -- SH-derived or precomputed --,
-  // the radius as int bits, and `first`, the index of the Gaussian's first
-  // instance in the depth-ordered expansion, written by the duplicate pass).
+  // the radius as int bits, and `slot`, the offset of the Gaussian's first
+  // gradient slot inside its 256-Gaussian block: the block's slots start at
+  // slot_base[block]).
   float4* rec;
   uint8_t* clamped;        // bit c set when channel c was clamped at 0
   uint32_t* tiles_touched; // rect area
+  uint32_t* slot_local;    // copy of the record's `slot` (dense, for the gradient gather)
+  uint32_t* block_slots;   // per 256-Gaussian block: its Gaussians' total rect area
+  uint32_t* slot_base;     // exclusive scan of block_slots (written by the forward scan)
   uint32_t* order;         // Gaussians sorted by (depth bits, index)
   uint32_t* order_alt;     // ping-pong for the depth sort
   uint32_t* key_alt;       // ping-pong keys for the depth sort
@@ -176,6 +220,9 @@ inline GeomState carve_geom(char* base, size_t P) {
   g.rec = c.take<float4>(3 * P);
   g.clamped = c.take<uint8_t>(P);
   g.tiles_touched = c.take<uint32_t>(P);
+  g.slot_local = c.take<uint32_t>(P);
+  g.block_slots = c.take<uint32_t>(slot_blocks(P) + 1);
+  g.slot_base = c.take<uint32_t>(slot_blocks(P) + 1);
   g.order = c.take<uint32_t>(P);
   g.order_alt = c.take<uint32_t>(P);
   g.key_alt = c.take<uint32_t>(P);
@@ -187,8 +234,11 @@ inline GeomState carve_geom(char* base, size_t P) {
 }
 
 // Per-instance state: tile keys and Gaussian ids, ping-pong for the tile sort;
-// the backward's per-instance gradient terms, indexed like the (unsorted)
-// depth-ordered expansion so that each Gaussian's terms are contiguous.
+// the backward's per-(tile, Gaussian) gradient terms.  Gaussian g's instances
+// own the slots [slot_base[g / 256] + slot(g), ... + tiles_touched[g]), one per
+// tile of its rect in row-major order, so that consecutive Gaussians' slots
+// are adjacent; the opacity word of a slot the backward did not write holds
+// kNoRecord.
 constexpr int kGradTerms = 9;  // dcolor x3, dmean2D x2, dconic x3, dopacity
 struct BinState {
   uint32_t* tile_key;
@@ -196,7 +246,7 @@ struct BinState {
   uint32_t* gid;
   uint32_t* gid_alt;
   float4* grad_rec;       // [R][2]: dcolor x3, dmean2D x2, dconic x3 (32 B, one sector)
-  float* grad_opacity;    // [R]: dopacity
+  float* grad_opacity;    // [R]: dopacity, or kNoRecord
   uint32_t* sort_counts;  // radix sort workspace (sort_workspace_words)
   uint32_t* scan_partials;
   size_t bytes;
@@ -212,13 +262,11 @@ __device__ __forceinline__ RenderRec load_rec(const float4* __restrict__ rec, ui
   return {make_float2(a.x, a.y), make_float4(a.z, a.w, b.x, b.y), make_float3(b.z, b.w, c.x)};
 }
 __device__ __forceinline__ void store_rec(float4* __restrict__ rec, uint32_t g, float2 xy, float4 co, float3 rgb,
-                                          int radius) {
+                                          int radius, uint32_t slot) {
   rec[3 * g] = make_float4(xy.x, xy.y, co.x, co.y);
   rec[3 * g + 1] = make_float4(co.z, co.w, rgb.x, rgb.y);
-  rec[3 * g + 2] = make_float4(rgb.z, __int_as_float(radius), 0.f, 0.f);
+  rec[3 * g + 2] = make_float4(rgb.z, __int_as_float(radius), __uint_as_float(slot), 0.f);
 }
-// Words 9 and 10 of a record (radius, first instance).
-constexpr int kRecRadiusWord = 9, kRecFirstWord = 10;
 
 inline BinState carve_bin(char* base, size_t R) {
   BinState b;
@@ -240,10 +288,6 @@ struct ImgState {
   float* final_T;
   uint32_t* n_contrib;
   uint2* ranges;
-  // Per tile, written by the backward: 1 + `first` of the deepest instance it
-  // staged (the tile's lists run in depth order, so exactly the instances
-  // whose Gaussian's first < this value got a gradient record); 0: none.
-  uint32_t* live_first;
   size_t bytes;
 };
 
@@ -254,7 +298,6 @@ inline ImgState carve_img(char* base, int W, int H) {
   s.final_T = c.take<float>(N);
   s.n_contrib = c.take<uint32_t>(N);
   s.ranges = c.take<uint2>((size_t)tile_grid(W, H).tiles());
-  s.live_first = c.take<uint32_t>((size_t)tile_grid(W, H).tiles());
   s.bytes = align_up(c.off) + kAlign;
   return s;
 }

@@ -58,8 +58,11 @@ hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmat
 // sort.hip
 // Inclusive (or exclusive) scan of n u32 values, optionally gathered through
 // idx (out[i] = sum of in[idx[j]] for j <= i).  partials needs scan_blocks(n)+1 entries.
+// side_in (optional): a short array (a few thousand values) whose exclusive
+// scan goes to side_out in the same launches (the gradient slot bases).
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
-                       uint32_t* partials, hipStream_t s, uint32_t* gather_buf = nullptr);
+                       uint32_t* partials, hipStream_t s, uint32_t* gather_buf = nullptr,
+                       const uint32_t* side_in = nullptr, size_t side_n = 0, uint32_t* side_out = nullptr);
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned
@@ -72,16 +75,19 @@ struct SortResult {
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
                              hipStream_t s, SortResult* out, const uint32_t* n_dev = nullptr);
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, float4* rec,
-                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
+// no_record: the gradient slots' opacity words, set to kNoRecord (one per instance).
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
+                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
+                            hipStream_t s);
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s, const uint32_t* R_dev = nullptr);
 // Batched expansion: items = V P, capacity instances at most; R_dev is the
 // device-side instance count (last inclusive offset); status [2] receives
 // {count needed, 1 if it exceeds the capacity}.
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
-                                  const uint32_t* order, const uint32_t* offsets, float4* rec, const int* radii,
-                                  const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, hipStream_t s);
+                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
+                                  hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // render.hip
@@ -89,16 +95,20 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
                                  const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
                                  float* out_color, hipStream_t s);
 // The backward writes each staged (tile, Gaussian) instance's nine gradient
-// terms to grad_rec [R][2] float4 + grad_opacity [R] and each tile's live_first; gather_grads sums them
-// per Gaussian into the reference's per-Gaussian arrays (every entry written).
+// terms to its slot -- grad_rec [nslots][2] float4 + grad_opacity [nslots] --
+// at slot_base[Gaussian / kSlotBlock] + the record's slot word + the tile's
+// position in the Gaussian's rect; slots it does not write must hold
+// kNoRecord in grad_opacity.  gather_grads sums each item's slots into the
+// reference's per-Gaussian arrays (every entry written).  P: Gaussians per
+// view (items = V P); R_dev (optional): the live slot count.
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
-                                  const float* dL_dpix, float4* grad_rec, float* grad_opacity, uint32_t* live_first,
-                                  hipStream_t s);
-hipError_t launch_gather_grads(int P, uint32_t R, const float4* rec, const uint32_t* tiles_touched, TileGrid grid,
-                               const uint32_t* live_first, const float4* grad_rec, const float* grad_opacity,
-                               float* dL_dmean2D, float* dL_dconic, float* dL_dopacity, float* dL_dcolors,
-                               hipStream_t s);
+                                  const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
+                                  float* grad_opacity, hipStream_t s);
+hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
+                               const uint32_t* tiles_touched, const uint32_t* slot_local, const uint32_t* slot_base,
+                               const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
+                               float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
 
 // Batched render over every tile of every view of vt; pixel arrays are the
 // views' slices laid out one after the other (colours [3, H_v, W_v] per view).
@@ -107,13 +117,8 @@ hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges,
                                        float* out_color, hipStream_t s);
 hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                         const float* bg, const float4* rec, const float* final_T,
-                                        const uint32_t* n_contrib, const float* dL_dpix, float4* grad_rec,
-                                        float* grad_opacity, uint32_t* live_first, hipStream_t s);
-// capacity: the instance capacity; R_dev: the live instance count on the device.
-hipError_t launch_gather_grads_views(const ViewTable& vt, uint32_t capacity, const uint32_t* R_dev,
-                                     const float4* rec, const uint32_t* tiles_touched, const uint32_t* live_first,
-                                     const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
-                                     float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
+                                        const uint32_t* n_contrib, const float* dL_dpix, const uint32_t* slot_base,
+                                        uint32_t nslots, float4* grad_rec, float* grad_opacity, hipStream_t s);
 bool tile_shape_supported(int bx, int by);
 
 // preprocess_backward.hip

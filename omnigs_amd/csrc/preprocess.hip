@@ -133,7 +133,6 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
     __syncthreads();
   }
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((kMlp ? (idx & ~63) : idx) >= a.P) return;
   const bool in_range = idx < a.P;
 
   if (in_range) {
@@ -143,6 +142,13 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
   }
   GeomOut o;
   const bool live = in_range && preprocess_geometry(a, idx, grid, o);
+  // gradient slots: the block's Gaussians in index order, one slot per tile
+  // of the rect (every thread of the block takes part in the scan)
+  uint32_t block_total;
+  const uint32_t slot = block_exclusive_scan(live ? (uint32_t)o.area : 0u, &block_total);
+  if (threadIdx.x == 0) g.block_slots[blockIdx.x] = block_total;
+  if (in_range) g.slot_local[idx] = slot;
+  if ((kMlp ? (idx & ~63) : idx) >= a.P) return;  // MLP: whole waves evaluate the decoder
 
   float3 col;
   if constexpr (kMlp) {
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
 
   g.depth_key[idx] = __float_as_uint(o.r);
   radii_out[idx] = o.radius;
-  store_rec(g.rec, idx, o.pix, o.conic, col, o.radius);
+  store_rec(g.rec, idx, o.pix, o.conic, col, o.radius, slot);
   g.tiles_touched[idx] = (uint32_t)o.area;
 }
 
@@ -187,8 +193,8 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
     __syncthreads();
   }
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((kMlp ? (idx & ~63) : idx) >= a.P) return;
   const bool in_range = idx < a.P;
+  const bool wave_has = (idx & ~63) < a.P;  // MLP: whole waves evaluate the decoder
   GaussGeo gg{};
   float3 c[16];
   const int nc = (kMlp || a.colors_precomp) ? 0 : min(a.M, sh_coeffs(a.D));
@@ -196,6 +202,9 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
     gg = load_gauss(a, idx);
     if (!kMlp && !a.colors_precomp) load_sh(a.shs, a.M, idx, nc, c);
   }
+  // gradient slots of the block: view by view, the block's Gaussians in index
+  // order within each view (every thread of the block takes part in the scans)
+  uint32_t view_prefix = 0;
   for (int v = 0; v < vt.V; ++v) {
     const size_t item = (size_t)v * a.P + idx;
     if (in_range) {
@@ -205,9 +214,14 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
     }
     GeomOut o;
     const bool live = in_range && project_view(gg, viewmats + 16 * v, vt.W[v], vt.H[v], vt.grid(v), o);
+    uint32_t view_total;
+    const uint32_t slot = view_prefix + block_exclusive_scan(live ? (uint32_t)o.area : 0u, &view_total);
+    view_prefix += view_total;
+    if (in_range) g.slot_local[item] = slot;
     const float3 cp = make_float3(camposv[3 * v], camposv[3 * v + 1], camposv[3 * v + 2]);
     float3 col;
     if constexpr (kMlp) {
+      if (!wave_has) continue;
       float3 dir = make_float3(0.f, 0.f, 1.f);
       if (in_range) {
         const float3 d = make_float3(gg.p.x - cp.x, gg.p.y - cp.y, gg.p.z - cp.z);
@@ -227,9 +241,10 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
     }
     g.depth_key[item] = __float_as_uint(o.r);
     radii_out[item] = o.radius;
-    store_rec(g.rec, item, o.pix, o.conic, col, o.radius);
+    store_rec(g.rec, item, o.pix, o.conic, col, o.radius, slot);
     g.tiles_touched[item] = (uint32_t)o.area;
   }
+  if (threadIdx.x == 0) g.block_slots[blockIdx.x] = view_prefix;
 }
 
 hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmats, const float* campos,

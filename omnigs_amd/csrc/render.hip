@@ -338,13 +338,13 @@ template <int TX, int TY, bool kWrap, bool kMulti>
 __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
     ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
-    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, float4* __restrict__ grad_rec,
-    float* __restrict__ grad_opacity, uint32_t* __restrict__ live_first) {
+    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, const uint32_t* __restrict__ slot_base,
+    uint32_t nslots, float4* __restrict__ grad_rec, float* __restrict__ grad_opacity) {
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
   const TileGrid tg{gx, (H + TY - 1) / TY, kWrap ? 1 : 0, TX, TY};
-  __shared__ uint32_t s_inst[kBatch];  // index of the staged instance in the depth-ordered expansion
+  __shared__ uint32_t s_inst[kBatch];  // gradient slot of the staged instance
   __shared__ float4 s_co[kBatch];
   // the forward's layout: {x, y, a, b}, {c, opacity, r, g}, blue
   __shared__ float4 s_pa[kBatch];
@@ -381,7 +381,6 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   if (lane == 0) atomicMax(&s_nlive, wlast);
   __syncthreads();
   const int n = s_nlive;
-  if (tid == 0 && n == 0) live_first[blockIdx.x] = 0u;
   float dp0 = 0.f, dp1 = 0.f, dp2 = 0.f;
   if (inside) {
     dp0 = dL_dpix[pid];
@@ -450,13 +449,13 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const float2 m = rr.xy;
       const float4 co = rr.co;
       const float3 q = prescale_conic(co);
-      // this instance's slot in the expansion: the Gaussian's first instance
-      // plus the tile's position in its rect (the duplicate pass's order)
+      // this instance's gradient slot: the Gaussian's first slot (its block's
+      // base plus its offset in the block) plus the tile's position in its rect
       const float4 r2 = rec[3 * g + 2];
       const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
-      const uint32_t first = __float_as_uint(r2.z);
-      s_inst[tid] = first + rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
-      if (k == 0) live_first[blockIdx.x] = first + 1u;  // position n - 1, the deepest staged
+      const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
+      s_inst[tid] = slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z) +
+                    rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
       s_co[tid] = co;
       s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
       s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
@@ -526,25 +525,30 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
                                    -0.5f * co.w * swdy2,
                                    sw};
       // plain stores, zeros included: every staged instance owns its record
-      const size_t j = s_inst[tid];
-      grad_rec[2 * j] = make_float4(v[0], v[1], v[2], v[3]);
-      grad_rec[2 * j + 1] = make_float4(v[4], v[5], v[6], v[7]);
-      grad_opacity[j] = v[8];
+      // (slots past the capacity exist only when a batch overflowed)
+      const uint32_t j = s_inst[tid];
+      if (j < nslots) {
+        grad_rec[2 * (size_t)j] = make_float4(v[0], v[1], v[2], v[3]);
+        grad_rec[2 * (size_t)j + 1] = make_float4(v[4], v[5], v[6], v[7]);
+        grad_opacity[j] = v[8];
+      }
     }
   }
 }
 
-// Per-Gaussian (per-item) sum of the backward's gradient records: the
-// instances of item i are the contiguous slots [first, first + tiles_touched)
-// of the expansion, one per tile of its rect in the duplicate pass's order; a
-// slot holds a record iff the tile's backward staged it (first < live_first of
-// the tile).  Items with at most kGatherSerial instances are summed by their
-// own lane; larger ones (a few Gaussians cover thousands of tiles) one at a
-// time by the whole wave, lanes striding over the instances, so no lane walks
-// a long list alone.  Writes every output entry, zeros included, in the
-// reference's layouts: dL_dmean2D [.,3] (z = 0), dL_dconic [.,2,2] ((1,0) = 0),
-// dL_dopacity [.], dL_dcolors [.,3].  R (with R_dev, the live count) bounds
-// the slots read.
+// Per-Gaussian (per-item) sum of the backward's gradient records.  Item i
+// owns the slots [base, base + tiles_touched) with base = slot_base[block of
+// its Gaussian] + slot_local[i]; consecutive items' slots are adjacent, so
+// neighbouring lanes read neighbouring records.  A slot the backward did not
+// write (its tile never staged the instance) holds kNoRecord in its opacity
+// word.  Slots are fetched kGatherGroup per round trip (the opacity words,
+// then the records that exist).  Items with more than kGatherSerial instances
+// (a few Gaussians cover thousands of tiles) are summed one at a time by the
+// whole wave, lanes striding over the slots, so no lane walks a long list
+// alone.  Writes every output entry, zeros included, in the reference's
+// layouts: dL_dmean2D [.,3] (z = 0), dL_dconic [.,2,2] ((1,0) = 0),
+// dL_dopacity [.], dL_dcolors [.,3].  nslots (with R_dev, the live count)
+// bounds the slots read.
 constexpr uint32_t kGatherSerial = 16;
 constexpr int kGatherGroup = 8;  // slots fetched per round trip
 
@@ -552,11 +556,11 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 
-template <bool kMulti>
-__global__ __launch_bounds__(256) void gather_grads_kernel(int items, uint32_t R, const uint32_t* __restrict__ R_dev,
-                                                           const float4* __restrict__ rec,
-                                                           const uint32_t* __restrict__ tiles_touched, TileGrid grid,
-                                                           ViewTable vt, const uint32_t* __restrict__ live_first,
+__global__ __launch_bounds__(256) void gather_grads_kernel(int items, int P, uint32_t nslots,
+                                                           const uint32_t* __restrict__ R_dev,
+                                                           const uint32_t* __restrict__ tiles_touched,
+                                                           const uint32_t* __restrict__ slot_local,
+                                                           const uint32_t* __restrict__ slot_base,
                                                            const float4* __restrict__ grad_rec,
                                                            const float* __restrict__ grad_opacity,
                                                            float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
@@ -565,115 +569,55 @@ __global__ __launch_bounds__(256) void gather_grads_kernel(int items, uint32_t R
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const bool valid = i < items;  // every lane stays to the end: the wave-wide path needs them all
-  if (R_dev) R = min(R, *R_dev);
-  const bool wrap = (kMulti ? vt.wrap : grid.wrap) != 0;
-  uint32_t cnt = valid ? tiles_touched[i] : 0u;
-  uint32_t first = 0, tile_base = 0;
-  int x0 = 0, y0 = 0, w = 1, gx = 1;
-  if (cnt > 0) {
-    TileGrid gv = grid;
-    if constexpr (kMulti) {
-      const int v = i / vt.P;
-      gv = vt.grid(v);
-      tile_base = vt.tile_base[v];
-    }
-    const float4 r0 = rec[3 * i], r2 = rec[3 * i + 2];
-    const Rect rc = tile_rect(make_float2(r0.x, r0.y), __float_as_int(r2.y), gv);
-    x0 = rc.x0;
-    y0 = rc.y0;
-    w = max(1, rc.x1 - rc.x0);
-    gx = gv.gx;
-    first = __float_as_uint(r2.z);
-  }
-  // Slot (first + l) sits at tile (y, x0 + c).  A group of slots is fetched
-  // in two rounds, each with all of its loads in flight: the tiles'
-  // live_first, then the records of the slots that hold one (first <
-  // live_first).  Tile 0 stands in for slots out of range.
-  auto tile_at = [&](int y, int c, int x0_, int gx_, uint32_t tb) {
-    int col = x0_ + c;
-    if (wrap) col = ((col % gx_) + gx_) % gx_;
-    return tb + (uint32_t)(y * gx_ + col);
-  };
-  auto add_group = [&](const bool* in, const uint32_t* tt, const uint32_t* jj, uint32_t fst, float* a) {
-    bool ok[kGatherGroup];
+  if (R_dev) nslots = min(nslots, *R_dev);
+  const uint32_t cnt = valid ? tiles_touched[i] : 0u;
+  const uint32_t base = cnt > 0 ? slot_base[(uint32_t)(i % P) / kSlotBlock] + slot_local[i] : 0u;
+
+  // adds the records of slots base + l for the group's l (l0, l0 + stride, ...)
+  auto add_group = [&](uint32_t b, uint32_t n, uint32_t l0, uint32_t stride, float* a) {
+    uint32_t jj[kGatherGroup];
+    bool in[kGatherGroup];
 #pragma unroll
-    for (int u = 0; u < kGatherGroup; ++u) ok[u] = in[u] && fst < live_first[tt[u]];
+    for (int u = 0; u < kGatherGroup; ++u) {
+      const uint32_t l = l0 + (uint32_t)u * stride;
+      in[u] = l < n && b + l < nslots;
+      jj[u] = in[u] ? b + l : 0u;
+    }
+    uint32_t op[kGatherGroup];
+#pragma unroll
+    for (int u = 0; u < kGatherGroup; ++u)
+      op[u] = in[u] ? __float_as_uint(grad_opacity[jj[u]]) : kNoRecord;
     float4 r0[kGatherGroup], r1[kGatherGroup];
-    float ro[kGatherGroup];
 #pragma unroll
     for (int u = 0; u < kGatherGroup; ++u) {
       r0[u] = r1[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      ro[u] = 0.f;
-      if (ok[u]) {
+      if (op[u] != kNoRecord) {
         r0[u] = grad_rec[2 * (size_t)jj[u]];
         r1[u] = grad_rec[2 * (size_t)jj[u] + 1];
-        ro[u] = grad_opacity[jj[u]];
       }
     }
 #pragma unroll
     for (int u = 0; u < kGatherGroup; ++u) {
       a[0] += r0[u].x; a[1] += r0[u].y; a[2] += r0[u].z; a[3] += r0[u].w;
       a[4] += r1[u].x; a[5] += r1[u].y; a[6] += r1[u].z; a[7] += r1[u].w;
-      a[8] += ro[u];
+      a[8] += op[u] != kNoRecord ? __uint_as_float(op[u]) : 0.f;
     }
   };
 
   float acc[kGradTerms];
 #pragma unroll
   for (int q = 0; q < kGradTerms; ++q) acc[q] = 0.f;
-  if (cnt <= kGatherSerial) {
-    int y = y0, c = 0;
-    for (uint32_t l0 = 0; l0 < cnt; l0 += kGatherGroup) {
-      bool in[kGatherGroup];
-      uint32_t tt[kGatherGroup], jj[kGatherGroup];
-#pragma unroll
-      for (int u = 0; u < kGatherGroup; ++u) {
-        const uint32_t l = l0 + u, j = first + l;
-        in[u] = l < cnt && j < R;
-        tt[u] = in[u] ? tile_at(y, c, x0, gx, tile_base) : 0u;
-        jj[u] = in[u] ? j : 0u;
-        if (++c == w) {
-          c = 0;
-          ++y;
-        }
-      }
-      add_group(in, tt, jj, first, acc);
-    }
-  }
+  if (cnt <= kGatherSerial)
+    for (uint32_t l0 = 0; l0 < cnt; l0 += kGatherGroup) add_group(base, cnt, l0, 1, acc);
   uint64_t big = __ballot(cnt > kGatherSerial);
   while (big) {
     const int src = __ffsll((unsigned long long)big) - 1;
     big &= big - 1;
-    const uint32_t bcnt = readlane_u32(cnt, src), bfirst = readlane_u32(first, src);
-    const uint32_t btb = readlane_u32(tile_base, src);
-    const int bx0 = (int)readlane_u32((uint32_t)x0, src), by0 = (int)readlane_u32((uint32_t)y0, src);
-    const int bw = (int)readlane_u32((uint32_t)w, src), bgx = (int)readlane_u32((uint32_t)gx, src);
-    const float inv_w = 1.0f / (float)bw;
+    const uint32_t bcnt = readlane_u32(cnt, src), bbase = readlane_u32(base, src);
     float part[kGradTerms];
 #pragma unroll
     for (int q = 0; q < kGradTerms; ++q) part[q] = 0.f;
-    for (uint32_t l0 = lane; l0 < bcnt; l0 += kWave * kGatherGroup) {
-      bool in[kGatherGroup];
-      uint32_t tt[kGatherGroup], jj[kGatherGroup];
-#pragma unroll
-      for (int u = 0; u < kGatherGroup; ++u) {
-        const uint32_t l = l0 + (uint32_t)(u * kWave), j = bfirst + l;
-        // l / w via a float estimate and one integer correction (exact for l < 2^23)
-        int qy = (int)((float)l * inv_w);
-        int c = (int)l - qy * bw;
-        if (c < 0) {
-          --qy;
-          c += bw;
-        } else if (c >= bw) {
-          ++qy;
-          c -= bw;
-        }
-        in[u] = l < bcnt && j < R;
-        tt[u] = in[u] ? tile_at(by0 + qy, c, bx0, bgx, btb) : 0u;
-        jj[u] = in[u] ? j : 0u;
-      }
-      add_group(in, tt, jj, bfirst, part);
-    }
+    for (uint32_t l0 = lane; l0 < bcnt; l0 += kWave * kGatherGroup) add_group(bbase, bcnt, l0, kWave, part);
     const float grp = wave_reduce8(part, lane);  // term q in lanes 8q .. 8q + 7
     const float sw = wave_reduce1(part[8]);
 #pragma unroll
@@ -744,25 +688,24 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
 
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
-                                  const float* dL_dpix, float4* grad_rec, float* grad_opacity, uint32_t* live_first,
-                                  hipStream_t s) {
+                                  const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
+                                  float* grad_opacity, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
   ViewTable none{};
   return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
-                                     point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, grad_rec,
-                                     grad_opacity, live_first);
+                                     point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, slot_base, nslots,
+                                     grad_rec, grad_opacity);
 }
 
-hipError_t launch_gather_grads(int P, uint32_t R, const float4* rec, const uint32_t* tiles_touched, TileGrid grid,
-                               const uint32_t* live_first, const float4* grad_rec, const float* grad_opacity,
-                               float* dL_dmean2D, float* dL_dconic, float* dL_dopacity, float* dL_dcolors,
-                               hipStream_t s) {
-  if (P == 0) return hipSuccess;
-  ViewTable none{};
-  hipLaunchKernelGGL(gather_grads_kernel<false>, dim3((P + 255) / 256), dim3(256), 0, s, P, R,
-                     (const uint32_t*)nullptr, rec, tiles_touched, grid, none, live_first, grad_rec, grad_opacity,
-                     dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
+                               const uint32_t* tiles_touched, const uint32_t* slot_local, const uint32_t* slot_base,
+                               const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
+                               float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
+  if (items == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_grads_kernel, dim3((items + 255) / 256), dim3(256), 0, s, items, P, nslots, R_dev,
+                     tiles_touched, slot_local, slot_base, grad_rec, grad_opacity, dL_dmean2D, dL_dconic, dL_dopacity,
+                     dL_dcolors);
   return hipGetLastError();
 }
 
@@ -777,24 +720,12 @@ hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges,
 
 hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                         const float* bg, const float4* rec, const float* final_T,
-                                        const uint32_t* n_contrib, const float* dL_dpix, float4* grad_rec,
-                                        float* grad_opacity, uint32_t* live_first, hipStream_t s) {
+                                        const uint32_t* n_contrib, const float* dL_dpix, const uint32_t* slot_base,
+                                        uint32_t nslots, float4* grad_rec, float* grad_opacity, hipStream_t s) {
   const uint32_t tiles = vt.tile_base[vt.V];
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
-                                    final_T, n_contrib, dL_dpix, grad_rec, grad_opacity, live_first);
-}
-
-hipError_t launch_gather_grads_views(const ViewTable& vt, uint32_t capacity, const uint32_t* R_dev,
-                                     const float4* rec, const uint32_t* tiles_touched, const uint32_t* live_first,
-                                     const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
-                                     float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
-  const int items = vt.V * vt.P;
-  if (items == 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_grads_kernel<true>, dim3((items + 255) / 256), dim3(256), 0, s, items, capacity, R_dev,
-                     rec, tiles_touched, vt.grid(0), vt, live_first, grad_rec, grad_opacity, dL_dmean2D, dL_dconic,
-                     dL_dopacity, dL_dcolors);
-  return hipGetLastError();
+                                    final_T, n_contrib, dL_dpix, slot_base, nslots, grad_rec, grad_opacity);
 }
 
 }  // namespace omnigs

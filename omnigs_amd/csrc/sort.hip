@@ -16,49 +16,55 @@
 
 namespace omnigs {
 
-// ---------------------------------------------------------------------------
-// Wave / block scans (64-lane wavefronts, 4 waves per 256-thread block).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t n = __shfl_up(v, o, kWave);
-    if (lane >= o) v += n;
-  }
-  return v;
-}
-
-// Exclusive scan of one value per thread across a 256-thread block.
-// Returns the exclusive prefix; *total receives the block sum.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t wsum[kScanThreads / kWave];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const uint32_t inc = wave_inclusive_scan(v, lane);
-  if (lane == kWave - 1) wsum[w] = inc;
-  __syncthreads();
-  uint32_t woff = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < kScanThreads / kWave; ++i) {
-    const uint32_t s = wsum[i];
-    woff += i < w ? s : 0u;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return woff + inc - v;
-}
-
 __device__ __forceinline__ uint32_t load_in(const uint32_t* __restrict__ in, const uint32_t* __restrict__ idx,
                                             size_t i) {
   return idx ? in[idx[i]] : in[i];
 }
 
+// Exclusive scan of a short array by one block, kScanTile values at a time:
+// the chunk is loaded into LDS coalesced, thread t scans its kScanItems
+// consecutive values, the block scans the run sums (plus the carry of earlier
+// chunks), and the chunk leaves coalesced.
+__device__ void block_scan_array(const uint32_t* __restrict__ in, size_t n, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_v[kScanTile];
+  uint32_t carry = 0;
+  for (size_t c0 = 0; c0 < n; c0 += kScanTile) {
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) s_v[k] = c0 + k < n ? in[c0 + k] : 0u;
+    __syncthreads();
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) run += s_v[threadIdx.x * kScanItems + k];
+    uint32_t tot;
+    uint32_t pre = carry + block_exclusive_scan(run, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const uint32_t v = s_v[threadIdx.x * kScanItems + k];
+      s_v[threadIdx.x * kScanItems + k] = pre;
+      pre += v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
+      if (c0 + k < n) out[c0 + k] = s_v[k];
+    carry += tot;
+    __syncthreads();
+  }
+}
+
 // Block sums; with a gather index the gathered values are also written to
-// `gathered` so that the downsweep reads them contiguously.
+// `gathered` so that the downsweep reads them contiguously.  One block past
+// the last (launched when side_in is set) scans the unrelated short array
+// side_in -> side_out (the per-block gradient slot counts) while the others
+// reduce, so its serial chunks cost no launch and no time of their own.
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                    const uint32_t* __restrict__ idx, size_t n,
                                                                    uint32_t* __restrict__ partials,
-                                                                   uint32_t* __restrict__ gathered) {
+                                                                   uint32_t* __restrict__ gathered,
+                                                                   const uint32_t* __restrict__ side_in, size_t side_n,
+                                                                   uint32_t* __restrict__ side_out) {
+  if (side_in && blockIdx.x == gridDim.x - 1) {
+    block_scan_array(side_in, side_n, side_out);
+    return;
+  }
   const size_t base = (size_t)blockIdx.x * kScanTile;
   uint32_t s = 0;
 #pragma unroll
@@ -75,7 +81,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// Exclusive scan of the block partials in place, one block looping over chunks.
+// Exclusive scan of the block partials in place, one block looping over
+// chunks; their total goes to partials[nb].
 __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ partials, size_t nb) {
   uint32_t carry = 0;
   for (size_t c = 0; c < nb; c += kScanThreads) {
@@ -115,11 +122,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_downsweep_kernel(const uint
 }
 
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
-                       uint32_t* partials, hipStream_t s, uint32_t* gather_buf) {
+                       uint32_t* partials, hipStream_t s, uint32_t* gather_buf, const uint32_t* side_in,
+                       size_t side_n, uint32_t* side_out) {
   if (n == 0) return hipSuccess;
   const size_t nb = scan_blocks(n);
   uint32_t* gathered = idx ? gather_buf : nullptr;
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, in, idx, n, partials, gathered);
+  const bool side = side_in && side_n;
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb + (side ? 1 : 0)), dim3(kScanThreads), 0, s, in, idx, n, partials,
+                     gathered, side ? side_in : nullptr, side_n, side_out);
   hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, nb);
   const uint32_t* din = gathered ? gathered : in;
   const uint32_t* didx = gathered ? nullptr : idx;
@@ -340,10 +350,11 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ status,
                                                                 const uint32_t* __restrict__ order,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                float4* __restrict__ rec,
+                                                                const float4* __restrict__ rec,
                                                                 const int* __restrict__ radii, TileGrid grid,
                                                                 ViewTable vt, uint32_t* __restrict__ tile_key,
-                                                                uint32_t* __restrict__ gid) {
+                                                                uint32_t* __restrict__ gid,
+                                                                uint32_t* __restrict__ no_record) {
   constexpr int kViewSlots = kMulti ? kDupChunk : 1;
   __shared__ uint32_t s_end[kDupChunk];
   __shared__ uint32_t s_gid[kDupChunk];
@@ -376,13 +387,9 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     const uint32_t gg = order[s];
     s_end[j] = offsets[s];
     s_gid[j] = gg;
-    // the radius is read from the record (the preprocess stores it there, as
-    // it does in radii[]) so that the line this pass writes `first` into --
-    // the Gaussian's first instance, kept for the backward -- is the one it
-    // already reads
+    // the radius as the preprocess stored it in the record (and in radii[])
     const float4 r0 = rec[3 * gg];
     const int radius = __float_as_int(rec[3 * gg + 2].y);
-    reinterpret_cast<uint32_t*>(rec)[12 * (size_t)gg + kRecFirstWord] = s == 0 ? 0u : offsets[s - 1];
     TileGrid gv = grid;
     if constexpr (kMulti) {
       const int v = (int)(gg / (uint32_t)vt.P);
@@ -431,41 +438,49 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
       ++q;
     }
   }
+  // The gradient slots number as many as the instances: this pass also marks
+  // every one as holding no record (kNoRecord in its opacity word) for the
+  // backward, which overwrites the slots it stages.
   if (kb + kDupRun <= k1) {  // whole run: 16-byte stores (kb is a multiple of kDupRun)
     uint4* tk = reinterpret_cast<uint4*>(tile_key + kb);
     uint4* gd = reinterpret_cast<uint4*>(gid + kb);
+    uint4* nr = reinterpret_cast<uint4*>(no_record + kb);
     tk[0] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
     tk[1] = make_uint4(keys[4], keys[5], keys[6], keys[7]);
     gd[0] = make_uint4(ids[0], ids[1], ids[2], ids[3]);
     gd[1] = make_uint4(ids[4], ids[5], ids[6], ids[7]);
+    nr[0] = nr[1] = make_uint4(kNoRecord, kNoRecord, kNoRecord, kNoRecord);
   } else {
 #pragma unroll
     for (int u = 0; u < kDupRun; ++u) {
       if (kb + (uint32_t)u < k1) {
         tile_key[kb + u] = keys[u];
         gid[kb + u] = ids[u];
+        no_record[kb + u] = kNoRecord;
       }
     }
   }
 }
 
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, float4* rec,
-                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
+                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
+                            hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
   hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
-                     (uint32_t*)nullptr, order, offsets, rec, radii, grid, none, tile_key, gid);
+                     (uint32_t*)nullptr, order, offsets, rec, radii, grid, none, tile_key, gid, no_record);
   return hipGetLastError();
 }
 
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
-                                  const uint32_t* order, const uint32_t* offsets, float4* rec, const int* radii,
-                                  const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, hipStream_t s) {
+                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
+                                  hipStream_t s) {
   if (items == 0 || capacity == 0) return hipSuccess;
   const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
-                     order, offsets, rec, radii, vt.grid(0), vt, tile_key, gid);
+                     order, offsets, rec, radii, vt.grid(0), vt, tile_key, gid, no_record);
   return hipGetLastError();
 }
 
