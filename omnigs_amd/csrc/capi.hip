@@ -261,7 +261,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     // The same launches scan the preprocess blocks' slot counts into the first
     // gradient slot of every block (the backward's records).
     OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s, g.key_alt,
-                           g.block_slots, slot_blocks(P), g.slot_base),
+                           g.block_slots, slot_blocks(P), g.slot_base, g.dup_rect, g.rect_depth),
                "scan");
   }
 
@@ -291,7 +291,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   const BinState b = carve_bin(bbuf, R);
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
-    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rec, radii, grid, b.tile_key, b.gid,
+    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rect_depth, grid, b.tile_key, b.gid,
                                 reinterpret_cast<uint32_t*>(b.grad_opacity), s),
                "duplicate");
   }
@@ -522,14 +522,14 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   {
     StageTimer tm(OMNIGS_STAGE_SCAN, s);
     OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, items, false, g.scan_partials, s, g.key_alt,
-                           g.block_slots, slot_blocks(P), g.slot_base),
+                           g.block_slots, slot_blocks(P), g.slot_base, g.dup_rect, g.rect_depth),
                "batch scan");
   }
   const uint32_t* R_dev = g.offsets + (items - 1);
   const BinState& b = bs.b;
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
-    OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rec, g.radii,
+    OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rect_depth,
                                       vt, b.tile_key, b.gid, reinterpret_cast<uint32_t*>(b.grad_opacity), s),
                "batch duplicate");
   }

@@ -94,6 +94,13 @@ __device__ __forceinline__ Rect tile_rect(float2 p, int r, TileGrid g) {
   return {rect_lo(p.x, r, g.bx, g.gx), rect_lo(p.y, r, g.by, g.gy), rect_hi(p.x, r, g.bx, g.gx),
           rect_hi(p.y, r, g.by, g.gy)};
 }
+// A Gaussian's expansion record (see GeomState::dup_rect); a culled one has
+// area 0 and a one-column rect.
+__device__ __forceinline__ uint4 pack_dup_rect(bool live, int area, const Rect& rc) {
+  if (!live) return make_uint4(0u, 0u, 1u << 16, 0u);
+  const uint32_t w = (uint32_t)max(1, rc.x1 - rc.x0);
+  return make_uint4((uint32_t)area, (uint32_t)rc.x0, (uint32_t)rc.y0 | (w << 16), 0u);
+}
 // Tile column of rect column x (identity in clamp mode).
 __device__ __forceinline__ int wrap_col(int x, TileGrid g) { return g.wrap ? ((x % g.gx) + g.gx) % g.gx : x; }
 
@@ -200,6 +207,12 @@ struct GeomState {
   float4* rec;
   uint8_t* clamped;        // bit c set when channel c was clamped at 0
   uint32_t* tiles_touched; // rect area
+  // Expansion input per Gaussian, in index order: {area, rect x0 (int bits),
+  // y0 | w << 16, 0} (pack_dup_rect); the offset scan gathers it once through
+  // the depth order into rect_depth {x0, y0 | w << 16}, which the expansion
+  // then reads contiguously.
+  uint4* dup_rect;
+  uint2* rect_depth;
   uint32_t* slot_local;    // copy of the record's `slot` (dense, for the gradient gather)
   uint32_t* block_slots;   // per 256-Gaussian block: its Gaussians' total rect area
   uint32_t* slot_base;     // exclusive scan of block_slots (written by the forward scan)
@@ -220,6 +233,8 @@ inline GeomState carve_geom(char* base, size_t P) {
   g.rec = c.take<float4>(3 * P);
   g.clamped = c.take<uint8_t>(P);
   g.tiles_touched = c.take<uint32_t>(P);
+  g.dup_rect = c.take<uint4>(P);
+  g.rect_depth = c.take<uint2>(P);
   g.slot_local = c.take<uint32_t>(P);
   g.block_slots = c.take<uint32_t>(slot_blocks(P) + 1);
   g.slot_base = c.take<uint32_t>(slot_blocks(P) + 1);

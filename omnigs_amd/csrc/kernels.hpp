@@ -60,9 +60,13 @@ hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmat
 // idx (out[i] = sum of in[idx[j]] for j <= i).  partials needs scan_blocks(n)+1 entries.
 // side_in (optional): a short array (a few thousand values) whose exclusive
 // scan goes to side_out in the same launches (the gradient slot bases).
+// rect_in (optional, with idx and gather_buf): the values are the area words
+// of these expansion records, gathered through idx (in is then unused), and
+// the rest of each gathered record is written to rect_out in scan order.
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
                        uint32_t* partials, hipStream_t s, uint32_t* gather_buf = nullptr,
-                       const uint32_t* side_in = nullptr, size_t side_n = 0, uint32_t* side_out = nullptr);
+                       const uint32_t* side_in = nullptr, size_t side_n = 0, uint32_t* side_out = nullptr,
+                       const uint4* rect_in = nullptr, uint2* rect_out = nullptr);
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned
@@ -76,16 +80,17 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
                              hipStream_t s, SortResult* out, const uint32_t* n_dev = nullptr);
 // no_record: the gradient slots' opacity words, set to kNoRecord (one per instance).
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
-                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                            hipStream_t s);
+// rect_depth: the Gaussians' rects in depth order (written by launch_scan).
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
+                            const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
+                            uint32_t* no_record, hipStream_t s);
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s, const uint32_t* R_dev = nullptr);
 // Batched expansion: items = V P, capacity instances at most; R_dev is the
 // device-side instance count (last inclusive offset); status [2] receives
 // {count needed, 1 if it exceeds the capacity}.
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
-                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
                                   hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);

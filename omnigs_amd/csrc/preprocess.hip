@@ -24,6 +24,7 @@ struct GeomOut {
   float2 pix;
   float4 conic;
   int radius, area;
+  Rect rc;  // tile rect
 };
 
 // Per-Gaussian inputs of the projection (read once per Gaussian).
@@ -87,6 +88,7 @@ __device__ __forceinline__ bool project_view(const GaussGeo& gg, const float* vi
   const float py = (float)((((double)sy + 1.0) * (double)H - 1.0) * 0.5);
   o.pix = make_float2(px, py);
   const Rect rc = tile_rect(o.pix, o.radius, grid);
+  o.rc = rc;
   o.area = (rc.x1 - rc.x0) * (rc.y1 - rc.y0);
   o.r = r;
   return o.area != 0;
@@ -147,7 +149,10 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
   uint32_t block_total;
   const uint32_t slot = block_exclusive_scan(live ? (uint32_t)o.area : 0u, &block_total);
   if (threadIdx.x == 0) g.block_slots[blockIdx.x] = block_total;
-  if (in_range) g.slot_local[idx] = slot;
+  if (in_range) {
+    g.slot_local[idx] = slot;
+    g.dup_rect[idx] = pack_dup_rect(live, o.area, o.rc);
+  }
   if ((kMlp ? (idx & ~63) : idx) >= a.P) return;  // MLP: whole waves evaluate the decoder
 
   float3 col;
@@ -217,7 +222,10 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
     uint32_t view_total;
     const uint32_t slot = view_prefix + block_exclusive_scan(live ? (uint32_t)o.area : 0u, &view_total);
     view_prefix += view_total;
-    if (in_range) g.slot_local[item] = slot;
+    if (in_range) {
+      g.slot_local[item] = slot;
+      g.dup_rect[item] = pack_dup_rect(live, o.area, o.rc);
+    }
     const float3 cp = make_float3(camposv[3 * v], camposv[3 * v + 1], camposv[3 * v + 2]);
     float3 col;
     if constexpr (kMlp) {

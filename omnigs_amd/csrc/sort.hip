@@ -55,12 +55,17 @@ __device__ void block_scan_array(const uint32_t* __restrict__ in, size_t n, uint
 // the last (launched when side_in is set) scans the unrelated short array
 // side_in -> side_out (the per-block gradient slot counts) while the others
 // reduce, so its serial chunks cost no launch and no time of their own.
+// kRect: the values are the area words of the expansion records rect_in,
+// gathered through idx; the rest of each record goes to rect_out[i].
+template <bool kRect>
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                    const uint32_t* __restrict__ idx, size_t n,
                                                                    uint32_t* __restrict__ partials,
                                                                    uint32_t* __restrict__ gathered,
                                                                    const uint32_t* __restrict__ side_in, size_t side_n,
-                                                                   uint32_t* __restrict__ side_out) {
+                                                                   uint32_t* __restrict__ side_out,
+                                                                   const uint4* __restrict__ rect_in,
+                                                                   uint2* __restrict__ rect_out) {
   if (side_in && blockIdx.x == gridDim.x - 1) {
     block_scan_array(side_in, side_n, side_out);
     return;
@@ -71,7 +76,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
   for (int k = 0; k < kScanItems; ++k) {
     const size_t i = base + (size_t)k * kScanThreads + threadIdx.x;
     if (i < n) {
-      const uint32_t v = load_in(in, idx, i);
+      uint32_t v;
+      if constexpr (kRect) {
+        const uint4 r = rect_in[idx[i]];
+        v = r.x;
+        rect_out[i] = make_uint2(r.y, r.z);
+      } else {
+        v = load_in(in, idx, i);
+      }
       if (gathered) gathered[i] = v;
       s += v;
     }
@@ -123,13 +135,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_downsweep_kernel(const uint
 
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
                        uint32_t* partials, hipStream_t s, uint32_t* gather_buf, const uint32_t* side_in,
-                       size_t side_n, uint32_t* side_out) {
+                       size_t side_n, uint32_t* side_out, const uint4* rect_in, uint2* rect_out) {
   if (n == 0) return hipSuccess;
   const size_t nb = scan_blocks(n);
   uint32_t* gathered = idx ? gather_buf : nullptr;
   const bool side = side_in && side_n;
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb + (side ? 1 : 0)), dim3(kScanThreads), 0, s, in, idx, n, partials,
-                     gathered, side ? side_in : nullptr, side_n, side_out);
+  const dim3 grid(nb + (side ? 1 : 0));
+  if (rect_in && idx && gathered) {
+    hipLaunchKernelGGL(scan_reduce_kernel<true>, grid, dim3(kScanThreads), 0, s, in, idx, n, partials, gathered,
+                       side ? side_in : nullptr, side_n, side_out, rect_in, rect_out);
+  } else {
+    hipLaunchKernelGGL(scan_reduce_kernel<false>, grid, dim3(kScanThreads), 0, s, in, idx, n, partials, gathered,
+                       side ? side_in : nullptr, side_n, side_out, (const uint4*)nullptr, (uint2*)nullptr);
+  }
   hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, nb);
   const uint32_t* din = gathered ? gathered : in;
   const uint32_t* didx = gathered ? nullptr : idx;
@@ -350,8 +368,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ status,
                                                                 const uint32_t* __restrict__ order,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                const float4* __restrict__ rec,
-                                                                const int* __restrict__ radii, TileGrid grid,
+                                                                const uint2* __restrict__ rect_depth, TileGrid grid,
                                                                 ViewTable vt, uint32_t* __restrict__ tile_key,
                                                                 uint32_t* __restrict__ gid,
                                                                 uint32_t* __restrict__ no_record) {
@@ -387,20 +404,12 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     const uint32_t gg = order[s];
     s_end[j] = offsets[s];
     s_gid[j] = gg;
-    // the radius as the preprocess stored it in the record (and in radii[])
-    const float4 r0 = rec[3 * gg];
-    const int radius = __float_as_int(rec[3 * gg + 2].y);
-    TileGrid gv = grid;
-    if constexpr (kMulti) {
-      const int v = (int)(gg / (uint32_t)vt.P);
-      s_view[j] = (uint8_t)v;
-      gv = vt.grid(v);
-    }
-    const Rect rc = tile_rect(make_float2(r0.x, r0.y), radius, gv);
-    const int w = max(1, rc.x1 - rc.x0);
-    s_x0[j] = rc.x0;
-    s_y0[j] = (uint16_t)rc.y0;
-    s_w[j] = (uint16_t)w;
+    if constexpr (kMulti) s_view[j] = (uint8_t)(gg / (uint32_t)vt.P);
+    // the rect in depth order, as the offset scan gathered it
+    const uint2 rd = rect_depth[s];
+    s_x0[j] = (int)rd.x;
+    s_y0[j] = (uint16_t)(rd.y & 0xFFFFu);
+    s_w[j] = (uint16_t)(rd.y >> 16);
   }
   __syncthreads();
   // Each thread expands kDupRun consecutive instances: one search finds the
@@ -462,25 +471,25 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   }
 }
 
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets, const float4* rec,
-                            const int* radii, TileGrid grid, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                            hipStream_t s) {
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
+                            const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
+                            uint32_t* no_record, hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
   hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
-                     (uint32_t*)nullptr, order, offsets, rec, radii, grid, none, tile_key, gid, no_record);
+                     (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record);
   return hipGetLastError();
 }
 
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
-                                  const uint32_t* order, const uint32_t* offsets, const float4* rec, const int* radii,
+                                  const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
                                   hipStream_t s) {
   if (items == 0 || capacity == 0) return hipSuccess;
   const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
-                     order, offsets, rec, radii, vt.grid(0), vt, tile_key, gid, no_record);
+                     order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record);
   return hipGetLastError();
 }
 
