@@ -408,8 +408,13 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     const float power2 = power2_of(make_float3(pa.z, pa.w, pb.x), dx, dy);
     const float G = exp2_hw(power2);
     const float alpha = fminf(0.99f, pb.y * G);
-    const bool ok = j > lim && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
-    if (__builtin_amdgcn_ballot_w64(ok) == 0ull) return false;
+    // three compares whose lane masks combine on the scalar unit (a ballot of
+    // the combined bool would rematerialise it as a vector value first)
+    const bool c_pos = j > lim, c_pow = power2 <= 0.0f, c_alpha = alpha >= 1.0f / 255.0f;
+    const bool ok = c_pos && c_pow && c_alpha;
+    if ((__builtin_amdgcn_ballot_w64(c_pos) & __builtin_amdgcn_ballot_w64(c_pow) &
+         __builtin_amdgcn_ballot_w64(c_alpha)) == 0ull)
+      return false;
     wgt = 0.f;
     v = 0.f;
     if (ok) {
@@ -491,16 +496,20 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       float ta[8];
       moments(wa, va, xa, ya, ta);
       if (!fb) {
-        const float grp = wave_reduce8(ta, lane);
-        const float sw = wave_reduce1(wa);
+        float grp = wave_reduce8(ta, lane);
+        float sw = wave_reduce1(wa);
+        asm volatile("" : "+v"(grp), "+v"(sw));
         if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][ja], grp);
         if (lane == 0) atomicAdd(&s_acc[8][ja], sw);
         break;
       }
       float tb[8];
       moments(wb, vb, xb, yb, tb);
-      const float grp = wave_reduce16(ta, tb, lane);
-      const float sw = wave_reduce2(wa, wb);
+      float grp = wave_reduce16(ta, tb, lane);
+      float sw = wave_reduce2(wa, wb);
+      // materialise both sums here, so that their last DPP add stays fused
+      // instead of being sunk into the atomics' lane branches
+      asm volatile("" : "+v"(grp), "+v"(sw));
       const int jj = (lane & 32) ? jb : ja;
       if ((lane & 3) == 0) atomicAdd(&s_acc[(lane >> 2) & 7][jj], grp);
       if ((lane & 31) == 0) atomicAdd(&s_acc[8][jj], sw);
