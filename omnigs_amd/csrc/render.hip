@@ -172,13 +172,16 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   const uint2 range = ranges[blockIdx.x];
   const int n = (int)(range.y - range.x);
 
-  bool done = !inside;
+  // Lanes that are done (outside the image, or stopped), as a wave-uniform
+  // lane mask: the per-pair decisions below are lane masks combined on the
+  // scalar unit rather than per-lane booleans.
+  uint64_t done_mask = __builtin_amdgcn_ballot_w64(!inside);
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
   uint32_t last = 0;
 
   for (int base = 0; base < n; base += kBatch) {
-    if (__syncthreads_count(done) == kTilePixels) break;
+    if (__syncthreads_count(__builtin_amdgcn_inverse_ballot_w64(done_mask)) == kTilePixels) break;
     const int k = base + tid;
     if (k < n) {
       const uint32_t g = point_list[range.x + k];
@@ -208,10 +211,11 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const float alpha = fminf(0.99f, pb.y * exp2_hw(power2));
       const float test_T = T * (1.0f - alpha);
       // power2 > 0 or alpha < 1/255: skipped; test_T < 1e-4: stop before blending
-      const bool ok = !done && power2 <= 0.0f && alpha >= 1.0f / 255.0f;
-      const bool stop = ok && test_T < 0.0001f;
-      const bool blend = ok && !stop;
-      done = done || stop;
+      const uint64_t ok = __builtin_amdgcn_ballot_w64(power2 <= 0.0f) &
+                          __builtin_amdgcn_ballot_w64(alpha >= 1.0f / 255.0f) & ~done_mask;
+      const uint64_t stop = ok & __builtin_amdgcn_ballot_w64(test_T < 0.0001f);
+      const bool blend = __builtin_amdgcn_inverse_ballot_w64(ok & ~stop);
+      done_mask |= stop;
       const float wgt = blend ? alpha * T : 0.f;
       C0 += pb.z * wgt;
       C1 += pb.w * wgt;
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       pa = s_pa[j];
       pb = s_pb[j];
       c2 = s_blue[j];
-      if (__ballot(!done) == 0ull) break;
+      if (done_mask == ~0ull) break;  // all 64 lanes done
     }
   }
 
