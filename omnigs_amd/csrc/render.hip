@@ -397,8 +397,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   // The reference carries the colour accumulated behind each instance per
   // channel, acc_c <- la lc_c + (1 - la) acc_c, and only ever uses
   // sum_c (c_c - acc_c) dL/dpix_c; the same recurrence on the dot product
-  // s = acc . dL/dpix (with lcd = lc . dL/dpix) gives that sum directly.
-  float acc_dot = 0.f, last_alpha = 0.f, last_cd = 0.f;
+  // s = acc . dL/dpix (with lcd = lc . dL/dpix) gives that sum directly.  It
+  // is advanced right after each blended instance rather than at the next
+  // one, which reuses c . dL/dpix - s and carries no last alpha or colour.
+  float acc_dot = 0.f;
 
   // One (wave, instance) pair: updates the lanes' blending state and returns,
   // wave-uniformly, whether any lane blends it.  If one does, (wgt, v) are
@@ -428,10 +430,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       T = T * r;
       v = alpha * T;
       const float cd = pb.z * dp0 + pb.w * dp1 + s_blue[j] * dp2;
-      acc_dot += last_alpha * (last_cd - acc_dot);  // = la lcd + (1 - la) acc
-      const float dL_dalpha = (cd - acc_dot) * T + neg_tb * r;
-      last_alpha = alpha;
-      last_cd = cd;
+      const float diff = cd - acc_dot;
+      const float dL_dalpha = diff * T + neg_tb * r;
+      // the colour behind the next blended instance: alpha cd + (1 - alpha) acc
+      acc_dot += alpha * diff;
       wgt = G * dL_dalpha;
     }
     return true;
