@@ -57,7 +57,7 @@ typedef struct omnigs_buffer {
 } omnigs_buffer;
 
 typedef struct omnigs_lonlat_forward_args {
-  int P;                       /* number of Gaussians */
+  int P;                       /* number of Gaussians (at most 89478485 per call) */
   int D;                       /* active SH degree (0..3) */
   int M;                       /* SH coefficients per Gaussian (shs is [P,M,3]) */
   int width, height;           /* equirectangular image size in pixels */

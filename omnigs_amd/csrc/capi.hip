@@ -215,6 +215,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   *num_rendered = 0;
   const int P = a->P, W = a->width, H = a->height;
   if (P < 0 || W <= 0 || H <= 0) return fail_arg("P < 0 or empty image");
+  if (P > kMaxGaussians) return fail_arg("more than 89478485 Gaussians in one call");
   if (P == 0) return OMNIGS_OK;  // the reference boundary skips the rasterizer entirely
   if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || (render && !a->out_color))
     return fail_arg("missing required pointer");
@@ -481,6 +482,7 @@ size_t omnigs_lonlat_batch_workspace_bytes(int P, int V, const int* widths, cons
 int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_) {
   if (!a) return fail_arg("NULL args");
   if (a->P < 0) return fail_arg("P < 0");
+  if (a->P > kMaxGaussians) return fail_arg("more than 89478485 Gaussians in one call");
   ViewTable vt;
   const int st = make_view_table(a->P, a->V, a->widths, a->heights, a->tile_x, a->tile_y, a->seam_wrap, vt);
   if (st != OMNIGS_OK) return st;

@@ -189,6 +189,9 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 // Gradient slots are numbered per block of kSlotBlock Gaussians (the
 // preprocess's thread blocks): block b's slots start at slot_base[b].
 constexpr int kSlotBlock = 256;
+// The render kernels hold a tile-list position times 48 (a staging byte
+// offset) in 32 bits, so a tile's list -- at most P entries -- is bounded.
+constexpr int kMaxGaussians = 89478485;  // floor((2^32 - 1) / 48)
 inline size_t slot_blocks(size_t n) { return (n + kSlotBlock - 1) / kSlotBlock; }
 // Opacity word of a gradient slot that holds no record (a NaN pattern the
 // hardware never produces: its NaNs are the canonical 0x7FC00000).

@@ -127,11 +127,24 @@ __device__ __forceinline__ uint32_t quadrant_mask(float2 m, float4 co, int tile,
   return mk;
 }
 
+// A staged instance: {x, y, a, b}, {c, opacity, r, g}, {blue, ...} with
+// (a, b, c) the prescaled conic, 48 bytes, so that one LDS address register
+// (the list entry, a byte offset) reaches every operand of a pair.
+constexpr int kStageBytes = 48;
+struct alignas(16) Staged {
+  float4 a, b, c;
+};
+__device__ __forceinline__ const Staged& staged_at(const Staged* base, int off) {
+  return *reinterpret_cast<const Staged*>(reinterpret_cast<const char*>(base) + off);
+}
+
 // Each wave compacts the batch positions whose mask has its bit set into its
-// own ascending list; returns the list length (wave-uniform).  Slots below
-// `first_slot` are dropped (backward: positions past every lane's last
-// contributor).
-__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* list, int cnt, int w, int lane,
+// own ascending list, entry j * kStride (the forward stores staging byte
+// offsets, the backward positions); returns the list length (wave-uniform).
+// Slots below `first_slot` are dropped (backward: positions past every lane's
+// last contributor).
+template <int kStride, typename Entry>
+__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, Entry* list, int cnt, int w, int lane,
                                                int first_slot = 0) {
   int count = 0;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -140,7 +153,7 @@ __device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, uint8_t* l
     const int j = c * kWave + lane;
     const bool on = j >= first_slot && j < cnt && ((s_mask[j] >> w) & 1u);
     const uint64_t bal = __ballot(on);
-    if (on) list[count + __popcll(bal & below)] = (uint8_t)j;
+    if (on) list[count + __popcll(bal & below)] = (Entry)(j * kStride);
     count += __popcll(bal);
   }
   return count;
@@ -154,13 +167,9 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
-  // a pair's operands in two 16-byte reads at one address, plus blue:
-  // {x, y, a, b}, {c, opacity, r, g} with (a, b, c) the prescaled conic
-  __shared__ float4 s_pa[kBatch];
-  __shared__ float4 s_pb[kBatch];
-  __shared__ float s_blue[kBatch];
+  __shared__ Staged s_op[kBatch];
   __shared__ uint8_t s_mask[kBatch];
-  __shared__ uint8_t s_list[kWavesPerTile][kBatch];
+  __shared__ uint16_t s_list[kWavesPerTile][kBatch];
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), w = tid / kWave;
@@ -178,7 +187,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   uint64_t done_mask = __builtin_amdgcn_ballot_w64(!inside);
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
-  uint32_t last = 0;
+  uint32_t last_b = 0;  // n_contrib times kStageBytes
 
   for (int base = 0; base < n; base += kBatch) {
     if (__syncthreads_count(__builtin_amdgcn_inverse_ballot_w64(done_mask)) == kTilePixels) break;
@@ -189,21 +198,23 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const float2 m = rr.xy;
       const float4 co = rr.co;
       const float3 q = prescale_conic(co);
-      s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
-      s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
-      s_blue[tid] = rr.rgb.z;
+      s_op[tid].a = make_float4(m.x, m.y, q.x, q.y);
+      s_op[tid].b = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
+      s_op[tid].c = make_float4(rr.rgb.z, 0.f, 0.f, 0.f);
       s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
     __syncthreads();
     const int cnt = min(kBatch, n - base);
-    const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane);
+    const int len = build_wave_list<kStageBytes>(s_mask, s_list[w], cnt, w, lane);
+    // a blended pair's 1-based position base + j + 1, in staging bytes
+    const uint32_t pos_bias = (uint32_t)(base + 1) * kStageBytes;
     // Branch-free over the wave's list: every lane evaluates every pair and
     // the reference's skip / stop decisions become selects, so the loop stays
     // wave-uniform; it ends once every lane of the wave is done.  The next
     // pair's operands are fetched from LDS while the current one computes.
-    int j = s_list[w][0];  // a stale slot when len == 0: never used
-    float4 pa = s_pa[j], pb = s_pb[j];
-    float c2 = s_blue[j];
+    int jo = s_list[w][0];  // a stale entry when len == 0: never used
+    float4 pa = staged_at(s_op, jo).a, pb = staged_at(s_op, jo).b;
+    float c2 = staged_at(s_op, jo).c.x;
     for (int ii = 0; ii < len; ++ii) {
       const int jn = s_list[w][min(ii + 1, len - 1)];
       const float dx = seam_dx<kWrap>(pa.x - fx, Wf, invW), dy = pa.y - fy;
@@ -221,11 +232,11 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       C1 += pb.w * wgt;
       C2 += c2 * wgt;
       T = blend ? test_T : T;
-      last = blend ? (uint32_t)(base + j + 1) : last;  // 1-based position in the tile's list
-      j = jn;
-      pa = s_pa[j];
-      pb = s_pb[j];
-      c2 = s_blue[j];
+      last_b = blend ? pos_bias + (uint32_t)jo : last_b;  // 1-based position in the tile's list, in bytes
+      jo = jn;
+      pa = staged_at(s_op, jo).a;
+      pb = staged_at(s_op, jo).b;
+      c2 = staged_at(s_op, jo).c.x;
       if (done_mask == ~0ull) break;  // all 64 lanes done
     }
   }
@@ -234,7 +245,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     const size_t pid = (size_t)py * W + px;
     const size_t N = (size_t)W * H;
     final_T[loc.pix + pid] = T;
-    n_contrib[loc.pix + pid] = last;
+    n_contrib[loc.pix + pid] = last_b / (uint32_t)kStageBytes;
     float* out = out_color + 3 * loc.pix;
     out[pid] = C0 + T * bg[0];
     out[N + pid] = C1 + T * bg[1];
@@ -480,7 +491,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     const int cnt = min(kBatch, n - base);
     // slot j holds position n - 1 - (base + j); keep positions < wlast
     const int first_slot = n - base - wlast;
-    const int len = build_wave_list(s_mask, s_list[w], cnt, w, lane, first_slot);
+    const int len = build_wave_list<1>(s_mask, s_list[w], cnt, w, lane, first_slot);
     // Pairs that some lane blends are reduced two at a time (wave_reduce16);
     // a last odd one alone.  The list order (back to front) is kept.
     // lanes outside the image have last = 0, so no slot passes their limit
