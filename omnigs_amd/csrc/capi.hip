@@ -290,18 +290,19 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     return OMNIGS_ERR_ALLOC;
   }
   const BinState b = carve_bin(bbuf, R);
+  const int tile_bits = bit_width_u32((uint32_t)grid.tiles());
   {
+    // the expansion also counts the tile sort's first digit
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rect_depth, grid, b.tile_key, b.gid,
-                                reinterpret_cast<uint32_t*>(b.grad_opacity), s),
+                                reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, s),
                "duplicate");
   }
   SortResult ts{b.tile_key, b.gid};
   if (R > 0) {
     StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
-    const int bits = bit_width_u32((uint32_t)grid.tiles());
-    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, R, bits, b.sort_counts,
-                                 b.scan_partials, s, &ts),
+    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, R, tile_bits, b.sort_counts,
+                                 b.scan_partials, s, &ts, nullptr, true),
                "tile sort");
   }
   {
@@ -529,17 +530,19 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   }
   const uint32_t* R_dev = g.offsets + (items - 1);
   const BinState& b = bs.b;
+  const int tile_bits = bit_width_u32(vt.tile_base[vt.V]);
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rect_depth,
-                                      vt, b.tile_key, b.gid, reinterpret_cast<uint32_t*>(b.grad_opacity), s),
+                                      vt, b.tile_key, b.gid, reinterpret_cast<uint32_t*>(b.grad_opacity),
+                                      b.sort_counts, tile_bits, s),
                "batch duplicate");
   }
   SortResult ts;
   {
     StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
-    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, a->capacity,
-                                 bit_width_u32(vt.tile_base[vt.V]), b.sort_counts, b.scan_partials, s, &ts, R_dev),
+    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, a->capacity, tile_bits,
+                                 b.sort_counts, b.scan_partials, s, &ts, R_dev, true),
                "batch tile sort");
   }
   {

@@ -76,14 +76,21 @@ struct SortResult {
   uint32_t* vals;
 };
 // n_dev (optional): the element count on the device, n then being the capacity.
+// first_counted: the first pass's count table is already in counts (written
+// by the expansion, whose blocks are the sort's tiles).
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* partials,
-                             hipStream_t s, SortResult* out, const uint32_t* n_dev = nullptr);
+                             hipStream_t s, SortResult* out, const uint32_t* n_dev = nullptr,
+                             bool first_counted = false);
+// Digit mask of the first pass of a `bits`-bit sort.
+uint32_t radix_first_mask(int bits);
 // no_record: the gradient slots' opacity words, set to kNoRecord (one per instance).
 // rect_depth: the Gaussians' rects in depth order (written by launch_scan).
+// counts (optional): receives the first-pass count table of a key_bits-bit
+// tile sort of the expanded keys (launch_radix_sort's first_counted).
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
-                            uint32_t* no_record, hipStream_t s);
+                            uint32_t* no_record, uint32_t* counts, int key_bits, hipStream_t s);
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s, const uint32_t* R_dev = nullptr);
 // Batched expansion: items = V P, capacity instances at most; R_dev is the
@@ -92,7 +99,7 @@ hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2*
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
                                   const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                                  hipStream_t s);
+                                  uint32_t* counts, int key_bits, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // render.hip

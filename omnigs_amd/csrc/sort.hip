@@ -293,9 +293,16 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   }
 }
 
+uint32_t radix_first_mask(int bits) {
+  if (bits < 1) bits = 1;
+  const int passes = radix_passes(bits);
+  const int width = (bits + passes - 1) / passes;
+  return (1u << (bits < width ? bits : width)) - 1u;
+}
+
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t* vals_in, uint32_t* vals,
                              uint32_t* vals_alt, size_t n, int bits, uint32_t* counts, uint32_t* /*partials*/,
-                             hipStream_t s, SortResult* out, const uint32_t* n_dev) {
+                             hipStream_t s, SortResult* out, const uint32_t* n_dev, bool first_counted) {
   // Pass p reads (src) and writes (dst); the first pass reads vals_in (or
   // indices) and writes vals_alt, so an odd pass count ends in the *_alt arrays.
   uint32_t* ksrc = keys;
@@ -317,8 +324,9 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* keys_alt, const uint32_t*
   for (int shift = 0; shift < bits; shift += width) {
     const int pb = bits - shift < width ? bits - shift : width;
     const uint32_t mask = (1u << pb) - 1u;
-    hipLaunchKernelGGL(radix_count_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, n, n_dev,
-                       shift, mask, counts, nb);
+    if (!(first_counted && shift == 0))
+      hipLaunchKernelGGL(radix_count_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, n, n_dev,
+                         shift, mask, counts, nb);
     hipLaunchKernelGGL(radix_rowscan_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts, nb);
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ksrc, vsrc, kdst,
                        vdst, n, n_dev, shift, pb, (const uint32_t*)counts, nb);
@@ -371,8 +379,12 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 const uint2* __restrict__ rect_depth, TileGrid grid,
                                                                 ViewTable vt, uint32_t* __restrict__ tile_key,
                                                                 uint32_t* __restrict__ gid,
-                                                                uint32_t* __restrict__ no_record) {
+                                                                uint32_t* __restrict__ no_record,
+                                                                uint32_t* __restrict__ counts, uint32_t count_mask,
+                                                                uint32_t nblocks) {
+  static_assert(kDupChunk == kSortTile, "expansion blocks are the tile sort's tiles");
   constexpr int kViewSlots = kMulti ? kDupChunk : 1;
+  __shared__ uint32_t s_hist[kRadix];
   __shared__ uint32_t s_end[kDupChunk];
   __shared__ uint32_t s_gid[kDupChunk];
   __shared__ int s_x0[kDupChunk];
@@ -389,8 +401,13 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     R = min(R, want);
   }
   const uint32_t k0 = blockIdx.x * (uint32_t)kDupChunk;
-  if (k0 >= R) return;  // past the live instances (capacity-sized grids)
+  if (k0 >= R) {  // past the live instances (capacity-sized grids): an empty sort tile
+    if (counts)
+      for (int d = tid; d < kRadix; d += kDupThreads) counts[(size_t)d * nblocks + blockIdx.x] = 0u;
+    return;
+  }
   const uint32_t k1 = min(R, k0 + (uint32_t)kDupChunk);
+  for (int d = tid; d < kRadix; d += kDupThreads) s_hist[d] = 0u;
   if (tid == 0) {
     s_span[0] = first_greater(offsets, (uint32_t)P, k0);
     s_span[1] = first_greater(offsets, (uint32_t)P, k1 - 1);
@@ -417,36 +434,48 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   // advance one instance at a time; a Gaussian's first instance is its rect's
   // (0, 0), and owners with no instance are stepped over.
   const uint32_t kb = k0 + (uint32_t)tid * kDupRun;
-  if (kb >= k1) return;
-  uint32_t j = first_greater(s_end, ns, kb);
-  uint32_t w = s_w[j];
-  const uint32_t local = kb - (j == 0 ? begin0 : s_end[j - 1]);
-  uint32_t q = local / w, r = local - q * w;
   uint32_t keys[kDupRun], ids[kDupRun];
 #pragma unroll
-  for (int u = 0; u < kDupRun; ++u) {
-    const uint32_t k = kb + (uint32_t)u;
-    keys[u] = ids[u] = 0u;
-    if (k >= k1) continue;
-    while (k >= s_end[j]) {  // stops at the span's last owner: s_end[ns - 1] >= k1
-      ++j;
-      w = s_w[j];
-      q = r = 0;
-    }
-    const uint32_t y = (uint32_t)s_y0[j] + q;
-    if constexpr (kMulti) {
-      const int v = s_view[j];
-      const TileGrid gv = vt.grid(v);
-      keys[u] = vt.tile_base[v] + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, gv);
-    } else {
-      keys[u] = y * (uint32_t)grid.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, grid);
-    }
-    ids[u] = s_gid[j];
-    if (++r == w) {
-      r = 0;
-      ++q;
+  for (int u = 0; u < kDupRun; ++u) keys[u] = ids[u] = 0u;
+  if (kb < k1) {
+    uint32_t j = first_greater(s_end, ns, kb);
+    uint32_t w = s_w[j];
+    const uint32_t local = kb - (j == 0 ? begin0 : s_end[j - 1]);
+    uint32_t q = local / w, r = local - q * w;
+#pragma unroll
+    for (int u = 0; u < kDupRun; ++u) {
+      const uint32_t k = kb + (uint32_t)u;
+      if (k >= k1) continue;
+      while (k >= s_end[j]) {  // stops at the span's last owner: s_end[ns - 1] >= k1
+        ++j;
+        w = s_w[j];
+        q = r = 0;
+      }
+      const uint32_t y = (uint32_t)s_y0[j] + q;
+      if constexpr (kMulti) {
+        const int v = s_view[j];
+        const TileGrid gv = vt.grid(v);
+        keys[u] = vt.tile_base[v] + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, gv);
+      } else {
+        keys[u] = y * (uint32_t)grid.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, grid);
+      }
+      ids[u] = s_gid[j];
+      if (++r == w) {
+        r = 0;
+        ++q;
+      }
     }
   }
+  // The tile sort's first digit, counted here for its first pass (this block
+  // is that pass's tile), in place of a count kernel re-reading the keys.
+  if (counts) {
+#pragma unroll
+    for (int u = 0; u < kDupRun; ++u)
+      if (kb + (uint32_t)u < k1) atomicAdd(&s_hist[keys[u] & count_mask], 1u);
+    __syncthreads();
+    for (int d = tid; d < kRadix; d += kDupThreads) counts[(size_t)d * nblocks + blockIdx.x] = s_hist[d];
+  }
+  if (kb >= k1) return;
   // The gradient slots number as many as the instances: this pass also marks
   // every one as holding no record (kNoRecord in its opacity word) for the
   // backward, which overwrites the slots it stages.
@@ -473,23 +502,25 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
 
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
-                            uint32_t* no_record, hipStream_t s) {
+                            uint32_t* no_record, uint32_t* counts, int key_bits, hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
   hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
-                     (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record);
+                     (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record, counts,
+                     radix_first_mask(key_bits), (uint32_t)sort_blocks(R));
   return hipGetLastError();
 }
 
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
                                   const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                                  hipStream_t s) {
+                                  uint32_t* counts, int key_bits, hipStream_t s) {
   if (items == 0 || capacity == 0) return hipSuccess;
   const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
-                     order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record);
+                     order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record, counts,
+                     radix_first_mask(key_bits), (uint32_t)sort_blocks(capacity));
   return hipGetLastError();
 }
 
