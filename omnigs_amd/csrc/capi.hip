@@ -295,7 +295,8 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     // the expansion also counts the tile sort's first digit
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rect_depth, grid, b.tile_key, b.gid,
-                                reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, s),
+                                reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, img.ranges,
+                                s),
                "duplicate");
   }
   SortResult ts{b.tile_key, b.gid};
@@ -307,7 +308,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   }
   {
     StageTimer tm(OMNIGS_STAGE_RANGES, s);
-    OMNIGS_TRY(launch_tile_ranges(ts.keys, R, img.ranges, grid.tiles(), s), "ranges");
+    OMNIGS_TRY(launch_tile_ranges(ts.keys, R, img.ranges, grid.tiles(), s, nullptr, R > 0), "ranges");
   }
   *num_rendered = (int)R;
   if (!render) return OMNIGS_OK;
@@ -535,7 +536,7 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rect_depth,
                                       vt, b.tile_key, b.gid, reinterpret_cast<uint32_t*>(b.grad_opacity),
-                                      b.sort_counts, tile_bits, s),
+                                      b.sort_counts, tile_bits, bs.ranges, s),
                "batch duplicate");
   }
   SortResult ts;
@@ -547,7 +548,8 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   }
   {
     StageTimer tm(OMNIGS_STAGE_RANGES, s);
-    OMNIGS_TRY(launch_tile_ranges(ts.keys, a->capacity, bs.ranges, (int)vt.tile_base[vt.V], s, R_dev),
+    OMNIGS_TRY(launch_tile_ranges(ts.keys, a->capacity, bs.ranges, (int)vt.tile_base[vt.V], s, R_dev,
+                                  items > 0 && a->capacity > 0),
                "batch ranges");
   }
   {

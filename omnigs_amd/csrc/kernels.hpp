@@ -88,18 +88,20 @@ uint32_t radix_first_mask(int bits);
 // rect_depth: the Gaussians' rects in depth order (written by launch_scan).
 // counts (optional): receives the first-pass count table of a key_bits-bit
 // tile sort of the expanded keys (launch_radix_sort's first_counted).
+// ranges: cleared for launch_tile_ranges (its `zeroed`).
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
-                            uint32_t* no_record, uint32_t* counts, int key_bits, hipStream_t s);
+                            uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s);
+// zeroed: ranges were cleared by the expansion (else they are cleared here).
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
-                              hipStream_t s, const uint32_t* R_dev = nullptr);
+                              hipStream_t s, const uint32_t* R_dev = nullptr, bool zeroed = false);
 // Batched expansion: items = V P, capacity instances at most; R_dev is the
 // device-side instance count (last inclusive offset); status [2] receives
 // {count needed, 1 if it exceeds the capacity}.
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
                                   const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                                  uint32_t* counts, int key_bits, hipStream_t s);
+                                  uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // render.hip

@@ -108,28 +108,46 @@ __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* _
   if (threadIdx.x == 0) partials[nb] = carry;
 }
 
-// Each thread owns kScanItems consecutive elements (blocked layout).
+// Each thread scans kScanItems consecutive elements (blocked layout); the
+// tile enters and leaves through LDS (rows padded to kScanItems + 1 words) so
+// that the global loads and stores stay coalesced.
 __global__ __launch_bounds__(kScanThreads) void scan_downsweep_kernel(const uint32_t* __restrict__ in,
                                                                       const uint32_t* __restrict__ idx, size_t n,
                                                                       const uint32_t* __restrict__ partials,
                                                                       uint32_t* __restrict__ out, int exclusive) {
-  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
-  uint32_t s = 0;
+  constexpr int kRow = kScanItems + 1;
+  __shared__ uint32_t s_v[kScanThreads * kRow];
+  const size_t base = (size_t)blockIdx.x * kScanTile;
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const size_t i = base + k;
-    v[k] = i < n ? load_in(in, idx, i) : 0u;
-    s += v[k];
+    const int il = k * kScanThreads + tid;
+    const size_t i = base + (size_t)il;
+    s_v[(il / kScanItems) * kRow + il % kScanItems] = i < n ? load_in(in, idx, i) : 0u;
+  }
+  __syncthreads();
+  uint32_t* row = s_v + tid * kRow;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = row[k];
+    sum += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_exclusive_scan(s, &tot) + partials[blockIdx.x];
+  uint32_t run = block_exclusive_scan(sum, &tot) + partials[blockIdx.x];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const size_t i = base + k;
     const uint32_t inc = run + v[k];
-    if (i < n) out[i] = exclusive ? run : inc;
+    row[k] = exclusive ? run : inc;
     run = inc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int il = k * kScanThreads + tid;
+    const size_t i = base + (size_t)il;
+    if (i < n) out[i] = s_v[(il / kScanItems) * kRow + il % kScanItems];
   }
 }
 
@@ -381,7 +399,8 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ gid,
                                                                 uint32_t* __restrict__ no_record,
                                                                 uint32_t* __restrict__ counts, uint32_t count_mask,
-                                                                uint32_t nblocks) {
+                                                                uint32_t nblocks, uint32_t* __restrict__ zero,
+                                                                uint32_t zero_n) {
   static_assert(kDupChunk == kSortTile, "expansion blocks are the tile sort's tiles");
   constexpr int kViewSlots = kMulti ? kDupChunk : 1;
   __shared__ uint32_t s_hist[kRadix];
@@ -392,6 +411,9 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   __shared__ uint8_t s_view[kViewSlots];
   __shared__ uint32_t s_span[2];
   const int tid = threadIdx.x;
+  // the tile ranges, cleared here for the ranges pass (which sets the
+  // occupied tiles only) instead of by a launch of their own
+  for (uint32_t i = blockIdx.x * kDupThreads + tid; i < zero_n; i += gridDim.x * kDupThreads) zero[i] = 0u;
   if (R_dev) {
     const uint32_t want = *R_dev;
     if (status && blockIdx.x == 0 && tid == 0) {
@@ -502,25 +524,27 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
 
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
-                            uint32_t* no_record, uint32_t* counts, int key_bits, hipStream_t s) {
+                            uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
   hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
                      (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record, counts,
-                     radix_first_mask(key_bits), (uint32_t)sort_blocks(R));
+                     radix_first_mask(key_bits), (uint32_t)sort_blocks(R), reinterpret_cast<uint32_t*>(ranges),
+                     (uint32_t)(2 * grid.tiles()));
   return hipGetLastError();
 }
 
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
                                   const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                                  uint32_t* counts, int key_bits, hipStream_t s) {
+                                  uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s) {
   if (items == 0 || capacity == 0) return hipSuccess;
   const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
                      order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record, counts,
-                     radix_first_mask(key_bits), (uint32_t)sort_blocks(capacity));
+                     radix_first_mask(key_bits), (uint32_t)sort_blocks(capacity), reinterpret_cast<uint32_t*>(ranges),
+                     2u * vt.tile_base[vt.V]);
   return hipGetLastError();
 }
 
@@ -558,9 +582,12 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t R
 }
 
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
-                              hipStream_t s, const uint32_t* R_dev) {
-  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(ranges), 2 * (size_t)tiles, 0u, s);
-  if (e != hipSuccess || R == 0) return e;
+                              hipStream_t s, const uint32_t* R_dev, bool zeroed) {
+  if (!zeroed) {
+    const hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(ranges), 2 * (size_t)tiles, 0u, s);
+    if (e != hipSuccess) return e;
+  }
+  if (R == 0) return hipSuccess;
   hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, sorted_tile_keys,
                      (uint32_t)R, R_dev, ranges);
   return hipGetLastError();
