@@ -45,12 +45,11 @@ def algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes):
         "render": 40 * R + 8 * tiles + 20 * N,
         # per instance gather 40; per pixel T, n_contrib, dL/dpix read; 9 gradient floats per visible Gaussian
         "render_backward": 40 * R + 8 * tiles + 20 * N + 36 * Pv,
-        # per Gaussian: rec 32 + tiles_touched 4 read, dmean2D 12 + dconic 16 + dopacity 4 + dcolor 12 written
-        # (the per-instance records between the two stages are this design's own traffic, not counted)
-        "gather_grads": 80 * P,
-        # read: means, scales, rot, SH, radii, clamped, dmean2D, dconic, dcolor (277 B);
-        # write: dmeans, dcov3D, dSH, dscale, drot (256 B)
-        "preprocess_backward": 533 * P,
+        # the 2D gradients summed per Gaussian (rec 32 + tiles_touched 4 read; dmean2D 12 + dconic 16 +
+        # dopacity 4 + dcolor 12 written; the per-instance records between the two kernels are this
+        # design's own traffic, not counted), then the 3D terms: means, scales, rot, SH, radii, clamped
+        # read (237 B), dmeans, dcov3D, dSH, dscale, drot written (256 B); one fused kernel
+        "preprocess_backward": 573 * P,
     }
 
 

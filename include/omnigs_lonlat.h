@@ -377,7 +377,8 @@ enum omnigs_stage {
   OMNIGS_STAGE_TILE_SORT = 4,
   OMNIGS_STAGE_RANGES = 5,
   OMNIGS_STAGE_RENDER = 6,
-  OMNIGS_STAGE_GATHER_GRADS = 7, /* per-Gaussian sum of the render backward's per-instance records */
+  OMNIGS_STAGE_GATHER_GRADS = 7, /* per-Gaussian sum of the per-instance records (batched calls; the
+                                    single-view backward sums them inside the preprocess backward) */
   OMNIGS_STAGE_RENDER_BACKWARD = 8,
   OMNIGS_STAGE_PREPROCESS_BACKWARD = 9,
   OMNIGS_STAGE_DECODER_BACKWARD = 10,

@@ -378,19 +378,17 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
                                       img.n_contrib, a->dL_dpix, g.slot_base, (uint32_t)R, grad_rec, grad_opacity, s),
                "render backward");
   }
-  {
-    // every entry of the four per-Gaussian arrays is written (zeros where no
-    // tile touched the Gaussian), so they need no clearing
-    StageTimer tm(OMNIGS_STAGE_GATHER_GRADS, s);
-    OMNIGS_TRY(launch_gather_grads(P, P, (uint32_t)R, nullptr, g.tiles_touched, g.slot_local, g.slot_base, grad_rec,
-                                   grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor, s),
-               "gather grads");
-  }
+  // The per-Gaussian sums of the records (the four 2D gradient outputs, every
+  // entry written, zeros where no tile touched the Gaussian) are gathered
+  // inside the preprocess backward, which uses them from registers.
   PreprocessBackwardArgs pb{P, a->D, a->M, W, H, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,
                             a->rotations,
                             a->cov3D_precomp, a->viewmatrix, a->campos, radii, g.clamped, a->dL_dmean2D,
                             a->dL_dconic, a->dL_dcolor, a->dL_dmean3D, a->dL_dcov3D, a->dL_dsh,
-                            a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt};
+                            a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt,
+                            GatherSrc{P, (uint32_t)R, nullptr, g.tiles_touched, g.slot_local, g.slot_base, grad_rec,
+                                      grad_opacity},
+                            a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor};
   {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
     OMNIGS_TRY(launch_preprocess_backward(pb, s), "preprocess backward");

@@ -251,6 +251,19 @@ inline GeomState carve_geom(char* base, size_t P) {
   return g;
 }
 
+// Where the per-Gaussian gradient sums come from: the backward's records and
+// the slot layout (see gather_sum).  tiles_touched == nullptr: not used.
+struct GatherSrc {
+  int P;
+  uint32_t nslots;
+  const uint32_t* R_dev;
+  const uint32_t* tiles_touched;
+  const uint32_t* slot_local;
+  const uint32_t* slot_base;
+  const float4* grad_rec;
+  const float* grad_opacity;
+};
+
 // Per-instance state: tile keys and Gaussian ids, ping-pong for the tile sort;
 // the backward's per-(tile, Gaussian) gradient terms.  Gaussian g's instances
 // own the slots [slot_base[g / 256] + slot(g), ... + tiles_touched[g]), one per

@@ -46,6 +46,14 @@ struct PreprocessBackwardArgs {
   float* dL_drot;
   float* dpx_dt;
   float* dpy_dt;
+  // Fused gather (gather.tiles_touched set): the 2D gradients are summed
+  // from the render backward's records in this kernel and written to these
+  // outputs, instead of read from dL_dmean2D / dL_dconic / dL_dcolor.
+  GatherSrc gather;
+  float* out_dmean2D;
+  float* out_dconic;
+  float* out_dopacity;
+  float* out_dcolor;
 };
 
 // preprocess.hip

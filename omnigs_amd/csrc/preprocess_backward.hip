@@ -11,6 +11,7 @@
 // HBM-bound: ~290 B read and ~250 B written per visible Gaussian.
 #include "kernels.hpp"
 #include "lonlat_math.hpp"
+#include "grad_gather.hpp"
 
 namespace omnigs {
 
@@ -228,8 +229,17 @@ __device__ __forceinline__ void store_dsh(float* dsh, int M, const float f[48]) 
   }
 }
 
+// kGather: the 2D gradients are first summed from the render backward's
+// records (every lane of a wave takes part) and written out, then used here
+// from registers.
+template <bool kGather>
 __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBackwardArgs a) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  float gs[kGradTerms];
+  if constexpr (kGather) {
+    gather_sum(a.gather, idx, idx < a.P, gs);
+    if (idx < a.P) store_gathered((size_t)idx, gs, a.out_dmean2D, a.out_dconic, a.out_dopacity, a.out_dcolor);
+  }
   if (idx >= a.P) return;
   const int M = a.M;
 
@@ -276,8 +286,11 @@ __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBack
   float dcov[6];
   float3 dmean;
   Jac j;
-  cov_mean_backward(v, p, S, a.W, a.H, reinterpret_cast<const float4*>(a.dL_dconic)[idx], a.dL_dmean2D[3 * idx],
-                    a.dL_dmean2D[3 * idx + 1], dcov, dmean, j);
+  if constexpr (kGather)
+    cov_mean_backward(v, p, S, a.W, a.H, make_float4(gs[5], gs[6], 0.f, gs[7]), gs[3], gs[4], dcov, dmean, j);
+  else
+    cov_mean_backward(v, p, S, a.W, a.H, reinterpret_cast<const float4*>(a.dL_dconic)[idx], a.dL_dmean2D[3 * idx],
+                      a.dL_dmean2D[3 * idx + 1], dcov, dmean, j);
   if (a.dpx_dt) {
     a.dpx_dt[3 * idx] = j.px_x;
     a.dpx_dt[3 * idx + 1] = 0.f;
@@ -297,9 +310,10 @@ __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBack
     float3 c[16];
     load_sh(a.shs, M, idx, nc, c);
     float f[48];
+    const float3 dcol = kGather ? make_float3(gs[0], gs[1], gs[2])
+                                : make_float3(a.dL_dcolor[3 * idx], a.dL_dcolor[3 * idx + 1], a.dL_dcolor[3 * idx + 2]);
     const float3 dm = sh_backward(a.D, nc, c, p, make_float3(a.campos[0], a.campos[1], a.campos[2]), a.clamped[idx],
-                                  make_float3(a.dL_dcolor[3 * idx], a.dL_dcolor[3 * idx + 1], a.dL_dcolor[3 * idx + 2]),
-                                  f);
+                                  dcol, f);
     store_dsh(a.dL_dsh + (size_t)3 * M * idx, M, f);
     if (a.D > 0) {
       dmean.x += dm.x;
@@ -415,7 +429,10 @@ hipError_t launch_preprocess_backward_views(const PreprocessBackwardArgs& a, con
 
 hipError_t launch_preprocess_backward(const PreprocessBackwardArgs& a, hipStream_t s) {
   if (a.P == 0) return hipSuccess;
-  hipLaunchKernelGGL(preprocess_backward_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  if (a.gather.tiles_touched)
+    hipLaunchKernelGGL(preprocess_backward_kernel<true>, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(preprocess_backward_kernel<false>, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

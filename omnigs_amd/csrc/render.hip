@@ -18,6 +18,8 @@
 // with plain stores, and a per-Gaussian pass (gather_grads) sums the records,
 // in place of device-scope atomics on the per-Gaussian arrays.
 #include "kernels.hpp"
+#include "wave_ops.hpp"
+#include "grad_gather.hpp"
 
 namespace omnigs {
 
@@ -251,53 +253,6 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     out[N + pid] = C1 + T * bg[1];
     out[2 * N + pid] = C2 + T * bg[2];
   }
-}
-
-// Cross-lane helpers (gfx950): DPP moves, and the permlane swaps that trade
-// the upper 32 lanes of one VGPR with the lower 32 of another (32) or the odd
-// 16-lane rows of one with the even rows of another (16).
-template <int Ctrl>
-__device__ __forceinline__ float dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xf, 0xf, false));
-}
-// x + y of a 32-lane swap: lanes 0-31 hold x[i] + x[i+32], lanes 32-63 hold y[i-32] + y[i].
-__device__ __forceinline__ float fold32(float x, float y) {
-  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(y), false, false);
-  return __int_as_float(r[0]) + __int_as_float(r[1]);
-}
-// Rows 0,2 hold x summed over row pairs (0+1, 2+3); rows 1,3 hold y likewise.
-__device__ __forceinline__ float fold16(float x, float y) {
-  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(y), false, false);
-  return __int_as_float(r[0]) + __int_as_float(r[1]);
-}
-
-// Reduces eight per-lane terms t[0..7] over the wave by recursive halving:
-// on return every lane of 8-lane group g = lane >> 3 holds sum(t[g]).
-__device__ __forceinline__ float wave_reduce8(const float t[8], int lane) {
-  // lanes 0-31 carry t[0..3], lanes 32-63 carry t[4..7]
-  const float a0 = fold32(t[0], t[4]), a1 = fold32(t[1], t[5]);
-  const float a2 = fold32(t[2], t[6]), a3 = fold32(t[3], t[7]);
-  // bit 4 of the lane picks between (a0, a2) and (a1, a3)
-  const float b0 = fold16(a0, a2), b1 = fold16(a1, a3);
-  // bit 3 of the lane keeps b1 (else b0); the partner 8 lanes away gives the other
-  const bool hi = (lane >> 3) & 1;
-  const float keep = hi ? b1 : b0, give = hi ? b0 : b1;
-  float c = keep + dpp<0x128>(give);  // row_ror:8 (lane i <-> i^8 within a 16-lane row)
-  c += dpp<0x141>(c);                 // row_half_mirror: i + (7 - i) within 8 lanes
-  c += dpp<0x4e>(c);                  // quad_perm [2,3,0,1]
-  c += dpp<0xb1>(c);                  // quad_perm [1,0,3,2]
-  return c;
-}
-
-// Full wave sum of one term; every lane receives it.
-__device__ __forceinline__ float wave_reduce1(float v) {
-  float c = fold32(v, v);
-  c = fold16(c, c);
-  c += dpp<0x128>(c);
-  c += dpp<0x141>(c);
-  c += dpp<0x4e>(c);
-  c += dpp<0xb1>(c);
-  return c;
 }
 
 // Reduce-scatter of two instances' eight terms (a[0..7] of instance A, b[0..7]
@@ -562,109 +517,17 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   }
 }
 
-// Per-Gaussian (per-item) sum of the backward's gradient records.  Item i
-// owns the slots [base, base + tiles_touched) with base = slot_base[block of
-// its Gaussian] + slot_local[i]; consecutive items' slots are adjacent, so
-// neighbouring lanes read neighbouring records.  A slot the backward did not
-// write (its tile never staged the instance) holds kNoRecord in its opacity
-// word.  Slots are fetched kGatherGroup per round trip (the opacity words,
-// then the records that exist).  Items with more than kGatherSerial instances
-// (a few Gaussians cover thousands of tiles) are summed one at a time by the
-// whole wave, lanes striding over the slots, so no lane walks a long list
-// alone.  Writes every output entry, zeros included, in the reference's
-// layouts: dL_dmean2D [.,3] (z = 0), dL_dconic [.,2,2] ((1,0) = 0),
-// dL_dopacity [.], dL_dcolors [.,3].  nslots (with R_dev, the live count)
-// bounds the slots read.
-constexpr uint32_t kGatherSerial = 16;
-constexpr int kGatherGroup = 8;  // slots fetched per round trip
-
-__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
-}
-
-__global__ __launch_bounds__(256) void gather_grads_kernel(int items, int P, uint32_t nslots,
-                                                           const uint32_t* __restrict__ R_dev,
-                                                           const uint32_t* __restrict__ tiles_touched,
-                                                           const uint32_t* __restrict__ slot_local,
-                                                           const uint32_t* __restrict__ slot_base,
-                                                           const float4* __restrict__ grad_rec,
-                                                           const float* __restrict__ grad_opacity,
-                                                           float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
+// Per-Gaussian (per-item) sums of the gradient records (gather_sum), written
+// in full, zeros included, so the outputs need no clearing.
+__global__ __launch_bounds__(256) void gather_grads_kernel(GatherSrc src, int items, float* __restrict__ dL_dmean2D,
+                                                           float* __restrict__ dL_dconic,
                                                            float* __restrict__ dL_dopacity,
                                                            float* __restrict__ dL_dcolors) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1);
   const bool valid = i < items;  // every lane stays to the end: the wave-wide path needs them all
-  if (R_dev) nslots = min(nslots, *R_dev);
-  const uint32_t cnt = valid ? tiles_touched[i] : 0u;
-  const uint32_t base = cnt > 0 ? slot_base[(uint32_t)(i % P) / kSlotBlock] + slot_local[i] : 0u;
-
-  // adds the records of slots base + l for the group's l (l0, l0 + stride, ...)
-  auto add_group = [&](uint32_t b, uint32_t n, uint32_t l0, uint32_t stride, float* a) {
-    uint32_t jj[kGatherGroup];
-    bool in[kGatherGroup];
-#pragma unroll
-    for (int u = 0; u < kGatherGroup; ++u) {
-      const uint32_t l = l0 + (uint32_t)u * stride;
-      in[u] = l < n && b + l < nslots;
-      jj[u] = in[u] ? b + l : 0u;
-    }
-    uint32_t op[kGatherGroup];
-#pragma unroll
-    for (int u = 0; u < kGatherGroup; ++u)
-      op[u] = in[u] ? __float_as_uint(grad_opacity[jj[u]]) : kNoRecord;
-    float4 r0[kGatherGroup], r1[kGatherGroup];
-#pragma unroll
-    for (int u = 0; u < kGatherGroup; ++u) {
-      r0[u] = r1[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (op[u] != kNoRecord) {
-        r0[u] = grad_rec[2 * (size_t)jj[u]];
-        r1[u] = grad_rec[2 * (size_t)jj[u] + 1];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kGatherGroup; ++u) {
-      a[0] += r0[u].x; a[1] += r0[u].y; a[2] += r0[u].z; a[3] += r0[u].w;
-      a[4] += r1[u].x; a[5] += r1[u].y; a[6] += r1[u].z; a[7] += r1[u].w;
-      a[8] += op[u] != kNoRecord ? __uint_as_float(op[u]) : 0.f;
-    }
-  };
-
   float acc[kGradTerms];
-#pragma unroll
-  for (int q = 0; q < kGradTerms; ++q) acc[q] = 0.f;
-  if (cnt <= kGatherSerial)
-    for (uint32_t l0 = 0; l0 < cnt; l0 += kGatherGroup) add_group(base, cnt, l0, 1, acc);
-  uint64_t big = __ballot(cnt > kGatherSerial);
-  while (big) {
-    const int src = __ffsll((unsigned long long)big) - 1;
-    big &= big - 1;
-    const uint32_t bcnt = readlane_u32(cnt, src), bbase = readlane_u32(base, src);
-    float part[kGradTerms];
-#pragma unroll
-    for (int q = 0; q < kGradTerms; ++q) part[q] = 0.f;
-    for (uint32_t l0 = lane; l0 < bcnt; l0 += kWave * kGatherGroup) add_group(bbase, bcnt, l0, kWave, part);
-    const float grp = wave_reduce8(part, lane);  // term q in lanes 8q .. 8q + 7
-    const float sw = wave_reduce1(part[8]);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float tot = __uint_as_float(readlane_u32(__float_as_uint(grp), 8 * q));
-      if (lane == src) acc[q] = tot;
-    }
-    if (lane == src) acc[8] = sw;
-  }
-  if (!valid) return;
-  dL_dcolors[3 * (size_t)i] = acc[0];
-  dL_dcolors[3 * (size_t)i + 1] = acc[1];
-  dL_dcolors[3 * (size_t)i + 2] = acc[2];
-  dL_dmean2D[3 * (size_t)i] = acc[3];
-  dL_dmean2D[3 * (size_t)i + 1] = acc[4];
-  dL_dmean2D[3 * (size_t)i + 2] = 0.f;
-  dL_dconic[4 * (size_t)i] = acc[5];
-  dL_dconic[4 * (size_t)i + 1] = acc[6];
-  dL_dconic[4 * (size_t)i + 2] = 0.f;
-  dL_dconic[4 * (size_t)i + 3] = acc[7];
-  dL_dopacity[i] = acc[8];
+  gather_sum(src, i, valid, acc);
+  if (valid) store_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
 }
 
 namespace {
@@ -729,9 +592,9 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
                                const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
                                float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
   if (items == 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_grads_kernel, dim3((items + 255) / 256), dim3(256), 0, s, items, P, nslots, R_dev,
-                     tiles_touched, slot_local, slot_base, grad_rec, grad_opacity, dL_dmean2D, dL_dconic, dL_dopacity,
-                     dL_dcolors);
+  const GatherSrc src{P, nslots, R_dev, tiles_touched, slot_local, slot_base, grad_rec, grad_opacity};
+  hipLaunchKernelGGL(gather_grads_kernel, dim3((items + 255) / 256), dim3(256), 0, s, src, items, dL_dmean2D,
+                     dL_dconic, dL_dopacity, dL_dcolors);
   return hipGetLastError();
 }
 
