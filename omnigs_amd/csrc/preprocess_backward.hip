@@ -229,27 +229,93 @@ __device__ __forceinline__ void store_dsh(float* dsh, int M, const float f[48]) 
   }
 }
 
+// Wave-cooperative SH input and output for M = 16 (48 floats per Gaussian):
+// the coefficients of half a wave's Gaussians at a time pass through an LDS
+// tile, so that the global reads and writes are contiguous float4 runs (a
+// lane's own 192-byte row would put each float4 of one instruction in a
+// different cache line).  Every lane of the wave must call these; rows past
+// P are neither read nor written.
+constexpr int kShRow = 52;  // floats per LDS row: 48 + 4, so 8 lanes' 16-byte accesses hit distinct banks
+constexpr int kShTile = 32 * kShRow;
+
+__device__ __forceinline__ void sh_load_wave(const float* __restrict__ shs, int P, int wave_base, int lane,
+                                             float* tile, float3 c[16]) {
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int g0 = wave_base + 32 * half;
+    const float4* src = reinterpret_cast<const float4*>(shs) + (size_t)g0 * 12;
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int ch = k * kWave + lane, r = ch / 12, col = ch % 12;
+      const float4 v = g0 + r < P ? src[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4*>(tile + r * kShRow + 4 * col) = v;
+    }
+    wave_lds_sync();
+    if ((lane >> 5) == half) {
+      const float* row = tile + (lane & 31) * kShRow;
+      float f[48];
+#pragma unroll
+      for (int i = 0; i < 12; ++i) {
+        const float4 v = *reinterpret_cast<const float4*>(row + 4 * i);
+        f[4 * i] = v.x; f[4 * i + 1] = v.y; f[4 * i + 2] = v.z; f[4 * i + 3] = v.w;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) c[i] = make_float3(f[3 * i], f[3 * i + 1], f[3 * i + 2]);
+    }
+  }
+}
+
+__device__ __forceinline__ void sh_store_wave(float* __restrict__ dsh, int P, int wave_base, int lane, float* tile,
+                                              const float f[48]) {
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int g0 = wave_base + 32 * half;
+    float4* dst = reinterpret_cast<float4*>(dsh) + (size_t)g0 * 12;
+    wave_lds_sync();
+    if ((lane >> 5) == half) {
+      float* row = tile + (lane & 31) * kShRow;
+#pragma unroll
+      for (int i = 0; i < 12; ++i)
+        *reinterpret_cast<float4*>(row + 4 * i) = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int ch = k * kWave + lane, r = ch / 12, col = ch % 12;
+      if (g0 + r < P) dst[ch] = *reinterpret_cast<const float4*>(tile + r * kShRow + 4 * col);
+    }
+  }
+}
+
 // kGather: the 2D gradients are first summed from the render backward's
 // records (every lane of a wave takes part) and written out, then used here
-// from registers.
+// from registers.  With M = 16 SH coefficients the SH input and output are
+// wave-cooperative (sh_load_wave / sh_store_wave), so every lane runs to the
+// end, with `live` predicating the per-Gaussian work.
 template <bool kGather>
 __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBackwardArgs a) {
+  __shared__ __attribute__((aligned(16))) float s_sh[256 / kWave][kShTile];
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   float gs[kGradTerms];
   if constexpr (kGather) {
     gather_sum(a.gather, idx, idx < a.P, gs);
     if (idx < a.P) store_gathered((size_t)idx, gs, a.out_dmean2D, a.out_dconic, a.out_dopacity, a.out_dcolor);
   }
-  if (idx >= a.P) return;
   const int M = a.M;
+  const bool in = idx < a.P;
+  const bool staged = a.shs && a.dL_dsh && M == 16;  // uniform over the launch
+  if (!staged && !in) return;
+  const bool live = in && a.radii[idx] > 0;
 
-  if (!(a.radii[idx] > 0)) {
+  if (!live && in) {
     a.dL_dmean3D[3 * idx] = 0.f;
     a.dL_dmean3D[3 * idx + 1] = 0.f;
     a.dL_dmean3D[3 * idx + 2] = 0.f;
 #pragma unroll
     for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = 0.f;
-    if (a.dL_dsh)
+    if (a.dL_dsh && !staged)
       for (int k = 0; k < 3 * M; ++k) a.dL_dsh[(size_t)3 * M * idx + k] = 0.f;
     if (a.dL_dscale) {
 #pragma unroll
@@ -264,77 +330,87 @@ __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBack
 #pragma unroll
       for (int k = 0; k < 3; ++k) a.dpy_dt[3 * idx + k] = 0.f;
     }
-    return;
   }
+  if (!staged && !live) return;
 
-  const View v = load_view(a.viewmatrix);
-  const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
-
-  // ---- 3D covariance (recomputed; identical to the forward's) ----
-  Sym3 S;
-  float3 s = make_float3(0.f, 0.f, 0.f);
-  float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
-  if (a.cov3D_precomp) {
+  const int wave_base = idx - lane;
+  float3 c[16];
+  if (staged) sh_load_wave(a.shs, a.P, wave_base, lane, s_sh[threadIdx.x / kWave], c);
+  float f[48];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
-  } else {
-    s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
-    q = reinterpret_cast<const float4*>(a.rotations)[idx];
-    S = cov3d_from(s, a.scale_modifier, q);
-  }
+  for (int k = 0; k < 48; ++k) f[k] = 0.f;
 
-  float dcov[6];
-  float3 dmean;
-  Jac j;
-  if constexpr (kGather)
-    cov_mean_backward(v, p, S, a.W, a.H, make_float4(gs[5], gs[6], 0.f, gs[7]), gs[3], gs[4], dcov, dmean, j);
-  else
-    cov_mean_backward(v, p, S, a.W, a.H, reinterpret_cast<const float4*>(a.dL_dconic)[idx], a.dL_dmean2D[3 * idx],
-                      a.dL_dmean2D[3 * idx + 1], dcov, dmean, j);
-  if (a.dpx_dt) {
-    a.dpx_dt[3 * idx] = j.px_x;
-    a.dpx_dt[3 * idx + 1] = 0.f;
-    a.dpx_dt[3 * idx + 2] = j.px_z;
-  }
-  if (a.dpy_dt) {
-    a.dpy_dt[3 * idx] = j.py_x;
-    a.dpy_dt[3 * idx + 1] = j.py_y;
-    a.dpy_dt[3 * idx + 2] = j.py_z;
-  }
+  if (live) {
+    const View v = load_view(a.viewmatrix);
+    const float3 p = make_float3(a.means3D[3 * idx], a.means3D[3 * idx + 1], a.means3D[3 * idx + 2]);
+
+    // ---- 3D covariance (recomputed; identical to the forward's) ----
+    Sym3 S;
+    float3 s = make_float3(0.f, 0.f, 0.f);
+    float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
+    if (a.cov3D_precomp) {
 #pragma unroll
-  for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = dcov[k];
+      for (int k = 0; k < 6; ++k) S.m[k] = a.cov3D_precomp[6 * idx + k];
+    } else {
+      s = make_float3(a.scales[3 * idx], a.scales[3 * idx + 1], a.scales[3 * idx + 2]);
+      q = reinterpret_cast<const float4*>(a.rotations)[idx];
+      S = cov3d_from(s, a.scale_modifier, q);
+    }
 
-  // ---- SH term ----
-  if (a.shs) {
-    const int nc = min(M, sh_coeffs(a.D));
-    float3 c[16];
-    load_sh(a.shs, M, idx, nc, c);
-    float f[48];
-    const float3 dcol = kGather ? make_float3(gs[0], gs[1], gs[2])
-                                : make_float3(a.dL_dcolor[3 * idx], a.dL_dcolor[3 * idx + 1], a.dL_dcolor[3 * idx + 2]);
-    const float3 dm = sh_backward(a.D, nc, c, p, make_float3(a.campos[0], a.campos[1], a.campos[2]), a.clamped[idx],
-                                  dcol, f);
-    store_dsh(a.dL_dsh + (size_t)3 * M * idx, M, f);
-    if (a.D > 0) {
-      dmean.x += dm.x;
-      dmean.y += dm.y;
-      dmean.z += dm.z;
+    float dcov[6];
+    float3 dmean;
+    Jac j;
+    if constexpr (kGather)
+      cov_mean_backward(v, p, S, a.W, a.H, make_float4(gs[5], gs[6], 0.f, gs[7]), gs[3], gs[4], dcov, dmean, j);
+    else
+      cov_mean_backward(v, p, S, a.W, a.H, reinterpret_cast<const float4*>(a.dL_dconic)[idx], a.dL_dmean2D[3 * idx],
+                        a.dL_dmean2D[3 * idx + 1], dcov, dmean, j);
+    if (a.dpx_dt) {
+      a.dpx_dt[3 * idx] = j.px_x;
+      a.dpx_dt[3 * idx + 1] = 0.f;
+      a.dpx_dt[3 * idx + 2] = j.px_z;
+    }
+    if (a.dpy_dt) {
+      a.dpy_dt[3 * idx] = j.py_x;
+      a.dpy_dt[3 * idx + 1] = j.py_y;
+      a.dpy_dt[3 * idx + 2] = j.py_z;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.dL_dcov3D[6 * idx + k] = dcov[k];
+
+    // ---- SH term ----
+    if (a.shs) {
+      const int nc = min(M, sh_coeffs(a.D));
+      if (!staged) load_sh(a.shs, M, idx, nc, c);
+      const float3 dcol = kGather ? make_float3(gs[0], gs[1], gs[2])
+                                  : make_float3(a.dL_dcolor[3 * idx], a.dL_dcolor[3 * idx + 1],
+                                                a.dL_dcolor[3 * idx + 2]);
+      const float3 dm = sh_backward(a.D, nc, c, p, make_float3(a.campos[0], a.campos[1], a.campos[2]),
+                                    a.clamped[idx], dcol, f);
+      if (!staged) store_dsh(a.dL_dsh + (size_t)3 * M * idx, M, f);
+      if (a.D > 0) {
+        dmean.x += dm.x;
+        dmean.y += dm.y;
+        dmean.z += dm.z;
+      }
+    }
+    a.dL_dmean3D[3 * idx] = dmean.x;
+    a.dL_dmean3D[3 * idx + 1] = dmean.y;
+    a.dL_dmean3D[3 * idx + 2] = dmean.z;
+
+    // ---- scale / rotation term ----
+    if (a.scales && a.dL_dscale && a.dL_drot) {
+      float dsc[3];
+      float4 dq;
+      scale_rot_backward(dcov, s, q, a.scale_modifier, dsc, dq);
+      a.dL_dscale[3 * idx] = dsc[0];
+      a.dL_dscale[3 * idx + 1] = dsc[1];
+      a.dL_dscale[3 * idx + 2] = dsc[2];
+      reinterpret_cast<float4*>(a.dL_drot)[idx] = dq;
     }
   }
-  a.dL_dmean3D[3 * idx] = dmean.x;
-  a.dL_dmean3D[3 * idx + 1] = dmean.y;
-  a.dL_dmean3D[3 * idx + 2] = dmean.z;
-
-  // ---- scale / rotation term ----
-  if (a.scales && a.dL_dscale && a.dL_drot) {
-    float dsc[3];
-    float4 dq;
-    scale_rot_backward(dcov, s, q, a.scale_modifier, dsc, dq);
-    a.dL_dscale[3 * idx] = dsc[0];
-    a.dL_dscale[3 * idx + 1] = dsc[1];
-    a.dL_dscale[3 * idx + 2] = dsc[2];
-    reinterpret_cast<float4*>(a.dL_drot)[idx] = dq;
-  }
+  // zeros for Gaussians that are not live
+  if (staged) sh_store_wave(a.dL_dsh, a.P, wave_base, lane, s_sh[threadIdx.x / kWave], f);
 }
 
 // Batched backward: one thread per Gaussian sums every view's terms (items

@@ -52,4 +52,13 @@ __device__ __forceinline__ float wave_reduce1(float v) {
   return c;
 }
 
+// Orders one wave's LDS accesses at this point for the compiler (the
+// hardware executes a wave's LDS operations in order), for data that lanes of
+// the same wave exchange through LDS.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace omnigs
