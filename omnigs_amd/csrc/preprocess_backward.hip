@@ -240,16 +240,25 @@ constexpr int kShTile = 32 * kShRow;
 
 __device__ __forceinline__ void sh_load_wave(const float* __restrict__ shs, int P, int wave_base, int lane,
                                              float* tile, float3 c[16]) {
+  // both halves' loads are issued before the first LDS round (one memory latency)
+  float4 ld[2][6];
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int g0 = wave_base + 32 * half;
     const float4* src = reinterpret_cast<const float4*>(shs) + (size_t)g0 * 12;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int ch = k * kWave + lane, r = ch / 12;
+      ld[half][k] = g0 + r < P ? src[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
     wave_lds_sync();
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const int ch = k * kWave + lane, r = ch / 12, col = ch % 12;
-      const float4 v = g0 + r < P ? src[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
-      *reinterpret_cast<float4*>(tile + r * kShRow + 4 * col) = v;
+      *reinterpret_cast<float4*>(tile + r * kShRow + 4 * col) = ld[half][k];
     }
     wave_lds_sync();
     if ((lane >> 5) == half) {
