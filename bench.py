@@ -24,6 +24,8 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS = 1024          # 256 CUs x 4 SIMDs
+VALU_CLOCK_GHZ = 2.4       # peak engine clock
 
 
 def algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes):
@@ -119,10 +121,19 @@ def main() -> int:
     achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
     whole_bytes = 848 * P + 124 * R + 40 * N
     traffic = None
+    valu_issue = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(dominant)
+            tj = json.load(f)
+        traffic = tj.get(dominant)
+        # the render kernels are VALU-issue-bound: wave64 VALU instructions per launch (rocprofv3
+        # SQ_INSTS_VALU, same workload) at 4 cycles each on 1024 SIMDs, against the launch time
+        valu = tj.get("_valu_instructions", {}).get(dominant)
+        if valu:
+            cap = VALU_SIMDS * VALU_CLOCK_GHZ * 1e9 * dom_ms * 1e-3 / 4.0
+            valu_issue = {"instructions": valu, "frac": round(valu / cap, 4), "simds": VALU_SIMDS,
+                          "clock_ghz": VALU_CLOCK_GHZ, "cycles_per_instruction": 4}
 
     out = {
         "metric": "fwd+bwd ms/iter, 1.5M Gaussians @1920x960 lonlat (plain SH path)",
@@ -142,7 +153,8 @@ def main() -> int:
                    "sh_degree": 3, "peak_hbm_bytes": peak_hbm},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": alg[dominant], "avg_launch_ms": round(dom_ms, 5)},
+                     "algorithmic_bytes": alg[dominant], "avg_launch_ms": round(dom_ms, 5),
+                     "valu_issue": valu_issue},
         "whole_path": {"algorithmic_bytes": whole_bytes,
                        "achieved_GBps": round(whole_bytes / (median_ms * 1e-3) / 1e9, 2),
                        "frac_of_hbm_roofline": round(whole_bytes / (median_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
