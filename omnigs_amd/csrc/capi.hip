@@ -296,7 +296,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rect_depth, grid, b.tile_key, b.gid,
                                 reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, img.ranges,
-                                s),
+                                b.blend_ok, s),
                "duplicate");
   }
   SortResult ts{b.tile_key, b.gid};
@@ -315,7 +315,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   {
     StageTimer tm(OMNIGS_STAGE_RENDER, s);
     OMNIGS_TRY(launch_render_forward(grid, img.ranges, ts.vals, W, H, g.rec, img.final_T, img.n_contrib,
-                                     a->background, a->out_color, s),
+                                     a->background, a->out_color, b.blend, b.blend_ok, s),
                "render");
   }
   return OMNIGS_OK;
@@ -364,18 +364,23 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   const uint32_t* point_list = nullptr;
   float4* grad_rec = nullptr;
   float* grad_opacity = nullptr;
+  const uint8_t* blend = nullptr;
+  const uint32_t* blend_ok = nullptr;
   if (R > 0) {
     const BinState b = carve_bin(a->binning_buffer, R);
     const int passes = radix_passes(bit_width_u32((uint32_t)grid.tiles()));
     point_list = (passes % 2) ? b.gid_alt : b.gid;
     grad_rec = b.grad_rec;
     grad_opacity = b.grad_opacity;
+    blend = b.blend;
+    blend_ok = b.blend_ok;
   }
 
   if (R > 0) {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward(grid, img.ranges, point_list, W, H, a->background, g.rec, img.final_T,
-                                      img.n_contrib, a->dL_dpix, g.slot_base, (uint32_t)R, grad_rec, grad_opacity, s),
+                                      img.n_contrib, a->dL_dpix, g.slot_base, (uint32_t)R, grad_rec, grad_opacity,
+                                      blend, blend_ok, s),
                "render backward");
   }
   // The per-Gaussian sums of the records (the four 2D gradient outputs, every
@@ -534,7 +539,7 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rect_depth,
                                       vt, b.tile_key, b.gid, reinterpret_cast<uint32_t*>(b.grad_opacity),
-                                      b.sort_counts, tile_bits, bs.ranges, s),
+                                      b.sort_counts, tile_bits, bs.ranges, b.blend_ok, s),
                "batch duplicate");
   }
   SortResult ts;
@@ -553,7 +558,7 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   {
     StageTimer tm(OMNIGS_STAGE_RENDER, s);
     OMNIGS_TRY(launch_render_forward_views(vt, bs.ranges, ts.vals, g.rec, bs.final_T, bs.n_contrib, a->background,
-                                           a->out_color, s),
+                                           a->out_color, b.blend, b.blend_ok, s),
                "batch render");
   }
   return OMNIGS_OK;
@@ -587,7 +592,7 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward_views(vt, bs.ranges, point_list, a->background, g.rec, bs.final_T,
                                             bs.n_contrib, gr->dL_dpix, g.slot_base, a->capacity, b.grad_rec,
-                                            b.grad_opacity, s),
+                                            b.grad_opacity, b.blend, b.blend_ok, s),
                "batch render backward");
   }
   {

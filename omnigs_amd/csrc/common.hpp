@@ -280,6 +280,12 @@ struct BinState {
   float* grad_opacity;    // [R]: dopacity, or kNoRecord
   uint32_t* sort_counts;  // radix sort workspace (sort_workspace_words)
   uint32_t* scan_partials;
+  // Per instance (in the tile lists' order): bit w set when some lane of the
+  // tile's wave w blended it in the render forward.  Valid only when
+  // *blend_ok == 1 (set by the render forward, cleared by the expansion, so
+  // a binning recomputed without a render forward falls back to the masks).
+  uint8_t* blend;
+  uint32_t* blend_ok;
   size_t bytes;
 };
 
@@ -310,6 +316,8 @@ inline BinState carve_bin(char* base, size_t R) {
   b.grad_opacity = c.take<float>(R);
   b.sort_counts = c.take<uint32_t>(sort_workspace_words(R));
   b.scan_partials = c.take<uint32_t>(1);
+  b.blend = c.take<uint8_t>(R);
+  b.blend_ok = c.take<uint32_t>(1);
   b.bytes = align_up(c.off) + kAlign;
   return b;
 }

@@ -96,10 +96,12 @@ uint32_t radix_first_mask(int bits);
 // rect_depth: the Gaussians' rects in depth order (written by launch_scan).
 // counts (optional): receives the first-pass count table of a key_bits-bit
 // tile sort of the expanded keys (launch_radix_sort's first_counted).
-// ranges: cleared for launch_tile_ranges (its `zeroed`).
+// ranges: cleared for launch_tile_ranges (its `zeroed`).  blend_ok: cleared
+// (BinState::blend).
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
-                            uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s);
+                            uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok,
+                            hipStream_t s);
 // zeroed: ranges were cleared by the expansion (else they are cleared here).
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s, const uint32_t* R_dev = nullptr, bool zeroed = false);
@@ -109,24 +111,28 @@ hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2*
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
                                   const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                                  uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s);
+                                  uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // render.hip
+// blend (optional, R bytes): bit w of instance i set when wave w of its tile
+// blended it; *blend_ok is set to 1 (BinState::blend).
 hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                  const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
-                                 float* out_color, hipStream_t s);
+                                 float* out_color, uint8_t* blend, uint32_t* blend_ok, hipStream_t s);
 // The backward writes each staged (tile, Gaussian) instance's nine gradient
 // terms to its slot -- grad_rec [nslots][2] float4 + grad_opacity [nslots] --
 // at slot_base[Gaussian / kSlotBlock] + the record's slot word + the tile's
 // position in the Gaussian's rect; slots it does not write must hold
 // kNoRecord in grad_opacity.  gather_grads sums each item's slots into the
 // reference's per-Gaussian arrays (every entry written).  P: Gaussians per
-// view (items = V P); R_dev (optional): the live slot count.
+// view (items = V P); R_dev (optional): the live slot count.  blend (with
+// *blend_ok == 1): the forward's per-wave blend bits, which select the
+// (wave, instance) pairs to evaluate; else the quadrant masks do.
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
-                                  float* grad_opacity, hipStream_t s);
+                                  float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s);
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
                                const uint32_t* tiles_touched, const uint32_t* slot_local, const uint32_t* slot_base,
                                const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
@@ -136,11 +142,12 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
 // views' slices laid out one after the other (colours [3, H_v, W_v] per view).
 hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                        const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
-                                       float* out_color, hipStream_t s);
+                                       float* out_color, uint8_t* blend, uint32_t* blend_ok, hipStream_t s);
 hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                         const float* bg, const float4* rec, const float* final_T,
                                         const uint32_t* n_contrib, const float* dL_dpix, const uint32_t* slot_base,
-                                        uint32_t nslots, float4* grad_rec, float* grad_opacity, hipStream_t s);
+                                        uint32_t nslots, float4* grad_rec, float* grad_opacity, const uint8_t* blend,
+                                        const uint32_t* blend_ok, hipStream_t s);
 bool tile_shape_supported(int bx, int by);
 
 // preprocess_backward.hip

@@ -165,7 +165,8 @@ template <int TX, int TY, bool kWrap, bool kMulti>
 __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
     int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
     ViewTable vt, const float4* __restrict__ rec, float* __restrict__ final_T, uint32_t* __restrict__ n_contrib,
-    const float* __restrict__ bg, float* __restrict__ out_color) {
+    const float* __restrict__ bg, float* __restrict__ out_color, uint8_t* __restrict__ blend_out,
+    uint32_t* __restrict__ blend_ok) {
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
@@ -191,8 +192,17 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
   uint32_t last_b = 0;  // n_contrib times kStageBytes
 
-  for (int base = 0; base < n; base += kBatch) {
-    if (__syncthreads_count(__builtin_amdgcn_inverse_ballot_w64(done_mask)) == kTilePixels) break;
+  if (blend_out && blockIdx.x == 0 && tid == 0) *blend_ok = 1u;
+  for (int base = 0;; base += kBatch) {
+    const int all_done = __syncthreads_count(__builtin_amdgcn_inverse_ballot_w64(done_mask));
+    // the previous batch's blend bytes (every wave has finished it): byte w of
+    // the staged entry is 1 when some lane of wave w blended it
+    if (blend_out && base > 0 && base - kBatch + tid < n) {
+      const uint32_t f = __float_as_uint(s_op[tid].c.y);
+      blend_out[range.x + (base - kBatch + tid)] =
+          (uint8_t)((f & 1u) | ((f >> 7) & 2u) | ((f >> 14) & 4u) | ((f >> 21) & 8u));
+    }
+    if (base >= n || all_done == kTilePixels) break;
     const int k = base + tid;
     if (k < n) {
       const uint32_t g = point_list[range.x + k];
@@ -202,7 +212,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const float3 q = prescale_conic(co);
       s_op[tid].a = make_float4(m.x, m.y, q.x, q.y);
       s_op[tid].b = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
-      s_op[tid].c = make_float4(rr.rgb.z, 0.f, 0.f, 0.f);
+      s_op[tid].c = make_float4(rr.rgb.z, 0.f, 0.f, 0.f);  // y: one blend byte per wave
       s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
     __syncthreads();
@@ -229,6 +239,9 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
       const uint64_t stop = ok & __builtin_amdgcn_ballot_w64(test_T < 0.0001f);
       const bool blend = __builtin_amdgcn_inverse_ballot_w64(ok & ~stop);
       done_mask |= stop;
+      // whether some lane of this wave blended the instance: every lane
+      // stores the same byte at the same address (the wave's own byte)
+      reinterpret_cast<uint8_t*>(s_op)[jo + 36 + w] = (ok & ~stop) != 0ull ? 1 : 0;
       const float wgt = blend ? alpha * T : 0.f;
       C0 += pb.z * wgt;
       C1 += pb.w * wgt;
@@ -309,7 +322,8 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
     ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, const uint32_t* __restrict__ slot_base,
-    uint32_t nslots, float4* __restrict__ grad_rec, float* __restrict__ grad_opacity) {
+    uint32_t nslots, float4* __restrict__ grad_rec, float* __restrict__ grad_opacity,
+    const uint8_t* __restrict__ blend_in, const uint32_t* __restrict__ blend_ok) {
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
@@ -339,6 +353,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   n_contrib += loc.pix;
   dL_dpix += 3 * loc.pix;
 
+  const bool use_blend = blend_in && *blend_ok == 1u;
   const float T_final = inside ? final_Ts[pid] : 0.f;
   float T = T_final;
   const int last = inside ? (int)n_contrib[pid] : 0;
@@ -437,7 +452,11 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
       s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
       s_blue[tid] = rr.rgb.z;
-      s_mask[tid] = (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
+      // the waves that blended the instance in the forward (exactly the
+      // pairs whose evaluation here finds a blending lane), else the
+      // quadrants its ellipse reaches
+      s_mask[tid] = use_blend ? (uint8_t)(blend_in[range.x + (n - 1 - k)] & 0xFu)
+                              : (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
     }
 #pragma unroll
     for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
@@ -567,24 +586,24 @@ bool tile_shape_supported(int bx, int by) {
 
 hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                  const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
-                                 float* out_color, hipStream_t s) {
+                                 float* out_color, uint8_t* blend, uint32_t* blend_ok, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
   ViewTable none{};
   return dispatch_shape<false, FwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
-                                     point_list, W, H, none, rec, final_T, n_contrib, bg, out_color);
+                                     point_list, W, H, none, rec, final_T, n_contrib, bg, out_color, blend, blend_ok);
 }
 
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
-                                  float* grad_opacity, hipStream_t s) {
+                                  float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s) {
   const int tiles = grid.tiles();
   if (tiles == 0) return hipSuccess;
   ViewTable none{};
   return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
                                      point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, slot_base, nslots,
-                                     grad_rec, grad_opacity);
+                                     grad_rec, grad_opacity, blend, blend_ok);
 }
 
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
@@ -600,21 +619,23 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
 
 hipError_t launch_render_forward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                        const float4* rec, float* final_T, uint32_t* n_contrib, const float* bg,
-                                       float* out_color, hipStream_t s) {
+                                       float* out_color, uint8_t* blend, uint32_t* blend_ok, hipStream_t s) {
   const uint32_t tiles = vt.tile_base[vt.V];
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, FwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, rec,
-                                    final_T, n_contrib, bg, out_color);
+                                    final_T, n_contrib, bg, out_color, blend, blend_ok);
 }
 
 hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges, const uint32_t* point_list,
                                         const float* bg, const float4* rec, const float* final_T,
                                         const uint32_t* n_contrib, const float* dL_dpix, const uint32_t* slot_base,
-                                        uint32_t nslots, float4* grad_rec, float* grad_opacity, hipStream_t s) {
+                                        uint32_t nslots, float4* grad_rec, float* grad_opacity, const uint8_t* blend,
+                                        const uint32_t* blend_ok, hipStream_t s) {
   const uint32_t tiles = vt.tile_base[vt.V];
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
-                                    final_T, n_contrib, dL_dpix, slot_base, nslots, grad_rec, grad_opacity);
+                                    final_T, n_contrib, dL_dpix, slot_base, nslots, grad_rec, grad_opacity, blend,
+                                    blend_ok);
 }
 
 }  // namespace omnigs

@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ no_record,
                                                                 uint32_t* __restrict__ counts, uint32_t count_mask,
                                                                 uint32_t nblocks, uint32_t* __restrict__ zero,
-                                                                uint32_t zero_n) {
+                                                                uint32_t zero_n, uint32_t* __restrict__ blend_ok) {
   static_assert(kDupChunk == kSortTile, "expansion blocks are the tile sort's tiles");
   constexpr int kViewSlots = kMulti ? kDupChunk : 1;
   __shared__ uint32_t s_hist[kRadix];
@@ -414,6 +414,9 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   // the tile ranges, cleared here for the ranges pass (which sets the
   // occupied tiles only) instead of by a launch of their own
   for (uint32_t i = blockIdx.x * kDupThreads + tid; i < zero_n; i += gridDim.x * kDupThreads) zero[i] = 0u;
+  // the forward's per-wave blend bits are stale until a render forward of
+  // this binning writes them
+  if (blend_ok && blockIdx.x == 0 && tid == 0) *blend_ok = 0u;
   if (R_dev) {
     const uint32_t want = *R_dev;
     if (status && blockIdx.x == 0 && tid == 0) {
@@ -524,27 +527,28 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
 
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
-                            uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s) {
+                            uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok,
+                            hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
   hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
                      (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record, counts,
                      radix_first_mask(key_bits), (uint32_t)sort_blocks(R), reinterpret_cast<uint32_t*>(ranges),
-                     (uint32_t)(2 * grid.tiles()));
+                     (uint32_t)(2 * grid.tiles()), blend_ok);
   return hipGetLastError();
 }
 
 hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* R_dev, uint32_t* status,
                                   const uint32_t* order, const uint32_t* offsets, const uint2* rect_depth,
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
-                                  uint32_t* counts, int key_bits, uint2* ranges, hipStream_t s) {
+                                  uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok, hipStream_t s) {
   if (items == 0 || capacity == 0) return hipSuccess;
   const uint32_t blocks = (capacity + kDupChunk - 1) / kDupChunk;
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
                      order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record, counts,
                      radix_first_mask(key_bits), (uint32_t)sort_blocks(capacity), reinterpret_cast<uint32_t*>(ranges),
-                     2u * vt.tile_base[vt.V]);
+                     2u * vt.tile_base[vt.V], blend_ok);
   return hipGetLastError();
 }
 
