@@ -399,11 +399,11 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     // the combined bool would rematerialise it as a vector value first)
     const bool c_pos = j > lim, c_pow = power2 <= 0.0f, c_alpha = alpha >= 1.0f / 255.0f;
     const bool ok = c_pos && c_pow && c_alpha;
+    wgt = 0.f;
+    v = 0.f;
     if ((__builtin_amdgcn_ballot_w64(c_pos) & __builtin_amdgcn_ballot_w64(c_pow) &
          __builtin_amdgcn_ballot_w64(c_alpha)) == 0ull)
       return false;
-    wgt = 0.f;
-    v = 0.f;
     if (ok) {
       // T and 1/(1 - alpha) only scale gradients here (no decision depends
       // on them), so the hardware reciprocal replaces two IEEE divisions.
@@ -466,35 +466,24 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     // slot j holds position n - 1 - (base + j); keep positions < wlast
     const int first_slot = n - base - wlast;
     const int len = build_wave_list<1>(s_mask, s_list[w], cnt, w, lane, first_slot);
-    // Pairs that some lane blends are reduced two at a time (wave_reduce16);
-    // a last odd one alone.  The list order (back to front) is kept.
-    // lanes outside the image have last = 0, so no slot passes their limit
+    // The list order (back to front) is kept.  Lanes outside the image have
+    // last = 0, so no slot passes their limit.
     const int lim = n - 1 - base - last;
-    int ii = 0;
-    for (;;) {
-      float wa, va, xa, ya, wb, vb, xb, yb;
-      int ja = 0, jb = 0;
-      bool fa = false, fb = false;
-      while (ii < len && !fa) {
-        ja = s_list[w][ii++];
-        fa = eval(ja, lim, wa, va, xa, ya);
-      }
-      if (!fa) break;
-      while (ii < len && !fb) {
-        jb = s_list[w][ii++];
-        fb = eval(jb, lim, wb, vb, xb, yb);
-      }
+    // sums of one instance (wave_reduce8 + 1) or of two (wave_reduce16 + 2),
+    // added to the block's accumulators
+    auto add_one = [&](int ja, float wa, float va, float xa, float ya) {
       float ta[8];
       moments(wa, va, xa, ya, ta);
-      if (!fb) {
-        float grp = wave_reduce8(ta, lane);
-        float sw = wave_reduce1(wa);
-        asm volatile("" : "+v"(grp), "+v"(sw));
-        if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][ja], grp);
-        if (lane == 0) atomicAdd(&s_acc[8][ja], sw);
-        break;
-      }
-      float tb[8];
+      float grp = wave_reduce8(ta, lane);
+      float sw = wave_reduce1(wa);
+      asm volatile("" : "+v"(grp), "+v"(sw));
+      if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][ja], grp);
+      if (lane == 0) atomicAdd(&s_acc[8][ja], sw);
+    };
+    auto add_two = [&](int ja, float wa, float va, float xa, float ya, int jb, float wb, float vb, float xb,
+                       float yb) {
+      float ta[8], tb[8];
+      moments(wa, va, xa, ya, ta);
       moments(wb, vb, xb, yb, tb);
       float grp = wave_reduce16(ta, tb, lane);
       float sw = wave_reduce2(wa, wb);
@@ -504,6 +493,47 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       const int jj = (lane & 32) ? jb : ja;
       if ((lane & 3) == 0) atomicAdd(&s_acc[(lane >> 2) & 7][jj], grp);
       if ((lane & 31) == 0) atomicAdd(&s_acc[8][jj], sw);
+    };
+    int ii = 0;
+    if (use_blend) {
+      // Every listed pair has a blending lane (the forward's bits), so the
+      // instances are reduced two consecutive entries at a time, with no
+      // search between them.
+      for (; ii + 1 < len; ii += 2) {
+        const int ja = s_list[w][ii], jb = s_list[w][ii + 1];
+        float wa, va, xa, ya, wb, vb, xb, yb;
+        eval(ja, lim, wa, va, xa, ya);
+        eval(jb, lim, wb, vb, xb, yb);
+        add_two(ja, wa, va, xa, ya, jb, wb, vb, xb, yb);
+      }
+      if (ii < len) {
+        const int ja = s_list[w][ii];
+        float wa, va, xa, ya;
+        eval(ja, lim, wa, va, xa, ya);
+        add_one(ja, wa, va, xa, ya);
+      }
+    } else {
+      // Pairs that some lane blends are found by evaluating the list in order
+      // and reduced two at a time; a last odd one alone.
+      for (;;) {
+        float wa, va, xa, ya, wb, vb, xb, yb;
+        int ja = 0, jb = 0;
+        bool fa = false, fb = false;
+        while (ii < len && !fa) {
+          ja = s_list[w][ii++];
+          fa = eval(ja, lim, wa, va, xa, ya);
+        }
+        if (!fa) break;
+        while (ii < len && !fb) {
+          jb = s_list[w][ii++];
+          fb = eval(jb, lim, wb, vb, xb, yb);
+        }
+        if (!fb) {
+          add_one(ja, wa, va, xa, ya);
+          break;
+        }
+        add_two(ja, wa, va, xa, ya, jb, wb, vb, xb, yb);
+      }
     }
     __syncthreads();
     if (tid < cnt) {
