@@ -388,7 +388,9 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   // G dL/dalpha and alpha T on blending lanes (zero elsewhere) and (dx, dy)
   // the offsets; the nine moments follow from these four (moments()).
   // `lim` = n - 1 - base - last: position n - 1 - (base + j) < last <=> j > lim.
-  auto eval = [&](int j, int lim, float& wgt, float& v, float& dx, float& dy) -> bool {
+  // `search`: the pair may have no blending lane (quadrant-mask lists); with
+  // the forward's blend bits it always has one and the test is skipped.
+  auto eval = [&](int j, int lim, float& wgt, float& v, float& dx, float& dy, bool search) -> bool {
     const float4 pa = s_pa[j], pb = s_pb[j];
     dx = seam_dx<kWrap>(pa.x - fx, Wf, invW);
     dy = pa.y - fy;
@@ -401,8 +403,8 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     const bool ok = c_pos && c_pow && c_alpha;
     wgt = 0.f;
     v = 0.f;
-    if ((__builtin_amdgcn_ballot_w64(c_pos) & __builtin_amdgcn_ballot_w64(c_pow) &
-         __builtin_amdgcn_ballot_w64(c_alpha)) == 0ull)
+    if (search && (__builtin_amdgcn_ballot_w64(c_pos) & __builtin_amdgcn_ballot_w64(c_pow) &
+                   __builtin_amdgcn_ballot_w64(c_alpha)) == 0ull)
       return false;
     if (ok) {
       // T and 1/(1 - alpha) only scale gradients here (no decision depends
@@ -502,14 +504,14 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       for (; ii + 1 < len; ii += 2) {
         const int ja = s_list[w][ii], jb = s_list[w][ii + 1];
         float wa, va, xa, ya, wb, vb, xb, yb;
-        eval(ja, lim, wa, va, xa, ya);
-        eval(jb, lim, wb, vb, xb, yb);
+        eval(ja, lim, wa, va, xa, ya, false);
+        eval(jb, lim, wb, vb, xb, yb, false);
         add_two(ja, wa, va, xa, ya, jb, wb, vb, xb, yb);
       }
       if (ii < len) {
         const int ja = s_list[w][ii];
         float wa, va, xa, ya;
-        eval(ja, lim, wa, va, xa, ya);
+        eval(ja, lim, wa, va, xa, ya, false);
         add_one(ja, wa, va, xa, ya);
       }
     } else {
@@ -521,12 +523,12 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
         bool fa = false, fb = false;
         while (ii < len && !fa) {
           ja = s_list[w][ii++];
-          fa = eval(ja, lim, wa, va, xa, ya);
+          fa = eval(ja, lim, wa, va, xa, ya, true);
         }
         if (!fa) break;
         while (ii < len && !fb) {
           jb = s_list[w][ii++];
-          fb = eval(jb, lim, wb, vb, xb, yb);
+          fb = eval(jb, lim, wb, vb, xb, yb, true);
         }
         if (!fb) {
           add_one(ja, wa, va, xa, ya);
