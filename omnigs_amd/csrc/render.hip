@@ -438,27 +438,33 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     __syncthreads();
     const int k = base + tid;
     if (k < n) {
-      const uint32_t g = point_list[range.x + (n - 1 - k)];
-      const RenderRec rr = load_rec(rec, g);
-      const float2 m = rr.xy;
-      const float4 co = rr.co;
-      const float3 q = prescale_conic(co);
-      // this instance's gradient slot: the Gaussian's first slot (its block's
-      // base plus its offset in the block) plus the tile's position in its rect
-      const float4 r2 = rec[3 * g + 2];
-      const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
-      const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
-      s_inst[tid] = slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z) +
-                    rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
-      s_co[tid] = co;
-      s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
-      s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
-      s_blue[tid] = rr.rgb.z;
       // the waves that blended the instance in the forward (exactly the
-      // pairs whose evaluation here finds a blending lane), else the
-      // quadrants its ellipse reaches
-      s_mask[tid] = use_blend ? (uint8_t)(blend_in[range.x + (n - 1 - k)] & 0xFu)
-                              : (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
+      // pairs whose evaluation here finds a blending lane); an instance no
+      // wave blended is in no list and writes no record (its slot keeps the
+      // expansion's no-record word)
+      const uint32_t nib = use_blend ? blend_in[range.x + (n - 1 - k)] & 0xFu : 0xFu;
+      s_mask[tid] = 0;
+      s_inst[tid] = kNoRecord;  // >= nslots: no record written
+      if (nib != 0u) {
+        const uint32_t g = point_list[range.x + (n - 1 - k)];
+        const RenderRec rr = load_rec(rec, g);
+        const float2 m = rr.xy;
+        const float4 co = rr.co;
+        const float3 q = prescale_conic(co);
+        // this instance's gradient slot: the Gaussian's first slot (its block's
+        // base plus its offset in the block) plus the tile's position in its rect
+        const float4 r2 = rec[3 * g + 2];
+        const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
+        const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
+        s_inst[tid] = slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z) +
+                      rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
+        s_co[tid] = co;
+        s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
+        s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
+        s_blue[tid] = rr.rgb.z;
+        // else the quadrants the ellipse reaches
+        s_mask[tid] = use_blend ? (uint8_t)nib : (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
+      }
     }
 #pragma unroll
     for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
