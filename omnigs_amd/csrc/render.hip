@@ -318,7 +318,10 @@ __device__ __forceinline__ int wave_max_int(int v) {
 constexpr int kAccStride = kBatch + 1;  // padded so one instance's terms sit in distinct banks
 
 template <int TX, int TY, bool kWrap, bool kMulti>
-__global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
+// Six blocks per CU fit the kernel's LDS; the register budget is held to
+// match (80 VGPRs), which the compiler otherwise exceeds by hoisting staging
+// loads.
+__global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 8))) void render_backward_kernel(
     int gx_, const uint2* __restrict__ ranges, const uint32_t* __restrict__ point_list, int W_, int H_,
     ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, const uint32_t* __restrict__ slot_base,
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   const float Wf = (float)W, invW = 1.0f / (float)W;
   const TileGrid tg{gx, (H + TY - 1) / TY, kWrap ? 1 : 0, TX, TY};
   __shared__ uint32_t s_inst[kBatch];  // gradient slot of the staged instance
-  __shared__ float4 s_co[kBatch];
+  __shared__ uint32_t s_pos[kBatch];   // its position in the tile's list
   // the forward's layout: {x, y, a, b}, {c, opacity, r, g}, blue
   __shared__ float4 s_pa[kBatch];
   __shared__ float4 s_pb[kBatch];
@@ -337,6 +340,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   __shared__ float s_acc[kGradTerms][kAccStride];
   __shared__ uint8_t s_mask[kBatch];
   __shared__ uint8_t s_list[kWavesPerTile][kBatch];
+  // positions still to stage, back to front, as (position << 4 | wave bits),
+  // in a ring of 2 kBatch entries from q_head
+  __shared__ uint32_t s_queue[2 * kBatch];
+  __shared__ int s_wlast[kWavesPerTile];
   __shared__ int s_nlive;
 
   const int tid = threadIdx.x;
@@ -363,7 +370,10 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   if (tid == 0) s_nlive = 0;
   __syncthreads();
   const int wlast = wave_max_int(last);
-  if (lane == 0) atomicMax(&s_nlive, wlast);
+  if (lane == 0) {
+    atomicMax(&s_nlive, wlast);
+    s_wlast[w] = wlast;
+  }
   __syncthreads();
   const int n = s_nlive;
   float dp0 = 0.f, dp1 = 0.f, dp2 = 0.f;
@@ -387,10 +397,12 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
   // wave-uniformly, whether any lane blends it.  If one does, (wgt, v) are
   // G dL/dalpha and alpha T on blending lanes (zero elsewhere) and (dx, dy)
   // the offsets; the nine moments follow from these four (moments()).
-  // `lim` = n - 1 - base - last: position n - 1 - (base + j) < last <=> j > lim.
-  // `search`: the pair may have no blending lane (quadrant-mask lists); with
-  // the forward's blend bits it always has one and the test is skipped.
-  auto eval = [&](int j, int lim, float& wgt, float& v, float& dx, float& dy, bool search) -> bool {
+  // A lane blends staged instance j only before its last contributor
+  // (s_pos[j] < last <=> j >= lo, positions falling with j).  `search`: the
+  // pair may have no blending lane
+  // (quadrant-mask lists); with the forward's blend bits it always has one
+  // and the test is skipped.
+  auto eval = [&](int j, int lo, float& wgt, float& v, float& dx, float& dy, bool search) -> bool {
     const float4 pa = s_pa[j], pb = s_pb[j];
     dx = seam_dx<kWrap>(pa.x - fx, Wf, invW);
     dy = pa.y - fy;
@@ -399,7 +411,7 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     const float alpha = fminf(0.99f, pb.y * G);
     // three compares whose lane masks combine on the scalar unit (a ballot of
     // the combined bool would rematerialise it as a vector value first)
-    const bool c_pos = j > lim, c_pow = power2 <= 0.0f, c_alpha = alpha >= 1.0f / 255.0f;
+    const bool c_pos = j >= lo, c_pow = power2 <= 0.0f, c_alpha = alpha >= 1.0f / 255.0f;
     const bool ok = c_pos && c_pow && c_alpha;
     wgt = 0.f;
     v = 0.f;
@@ -434,49 +446,76 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
     t[7] = wdy * dy;
   };
 
-  for (int base = 0; base < n; base += kBatch) {
+  // The list is walked from position n - 1 down.  Positions that some wave
+  // must evaluate -- with the forward's blend bits, those some wave blended
+  // (often under half of them) -- enter a queue in LDS, 256 positions per
+  // block scan, and batches of up to 256 are staged from its front, so
+  // batches hold no instance that no wave evaluates.
+  constexpr uint32_t kRing = 2 * kBatch - 1;
+  int q_head = 0, q_len = 0, scanned = 0;  // block-uniform
+  for (;;) {
     __syncthreads();
-    const int k = base + tid;
-    if (k < n) {
+    while (q_len < kBatch && scanned < n) {
+      const int k = scanned + tid;
+      uint32_t bits = 0u;
+      if (k < n) bits = use_blend ? blend_in[range.x + (n - 1 - k)] & 0xFu : 0xFu;
+      uint32_t added;
+      const uint32_t at = block_exclusive_scan(bits ? 1u : 0u, &added);
+      if (bits) s_queue[(q_head + q_len + (int)at) & kRing] = ((uint32_t)(n - 1 - k) << 4) | bits;
+      q_len += __builtin_amdgcn_readfirstlane((int)added);  // uniform: kept in a scalar register
+      scanned += kBatch;
+    }
+    if (q_len == 0) break;
+    __syncthreads();
+    const int cnt = min(kBatch, q_len);
+    s_mask[tid] = 0;
+    s_inst[tid] = kNoRecord;  // >= nslots: no record written
+    if (tid < cnt) {
+      const uint32_t e = s_queue[(q_head + tid) & kRing], p = e >> 4, bits = e & 0xFu;
+      s_pos[tid] = p;
+      const uint32_t g = point_list[range.x + p];
+      const RenderRec rr = load_rec(rec, g);
+      const float2 m = rr.xy;
+      const float4 co = rr.co;
+      const float3 q = prescale_conic(co);
+      // this instance's gradient slot: the Gaussian's first slot (its block's
+      // base plus its offset in the block) plus the tile's position in its rect
+      const float4 r2 = rec[3 * g + 2];
+      const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
+      const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
+      s_inst[tid] = slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z) +
+                    rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
+      s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
+      s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
+      s_blue[tid] = rr.rgb.z;
       // the waves that blended the instance in the forward (exactly the
-      // pairs whose evaluation here finds a blending lane); an instance no
-      // wave blended is in no list and writes no record (its slot keeps the
-      // expansion's no-record word)
-      const uint32_t nib = use_blend ? blend_in[range.x + (n - 1 - k)] & 0xFu : 0xFu;
-      s_mask[tid] = 0;
-      s_inst[tid] = kNoRecord;  // >= nslots: no record written
-      if (nib != 0u) {
-        const uint32_t g = point_list[range.x + (n - 1 - k)];
-        const RenderRec rr = load_rec(rec, g);
-        const float2 m = rr.xy;
-        const float4 co = rr.co;
-        const float3 q = prescale_conic(co);
-        // this instance's gradient slot: the Gaussian's first slot (its block's
-        // base plus its offset in the block) plus the tile's position in its rect
-        const float4 r2 = rec[3 * g + 2];
-        const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
-        const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
-        s_inst[tid] = slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z) +
-                      rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
-        s_co[tid] = co;
-        s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
-        s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
-        s_blue[tid] = rr.rgb.z;
-        // else the quadrants the ellipse reaches
-        s_mask[tid] = use_blend ? (uint8_t)nib : (uint8_t)quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
+      // pairs whose evaluation here finds a blending lane); else the
+      // quadrants the ellipse reaches, for the waves with a lane whose last
+      // contributor lies behind it
+      uint32_t mk = bits;
+      if (!use_blend) {
+        mk = quadrant_mask<TX, TY>(m, co, tile, gx, kWrap ? Wf : 0.f);
+#pragma unroll
+        for (int qw = 0; qw < kWavesPerTile; ++qw)
+          if ((int)p >= s_wlast[qw]) mk &= ~(1u << qw);
       }
+      s_mask[tid] = (uint8_t)mk;
     }
 #pragma unroll
     for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
     __syncthreads();
+    q_head = (q_head + cnt) & kRing;
+    q_len -= cnt;
+    // lo = the number of staged entries at or past this lane's last
+    // contributor (a prefix: positions fall along the batch)
+    int lo = 0;
+#pragma unroll
+    for (int step = kBatch; step > 0; step >>= 1)
+      if (lo + step <= cnt && (int)s_pos[lo + step - 1] >= last) lo += step;
 
-    const int cnt = min(kBatch, n - base);
-    // slot j holds position n - 1 - (base + j); keep positions < wlast
-    const int first_slot = n - base - wlast;
-    const int len = build_wave_list<1>(s_mask, s_list[w], cnt, w, lane, first_slot);
+    const int len = build_wave_list<1>(s_mask, s_list[w], cnt, w, lane);
     // The list order (back to front) is kept.  Lanes outside the image have
-    // last = 0, so no slot passes their limit.
-    const int lim = n - 1 - base - last;
+    // last = 0, so no position passes their limit.
     // sums of one instance (wave_reduce8 + 1) or of two (wave_reduce16 + 2),
     // added to the block's accumulators
     auto add_one = [&](int ja, float wa, float va, float xa, float ya) {
@@ -510,14 +549,14 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       for (; ii + 1 < len; ii += 2) {
         const int ja = s_list[w][ii], jb = s_list[w][ii + 1];
         float wa, va, xa, ya, wb, vb, xb, yb;
-        eval(ja, lim, wa, va, xa, ya, false);
-        eval(jb, lim, wb, vb, xb, yb, false);
+        eval(ja, lo, wa, va, xa, ya, false);
+        eval(jb, lo, wb, vb, xb, yb, false);
         add_two(ja, wa, va, xa, ya, jb, wb, vb, xb, yb);
       }
       if (ii < len) {
         const int ja = s_list[w][ii];
         float wa, va, xa, ya;
-        eval(ja, lim, wa, va, xa, ya, false);
+        eval(ja, lo, wa, va, xa, ya, false);
         add_one(ja, wa, va, xa, ya);
       }
     } else {
@@ -529,12 +568,12 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
         bool fa = false, fb = false;
         while (ii < len && !fa) {
           ja = s_list[w][ii++];
-          fa = eval(ja, lim, wa, va, xa, ya, true);
+          fa = eval(ja, lo, wa, va, xa, ya, true);
         }
         if (!fa) break;
         while (ii < len && !fb) {
           jb = s_list[w][ii++];
-          fb = eval(jb, lim, wb, vb, xb, yb, true);
+          fb = eval(jb, lo, wb, vb, xb, yb, true);
         }
         if (!fb) {
           add_one(ja, wa, va, xa, ya);
@@ -548,7 +587,9 @@ __global__ __launch_bounds__(kTilePixels) void render_backward_kernel(
       // moments -> the reference's per-pair terms, once per (tile, Gaussian):
       //   dL/dmean2D = -o (0.5W (A Swdx + B Swdy), 0.5H (C Swdy + B Swdx))
       //   dL/dconic  = -0.5 o (Swdx2, Swdxdy, Swdy2),  dL/dopacity = Sw,  dL/dcolor = Sv dL/dpix
-      const float4 co = s_co[tid];
+      // (A, B, C) from the staged prescaled conic
+      const float4 pa = s_pa[tid], pb = s_pb[tid];
+      const float4 co = make_float4(pa.z * (-2.0f / kLog2e), pa.w * (-1.0f / kLog2e), pb.x * (-2.0f / kLog2e), pb.y);
       const float sv0 = s_acc[0][tid], sv1 = s_acc[1][tid], sv2 = s_acc[2][tid];
       const float swdx = s_acc[3][tid], swdy = s_acc[4][tid];
       const float swdx2 = s_acc[5][tid], swdxdy = s_acc[6][tid], swdy2 = s_acc[7][tid];
