@@ -136,24 +136,31 @@ constexpr int kStageBytes = 48;
 struct alignas(16) Staged {
   float4 a, b, c;
 };
+// An element of an LDS array addressed by its byte offset.
+template <typename T, typename A>
+__device__ __forceinline__ T& lds_at(A* base, int off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
+template <typename T, typename A>
+__device__ __forceinline__ const T& lds_at(const A* base, int off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
 __device__ __forceinline__ const Staged& staged_at(const Staged* base, int off) {
   return *reinterpret_cast<const Staged*>(reinterpret_cast<const char*>(base) + off);
 }
 
-// Each wave compacts the batch positions whose mask has its bit set into its
-// own ascending list, entry j * kStride (the forward stores staging byte
-// offsets, the backward positions); returns the list length (wave-uniform).
-// Slots below `first_slot` are dropped (backward: positions past every lane's
-// last contributor).
+// Each wave compacts the batch slots whose mask has its bit set into its own
+// ascending list, entry j * kStride (the forward stores staging byte offsets,
+// the backward 4-byte-element offsets); returns the list length
+// (wave-uniform).
 template <int kStride, typename Entry>
-__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, Entry* list, int cnt, int w, int lane,
-                                               int first_slot = 0) {
+__device__ __forceinline__ int build_wave_list(const uint8_t* s_mask, Entry* list, int cnt, int w, int lane) {
   int count = 0;
   const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int c = 0; c < kBatch / kWave; ++c) {
     const int j = c * kWave + lane;
-    const bool on = j >= first_slot && j < cnt && ((s_mask[j] >> w) & 1u);
+    const bool on = j < cnt && ((s_mask[j] >> w) & 1u);
     const uint64_t bal = __ballot(on);
     if (on) list[count + __popcll(bal & below)] = (Entry)(j * kStride);
     count += __popcll(bal);
@@ -332,14 +339,15 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
   const float Wf = (float)W, invW = 1.0f / (float)W;
   const TileGrid tg{gx, (H + TY - 1) / TY, kWrap ? 1 : 0, TX, TY};
   __shared__ uint32_t s_inst[kBatch];  // gradient slot of the staged instance
-  __shared__ uint32_t s_pos[kBatch];   // its position in the tile's list
   // the forward's layout: {x, y, a, b}, {c, opacity, r, g}, blue
   __shared__ float4 s_pa[kBatch];
   __shared__ float4 s_pb[kBatch];
   __shared__ float s_blue[kBatch];
   __shared__ float s_acc[kGradTerms][kAccStride];
   __shared__ uint8_t s_mask[kBatch];
-  __shared__ uint8_t s_list[kWavesPerTile][kBatch];
+  // per wave, the slots it evaluates as 4 j (the byte offset of slot j in the
+  // 4-byte arrays; 16-byte ones take 4 times that)
+  __shared__ uint16_t s_list[kWavesPerTile][kBatch];
   // positions still to stage, back to front, as (position << 4 | wave bits),
   // in a ring of 2 kBatch entries from q_head
   __shared__ uint32_t s_queue[2 * kBatch];
@@ -397,13 +405,13 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
   // wave-uniformly, whether any lane blends it.  If one does, (wgt, v) are
   // G dL/dalpha and alpha T on blending lanes (zero elsewhere) and (dx, dy)
   // the offsets; the nine moments follow from these four (moments()).
-  // A lane blends staged instance j only before its last contributor
-  // (s_pos[j] < last <=> j >= lo, positions falling with j).  `search`: the
-  // pair may have no blending lane
+  // The instance is slot j, passed as j4 = 4 j (a list entry).  A lane blends
+  // it only before its last contributor (its position < last <=> j4 >= lo4,
+  // positions falling with j).  `search`: the pair may have no blending lane
   // (quadrant-mask lists); with the forward's blend bits it always has one
   // and the test is skipped.
-  auto eval = [&](int j, int lo, float& wgt, float& v, float& dx, float& dy, bool search) -> bool {
-    const float4 pa = s_pa[j], pb = s_pb[j];
+  auto eval = [&](int j4, int lo4, float& wgt, float& v, float& dx, float& dy, bool search) -> bool {
+    const float4 pa = lds_at<float4>(s_pa, 4 * j4), pb = lds_at<float4>(s_pb, 4 * j4);
     dx = seam_dx<kWrap>(pa.x - fx, Wf, invW);
     dy = pa.y - fy;
     const float power2 = power2_of(make_float3(pa.z, pa.w, pb.x), dx, dy);
@@ -411,20 +419,22 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
     const float alpha = fminf(0.99f, pb.y * G);
     // three compares whose lane masks combine on the scalar unit (a ballot of
     // the combined bool would rematerialise it as a vector value first)
-    const bool c_pos = j >= lo, c_pow = power2 <= 0.0f, c_alpha = alpha >= 1.0f / 255.0f;
+    const bool c_pos = j4 >= lo4, c_pow = power2 <= 0.0f, c_alpha = alpha >= 1.0f / 255.0f;
     const bool ok = c_pos && c_pow && c_alpha;
     wgt = 0.f;
     v = 0.f;
     if (search && (__builtin_amdgcn_ballot_w64(c_pos) & __builtin_amdgcn_ballot_w64(c_pow) &
                    __builtin_amdgcn_ballot_w64(c_alpha)) == 0ull)
       return false;
+    // evaluated by the whole wave (some lane always blends), which keeps the
+    // operand loads out of the branch
+    const float cd = pb.z * dp0 + pb.w * dp1 + lds_at<float>(s_blue, j4) * dp2;
     if (ok) {
       // T and 1/(1 - alpha) only scale gradients here (no decision depends
       // on them), so the hardware reciprocal replaces two IEEE divisions.
       const float r = __builtin_amdgcn_rcpf(1.f - alpha);
       T = T * r;
       v = alpha * T;
-      const float cd = pb.z * dp0 + pb.w * dp1 + s_blue[j] * dp2;
       const float diff = cd - acc_dot;
       const float dL_dalpha = diff * T + neg_tb * r;
       // the colour behind the next blended instance: alpha cd + (1 - alpha) acc
@@ -472,7 +482,6 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
     s_inst[tid] = kNoRecord;  // >= nslots: no record written
     if (tid < cnt) {
       const uint32_t e = s_queue[(q_head + tid) & kRing], p = e >> 4, bits = e & 0xFu;
-      s_pos[tid] = p;
       const uint32_t g = point_list[range.x + p];
       const RenderRec rr = load_rec(rec, g);
       const float2 m = rr.xy;
@@ -504,16 +513,18 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
 #pragma unroll
     for (int qq = 0; qq < kGradTerms; ++qq) s_acc[qq][tid] = 0.f;
     __syncthreads();
-    q_head = (q_head + cnt) & kRing;
-    q_len -= cnt;
     // lo = the number of staged entries at or past this lane's last
-    // contributor (a prefix: positions fall along the batch)
+    // contributor (a prefix: positions fall along the batch); the batch's
+    // queue entries stay in place until the next fill
     int lo = 0;
 #pragma unroll
     for (int step = kBatch; step > 0; step >>= 1)
-      if (lo + step <= cnt && (int)s_pos[lo + step - 1] >= last) lo += step;
+      if (lo + step <= cnt && (int)(s_queue[(q_head + lo + step - 1) & kRing] >> 4) >= last) lo += step;
+    const int lo4 = 4 * lo;
+    q_head = (q_head + cnt) & kRing;
+    q_len -= cnt;
 
-    const int len = build_wave_list<1>(s_mask, s_list[w], cnt, w, lane);
+    const int len = build_wave_list<4>(s_mask, s_list[w], cnt, w, lane);
     // The list order (back to front) is kept.  Lanes outside the image have
     // last = 0, so no position passes their limit.
     // sums of one instance (wave_reduce8 + 1) or of two (wave_reduce16 + 2),
@@ -524,8 +535,8 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
       float grp = wave_reduce8(ta, lane);
       float sw = wave_reduce1(wa);
       asm volatile("" : "+v"(grp), "+v"(sw));
-      if ((lane & 7) == 0) atomicAdd(&s_acc[lane >> 3][ja], grp);
-      if (lane == 0) atomicAdd(&s_acc[8][ja], sw);
+      if ((lane & 7) == 0) atomicAdd(&lds_at<float>(s_acc[lane >> 3], ja), grp);
+      if (lane == 0) atomicAdd(&lds_at<float>(s_acc[8], ja), sw);
     };
     auto add_two = [&](int ja, float wa, float va, float xa, float ya, int jb, float wb, float vb, float xb,
                        float yb) {
@@ -538,8 +549,8 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
       // instead of being sunk into the atomics' lane branches
       asm volatile("" : "+v"(grp), "+v"(sw));
       const int jj = (lane & 32) ? jb : ja;
-      if ((lane & 3) == 0) atomicAdd(&s_acc[(lane >> 2) & 7][jj], grp);
-      if ((lane & 31) == 0) atomicAdd(&s_acc[8][jj], sw);
+      if ((lane & 3) == 0) atomicAdd(&lds_at<float>(s_acc[(lane >> 2) & 7], jj), grp);
+      if ((lane & 31) == 0) atomicAdd(&lds_at<float>(s_acc[8], jj), sw);
     };
     int ii = 0;
     if (use_blend) {
@@ -547,16 +558,18 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
       // instances are reduced two consecutive entries at a time, with no
       // search between them.
       for (; ii + 1 < len; ii += 2) {
-        const int ja = s_list[w][ii], jb = s_list[w][ii + 1];
+        // ii is even: both entries in one 4-byte read
+        const uint32_t e = *reinterpret_cast<const uint32_t*>(&s_list[w][ii]);
+        const int ja = (int)(e & 0xFFFFu), jb = (int)(e >> 16);
         float wa, va, xa, ya, wb, vb, xb, yb;
-        eval(ja, lo, wa, va, xa, ya, false);
-        eval(jb, lo, wb, vb, xb, yb, false);
+        eval(ja, lo4, wa, va, xa, ya, false);
+        eval(jb, lo4, wb, vb, xb, yb, false);
         add_two(ja, wa, va, xa, ya, jb, wb, vb, xb, yb);
       }
       if (ii < len) {
         const int ja = s_list[w][ii];
         float wa, va, xa, ya;
-        eval(ja, lo, wa, va, xa, ya, false);
+        eval(ja, lo4, wa, va, xa, ya, false);
         add_one(ja, wa, va, xa, ya);
       }
     } else {
@@ -568,12 +581,12 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
         bool fa = false, fb = false;
         while (ii < len && !fa) {
           ja = s_list[w][ii++];
-          fa = eval(ja, lo, wa, va, xa, ya, true);
+          fa = eval(ja, lo4, wa, va, xa, ya, true);
         }
         if (!fa) break;
         while (ii < len && !fb) {
           jb = s_list[w][ii++];
-          fb = eval(jb, lo, wb, vb, xb, yb, true);
+          fb = eval(jb, lo4, wb, vb, xb, yb, true);
         }
         if (!fb) {
           add_one(ja, wa, va, xa, ya);
