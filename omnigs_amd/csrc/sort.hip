@@ -184,6 +184,22 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, s
 //              and writes each to base[d] + row prefix + rank (stable).
 // counts holds kRadix * blocks row entries followed by kRadix digit totals.
 // ---------------------------------------------------------------------------
+// Which sort tile block b of a grid of nb takes.  Blocks are dispatched
+// round-robin over the 8 XCDs, each with its own L2, and the digit-count
+// table holds one word per tile in each digit's row: with tile = b, the 32
+// words of a 128-byte line come from 32 blocks on all eight XCDs, and every
+// line is written back (and read again) eight times over.  Here each run of
+// 32 consecutive tiles goes to blocks on one XCD; the runs of a group of 256
+// tiles alternate over the XCDs, so live tiles stay spread over the chip when
+// the grid is sized for a capacity.  A partial last group keeps tile = b.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  constexpr uint32_t kXcds = 8, kRun = 32, kGroup = kXcds * kRun;
+  const uint32_t g = b / kGroup;
+  if ((g + 1) * kGroup > nb) return b;
+  const uint32_t r = b % kGroup;
+  return g * kGroup + (r % kXcds) * kRun + r / kXcds;
+}
+
 // n: element count, or the capacity when n_dev holds the count on the device
 // (min(*n_dev, n) elements are sorted; blocks past it count nothing).
 __device__ __forceinline__ size_t live_count(size_t n, const uint32_t* n_dev) {
@@ -198,14 +214,15 @@ __global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const uint32_
   n = live_count(n, n_dev);
   for (int d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d] = 0;
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * kSortTile;
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const size_t base = (size_t)t * kSortTile;
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
     const size_t i = base + (size_t)k * kSortThreads + threadIdx.x;
     if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[(size_t)d * nblocks + blockIdx.x] = hist[d];
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[(size_t)d * nblocks + t] = hist[d];
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_rowscan_kernel(uint32_t* __restrict__ counts,
@@ -244,7 +261,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __shared__ int32_t s_out[kRadix];          // global position of the key in slot i: s_out[d] + i
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
-  const size_t base = (size_t)blockIdx.x * kSortTile;
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const size_t base = (size_t)t * kSortTile;
   n = live_count(n, n_dev);
   if (base >= n) return;  // uniform across the block
   const uint32_t nloc = (uint32_t)min((size_t)kSortTile, n - base);
@@ -291,7 +309,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   const uint32_t local = block_exclusive_scan(here, &tot);
   const uint32_t digit_base = block_exclusive_scan(counts[(size_t)kRadix * nblocks + d], &tot);
   s_local[d] = local;
-  s_out[d] = (int32_t)(digit_base + counts[(size_t)d * nblocks + blockIdx.x]) - (int32_t)local;
+  s_out[d] = (int32_t)(digit_base + counts[(size_t)d * nblocks + t]) - (int32_t)local;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
@@ -425,10 +443,13 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     }
     R = min(R, want);
   }
-  const uint32_t k0 = blockIdx.x * (uint32_t)kDupChunk;
+  // the sort tile this block expands (xcd_tile: its count column shares
+  // lines with tiles on the same XCD)
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t k0 = t * (uint32_t)kDupChunk;
   if (k0 >= R) {  // past the live instances (capacity-sized grids): an empty sort tile
     if (counts)
-      for (int d = tid; d < kRadix; d += kDupThreads) counts[(size_t)d * nblocks + blockIdx.x] = 0u;
+      for (int d = tid; d < kRadix; d += kDupThreads) counts[(size_t)d * nblocks + t] = 0u;
     return;
   }
   const uint32_t k1 = min(R, k0 + (uint32_t)kDupChunk);
@@ -498,7 +519,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     for (int u = 0; u < kDupRun; ++u)
       if (kb + (uint32_t)u < k1) atomicAdd(&s_hist[keys[u] & count_mask], 1u);
     __syncthreads();
-    for (int d = tid; d < kRadix; d += kDupThreads) counts[(size_t)d * nblocks + blockIdx.x] = s_hist[d];
+    for (int d = tid; d < kRadix; d += kDupThreads) counts[(size_t)d * nblocks + t] = s_hist[d];
   }
   if (kb >= k1) return;
   // The gradient slots number as many as the instances: this pass also marks
