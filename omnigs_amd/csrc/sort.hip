@@ -225,17 +225,31 @@ __global__ __launch_bounds__(kSortThreads) void radix_count_kernel(const uint32_
   for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[(size_t)d * nblocks + t] = hist[d];
 }
 
+// One block per digit.  A round covers kRowItems consecutive entries per
+// thread (one 64-byte line each), summed serially around a single block scan,
+// so that a row of up to 4096 tiles takes one round instead of one per 256.
+constexpr int kRowItems = 16;
 __global__ __launch_bounds__(kSortThreads) void radix_rowscan_kernel(uint32_t* __restrict__ counts,
                                                                      uint32_t nblocks) {
   const uint32_t d = blockIdx.x;
   uint32_t* row = counts + (size_t)d * nblocks;
   uint32_t carry = 0;
-  for (uint32_t c = 0; c < nblocks; c += kSortThreads) {
-    const uint32_t i = c + threadIdx.x;
-    const uint32_t v = i < nblocks ? row[i] : 0u;
+  for (uint32_t c = 0; c < nblocks; c += kSortThreads * kRowItems) {
+    const uint32_t i0 = c + threadIdx.x * kRowItems;
+    uint32_t v[kRowItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kRowItems; ++k) {
+      v[k] = i0 + k < nblocks ? row[i0 + k] : 0u;
+      sum += v[k];
+    }
     uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, &tot);
-    if (i < nblocks) row[i] = carry + ex;
+    uint32_t run = carry + block_exclusive_scan(sum, &tot);
+#pragma unroll
+    for (int k = 0; k < kRowItems; ++k) {
+      if (i0 + k < nblocks) row[i0 + k] = run;
+      run += v[k];
+    }
     carry += tot;
   }
   if (threadIdx.x == 0) counts[(size_t)kRadix * nblocks + d] = carry;
