@@ -60,7 +60,8 @@ typedef struct omnigs_lonlat_forward_args {
   int P;                       /* number of Gaussians (at most 89478485 per call) */
   int D;                       /* active SH degree (0..3) */
   int M;                       /* SH coefficients per Gaussian (shs is [P,M,3]) */
-  int width, height;           /* equirectangular image size in pixels */
+  int width, height;           /* equirectangular image size in pixels (at most 32766 tile
+                                  columns and 65535 tile rows, else OMNIGS_ERR_ARG) */
   const float* background;     /* [3] */
   const float* means3D;        /* [P,3] world positions */
   const float* shs;            /* [P,M,3] or NULL (then colors_precomp is used) */

@@ -228,6 +228,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
   hipStream_t s = (hipStream_t)stream_;
   const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
+  if (grid.gx > kMaxGridCols || grid.gy > 65535) return fail_arg("tile grid over 32766 columns or 65535 rows");
 
   char* gbuf = grow(geometry_buffer, carve_geom(nullptr, P).bytes);
   char* ibuf = render ? grow(image_buffer, carve_img(nullptr, W, H).bytes) : kept_image;
@@ -459,6 +460,7 @@ int make_view_table(int P, int V, const int* widths, const int* heights, int bx,
   for (int v = 0; v < V; ++v) {
     if (widths[v] <= 0 || heights[v] <= 0) return fail_arg("empty view");
     const TileGrid g = tile_grid(widths[v], heights[v], wrap, bx, by);
+    if (g.gx > kMaxGridCols || g.gy > 65535) return fail_arg("view tile grid over 32766 columns or 65535 rows");
     vt.W[v] = widths[v];
     vt.H[v] = heights[v];
     vt.gx[v] = g.gx;

@@ -94,12 +94,21 @@ __device__ __forceinline__ Rect tile_rect(float2 p, int r, TileGrid g) {
   return {rect_lo(p.x, r, g.bx, g.gx), rect_lo(p.y, r, g.by, g.gy), rect_hi(p.x, r, g.bx, g.gx),
           rect_hi(p.y, r, g.by, g.gy)};
 }
-// A Gaussian's expansion record (see GeomState::dup_rect); a culled one has
-// area 0 and a one-column rect.
-__device__ __forceinline__ uint4 pack_dup_rect(bool live, int area, const Rect& rc) {
-  if (!live) return make_uint4(0u, 0u, 1u << 16, 0u);
-  const uint32_t w = (uint32_t)max(1, rc.x1 - rc.x0);
-  return make_uint4((uint32_t)area, (uint32_t)rc.x0, (uint32_t)rc.y0 | (w << 16), 0u);
+// A Gaussian's expansion record (see GeomState::dup_rect), 8 bytes:
+// {x0 (low 16 bits, signed) | h << 16, y0 | w << 16}, the area being w h; a
+// culled one has h = 0 and a one-column rect.  Fits while the tile grid is
+// below kMaxGridCols columns (|x0| <= gx + 1 in seam-wrap mode) and 65536
+// rows.
+constexpr int kMaxGridCols = 32766;
+__device__ __forceinline__ uint2 pack_dup_rect(bool live, int area, const Rect& rc) {
+  if (!live || area <= 0) return make_uint2(0u, 1u << 16);
+  const uint32_t w = (uint32_t)(rc.x1 - rc.x0), h = (uint32_t)(rc.y1 - rc.y0);  // area = w h > 0
+  return make_uint2(((uint32_t)rc.x0 & 0xFFFFu) | (h << 16), (uint32_t)rc.y0 | (w << 16));
+}
+// Its area and the rect {x0, y0 | w << 16} that the expansion reads.
+__device__ __forceinline__ uint32_t dup_rect_area(uint2 r) { return (r.x >> 16) * (r.y >> 16); }
+__device__ __forceinline__ uint2 dup_rect_xy(uint2 r) {
+  return make_uint2((uint32_t)(int32_t)(int16_t)(r.x & 0xFFFFu), r.y);
 }
 // Tile column of rect column x (identity in clamp mode).
 __device__ __forceinline__ int wrap_col(int x, TileGrid g) { return g.wrap ? ((x % g.gx) + g.gx) % g.gx : x; }
@@ -210,11 +219,10 @@ struct GeomState {
   float4* rec;
   uint8_t* clamped;        // bit c set when channel c was clamped at 0
   uint32_t* tiles_touched; // rect area
-  // Expansion input per Gaussian, in index order: {area, rect x0 (int bits),
-  // y0 | w << 16, 0} (pack_dup_rect); the offset scan gathers it once through
-  // the depth order into rect_depth {x0, y0 | w << 16}, which the expansion
-  // then reads contiguously.
-  uint4* dup_rect;
+  // Expansion input per Gaussian, in index order (pack_dup_rect, 8 bytes);
+  // the offset scan gathers it once through the depth order into rect_depth
+  // {x0, y0 | w << 16}, which the expansion then reads contiguously.
+  uint2* dup_rect;
   uint2* rect_depth;
   uint32_t* slot_local;    // copy of the record's `slot` (dense, for the gradient gather)
   uint32_t* block_slots;   // per 256-Gaussian block: its Gaussians' total rect area
@@ -236,7 +244,7 @@ inline GeomState carve_geom(char* base, size_t P) {
   g.rec = c.take<float4>(3 * P);
   g.clamped = c.take<uint8_t>(P);
   g.tiles_touched = c.take<uint32_t>(P);
-  g.dup_rect = c.take<uint4>(P);
+  g.dup_rect = c.take<uint2>(P);
   g.rect_depth = c.take<uint2>(P);
   g.slot_local = c.take<uint32_t>(P);
   g.block_slots = c.take<uint32_t>(slot_blocks(P) + 1);

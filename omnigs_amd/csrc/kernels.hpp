@@ -74,7 +74,7 @@ hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmat
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
                        uint32_t* partials, hipStream_t s, uint32_t* gather_buf = nullptr,
                        const uint32_t* side_in = nullptr, size_t side_n = 0, uint32_t* side_out = nullptr,
-                       const uint4* rect_in = nullptr, uint2* rect_out = nullptr);
+                       const uint2* rect_in = nullptr, uint2* rect_out = nullptr);
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned

@@ -55,8 +55,8 @@ __device__ void block_scan_array(const uint32_t* __restrict__ in, size_t n, uint
 // the last (launched when side_in is set) scans the unrelated short array
 // side_in -> side_out (the per-block gradient slot counts) while the others
 // reduce, so its serial chunks cost no launch and no time of their own.
-// kRect: the values are the area words of the expansion records rect_in,
-// gathered through idx; the rest of each record goes to rect_out[i].
+// kRect: the values are the areas of the expansion records rect_in,
+// gathered through idx; each record's rect goes to rect_out[i].
 template <bool kRect>
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                    const uint32_t* __restrict__ idx, size_t n,
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
                                                                    uint32_t* __restrict__ gathered,
                                                                    const uint32_t* __restrict__ side_in, size_t side_n,
                                                                    uint32_t* __restrict__ side_out,
-                                                                   const uint4* __restrict__ rect_in,
+                                                                   const uint2* __restrict__ rect_in,
                                                                    uint2* __restrict__ rect_out) {
   if (side_in && blockIdx.x == gridDim.x - 1) {
     block_scan_array(side_in, side_n, side_out);
@@ -78,9 +78,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
     if (i < n) {
       uint32_t v;
       if constexpr (kRect) {
-        const uint4 r = rect_in[idx[i]];
-        v = r.x;
-        rect_out[i] = make_uint2(r.y, r.z);
+        const uint2 r = rect_in[idx[i]];
+        v = dup_rect_area(r);
+        rect_out[i] = dup_rect_xy(r);
       } else {
         v = load_in(in, idx, i);
       }
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_downsweep_kernel(const uint
 
 hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, size_t n, bool exclusive,
                        uint32_t* partials, hipStream_t s, uint32_t* gather_buf, const uint32_t* side_in,
-                       size_t side_n, uint32_t* side_out, const uint4* rect_in, uint2* rect_out) {
+                       size_t side_n, uint32_t* side_out, const uint2* rect_in, uint2* rect_out) {
   if (n == 0) return hipSuccess;
   const size_t nb = scan_blocks(n);
   uint32_t* gathered = idx ? gather_buf : nullptr;
@@ -164,7 +164,7 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, s
                        side ? side_in : nullptr, side_n, side_out, rect_in, rect_out);
   } else {
     hipLaunchKernelGGL(scan_reduce_kernel<false>, grid, dim3(kScanThreads), 0, s, in, idx, n, partials, gathered,
-                       side ? side_in : nullptr, side_n, side_out, (const uint4*)nullptr, (uint2*)nullptr);
+                       side ? side_in : nullptr, side_n, side_out, (const uint2*)nullptr, (uint2*)nullptr);
   }
   hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, nb);
   const uint32_t* din = gathered ? gathered : in;
