@@ -67,6 +67,16 @@ def test_argument_validation_needs_no_gpu(lib):
     a.P = 10  # required pointers missing
     st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
     assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    # a tile grid wider than the expansion record holds is refused before any
+    # device work (the pointers are never dereferenced)
+    dummy = (ctypes.c_float * 64)()
+    addr = ctypes.addressof(dummy)
+    for f in ("background", "means3D", "shs", "opacities", "scales", "rotations", "viewmatrix", "campos",
+              "out_color"):
+        setattr(a, f, addr)
+    a.D, a.M, a.width, a.height = 0, 1, 16 * 32767, 32  # 32767 tile columns, one past the limit
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
+    assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT and b"tile grid" in lib.omnigs_last_error()
     b = capi.BackwardArgs()
     b.P, b.R, b.width, b.height = 10, 5, 64, 32
     assert lib.omnigs_lonlat_backward(ctypes.byref(b), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
