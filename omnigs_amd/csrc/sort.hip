@@ -602,22 +602,39 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Four consecutive keys per thread (one 16-byte load) and the key before
+// them; a tile's range starts where its key differs from the previous one.
+constexpr int kRangeKeys = 4;
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t R, const uint32_t* __restrict__ R_dev,
                                    uint2* __restrict__ ranges) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k0 = (blockIdx.x * blockDim.x + threadIdx.x) * (uint32_t)kRangeKeys;
   if (R_dev) R = min(R, *R_dev);
-  if (k >= R) return;
-  const uint32_t t = keys[k];
-  if (k == 0) {
-    ranges[t].x = 0;
+  if (k0 >= R) return;
+  uint32_t t[kRangeKeys];
+  if (k0 + kRangeKeys <= R) {
+    const uint4 q = *reinterpret_cast<const uint4*>(keys + k0);  // keys is 16-byte aligned
+    t[0] = q.x;
+    t[1] = q.y;
+    t[2] = q.z;
+    t[3] = q.w;
   } else {
-    const uint32_t pt = keys[k - 1];
-    if (pt != t) {
-      ranges[pt].y = k;
-      ranges[t].x = k;
-    }
+#pragma unroll
+    for (int j = 0; j < kRangeKeys; ++j) t[j] = k0 + j < R ? keys[k0 + j] : 0u;
   }
-  if (k == R - 1) ranges[t].y = R;
+  uint32_t prev = k0 > 0 ? keys[k0 - 1] : 0u;
+#pragma unroll
+  for (int j = 0; j < kRangeKeys; ++j) {
+    const uint32_t k = k0 + (uint32_t)j;
+    if (k >= R) break;
+    if (k == 0) {
+      ranges[t[j]].x = 0;
+    } else if (prev != t[j]) {
+      ranges[prev].y = k;
+      ranges[t[j]].x = k;
+    }
+    if (k == R - 1) ranges[t[j]].y = R;
+    prev = t[j];
+  }
 }
 
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
@@ -627,7 +644,8 @@ hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2*
     if (e != hipSuccess) return e;
   }
   if (R == 0) return hipSuccess;
-  hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, sorted_tile_keys,
+  const size_t threads = (R + kRangeKeys - 1) / kRangeKeys;
+  hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, sorted_tile_keys,
                      (uint32_t)R, R_dev, ranges);
   return hipGetLastError();
 }
