@@ -417,6 +417,45 @@ __device__ __forceinline__ uint32_t first_greater(const uint32_t* __restrict__ a
   return lo;
 }
 
+// first_greater for two values at once, by the whole 256-thread block: each
+// half runs a 128-ary search (threads [0, 128) for v[0], [128, 256) for
+// v[1]), one load per thread and round, so that 1.5 M entries take three
+// rounds instead of 21 dependent loads by one thread.  a is non-decreasing;
+// out[h] = the first i < n with a[i] > v[h], or n.  Every thread calls it.
+__device__ void block_first_greater2(const uint32_t* __restrict__ a, uint32_t n, uint32_t v0, uint32_t v1,
+                                     uint32_t* out) {
+  constexpr uint32_t kWays = kDupThreads / 2;
+  __shared__ uint32_t s_lo[2], s_hi[2], s_below[2];
+  const int tid = threadIdx.x, h = tid / (int)kWays;
+  const uint32_t j = (uint32_t)tid % kWays, v = h ? v1 : v0;
+  if (tid < 2) {
+    s_lo[tid] = 0;
+    s_hi[tid] = n;  // the answer lies in [lo, hi]
+  }
+  __syncthreads();
+  while (s_lo[0] < s_hi[0] || s_lo[1] < s_hi[1]) {  // block-uniform: shared values after a barrier
+    const uint32_t lo = s_lo[h], hi = s_hi[h];
+    const uint32_t stride = (hi - lo + kWays - 1) / kWays;  // 0 once this half is done
+    const uint32_t q = lo + j * stride;
+    // probes at or below v form a prefix of the half's 128 probes
+    const bool below = stride > 0 && q < hi && a[q] <= v;
+    const uint64_t bal = __ballot(below);
+    __syncthreads();  // every thread has read lo / hi
+    if (tid < 2) s_below[tid] = 0;
+    __syncthreads();
+    if ((tid & (kWave - 1)) == 0 && bal) atomicAdd(&s_below[h], (uint32_t)__popcll(bal));
+    __syncthreads();
+    if (j == 0 && stride > 0) {
+      const uint32_t nb = s_below[h];  // probes 0 .. nb-1 are <= v
+      s_lo[h] = nb == 0 ? lo : lo + (nb - 1) * stride + 1;
+      s_hi[h] = min(hi, lo + nb * stride);
+    }
+    __syncthreads();
+  }
+  if (tid < 2) out[tid] = s_lo[tid];
+  __syncthreads();
+}
+
 // kMulti: items of a batch (item i = Gaussian i % vt.P in view i / vt.P), each
 // keyed by its view's global tile id.  R_dev: instance count on the device
 // (clamped to R, the capacity); status[0] receives the count needed and
@@ -468,11 +507,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   }
   const uint32_t k1 = min(R, k0 + (uint32_t)kDupChunk);
   for (int d = tid; d < kRadix; d += kDupThreads) s_hist[d] = 0u;
-  if (tid == 0) {
-    s_span[0] = first_greater(offsets, (uint32_t)P, k0);
-    s_span[1] = first_greater(offsets, (uint32_t)P, k1 - 1);
-  }
-  __syncthreads();
+  block_first_greater2(offsets, (uint32_t)P, k0, k1 - 1, s_span);
   const uint32_t sb = s_span[0];
   const uint32_t ns = s_span[1] - sb + 1;  // <= kDupChunk
   const uint32_t begin0 = sb == 0 ? 0u : offsets[sb - 1];
