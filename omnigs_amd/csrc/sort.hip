@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   __shared__ uint32_t s_hist[kRadix];
   __shared__ uint32_t s_end[kDupChunk];
   __shared__ uint32_t s_gid[kDupChunk];
-  __shared__ int s_x0[kDupChunk];
+  __shared__ int16_t s_x0[kDupChunk];  // |x0| <= gx + 1 <= kMaxGridCols + 1 fits 16 bits
   __shared__ uint16_t s_y0[kDupChunk], s_w[kDupChunk];
   __shared__ uint8_t s_view[kViewSlots];
   __shared__ uint32_t s_span[2];
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
     if constexpr (kMulti) s_view[j] = (uint8_t)(gg / (uint32_t)vt.P);
     // the rect in depth order, as the offset scan gathered it
     const uint2 rd = rect_depth[s];
-    s_x0[j] = (int)rd.x;
+    s_x0[j] = (int16_t)(int)rd.x;
     s_y0[j] = (uint16_t)(rd.y & 0xFFFFu);
     s_w[j] = (uint16_t)(rd.y >> 16);
   }
