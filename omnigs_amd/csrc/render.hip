@@ -349,8 +349,8 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
   // 4-byte arrays; 16-byte ones take 4 times that)
   __shared__ uint16_t s_list[kWavesPerTile][kBatch];
   // positions still to stage, back to front, as (position << 4 | wave bits),
-  // in a ring of 2 kBatch entries from q_head
-  __shared__ uint32_t s_queue[2 * kBatch];
+  // in a ring of 4 kBatch entries from q_head
+  __shared__ uint32_t s_queue[4 * kBatch];
   __shared__ int s_wlast[kWavesPerTile];
   __shared__ int s_nlive;
 
@@ -458,22 +458,25 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
 
   // The list is walked from position n - 1 down.  Positions that some wave
   // must evaluate -- with the forward's blend bits, those some wave blended
-  // (often under half of them) -- enter a queue in LDS, 256 positions per
-  // block scan, and batches of up to 256 are staged from its front, so
-  // batches hold no instance that no wave evaluates.
-  constexpr uint32_t kRing = 2 * kBatch - 1;
+  // (often under half of them) -- enter a queue in LDS, 512 positions (two
+  // per thread) per block scan, and batches of up to 256 are staged from its
+  // front, so batches hold no instance that no wave evaluates.  The queue
+  // holds under kBatch entries before a fill, so at most 3 kBatch - 1 after.
+  constexpr uint32_t kRing = 4 * kBatch - 1;
   int q_head = 0, q_len = 0, scanned = 0;  // block-uniform
   for (;;) {
     __syncthreads();
     while (q_len < kBatch && scanned < n) {
-      const int k = scanned + tid;
-      uint32_t bits = 0u;
-      if (k < n) bits = use_blend ? blend_in[range.x + (n - 1 - k)] & 0xFu : 0xFu;
+      const int k = scanned + 2 * tid;  // positions n - 1 - k and n - 2 - k
+      uint32_t b0 = 0u, b1 = 0u;
+      if (k < n) b0 = use_blend ? blend_in[range.x + (n - 1 - k)] & 0xFu : 0xFu;
+      if (k + 1 < n) b1 = use_blend ? blend_in[range.x + (n - 2 - k)] & 0xFu : 0xFu;
       uint32_t added;
-      const uint32_t at = block_exclusive_scan(bits ? 1u : 0u, &added);
-      if (bits) s_queue[(q_head + q_len + (int)at) & kRing] = ((uint32_t)(n - 1 - k) << 4) | bits;
+      uint32_t at = block_exclusive_scan((b0 ? 1u : 0u) + (b1 ? 1u : 0u), &added);
+      if (b0) s_queue[(q_head + q_len + (int)at++) & kRing] = ((uint32_t)(n - 1 - k) << 4) | b0;
+      if (b1) s_queue[(q_head + q_len + (int)at) & kRing] = ((uint32_t)(n - 2 - k) << 4) | b1;
       q_len += __builtin_amdgcn_readfirstlane((int)added);  // uniform: kept in a scalar register
-      scanned += kBatch;
+      scanned += 2 * kBatch;
     }
     if (q_len == 0) break;
     __syncthreads();
