@@ -159,7 +159,7 @@ inline size_t sort_workspace_words(size_t n) { return (size_t)kRadix * (sort_blo
 
 // Scan geometry: 256 threads x 16 items per block.
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 inline size_t scan_blocks(size_t n) { return (n + kScanTile - 1) / kScanTile; }
 
