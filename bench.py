@@ -74,8 +74,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are out of scope "
-                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "only one GPU is measured: the rasterizer runs on one GPU per process, "
+                                   "and multi-GPU sharding is a design, not a measured path", "n_gpus": args.gpus}))
         return 2
 
     import numpy as np
