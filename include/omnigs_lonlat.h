@@ -61,7 +61,8 @@ typedef struct omnigs_lonlat_forward_args {
   int D;                       /* active SH degree (0..3) */
   int M;                       /* SH coefficients per Gaussian (shs is [P,M,3]) */
   int width, height;           /* equirectangular image size in pixels (at most 32766 tile
-                                  columns and 65535 tile rows, else OMNIGS_ERR_ARG) */
+                                  columns, 65535 tile rows and 2^21 - 1 tiles, else
+                                  OMNIGS_ERR_INVALID_ARGUMENT) */
   const float* background;     /* [3] */
   const float* means3D;        /* [P,3] world positions */
   const float* shs;            /* [P,M,3] or NULL (then colors_precomp is used) */
@@ -77,7 +78,8 @@ typedef struct omnigs_lonlat_forward_args {
   float* out_color;            /* [3,H,W] output image (planar CHW) */
   int* radii;                  /* [P] output radii in pixels, or NULL (kept internally) */
   int seam_wrap;               /* 0: clamp tile columns at the image edge (the reference's behaviour);
-                                  1: wrap them around the longitude seam (extension, no reference counterpart) */
+                                  1: wrap them around the longitude seam (extension, no reference
+                                  counterpart; the width must be a multiple of 16, the tile width) */
   /* Colour decoder (extension, no reference counterpart): when both are set
    * the colours come from the decoder (omnigs_mlp_*) evaluated inside
    * preprocess, and shs / colors_precomp are ignored. */
@@ -187,7 +189,8 @@ int omnigs_photometric_loss(const float* image, const float* gt_image, const flo
  * No host synchronisation and no allocation happen inside (the instance
  * count stays on the device, bounded by the workspace's capacity), so a
  * caller may capture forward + backward into one HIP graph.  status (device,
- * 2 uints) receives {instances needed, 1 if that exceeded capacity}; after
+ * 2 uints) receives {instances needed, 1 if that exceeded capacity}, with
+ * 0xFFFFFFFF needed when the count does not fit 32 bits; after
  * an overflow the images are incomplete and the caller re-runs with a larger
  * capacity.  Pixel arrays (out_color, dL_dpix) hold view v's [3, H_v, W_v]
  * block at float offset 3 * sum_{u<v} W_u H_u. */
@@ -346,9 +349,13 @@ size_t omnigs_lonlat_image_bytes(int width, int height);
  * pointers may be NULL; the matching fields are then NULL. */
 typedef struct omnigs_lonlat_state_view {
   /* geometry buffer, per Gaussian */
-  const uint32_t* depth_key;     /* float bits of |t|; 0xFFFFFFFF when culled */
+  const uint32_t* depth_key;     /* the depth-sort keys in depth order, aligned with depth_order:
+                                    depth_key[i] = float bits of |t| of Gaussian depth_order[i]
+                                    (0xFFFFFFFF when culled) */
   const int32_t* radii;          /* internal radii (written when the forward got radii == NULL) */
-  const float* render_records;   /* [P,12]: x, y, conic A, B, C, opacity, r, g, b, 0, 0, 0 */
+  const float* render_records;   /* [P,12]: x, y, conic A, B, C, opacity, r, g, b, then three 32-bit
+                                    integers stored in the float slots: radius, the Gaussian's first
+                                    gradient slot within its 256-Gaussian block, and 0 */
   const uint8_t* clamped;        /* [P] bit c = channel c clamped */
   const uint32_t* tiles_touched; /* [P] */
   const uint32_t* depth_order;   /* [P] Gaussian ids by (depth bits, id) */

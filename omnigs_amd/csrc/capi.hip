@@ -16,6 +16,12 @@ namespace {
 
 thread_local char g_last_error[512] = "";
 
+// Seam wrap reads tile columns modulo the grid width and pixel offsets modulo
+// the image width; the two rings agree only when the tiles tile the width
+// exactly, so other widths are refused rather than dropping instances near
+// the seam.
+constexpr const char* kWrapWidthMsg = "seam_wrap needs an image width that is a multiple of the tile width";
+
 int fail_hip(hipError_t e, const char* where) {
   std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s", where, hipGetErrorString(e));
   return OMNIGS_ERR_HIP;
@@ -194,8 +200,9 @@ int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream) {
 
 namespace {
 
-// One pinned host word per calling thread for the forward's read-back of the
-// instance count (allocated on first use, kept for the thread's lifetime).
+// Pinned host words per calling thread for the forward's read-back of the
+// instance count and its overflow flag (allocated on first use, kept for the
+// thread's lifetime).
 uint32_t* pinned_word() {
   thread_local uint32_t* word = nullptr;
   thread_local bool tried = false;
@@ -229,6 +236,8 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   hipStream_t s = (hipStream_t)stream_;
   const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
   if (grid.gx > kMaxGridCols || grid.gy > 65535) return fail_arg("tile grid over 32766 columns or 65535 rows");
+  if (grid.tiles() > kMaxTiles) return fail_arg("more than 2^21 tiles in one image");
+  if (a->seam_wrap && W % grid.bx != 0) return fail_arg(kWrapWidthMsg);
 
   char* gbuf = grow(geometry_buffer, carve_geom(nullptr, P).bytes);
   char* ibuf = render ? grow(image_buffer, carve_img(nullptr, W, H).bytes) : kept_image;
@@ -271,19 +280,23 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   // the rest of the forward can be queued.  It is copied into a pinned word
   // and the stream is polled rather than waited on: the GPU idles until the
   // next launch, and a blocking wait adds the thread's wake-up to that.
-  uint32_t R = 0;
+  // The scan leaves {total, 32-bit overflow flag} after its block partials.
+  const uint32_t* total = g.scan_partials + scan_blocks((size_t)P);
+  uint32_t Rw[2] = {0u, 0u};
   if (uint32_t* pw = pinned_word()) {
-    OMNIGS_TRY(hipMemcpyAsync(pw, g.offsets + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
+    OMNIGS_TRY(hipMemcpyAsync(pw, total, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
     hipError_t q;
     while ((q = hipStreamQuery(s)) == hipErrorNotReady) {
     }
     OMNIGS_TRY(q, "sync R");
-    R = *pw;
+    Rw[0] = pw[0];
+    Rw[1] = pw[1];
   } else {
-    OMNIGS_TRY(hipMemcpyAsync(&R, g.offsets + (P - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
+    OMNIGS_TRY(hipMemcpyAsync(Rw, total, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
     OMNIGS_TRY(hipStreamSynchronize(s), "sync R");
   }
-  if (R > 0x7fffffffu) return fail_arg("instance count exceeds int range");
+  if (Rw[1] != 0u || Rw[0] > 0x7fffffffu) return fail_arg("instance count exceeds int range");
+  const uint32_t R = Rw[0];
 
   char* bbuf = grow(binning_buffer, carve_bin(nullptr, R).bytes);
   if (!bbuf) {
@@ -357,6 +370,7 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   if (a->scales && (!a->dL_dscale || !a->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
   hipStream_t s = (hipStream_t)stream_;
   const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
+  if (a->seam_wrap && W % grid.bx != 0) return fail_arg(kWrapWidthMsg);
   const GeomState g = carve_geom(a->geom_buffer, P);
   const ImgState img = carve_img(a->image_buffer, W, H);
   const int* radii = a->radii ? a->radii : g.radii;
@@ -461,6 +475,8 @@ int make_view_table(int P, int V, const int* widths, const int* heights, int bx,
     if (widths[v] <= 0 || heights[v] <= 0) return fail_arg("empty view");
     const TileGrid g = tile_grid(widths[v], heights[v], wrap, bx, by);
     if (g.gx > kMaxGridCols || g.gy > 65535) return fail_arg("view tile grid over 32766 columns or 65535 rows");
+    if (g.tiles() > kMaxTiles) return fail_arg("more than 2^21 tiles in one view");
+    if (wrap && widths[v] % bx != 0) return fail_arg(kWrapWidthMsg);
     vt.W[v] = widths[v];
     vt.H[v] = heights[v];
     vt.gx[v] = g.gx;
@@ -534,7 +550,8 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
                            g.block_slots, slot_blocks(P), g.slot_base, g.dup_rect, g.rect_depth),
                "batch scan");
   }
-  const uint32_t* R_dev = g.offsets + (items - 1);
+  // {instance count, 32-bit overflow flag} after the offset scan's block partials
+  const uint32_t* R_dev = g.scan_partials + scan_blocks(items);
   const BinState& b = bs.b;
   const int tile_bits = bit_width_u32(vt.tile_base[vt.V]);
   {

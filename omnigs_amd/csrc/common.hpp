@@ -201,6 +201,10 @@ constexpr int kSlotBlock = 256;
 // The render kernels hold a tile-list position times 48 (a staging byte
 // offset) in 32 bits, so a tile's list -- at most P entries -- is bounded.
 constexpr int kMaxGaussians = 89478485;  // floor((2^32 - 1) / 48)
+// Tiles per image (per view in a batch).  A Gaussian touches at most this
+// many, so a scan block of kScanTile tile counts sums below 2^32 and only the
+// scan's grand total can wrap (scan_partials_kernel flags it).
+constexpr int kMaxTiles = (1 << 21) - 1;
 inline size_t slot_blocks(size_t n) { return (n + kSlotBlock - 1) / kSlotBlock; }
 // Opacity word of a gradient slot that holds no record (a NaN pattern the
 // hardware never produces: its NaNs are the canonical 0x7FC00000).
@@ -253,7 +257,7 @@ inline GeomState carve_geom(char* base, size_t P) {
   g.order_alt = c.take<uint32_t>(P);
   g.key_alt = c.take<uint32_t>(P);
   g.offsets = c.take<uint32_t>(P);
-  g.scan_partials = c.take<uint32_t>(scan_blocks(P) + 1);
+  g.scan_partials = c.take<uint32_t>(scan_blocks(P) + 2);  // + total, overflow flag
   g.sort_counts = c.take<uint32_t>(sort_workspace_words(P));
   g.bytes = align_up(c.off) + kAlign;
   return g;

@@ -65,7 +65,8 @@ hipError_t launch_preprocess_views(const PreprocessArgs& a, const float* viewmat
 
 // sort.hip
 // Inclusive (or exclusive) scan of n u32 values, optionally gathered through
-// idx (out[i] = sum of in[idx[j]] for j <= i).  partials needs scan_blocks(n)+1 entries.
+// idx (out[i] = sum of in[idx[j]] for j <= i).  partials needs scan_blocks(n)+2 entries:
+// the grand total lands in partials[nb] and a 32-bit overflow flag in partials[nb+1] (nb = scan_blocks(n)).
 // side_in (optional): a short array (a few thousand values) whose exclusive
 // scan goes to side_out in the same launches (the gradient slot bases).
 // rect_in (optional, with idx and gather_buf): the values are the area words

@@ -272,7 +272,9 @@ __global__ __launch_bounds__(kFinalThreads) void loss_final_kernel(const float2*
   }
 }
 
-static size_t ssim_map_floats(int Hc, int W, int C) { return (size_t)3 * C * Hc * W; }
+// The three SSIM gradient maps, rounded up to 64 floats so that the block
+// partials placed after them (read as float2) stay 256-byte aligned.
+static size_t ssim_map_floats(int Hc, int W, int C) { return ((size_t)3 * C * Hc * W + 63) & ~(size_t)63; }
 
 static size_t ssim_blocks(int C, int Hc, int W) {
   return (size_t)((W + kLTX - 1) / kLTX) * ((Hc + kLTY - 1) / kLTY) * C;

@@ -94,8 +94,15 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
 }
 
 // Exclusive scan of the block partials in place, one block looping over
-// chunks; their total goes to partials[nb].
+// chunks.  Their total goes to partials[nb] and an overflow flag to
+// partials[nb + 1]: the scan is 32-bit, so the total is also summed in 64
+// bits, and a total past 2^32 - 1 leaves {0, 1} there (no usable instance,
+// overflow) instead of a wrapped count.  A single block's partial cannot
+// wrap: its kScanTile values are each at most kMaxTiles.
 __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ partials, size_t nb) {
+  __shared__ unsigned long long s_total;
+  if (threadIdx.x == 0) s_total = 0ull;
+  __syncthreads();
   uint32_t carry = 0;
   for (size_t c = 0; c < nb; c += kScanThreads) {
     const size_t i = c + threadIdx.x;
@@ -104,8 +111,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* _
     const uint32_t ex = block_exclusive_scan(v, &tot);
     if (i < nb) partials[i] = carry + ex;
     carry += tot;
+    unsigned long long w = v;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) w += __shfl_xor(w, o);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&s_total, w);
   }
-  if (threadIdx.x == 0) partials[nb] = carry;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool ovf = s_total > 0xFFFFFFFFull;
+    partials[nb] = ovf ? 0u : carry;
+    partials[nb + 1] = ovf ? 1u : 0u;
+  }
 }
 
 // Each thread scans kScanItems consecutive elements (blocked layout); the
@@ -489,10 +505,12 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   // this binning writes them
   if (blend_ok && blockIdx.x == 0 && tid == 0) *blend_ok = 0u;
   if (R_dev) {
-    const uint32_t want = *R_dev;
+    // R_dev = {count, 32-bit overflow flag} from the offset scan's partials
+    const uint32_t want = R_dev[0];
+    const bool wrapped = R_dev[1] != 0u;
     if (status && blockIdx.x == 0 && tid == 0) {
-      status[0] = want;
-      status[1] = want > R ? 1u : 0u;
+      status[0] = wrapped ? 0xFFFFFFFFu : want;
+      status[1] = (wrapped || want > R) ? 1u : 0u;
     }
     R = min(R, want);
   }

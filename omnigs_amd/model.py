@@ -212,9 +212,15 @@ class GaussianModel:
         self.prune_points(prune)
 
     def reset_opacity(self) -> None:
-        """opacity = inverse_sigmoid(min(opacity, 0.01)); its moments restart
-        (gaussian_model.cpp:564-573, replaceTensorToOptimizer)."""
-        new = inverse_sigmoid(torch.minimum(self.get_opacity(), torch.full_like(self.get_opacity(), 0.01)))
+        """GaussianModel::resetOpacity (gaussian_model.cpp:564-573):
+        opacity = inverse_sigmoid(min(sigmoid(opacity), ones_like(sigmoid(opacity) * 0.01))).
+        The reference clamps against ones_like(...), a tensor of ones, so the
+        activated opacities (all below 1) pass through unchanged apart from
+        the sigmoid / inverse-sigmoid round trip; what the reset does there is
+        restart the Adam moments (replaceTensorToOptimizer, :575-594: zeroed
+        exp_avg and exp_avg_sq, step kept).  Kept as the reference behaves."""
+        act = self.get_opacity()
+        new = inverse_sigmoid(torch.minimum(act, torch.ones_like(act * 0.01)))
         self.p["opacity"] = new.contiguous()
         if self.state:
             st = self.state["opacity"]

@@ -77,9 +77,27 @@ def test_argument_validation_needs_no_gpu(lib):
     a.D, a.M, a.width, a.height = 0, 1, 16 * 32767, 32  # 32767 tile columns, one past the limit
     st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
     assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT and b"tile grid" in lib.omnigs_last_error()
+    # more tiles than a scan block can sum in 32 bits
+    a.width, a.height = 16 * 2048, 16 * 1024  # 2^21 tiles
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
+    assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT and b"2^21 tiles" in lib.omnigs_last_error()
+    # seam wrap needs whole tiles across the width: at W = 1000 the 63-tile ring
+    # would be 1008 px wide and instances near the seam would be lost
+    a.width, a.height, a.seam_wrap = 1000, 500, 1
+    st = lib.omnigs_lonlat_forward(ctypes.byref(a), null_buf, null_buf, null_buf, None, ctypes.byref(R))
+    assert st == capi.OMNIGS_ERR_INVALID_ARGUMENT and b"multiple of the tile width" in lib.omnigs_last_error()
+    a.seam_wrap = 0
     b = capi.BackwardArgs()
     b.P, b.R, b.width, b.height = 10, 5, 64, 32
     assert lib.omnigs_lonlat_backward(ctypes.byref(b), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    for f in ("background", "means3D", "shs", "scales", "rotations", "viewmatrix", "campos", "dL_dpix",
+              "dL_dmean2D", "dL_dconic", "dL_dopacity", "dL_dcolor", "dL_dmean3D", "dL_dcov3D", "dL_dsh",
+              "dL_dscale", "dL_drot"):
+        setattr(b, f, addr)
+    b.geom_buffer = b.binning_buffer = b.image_buffer = addr
+    b.D, b.M, b.width, b.height, b.seam_wrap = 0, 1, 1000, 500, 1
+    assert lib.omnigs_lonlat_backward(ctypes.byref(b), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+    assert b"multiple of the tile width" in lib.omnigs_last_error()
     assert lib.omnigs_lonlat_mark_visible(-1, None, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
     assert lib.omnigs_lonlat_mark_visible(0, None, None) == capi.OMNIGS_OK
 
