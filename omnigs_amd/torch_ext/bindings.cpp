@@ -2,6 +2,7 @@
 #include <torch/extension.h>
 
 #include "gaussian_rasterizer.h"
+#include "omnigs_rasterizer.h"
 #include "rasterize_points.h"
 
 namespace {
@@ -13,10 +14,13 @@ std::tuple<torch::Tensor, torch::Tensor> rasterize_gaussians(
     int image_height, int image_width, torch::Tensor bg, float scale_modifier, torch::Tensor viewmatrix,
     torch::Tensor projmatrix, int sh_degree, torch::Tensor campos, bool prefiltered, int camera_type,
     bool checkpoint, bool seam_wrap) {
-  omnigs_torch::GaussianRasterizationSettings rs(image_height, image_width, 0.f, 0.f, bg, scale_modifier, viewmatrix,
-                                                 projmatrix, torch::Tensor(), sh_degree, campos, prefiltered,
-                                                 (omnigs_torch::CameraModelType)camera_type, false, seam_wrap);
-  omnigs_torch::GaussianRasterizer rasterizer(rs, checkpoint);
+  torch::Tensor RwcT;
+  GaussianRasterizationSettings rs(image_height, image_width, 0.f, 0.f, bg, scale_modifier, viewmatrix, projmatrix,
+                                   RwcT, sh_degree, campos, prefiltered, (Camera::CameraModelType)camera_type, false);
+  omnigs::RasterizerOptions opt;
+  opt.checkpoint = checkpoint;
+  opt.seam_wrap = seam_wrap;
+  omnigs::LonlatRasterizer rasterizer(rs, opt);
   const bool has_shs = shs.defined() && shs.numel() > 0;
   const bool has_cols = colors_precomp.defined() && colors_precomp.numel() > 0;
   const bool has_sc = scales.defined() && scales.numel() > 0;
@@ -32,18 +36,52 @@ std::tuple<torch::Tensor, torch::Tensor> rasterize_gaussians(
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.doc() = "LibTorch-ROCm boundary of the gfx950 equirectangular Gaussian rasterizer";
-  m.def("RasterizeGaussiansCUDA", &RasterizeGaussiansCUDA, py::arg("background"), py::arg("means3D"),
-        py::arg("colors"), py::arg("opacity"), py::arg("scales"), py::arg("rotations"), py::arg("scale_modifier"),
-        py::arg("cov3D_precomp"), py::arg("viewmatrix"), py::arg("projmatrix"), py::arg("tan_fovx"),
-        py::arg("tan_fovy"), py::arg("image_height"), py::arg("image_width"), py::arg("sh"), py::arg("degree"),
-        py::arg("campos"), py::arg("prefiltered"), py::arg("camera_type") = 1, py::arg("render_depth") = false,
-        py::arg("seam_wrap") = false);
-  m.def("RasterizeGaussiansBackwardCUDA", &RasterizeGaussiansBackwardCUDA, py::arg("background"),
-        py::arg("means3D"), py::arg("radii"), py::arg("colors"), py::arg("scales"), py::arg("rotations"),
-        py::arg("scale_modifier"), py::arg("cov3D_precomp"), py::arg("viewmatrix"), py::arg("projmatrix"),
-        py::arg("tan_fovx"), py::arg("tan_fovy"), py::arg("dL_dout_color"), py::arg("sh"), py::arg("degree"),
-        py::arg("campos"), py::arg("geomBuffer"), py::arg("R"), py::arg("binningBuffer"), py::arg("imageBuffer"),
-        py::arg("camera_type") = 1, py::arg("seam_wrap") = false);
+  // the reference's functions; seam_wrap = true selects the omnigs:: extension
+  m.def(
+      "RasterizeGaussiansCUDA",
+      [](const torch::Tensor& background, const torch::Tensor& means3D, const torch::Tensor& colors,
+         const torch::Tensor& opacity, const torch::Tensor& scales, const torch::Tensor& rotations,
+         float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+         const torch::Tensor& projmatrix, float tan_fovx, float tan_fovy, int image_height, int image_width,
+         const torch::Tensor& sh, int degree, const torch::Tensor& campos, bool prefiltered, int camera_type,
+         bool render_depth, bool seam_wrap) {
+        if (!seam_wrap)
+          return RasterizeGaussiansCUDA(background, means3D, colors, opacity, scales, rotations, scale_modifier,
+                                        cov3D_precomp, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height,
+                                        image_width, sh, degree, campos, prefiltered, camera_type, render_depth);
+        return omnigs::RasterizeGaussiansCUDA(background, means3D, colors, opacity, scales, rotations, scale_modifier,
+                                              cov3D_precomp, viewmatrix, projmatrix, tan_fovx, tan_fovy,
+                                              image_height, image_width, sh, degree, campos, prefiltered,
+                                              camera_type, render_depth, true);
+      },
+      py::arg("background"), py::arg("means3D"), py::arg("colors"), py::arg("opacity"), py::arg("scales"),
+      py::arg("rotations"), py::arg("scale_modifier"), py::arg("cov3D_precomp"), py::arg("viewmatrix"),
+      py::arg("projmatrix"), py::arg("tan_fovx"), py::arg("tan_fovy"), py::arg("image_height"),
+      py::arg("image_width"), py::arg("sh"), py::arg("degree"), py::arg("campos"), py::arg("prefiltered"),
+      py::arg("camera_type") = 1, py::arg("render_depth") = false, py::arg("seam_wrap") = false);
+  m.def(
+      "RasterizeGaussiansBackwardCUDA",
+      [](const torch::Tensor& background, const torch::Tensor& means3D, const torch::Tensor& radii,
+         const torch::Tensor& colors, const torch::Tensor& scales, const torch::Tensor& rotations,
+         float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+         const torch::Tensor& projmatrix, float tan_fovx, float tan_fovy, const torch::Tensor& dL_dout_color,
+         const torch::Tensor& sh, int degree, const torch::Tensor& campos, const torch::Tensor& geomBuffer, int R,
+         const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer, int camera_type, bool seam_wrap) {
+        if (!seam_wrap)
+          return RasterizeGaussiansBackwardCUDA(background, means3D, radii, colors, scales, rotations,
+                                                scale_modifier, cov3D_precomp, viewmatrix, projmatrix, tan_fovx,
+                                                tan_fovy, dL_dout_color, sh, degree, campos, geomBuffer, R,
+                                                binningBuffer, imageBuffer, camera_type);
+        return omnigs::RasterizeGaussiansBackwardCUDA(background, means3D, radii, colors, scales, rotations,
+                                                      scale_modifier, cov3D_precomp, viewmatrix, projmatrix,
+                                                      tan_fovx, tan_fovy, dL_dout_color, sh, degree, campos,
+                                                      geomBuffer, R, binningBuffer, imageBuffer, camera_type, true);
+      },
+      py::arg("background"), py::arg("means3D"), py::arg("radii"), py::arg("colors"), py::arg("scales"),
+      py::arg("rotations"), py::arg("scale_modifier"), py::arg("cov3D_precomp"), py::arg("viewmatrix"),
+      py::arg("projmatrix"), py::arg("tan_fovx"), py::arg("tan_fovy"), py::arg("dL_dout_color"), py::arg("sh"),
+      py::arg("degree"), py::arg("campos"), py::arg("geomBuffer"), py::arg("R"), py::arg("binningBuffer"),
+      py::arg("imageBuffer"), py::arg("camera_type") = 1, py::arg("seam_wrap") = false);
   m.def("markVisible", &markVisible, py::arg("means3D"), py::arg("viewmatrix"), py::arg("projmatrix"),
         py::arg("camera_type") = 1);
   m.def("rasterize_gaussians", &rasterize_gaussians, py::arg("means3D"), py::arg("means2D"), py::arg("opacities"),

@@ -5,8 +5,11 @@
 // same output tuple layout, zero-filled outputs, kByte scratch buffers grown
 // through a resize callback, P == 0 skipping the rasterizer, and the same
 // exceptions for a bad means3D shape or an invalid camera_type.  Work runs on
-// the current HIP stream of means3D's device.
+// the current HIP stream of means3D's device.  The global functions carry the
+// reference's signatures; the omnigs:: overloads add the seam-wrap switch.
 #include "rasterize_points.h"
+
+#include "omnigs_rasterizer.h"
 
 #include <c10/core/DeviceGuard.h>
 #include <c10/hip/HIPStream.h>
@@ -57,6 +60,8 @@ void* current_stream(const torch::Tensor& ref) {
 }
 
 }  // namespace
+
+namespace omnigs {
 
 std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> RasterizeGaussiansCUDA(
     const torch::Tensor& background, const torch::Tensor& means3D, const torch::Tensor& colors,
@@ -182,6 +187,38 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
   }
   return std::make_tuple(dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales,
                          dL_drotations);
+}
+
+}  // namespace omnigs
+
+std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> RasterizeGaussiansCUDA(
+    const torch::Tensor& background, const torch::Tensor& means3D, const torch::Tensor& colors,
+    const torch::Tensor& opacity, const torch::Tensor& scales, const torch::Tensor& rotations,
+    const float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+    const torch::Tensor& projmatrix, const float tan_fovx, const float tan_fovy, const int image_height,
+    const int image_width, const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
+    const bool prefiltered, const int camera_type, const bool render_depth) {
+  return omnigs::RasterizeGaussiansCUDA(background, means3D, colors, opacity, scales, rotations, scale_modifier,
+                                        cov3D_precomp, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height,
+                                        image_width, sh, degree, campos, prefiltered, camera_type, render_depth,
+                                        /*seam_wrap=*/false);
+}
+
+std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,
+           torch::Tensor>
+RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Tensor& means3D,
+                               const torch::Tensor& radii, const torch::Tensor& colors, const torch::Tensor& scales,
+                               const torch::Tensor& rotations, const float scale_modifier,
+                               const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+                               const torch::Tensor& projmatrix, const float tan_fovx, const float tan_fovy,
+                               const torch::Tensor& dL_dout_color, const torch::Tensor& sh, const int degree,
+                               const torch::Tensor& campos, const torch::Tensor& geomBuffer, const int R,
+                               const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer,
+                               const int camera_type) {
+  return omnigs::RasterizeGaussiansBackwardCUDA(background, means3D, radii, colors, scales, rotations,
+                                                scale_modifier, cov3D_precomp, viewmatrix, projmatrix, tan_fovx,
+                                                tan_fovy, dL_dout_color, sh, degree, campos, geomBuffer, R,
+                                                binningBuffer, imageBuffer, camera_type, /*seam_wrap=*/false);
 }
 
 torch::Tensor markVisible(torch::Tensor& means3D, torch::Tensor& /*viewmatrix*/, torch::Tensor& /*projmatrix*/,

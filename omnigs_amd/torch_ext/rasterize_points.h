@@ -1,16 +1,18 @@
 // LibTorch-ROCm boundary of the equirectangular rasterizer.
 //
-// Same declarations as the reference's include/rasterize_points.h:29-80, so a
-// caller built against them (GaussianRasterizerFunction, GaussianRenderer)
-// links against this library unchanged.  Only camera_type 3 (LONLAT) is
-// served; camera_type 1 (pinhole) is outside this build's scope and raises,
-// as any other invalid value does in the reference.  The trailing seam_wrap
-// parameter (default false = the reference's clamp at the image edge) is an
-// extension: tile columns wrap around the longitude seam.
+// The reference's declarations (include/rasterize_points.h:29-80), unchanged:
+// same names, parameter types, defaults and global scope, so a caller built
+// against them (GaussianRasterizerFunction, GaussianRenderer) links against
+// this library as it is.  Only camera_type 3 (LONLAT) is served; camera_type
+// 1 (pinhole) is outside this build's scope and raises, as any other invalid
+// value does in the reference.  The longitude-seam wrap extension has its own
+// entry points in omnigs_rasterizer.h.
 #pragma once
 
 #include <torch/torch.h>
 
+#include <cstdio>
+#include <string>
 #include <tuple>
 
 std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> RasterizeGaussiansCUDA(
@@ -19,7 +21,7 @@ std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torc
     const float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
     const torch::Tensor& projmatrix, const float tan_fovx, const float tan_fovy, const int image_height,
     const int image_width, const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
-    const bool prefiltered, const int camera_type = 1, const bool render_depth = false, const bool seam_wrap = false);
+    const bool prefiltered, const int camera_type = 1, const bool render_depth = false);
 
 std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,
            torch::Tensor>
@@ -31,7 +33,7 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
                                const torch::Tensor& dL_dout_color, const torch::Tensor& sh, const int degree,
                                const torch::Tensor& campos, const torch::Tensor& geomBuffer, const int R,
                                const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer,
-                               const int camera_type = 1, const bool seam_wrap = false);
+                               const int camera_type = 1);
 
 torch::Tensor markVisible(torch::Tensor& means3D, torch::Tensor& viewmatrix, torch::Tensor& projmatrix,
                           const int camera_type = 1);
