@@ -8,11 +8,11 @@ loss kernel (omnigs_photometric_loss with lambda 0) + backward (render
 backward, preprocess backward), all through the C ABI with inputs resident in
 HBM.  Prints ONE JSON line.
 
-The roofline object prices the dominant stage (by measured time) against
-HBM: achieved = its algorithmic bytes per launch / its average launch time
-(HIP events recorded around each stage on the launch stream during the
-timed region).  The CPU baseline runs the oracle (FP32, one thread) over one
-full fwd+bwd iteration of the same workload.
+The roofline object prices the dominant stage (by time, from the warm-up
+steps' per-stage events) against HBM: achieved = its algorithmic bytes per
+launch / its average launch time, from HIP events the library records around
+that stage on its launch stream during the timed steps.  The CPU baseline runs the FP32 oracle over full fwd+bwd
+iterations of the same workload on the host's cores (and one thread beside).
 """
 from __future__ import annotations
 
@@ -58,7 +58,7 @@ def algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--gaussians", type=int, default=1_500_000)
@@ -108,7 +108,8 @@ def main() -> int:
         return R, radii, g
 
     torch.cuda.reset_peak_memory_stats(dev)
-    median_ms, ms_per_step, stages, (R, radii, g) = time_steps(step, args.warmup, args.steps, dev)
+    median_ms, ms_per_step, stages, dominant, dom_ms, (R, radii, g) = time_steps(step, args.warmup, args.steps,
+                                                                                 dev)
     peak_hbm = torch.cuda.max_memory_allocated(dev)
     Pv = int((radii > 0).sum())
 
@@ -116,8 +117,6 @@ def main() -> int:
     tile_passes = (int(tiles).bit_length() + 7) // 8
     alg = algorithmic_bytes(P, Pv, R, tiles, N, depth_passes, tile_passes)
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items() if v[1]}
-    dominant = max(stage_ms, key=lambda k: stage_ms[k])
-    dom_ms = stage_ms[dominant]
     achieved = alg[dominant] / (dom_ms * 1e-3) / 1e9
     whole_bytes = 848 * P + 124 * R + 40 * N
     traffic = None
@@ -171,22 +170,30 @@ def main() -> int:
     return 0
 
 
-def time_steps(step, warmup: int, steps: int, dev, stage_steps: int = 5):
-    """W untimed steps, then K timed ones bracketed by synchronisation, each
-    also bracketed by events on the launch stream, with the library's stage
-    timers off (their events cost the launch stream a few microseconds per
-    stage boundary); then `stage_steps` more with the stage timers on, for the
-    per-stage breakdown.  Returns (median event ms, wall ms per step, stage
-    totals, last step's result)."""
+def time_steps(step, warmup: int, steps: int, dev):
+    """W untimed steps with every stage timed by the library's HIP events (the
+    per-stage breakdown, and which stage dominates), then K timed steps
+    bracketed by synchronisation, each step also bracketed by events on the
+    launch stream, with the library timing only the dominant stage (two events
+    per step; timing every stage would cost the stream a few microseconds per
+    stage boundary).  Returns (median step ms, wall ms per step, breakdown from
+    the warm-up, dominant stage, its mean ms over the timed steps, last step's
+    result)."""
     import torch
 
     from omnigs_amd import capi
-    res = None
-    for _ in range(warmup):
+    res = step()  # the first step grows the buffers: not in the breakdown
+    torch.cuda.synchronize()
+    capi.profile_read()  # drop anything recorded so far
+    capi.profile_enable(True)
+    for _ in range(max(1, warmup - 1)):
         res = step()
     torch.cuda.synchronize()
     capi.profile_enable(False)
-    capi.profile_read()  # drop anything recorded so far
+    stages = capi.profile_read()
+    stage_ms = {k: v[0] / v[1] for k, v in stages.items() if v[1]}
+    dominant = max(stage_ms, key=lambda k: stage_ms[k])
+    capi.profile_enable({dominant})
     stream = torch.cuda.current_stream(dev)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
@@ -197,13 +204,10 @@ def time_steps(step, warmup: int, steps: int, dev, stage_steps: int = 5):
         evs[i][1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    capi.profile_enable(True)
-    for _ in range(max(1, stage_steps)):
-        step()
-    torch.cuda.synchronize()
     capi.profile_enable(False)
-    stages = capi.profile_read()
-    return (statistics.median(a.elapsed_time(b) for a, b in evs), (t1 - t0) * 1000.0 / steps, stages, res)
+    dom = capi.profile_read()[dominant]
+    return (statistics.median(a.elapsed_time(b) for a, b in evs), (t1 - t0) * 1000.0 / steps, stages, dominant,
+            dom[0] / dom[1], res)
 
 
 # PSNR of the reference on 360Roam (supp.pdf p.5, Table 3): the quality level
@@ -274,7 +278,7 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
         g = capi.backward_scene(ds, R, radii, bufs, dL, mlp=mlp, mlp_workspace=ws)
         return R, radii, g
 
-    median_ms, ms_per_step, stages, _ = time_steps(step, args.warmup, args.steps, dev)
+    median_ms, ms_per_step, stages, _, _, _ = time_steps(step, args.warmup, args.steps, dev)
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items() if v[1]}
     # training pass FLOPs per Gaussian: forward, backward chain and weight gradients, 5888 MACs each
     flop_per_pass = 3 * 2 * 5888 * P
@@ -461,11 +465,30 @@ def copy_calibration(dev, steps: int) -> int:
     return 0
 
 
-def cpu_baseline(sc) -> dict:
+def host_cpu() -> dict:
+    """The host's CPU: model name, logical CPUs, and those this process may
+    run on (its affinity mask) and is allotted (OMP_NUM_THREADS)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": omp or None}
+
+
+def cpu_baseline(sc, repeats: int = 3) -> dict:
     """The FP32 oracle timed on the host over one full fwd+bwd iteration of the
-    same workload: the OpenMP tile-parallel variant on the host's cores (the
-    reported baseline) and the single-thread line-for-line port beside it
-    (bounded: about 15 s + 2 s)."""
+    same workload: the OpenMP tile-parallel variant on every core this process
+    is allotted (its affinity mask, capped by OMP_NUM_THREADS where the host
+    sets one -- on a shared GPU host that is the CPU share of one GPU), median
+    of ``repeats`` iterations, as the reported baseline; and the single-thread
+    line-for-line port beside it (one iteration; about 20 s in all)."""
     import numpy as np
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -473,7 +496,8 @@ def cpu_baseline(sc) -> dict:
 
     target = sc["target"]
     sc = {k: v for k, v in sc.items() if k != "target"}
-    want = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, len(os.sched_getaffinity(0)))
+    cpu = host_cpu()
+    want = cpu["affinity"] if not cpu["omp_num_threads"] else min(cpu["affinity"], cpu["omp_num_threads"])
 
     def one(threads):
         o = Oracle(sc, 32)
@@ -485,11 +509,15 @@ def cpu_baseline(sc) -> dict:
         o.backward(dL)
         return time.perf_counter() - t, used
 
-    dt_par, used = one(want)
+    runs = [one(want) for _ in range(max(1, repeats))]
+    used = runs[0][1]
+    dt_par = statistics.median(r[0] for r in runs)
     dt_1, _ = one(1)
     return {"value": round(dt_par * 1000.0, 1), "unit": "ms/iter", "cores": used, "kind": "port",
-            "sample": f"one full fwd+bwd iteration (1.5M Gaussians, 1920x960) of the FP32 oracle, OpenMP "
-                      f"tile-parallel with deterministic reductions on {used} host threads",
+            "sample": f"full fwd+bwd iterations (1.5M Gaussians, 1920x960) of the FP32 oracle, OpenMP "
+                      f"tile-parallel with deterministic reductions on {used} host threads, median of "
+                      f"{len(runs)}",
+            "host": cpu,
             "single_thread": {"value": round(dt_1 * 1000.0, 1), "unit": "ms/iter", "cores": 1}}
 
 

@@ -372,8 +372,27 @@ typedef struct omnigs_lonlat_state_view {
 int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, const char* image_buffer, int P, int R,
                         int width, int height, omnigs_lonlat_state_view* out);
 
-/* Stage timers: when enabled, every stage of the forward and backward is
- * bracketed by HIP events recorded on the launch stream.  read() waits for
+/* The same view of a batched workspace (omnigs_lonlat_batch_forward /
+ * _backward), for stage-by-stage parity tests.  Per-item arrays hold V P
+ * items, item v P + g being Gaussian g in view v; point_list / tile_keys hold
+ * the batch's sorted instances (their count is status[0] of the forward),
+ * Gaussian ids being item ids and tile ids the views' global tiles, view v's
+ * tiles starting at the sum of the earlier views' tile counts; final_T /
+ * n_contrib hold every view's pixels in the layout of out_color (one plane
+ * each); ranges is per global tile.  The 2D gradients are the backward's
+ * per-item sums (the single-view backward's dL_dmean2D, dL_dconic [4],
+ * dL_dopacity and dL_dcolor, per item).  No counterpart in the reference. */
+typedef struct omnigs_lonlat_batch_state_view {
+  omnigs_lonlat_state_view s;
+  const float* dL_dmean2D;  /* [V P, 3] */
+  const float* dL_dconic;   /* [V P, 4] */
+  const float* dL_dopacity; /* [V P] */
+  const float* dL_dcolor;   /* [V P, 3] */
+} omnigs_lonlat_batch_state_view;
+int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* args, omnigs_lonlat_batch_state_view* out);
+
+/* Stage timers: when enabled, the selected stages of the forward and backward
+ * are bracketed by HIP events recorded on the launch stream.  read() waits for
  * the recorded events, adds each stage's elapsed milliseconds into
  * ms[stage] and count[stage] (arrays of OMNIGS_NUM_STAGES), and clears them.
  * Measurement support only; no counterpart in the reference. */
@@ -392,7 +411,11 @@ enum omnigs_stage {
   OMNIGS_STAGE_DECODER_BACKWARD = 10,
   OMNIGS_NUM_STAGES = 11
 };
-int omnigs_profile_enable(int on);
+/* mask: 0 = off, otherwise bit s (1 << s) times stage s; OMNIGS_ALL_STAGES
+ * times every stage.  Timing one stage costs its launch stream two event
+ * records per call. */
+#define OMNIGS_ALL_STAGES ((1 << OMNIGS_NUM_STAGES) - 1)
+int omnigs_profile_enable(int mask);
 int omnigs_profile_read(double* ms, int* count);
 const char* omnigs_stage_name(int stage);
 

@@ -165,6 +165,44 @@ class BatchRenderer:
         return self.capacity
 
     # ----------------------------------------------------------------- views
+    def state(self) -> dict:
+        """Torch views of the workspace's arrays (omnigs_lonlat_batch_state),
+        for stage-by-stage parity tests: per-item arrays over V P items,
+        point_list / tile_keys over the instances of the last forward."""
+        self._sync_args()
+        sv = capi.BatchStateView()
+        st = self.lib.omnigs_lonlat_batch_state(ctypes.byref(self.args), ctypes.byref(sv))
+        if st != capi.OMNIGS_OK:
+            raise capi.OmnigsError(st, "omnigs_lonlat_batch_state")
+        n_items = self.V * self.P
+        R = min(self.needed()[0], self.capacity)
+        tiles = int(sum(((w + self.tile[0] - 1) // self.tile[0]) * ((h + self.tile[1] - 1) // self.tile[1])
+                        for w, h in self.sizes))
+        spec = {  # field: (dtype, count)
+            "depth_key": (torch.int32, n_items), "depth_order": (torch.int32, n_items),
+            "radii": (torch.int32, n_items), "render_records": (torch.float32, 12 * n_items),
+            "clamped": (torch.uint8, n_items), "tiles_touched": (torch.int32, n_items),
+            "point_list": (torch.int32, R), "tile_keys": (torch.int32, R),
+            "final_T": (torch.float32, self.total_pixels), "n_contrib": (torch.int32, self.total_pixels),
+            "ranges": (torch.int32, 2 * tiles), "dL_dmean2D": (torch.float32, 3 * n_items),
+            "dL_dconic": (torch.float32, 4 * n_items), "dL_dopacity": (torch.float32, n_items),
+            "dL_dcolor": (torch.float32, 3 * n_items)}
+        base = self.workspace.data_ptr()
+        out = {}
+        for name, (dt, n) in spec.items():
+            addr = getattr(sv.s, name) if hasattr(sv.s, name) else getattr(sv, name)
+            if not addr or n == 0:
+                continue
+            off = addr - base
+            nbytes = n * torch.empty((), dtype=dt).element_size()
+            assert 0 <= off and off + nbytes <= self.workspace.numel(), name
+            out[name] = self.workspace[off:off + nbytes].view(dt)
+        rec = out["render_records"].view(n_items, 12)
+        out["means2D"] = rec[:, 0:2].reshape(-1)
+        out["conic_opacity"] = rec[:, 2:6].reshape(-1)
+        out["rgb"] = rec[:, 6:9].reshape(-1)
+        return out
+
     def image(self, v: int) -> torch.Tensor:
         W, H = self.sizes[v]
         o = 3 * self.pix_offsets[v]

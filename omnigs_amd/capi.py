@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "omnigs_mlp_eval", "omnigs_mlp_workspace_bytes", "omnigs_mlp_distill_step", "omnigs_mlp_seeded_direction",
     "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
     "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
-    "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr",
+    "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr", "omnigs_lonlat_batch_state",
 )
 NUM_STAGES = 11
 
@@ -122,6 +122,11 @@ class StateView(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in _STATE_FIELDS]
 
 
+class BatchStateView(ctypes.Structure):
+    _fields_ = [("s", StateView), ("dL_dmean2D", ctypes.c_void_p), ("dL_dconic", ctypes.c_void_p),
+                ("dL_dopacity", ctypes.c_void_p), ("dL_dcolor", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -192,6 +197,8 @@ def load() -> ctypes.CDLL:
     lib.omnigs_lonlat_batch_backward.argtypes = [ctypes.POINTER(BatchArgs), ctypes.POINTER(BatchGrads),
                                                  ctypes.c_void_p]
     lib.omnigs_lonlat_batch_backward.restype = ctypes.c_int
+    lib.omnigs_lonlat_batch_state.argtypes = [ctypes.POINTER(BatchArgs), ctypes.POINTER(BatchStateView)]
+    lib.omnigs_lonlat_batch_state.restype = ctypes.c_int
     lib.omnigs_densification_stats.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6
     lib.omnigs_densification_stats.restype = ctypes.c_int
     lib.omnigs_expon_lr.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
@@ -503,8 +510,25 @@ def backward_scene(ds: dict, R, radii, buffers, dL_dpix, **kw):
                     degree=ds.get("degree", 3), scale_modifier=ds.get("scale_modifier", 1.0), **kw)
 
 
-def profile_enable(on: bool = True) -> None:
-    load().omnigs_profile_enable(1 if on else 0)
+def profile_enable(on=True) -> None:
+    """True: time every stage; False: none; a set of stage names: those."""
+    if on is True:
+        mask = (1 << NUM_STAGES) - 1
+    elif not on:
+        mask = 0
+    else:
+        mask = 0
+        for name in on:
+            mask |= 1 << stage_index(name)
+    load().omnigs_profile_enable(mask)
+
+
+def stage_index(name: str) -> int:
+    lib = load()
+    for i in range(NUM_STAGES):
+        if lib.omnigs_stage_name(i).decode() == name:
+            return i
+    raise KeyError(name)
 
 
 def profile_read() -> dict:

@@ -43,7 +43,7 @@ struct StageEvent {
   hipEvent_t start, stop;
 };
 std::mutex g_prof_mu;
-bool g_prof_on = false;
+int g_prof_mask = 0;  // bit s: time stage s
 std::vector<StageEvent> g_prof_events;
 
 struct StageTimer {
@@ -54,7 +54,7 @@ struct StageTimer {
   StageTimer(int st, hipStream_t stream) : stage(st), s(stream) {
     {
       std::lock_guard<std::mutex> lk(g_prof_mu);
-      on = g_prof_on;
+      on = st >= 0 && st < OMNIGS_NUM_STAGES && ((g_prof_mask >> st) & 1);
     }
     if (on) {
       ev.stage = st;
@@ -105,9 +105,9 @@ const char* omnigs_last_error(void) { return g_last_error; }
 
 const char* omnigs_version(void) { return "omnigs-mi355x 0.1 (gfx950)"; }
 
-int omnigs_profile_enable(int on) {
+int omnigs_profile_enable(int mask) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  g_prof_on = on != 0;
+  g_prof_mask = mask & OMNIGS_ALL_STAGES;
   return OMNIGS_OK;
 }
 
@@ -616,7 +616,8 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
   }
   {
     StageTimer tm(OMNIGS_STAGE_GATHER_GRADS, s);
-    OMNIGS_TRY(launch_gather_grads((int)items, P, a->capacity, g.offsets + (items - 1), g.tiles_touched,
+    // the live instance count (0 after a 32-bit overflow), as the forward's kernels read it
+    OMNIGS_TRY(launch_gather_grads((int)items, P, a->capacity, g.scan_partials + scan_blocks(items), g.tiles_touched,
                                    g.slot_local, g.slot_base, b.grad_rec, b.grad_opacity, bs.dmean2D, bs.dconic,
                                    bs.dopacity, bs.dcolor, s),
                "batch gather grads");
@@ -637,6 +638,38 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
                                    s, a->V),
                "batch decoder backward");
   }
+  return OMNIGS_OK;
+}
+
+int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* a, omnigs_lonlat_batch_state_view* out) {
+  if (!a || !out) return fail_arg("NULL args");
+  std::memset(out, 0, sizeof(*out));
+  if (a->P < 0 || !a->workspace) return fail_arg("P < 0 or no workspace");
+  ViewTable vt;
+  const int st = make_view_table(a->P, a->V, a->widths, a->heights, a->tile_x, a->tile_y, a->seam_wrap, vt);
+  if (st != OMNIGS_OK) return st;
+  const size_t items = (size_t)a->V * a->P;
+  const BatchState bs =
+      carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V], vt.tile_base[vt.V], a->capacity);
+  if (a->workspace_bytes < bs.bytes) return fail_arg("workspace smaller than omnigs_lonlat_batch_workspace_bytes()");
+  omnigs_lonlat_state_view& v = out->s;
+  v.depth_key = bs.g.depth_key;
+  v.radii = bs.g.radii;
+  v.render_records = reinterpret_cast<const float*>(bs.g.rec);
+  v.clamped = bs.g.clamped;
+  v.tiles_touched = bs.g.tiles_touched;
+  v.depth_order = bs.g.order;
+  v.offsets = bs.g.offsets;
+  const bool alt = radix_passes(bit_width_u32(vt.tile_base[vt.V])) % 2;
+  v.point_list = alt ? bs.b.gid_alt : bs.b.gid;
+  v.tile_keys = alt ? bs.b.tile_key_alt : bs.b.tile_key;
+  v.final_T = bs.final_T;
+  v.n_contrib = bs.n_contrib;
+  v.ranges = reinterpret_cast<const uint32_t*>(bs.ranges);
+  out->dL_dmean2D = bs.dmean2D;
+  out->dL_dconic = bs.dconic;
+  out->dL_dopacity = bs.dopacity;
+  out->dL_dcolor = bs.dcolor;
   return OMNIGS_OK;
 }
 

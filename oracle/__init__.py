@@ -20,11 +20,18 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 STAGES = {"preprocess": 0, "binning": 1, "render": 2, "render_backward": 3, "cov2d_backward": 4,
-          "preprocess_backward": 5}
+          "preprocess_backward": 5, "cov3d": 6}
+
+# Precision code of the running-error oracle: FP64 values, each real array
+# with a first-order bound of the rounding error an FP32 evaluation of the
+# same operations accumulates (get("err:<name>"), in units of 2^-24).  It
+# runs the backward stages only, on state set from outside.
+RUNNING_ERROR = 65
 
 # dtype of each named state array (None = the handle's float precision)
 _INT_ARRAYS = {"radii": np.int32, "clamped": np.uint8, "tiles_touched": np.uint32, "offsets": np.uint32,
-               "keys": np.uint64, "point_list": np.uint32, "ranges": np.uint32, "n_contrib": np.uint32}
+               "keys": np.uint64, "point_list": np.uint32, "ranges": np.uint32, "n_contrib": np.uint32,
+               "fragile": np.uint8, "gsig": np.uint64, "psig": np.uint64}
 
 
 def build(force: bool = False) -> str:
@@ -70,6 +77,8 @@ def _load():
     lib.oracle_set_threads.restype = ctypes.c_int
     lib.oracle_time_fwd_bwd.argtypes = [vp, vp]
     lib.oracle_time_fwd_bwd.restype = ctypes.c_double
+    lib.oracle_fragile_pixels.argtypes = [vp]
+    lib.oracle_fragile_pixels.restype = ctypes.c_longlong
     _lib = lib
     return lib
 
@@ -79,12 +88,13 @@ def higher_msb(n: int) -> int:
 
 
 class Oracle:
-    """One scene in one precision (32 or 64).  Arrays are numpy, C-contiguous."""
+    """One scene in one precision (32, 64, or RUNNING_ERROR).  Arrays are
+    numpy, C-contiguous."""
 
     def __init__(self, scene: dict, precision: int = 32, block=(16, 16)):
         lib = _load()
         self.precision = precision
-        self.ftype = np.float64 if precision == 64 else np.float32
+        self.ftype = np.float32 if precision == 32 else np.float64
         self.scene = scene
         self._keep = []
 
@@ -138,6 +148,12 @@ class Oracle:
     @property
     def decision_sig(self) -> int:
         return int(_load().oracle_decision_sig(self.h))
+
+    @property
+    def fragile_pixels(self) -> int:
+        """Pixels with a render-backward decision within rounding of its
+        threshold (running-error handles only)."""
+        return int(_load().oracle_fragile_pixels(self.h))
 
     def get(self, name: str) -> np.ndarray:
         lib = _load()

@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -34,6 +35,71 @@
 namespace omnigs_oracle {
 
 constexpr int kChannels = 3;  // config.h:25 NUM_CHANNELS
+
+// Running rounding-error bound (Wilkinson's running error analysis, as in
+// Higham, "Accuracy and Stability of Numerical Algorithms", 2nd ed., Sec. 3.3):
+// v is an operation's result evaluated in FP64 (the exact value to first
+// order) and e a first-order bound on the error that carrying out the same
+// operations in FP32 accumulates, in units of the FP32 unit roundoff
+// u = 2^-24.  Every operation propagates its operands' bounds and adds one
+// rounding of its own result:
+//   z = x +- y:  e = ex + ey + |z|          z = x y:  e = |y| ex + |x| ey + |z|
+//   z = x / y:   e = (ex + |z| ey) / |y| + |z|
+//   z = sqrt(x): e = ex / (2 z) + |z|       z = exp(x): e = |z| (ex + |x| + 2)
+// (exp also covers an exp2 evaluation whose argument x log2(e) is rounded).
+// Inputs and constants carry e = 0: both sides of a comparison read the same
+// FP32 values.  Decisions compare v.  Used to instantiate the backward stages
+// and give each gradient entry its own rounding scale (e u).
+struct Rerr {
+  double v = 0.0, e = 0.0;
+  constexpr Rerr() = default;
+  constexpr Rerr(double x) : v(x), e(0.0) {}
+  constexpr Rerr(double x, double err) : v(x), e(err) {}
+  Rerr& operator+=(Rerr o);
+  Rerr& operator-=(Rerr o);
+  Rerr& operator*=(Rerr o);
+};
+inline Rerr operator+(Rerr a, Rerr b) {
+  const double z = a.v + b.v;
+  return {z, a.e + b.e + std::fabs(z)};
+}
+inline Rerr operator-(Rerr a, Rerr b) {
+  const double z = a.v - b.v;
+  return {z, a.e + b.e + std::fabs(z)};
+}
+inline Rerr operator-(Rerr a) { return {-a.v, a.e}; }
+inline Rerr operator*(Rerr a, Rerr b) {
+  const double z = a.v * b.v;
+  return {z, std::fabs(b.v) * a.e + std::fabs(a.v) * b.e + std::fabs(z)};
+}
+inline Rerr operator/(Rerr a, Rerr b) {
+  const double z = a.v / b.v;
+  return {z, (a.e + std::fabs(z) * b.e) / std::fabs(b.v) + std::fabs(z)};
+}
+inline Rerr& Rerr::operator+=(Rerr o) { return *this = *this + o; }
+inline Rerr& Rerr::operator-=(Rerr o) { return *this = *this - o; }
+inline Rerr& Rerr::operator*=(Rerr o) { return *this = *this * o; }
+inline bool operator<(Rerr a, Rerr b) { return a.v < b.v; }
+inline bool operator>(Rerr a, Rerr b) { return a.v > b.v; }
+inline bool operator==(Rerr a, Rerr b) { return a.v == b.v; }
+inline bool operator!=(Rerr a, Rerr b) { return a.v != b.v; }
+
+// Math functions of the backward stages: std:: for float / double (the same
+// overloads as before), the error-propagating forms for Rerr.
+template <class R> inline R osqrt(R x) { return std::sqrt(x); }
+template <class R> inline R oexp(R x) { return std::exp(x); }
+template <class R> inline R onearbyint(R x) { return std::nearbyint(x); }
+inline Rerr osqrt(Rerr x) {
+  const double z = std::sqrt(x.v);
+  return {z, (z > 0.0 ? x.e / (2.0 * z) : 0.0) + z};
+}
+inline Rerr oexp(Rerr x) {
+  const double z = std::exp(x.v);
+  return {z, z * (x.e + std::fabs(x.v) + 2.0)};
+}
+inline Rerr onearbyint(Rerr x) { return {std::nearbyint(x.v), 0.0}; }
+template <class R> struct is_rerr : std::false_type {};
+template <> struct is_rerr<Rerr> : std::true_type {};
 
 // Real SH constants, auxiliary.h:32-49 (textbook values of the real basis).
 template <class R> struct SH {
@@ -135,7 +201,7 @@ inline WRect tile_rect_wrap(float px, float py, int radius, int gx, int gy, int 
   return {x0, (int)c.y0, x1, (int)c.y1};
 }
 inline uint32_t wrap_col(int x, int gx) { return (uint32_t)(((x % gx) + gx) % gx); }
-template <class R> inline R seam_dx(R dx, int W) { return dx - R(W) * std::nearbyint(dx / R(W)); }
+template <class R> inline R seam_dx(R dx, int W) { return dx - R(W) * onearbyint(dx / R(W)); }
 
 // getHigherMsb, rasterizer_impl.cu:47-62 (bit count of n: highest set bit + 1).
 inline uint32_t higher_msb(uint32_t n) {
@@ -190,13 +256,34 @@ template <class R> struct State {
   // threshold may legitimately differ between two correct FP32 evaluations)
   std::vector<R> margin;
   uint64_t decision_sig = 0;  // hash of every discrete decision (FD masking)
+  // per-Gaussian and per-pixel blend signatures of the render: gsig[g] is
+  // the XOR over the pixels that blend Gaussian g of a hash of the pixel;
+  // psig[pixel] hashes the sequence of Gaussians the pixel blends.  Only
+  // blended pairs enter: a pair skipped (power > 0, alpha < 1/255, after the
+  // stop) or absent from the tile list contributes nothing to any output.
+  // Two evaluations whose signatures agree blended the same pairs for that
+  // Gaussian or pixel.
+  std::vector<uint64_t> gsig, psig;
   // backward
   std::vector<R> dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors;  // [P,3], [P,4], [P], [P,3]
   std::vector<R> dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales, dL_drot, dpx_dt, dpy_dt;
+  // running-error mode (R = Rerr) only: Gaussians blended in a pixel where a
+  // render-backward decision (power > 0, alpha >= 1/255) lies within rounding
+  // of its threshold, so that another correct FP32 evaluation may decide it
+  // the other way and change the pixel's recovered T for every contributor
+  std::vector<uint8_t> fragile;
+  uint64_t fragile_pixels = 0;
 };
 
 inline void sig_mix(uint64_t& h, uint64_t v) {
   h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+}
+// splitmix64 finaliser (Steele, Lea, Flood 2014), for order-independent XOR signatures
+inline uint64_t sig_hash(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
 }
 
 // computeCov3D, forward.cu:194-228: Sigma = (S R)^T (S R), q = (r,x,y,z) used unnormalised.
@@ -224,7 +311,7 @@ template <class R> LonlatJac<R> lonlat_jacobian(V3<R> t, int W, int H) {
   const R eps = R(1e-7);
   R rxz2 = t.x * t.x + t.z * t.z;
   R rxz2_inv = R(1) / (rxz2 + eps);
-  R rxz = std::sqrt(rxz2);
+  R rxz = osqrt(rxz2);
   R rxz_inv = R(1) / (rxz + eps);
   R r2 = rxz2 + t.y * t.y;
   R r2_inv = R(1) / (r2 + eps);
@@ -437,14 +524,23 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
   st.n_contrib.assign(N, 0);
   st.out_color.assign(kChannels * N, 0);
   st.margin.assign(N, R(1));
+  st.gsig.assign(in.P, 0);
+  st.psig.assign(N, 0);
   uint64_t sig = 1469598103934665603ULL;
   const bool par = in.threads > 1;
   sig_mix(sig, st.R_);
 #pragma omp parallel for schedule(dynamic, 1) if (par) num_threads(in.threads)
   for (int py = 0; py < H; ++py)
     for (int px = 0; px < W; ++px) {
+      const size_t pix = (size_t)py * W + px;
+      uint64_t ps = 1469598103934665603ULL;
+      uint32_t cur = 0;  // the Gaussian of the pair being decided
       auto mix = [&](uint64_t v) {
         if (!par) sig_mix(sig, v);
+        if (v >= 4) {  // blended (1: power > 0, 2: alpha < 1/255, 3: stop)
+          __atomic_fetch_xor(&st.gsig[cur], sig_hash(pix), __ATOMIC_RELAXED);
+          sig_mix(ps, cur);
+        }
       };
       const int tile = (py / by) * gx + (px / bx);
       const uint32_t b = st.ranges[2 * tile], e = st.ranges[2 * tile + 1];
@@ -455,6 +551,7 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
       for (uint32_t k = b; k < e; ++k) {
         ++contributor;
         const uint32_t g = st.point_list[k];
+        cur = g;
         R dx = st.xy[2 * g] - fx;
         const R dy = st.xy[2 * g + 1] - fy;
         if (in.seam_wrap) dx = seam_dx(dx, W);
@@ -476,6 +573,7 @@ template <class R> void render(const Inputs<R>& in, State<R>& st) {
       st.final_T[pid] = T;
       st.n_contrib[pid] = last;
       st.margin[pid] = mg;
+      st.psig[pid] = ps;
       for (int ch = 0; ch < kChannels; ++ch) st.out_color[ch * N + pid] = C[ch] + T * in.bg[ch];
     }
   for (int i = 0; i < in.P; ++i) sig_mix(sig, (uint64_t)st.radii[i] * 131 + st.tiles_touched[i]);
@@ -501,6 +599,8 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
   st.dL_dconic.assign(4 * (size_t)P, 0);
   st.dL_dopacity.assign(P, 0);
   st.dL_dcolors.assign(3 * (size_t)P, 0);
+  st.fragile.assign(is_rerr<R>::value ? P : 0, 0);
+  st.fragile_pixels = 0;
   const R ddelx_dx = R(0.5 * W), ddely_dy = R(0.5 * H);
   // One pixel's reverse traversal; sink(k, g, vals[9]) receives the
   // contributions of sorted instance k (Gaussian g): dcolor x3, dmean2D x2,
@@ -518,6 +618,7 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
     R last_alpha = 0;
     R bg_dot = 0;
     for (int ch = 0; ch < kChannels; ++ch) bg_dot += in.bg[ch] * dpix[ch];
+    bool frag = false;
     // contributor index k - b (0-based); only those before last are blended.
     for (uint32_t kk = e; kk > b; --kk) {
       const uint32_t k = kk - 1;
@@ -528,8 +629,15 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
       if (in.seam_wrap) dx = seam_dx(dx, W);
       const R* co = &st.conic[4 * g];
       const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
+      if constexpr (is_rerr<R>::value) {
+        constexpr double u = 0x1p-24;
+        if (std::fabs(power.v) <= 8.0 * power.e * u + 64.0 * u) frag = true;
+        const R oG = co[3] * oexp(power);
+        const double thr = 1.0 / 255.0;
+        if (std::fabs(oG.v - thr) <= 8.0 * oG.e * u + thr * u * (64.0 + 8.0 * std::fabs(power.v))) frag = true;
+      }
       if (power > R(0)) continue;
-      const R G = std::exp(power);
+      const R G = oexp(power);
       const R alpha = std::min(R(0.99), co[3] * G);
       if (alpha < R(1) / R(255)) continue;
       T = T / (R(1) - alpha);
@@ -557,6 +665,12 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
       v[7] = R(-0.5) * gdy * dy * dL_dG;
       v[8] = G * dL_dalpha;
       sink(k, g, v);
+    }
+    if constexpr (is_rerr<R>::value) {
+      if (frag) {
+        for (uint32_t k = b; k < e && k - b < last; ++k) __atomic_store_n(&st.fragile[st.point_list[k]], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&st.fragile_pixels, 1, __ATOMIC_RELAXED);
+      }
     }
   };
   auto add_to_gaussian = [&](uint32_t g, const R* v) {
@@ -619,7 +733,7 @@ template <class R> void cov2d_backward(const Inputs<R>& in, State<R>& st) {
     const R rxz2 = txtx + tztz;
     const R rxz2_inv = R(1) / (rxz2 + R(1e-7));
     const R rxz4_inv = rxz2_inv * rxz2_inv;
-    const R rxz = std::sqrt(rxz2);
+    const R rxz = osqrt(rxz2);
     const R rxz_inv = R(1) / (rxz + R(1e-7));
     const R r2 = rxz2 + tyty;
     const R r2_inv = R(1) / (r2 + R(1e-7));
@@ -695,7 +809,7 @@ template <class R> void sh_backward(int deg, int M, const R* sh_g, V3<R> pos, V3
                                      const R* dL_dcolor3, R* dL_dmean3, R* dL_dsh_g) {
   using K = SH<R>;
   V3<R> d0 = pos - campos;
-  R len = std::sqrt(dot(d0, d0));
+  R len = osqrt(dot(d0, d0));
   V3<R> dir{d0.x / len, d0.y / len, d0.z / len};
   auto s = [&](int i) { return V3<R>{sh_g[3 * i], sh_g[3 * i + 1], sh_g[3 * i + 2]}; };
   V3<R> g{dL_dcolor3[0] * (clamped3[0] ? R(0) : R(1)), dL_dcolor3[1] * (clamped3[1] ? R(0) : R(1)),
@@ -742,7 +856,7 @@ template <class R> void sh_backward(int deg, int M, const R* sh_g, V3<R> pos, V3
   const V3<R> ddir{dot(dx, g), dot(dy, g), dot(dz, g)};
   // dnormvdv, auxiliary.h:134-144
   const R sum2 = d0.x * d0.x + d0.y * d0.y + d0.z * d0.z;
-  const R invsum32 = R(1) / std::sqrt(sum2 * sum2 * sum2);
+  const R invsum32 = R(1) / osqrt(sum2 * sum2 * sum2);
   dL_dmean3[0] += ((sum2 - d0.x * d0.x) * ddir.x - d0.y * d0.x * ddir.y - d0.z * d0.x * ddir.z) * invsum32;
   dL_dmean3[1] += (-d0.x * d0.y * ddir.x + (sum2 - d0.y * d0.y) * ddir.y - d0.z * d0.y * ddir.z) * invsum32;
   dL_dmean3[2] += (-d0.x * d0.z * ddir.x - d0.y * d0.z * ddir.y + (sum2 - d0.z * d0.z) * ddir.z) * invsum32;

@@ -65,3 +65,31 @@ def gpu_state(bufs, P, R, W, H) -> dict:
 
 def np_(t: torch.Tensor) -> np.ndarray:
     return t.detach().cpu().numpy()
+
+
+def record(case: str, **metrics) -> None:
+    """Appends one JSON line of parity metrics to $OMNIGS_PARITY_RECORD (when
+    set) and prints it, so a run leaves the measured errors behind."""
+    import json
+    import os
+    line = json.dumps({"case": case, **{k: (float(v) if isinstance(v, (np.floating, np.integer)) else v)
+                                        for k, v in metrics.items()}})
+    print(line)
+    path = os.environ.get("OMNIGS_PARITY_RECORD")
+    if path:
+        with open(path, "a") as f:
+            f.write(line + "\n")
+
+
+def fp64_errors(name: str, gpu, o64, floor_frac: float = 1e-2) -> dict:
+    """Max |gpu - fp64| relative to the tensor's largest magnitude, and the
+    max and 99th-percentile relative error over the entries at least
+    floor_frac of that magnitude."""
+    g = np.asarray(gpu, np.float64).ravel()
+    r = np.asarray(o64, np.float64).ravel()
+    d = np.abs(g - r)
+    top = max(np.abs(r).max(), 1e-300) if r.size else 1.0
+    big = np.abs(r) >= floor_frac * top
+    rel = d[big] / np.abs(r[big]) if big.any() else np.zeros(1)
+    return {f"{name}_abs_vs_fp64_of_max": float(d.max() / top) if d.size else 0.0,
+            f"{name}_rel_vs_fp64_max": float(rel.max()), f"{name}_rel_vs_fp64_p99": float(np.percentile(rel, 99))}

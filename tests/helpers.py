@@ -71,3 +71,71 @@ def allclose_grad(a, b, rtol=1e-3, scale_atol=1e-4):
     tol = rtol * np.abs(b) + scale_atol * max(np.abs(b).max(), 1e-30)
     ratio = np.abs(a - b) / tol
     return bool(np.all(ratio <= 1.0)), float(ratio.max())
+
+
+# ---------------------------------------------------------------------------
+# Rounding-aware gradient bars.
+#
+# The running-error oracle (oracle.RUNNING_ERROR) gives every gradient entry a
+# first-order bound e of the rounding error an FP32 evaluation of the
+# reference's operations accumulates, in units of u = 2^-24 (Higham, Sec. 3.3).
+# A GPU entry a passes against the FP32 oracle o when
+#     |a - o| <= GRAD_RTOL |o| + ROUNDING_K u e,
+# i.e. within the north-star 1e-3 relative, or within a few times the rounding
+# error both evaluations are entitled to.  ROUNDING_K = 4: the oracle's own
+# FP32 error stays below 0.25 u e on the test scenes, and the GPU evaluates
+# the same functions in another order (tree sums, FMAs, rcp).
+U32 = 2.0 ** -24
+ROUNDING_K = 4.0
+GRAD_RTOL = 1e-3
+STATE_KEYS_FWD = ("radii", "xy", "conic", "rgb", "point_list", "ranges", "final_T", "n_contrib", "clamped")
+
+
+def rounding_check(a, o, err, rtol=GRAD_RTOL, k=ROUNDING_K, skip=None):
+    """(ok, worst ratio, count checked): |a - o| <= rtol |o| + k u err on
+    every element not in ``skip`` (a boolean mask of the same size)."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    o = np.asarray(o, dtype=np.float64).ravel()
+    err = np.asarray(err, dtype=np.float64).ravel()
+    tol = rtol * np.abs(o) + k * U32 * err
+    keep = np.ones(a.size, bool) if skip is None else ~np.asarray(skip, bool).ravel()
+    d = np.abs(a - o)[keep]
+    t = tol[keep]
+    if d.size == 0:
+        return True, 0.0, 0
+    bad = d > t
+    ratio = np.where(t > 0, d / np.where(t > 0, t, 1.0), np.where(d > 0, np.inf, 0.0))
+    return (not bool(bad.any())), float(ratio.max()), int(d.size)
+
+
+def per_gaussian_mask(flags, width):
+    """Expands a per-Gaussian flag to the entries of a [P, width] array."""
+    return np.repeat(np.asarray(flags, bool), width)
+
+
+def running_error_oracle(scene, state: dict, dL_dpix=None, grads2d: dict | None = None, threads: int = 1,
+                         block=(16, 16)):
+    """The running-error oracle on a scene whose forward state (and, for the
+    preprocess backward, whose 2D gradients) are given -- normally the GPU's
+    own arrays, so that both sides read identical FP32 inputs.  Runs the
+    render backward when dL_dpix is given, then cov2d / preprocess backward
+    (on the 2D gradients of ``grads2d`` when given, else its own)."""
+    from oracle import RUNNING_ERROR, Oracle
+    o = Oracle(scene, RUNNING_ERROR, block=block)
+    o.set_threads(threads)
+    if scene.get("scales") is not None and scene.get("cov3D_precomp") is None:
+        o.run("cov3d")
+    for k, v in state.items():
+        o.set(k, v)
+    if dL_dpix is not None:
+        o.run("render_backward", dL_dpix)
+    if grads2d:
+        for k, v in grads2d.items():
+            o.set(k, v)
+    return o.run("cov2d_backward").run("preprocess_backward")
+
+
+def host_threads() -> int:
+    """Threads for the OpenMP oracle: the process's CPU share, at most 16."""
+    import os
+    return max(1, min(16, len(os.sched_getaffinity(0))))

@@ -134,7 +134,7 @@ def test_ctypes_structs_match_header(tmp_path):
     structs = {"omnigs_lonlat_forward_args": capi.ForwardArgs, "omnigs_lonlat_backward_args": capi.BackwardArgs,
                "omnigs_mlp_distill_args": capi.DistillArgs, "omnigs_adam_group": capi.AdamGroup,
                "omnigs_lonlat_state_view": capi.StateView, "omnigs_lonlat_batch_args": capi.BatchArgs,
-               "omnigs_lonlat_batch_grads": capi.BatchGrads}
+               "omnigs_lonlat_batch_grads": capi.BatchGrads, "omnigs_lonlat_batch_state_view": capi.BatchStateView}
     lines = []
     for cname, ct in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -4,9 +4,12 @@ Bars:
 * integer work (radii, tile counts, instance lists, tile ranges, n_contrib):
   bit-exact given identical upstream outputs;
 * image / final_T: |gpu - oracle| <= 1e-4 absolute;
-* gradients: |gpu - oracle| <= 1e-3 |oracle| + 1e-4 max|oracle| per element
-  (the second term absorbs float-atomic reordering on cancelling sums);
-* the GPU's error against the FP64 oracle is reported beside the FP32 bar.
+* gradients given identical inputs: |gpu - o32| <= 1e-3 |o32| + 4 u e per
+  element, e being the running-error oracle's bound of that entry's FP32
+  rounding error (tests/helpers.py); Gaussians blended in a pixel whose
+  backward decision lies within rounding of its threshold ("fragile", a few
+  per cent at most) are held to 1e-3 |o32| + 1e-4 max|o32| instead;
+* the GPU's error against the FP64 oracle is recorded (gpu_util.record).
 """
 from __future__ import annotations
 
@@ -134,6 +137,23 @@ def _grad_check(name, g, o_val):
     assert ok, f"{name}: worst tolerance ratio {worst:.3g}"
 
 
+def _rounding_grad_check(name, g, o32, oerr, width=1, fragile=None, metrics=None):
+    """Rounding-aware bar (tests/helpers.py) on every entry; entries of
+    fragile Gaussians are held to the looser 1e-3 |o| + 1e-4 max|o| bar."""
+    from tests.gpu_util import fp64_errors
+    from tests.helpers import per_gaussian_mask, rounding_check
+    skip = per_gaussian_mask(fragile, width) if fragile is not None else None
+    ok, worst, n = rounding_check(g, o32, oerr.get("err:" + name), skip=skip)
+    assert ok, f"{name}: worst rounding-aware ratio {worst:.3g} over {n} entries"
+    if skip is not None and skip.any():
+        ok2, worst2 = allclose_grad(np.asarray(g).ravel()[skip], np.asarray(o32).ravel()[skip], GRAD_RTOL,
+                                    GRAD_SCALE_ATOL)
+        assert ok2, f"{name} (fragile Gaussians): worst ratio {worst2:.3g}"
+    if metrics is not None:
+        metrics[f"{name}_worst_ratio"] = round(worst, 4)
+        metrics.update(fp64_errors(name, g, oerr.get(name)))
+
+
 @pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0), (3000, 200, 90, 1)])
 def test_render_backward_given_identical_forward(gpu, P, W, H, seed):
     import torch
@@ -149,10 +169,19 @@ def test_render_backward_given_identical_forward(gpu, P, W, H, seed):
                  ("n_contrib", st["n_contrib"])):
         o.set(k, v)
     o.run("render_backward", sc["dL_dpix"])
-    _grad_check("dL_dmean2D", np_(g["dL_dmeans2D"]).reshape(-1), o.get("dL_dmean2D"))
-    _grad_check("dL_dconic", np_(g["dL_dconic"]).reshape(-1), o.get("dL_dconic"))
-    _grad_check("dL_dopacity", np_(g["dL_dopacity"]).reshape(-1), o.get("dL_dopacity"))
-    _grad_check("dL_dcolors", np_(g["dL_dcolors"]).reshape(-1), o.get("dL_dcolors"))
+    from oracle import RUNNING_ERROR, Oracle
+    from tests.gpu_util import record
+    oe = Oracle(sc, RUNNING_ERROR)
+    for k in ("xy", "conic", "rgb", "point_list", "ranges", "final_T", "n_contrib"):
+        oe.set(k, o.get(k))
+    oe.run("render_backward", sc["dL_dpix"])
+    fragile = oe.get("fragile").astype(bool)
+    assert fragile.mean() <= 0.05, fragile.mean()
+    m = {"fragile_gaussians": int(fragile.sum()), "fragile_pixels": oe.fragile_pixels}
+    for mine, ref, w in (("dL_dmeans2D", "dL_dmean2D", 3), ("dL_dconic", "dL_dconic", 4),
+                         ("dL_dopacity", "dL_dopacity", 1), ("dL_dcolors", "dL_dcolors", 3)):
+        _rounding_grad_check(ref, np_(g[mine]).reshape(-1), o.get(ref), oe, w, fragile, m)
+    record(f"render_backward_given_identical_forward P={P} {W}x{H}", **m)
 
 
 @pytest.mark.parametrize("P,W,H,seed,degree", [(20000, 512, 256, 0, 3), (3000, 200, 90, 1, 1)])
@@ -173,30 +202,155 @@ def test_preprocess_backward_given_identical_inputs(gpu, P, W, H, seed, degree):
     o.set("dL_dcolors", np_(g["dL_dcolors"]))
     o.set("dL_dopacity", np_(g["dL_dopacity"]))
     o.run("cov2d_backward").run("preprocess_backward")
+    from tests.gpu_util import record
+    from tests.helpers import running_error_oracle
+    clamp3 = np.stack([clamped & 1, (clamped >> 1) & 1, (clamped >> 2) & 1], 1).astype(np.uint8)
+    oe = running_error_oracle(sc, {"radii": np_(radii), "clamped": clamp3},
+                              grads2d={"dL_dmean2D": np_(g["dL_dmeans2D"]), "dL_dconic": np_(g["dL_dconic"]),
+                                       "dL_dcolors": np_(g["dL_dcolors"]), "dL_dopacity": np_(g["dL_dopacity"])})
+    m = {}
     for mine, ref in (("dpx_dt", "dpx_dt"), ("dpy_dt", "dpy_dt"), ("dL_dcov3D", "dL_dcov3D"),
                       ("dL_dmeans3D", "dL_dmeans3D"), ("dL_dsh", "dL_dsh"), ("dL_dscales", "dL_dscales"),
                       ("dL_drotations", "dL_drot")):
-        _grad_check(mine, np_(g[mine]).reshape(-1), o.get(ref))
+        _rounding_grad_check(ref, np_(g[mine]).reshape(-1), o.get(ref), oe, metrics=m)
+    record(f"preprocess_backward_given_identical_inputs P={P} {W}x{H} degree {degree}", **m)
 
 
 # End to end (independent preprocess on both sides), ulp-level differences in
 # the projection can move a pair across the reference's discrete thresholds
-# (alpha >= 1/255, T >= 1e-4, the 3-sigma tile rect); such a flip changes a
-# pixel by up to one contribution (~alpha c T, alpha ~ 1/255) and moves that
-# Gaussian's gradients.  The FP32 and FP64 oracles disagree the same way.  So
-# the end-to-end bar is: at most FLIP_PIXEL_FRACTION of the pixels outside
-# 1e-4 and none beyond one 1/255-sized contribution, and at most
-# FLIP_GRAD_FRACTION of the gradient entries outside the gradient tolerance.
-# The strict bars hold stage by stage given identical upstream outputs (above).
+# (alpha >= 1/255, T >= 1e-4, the 3-sigma radius and tile rect); such a flip
+# changes a pixel by up to one contribution (~alpha c T, alpha ~ 1/255) and the
+# recovered T of every other Gaussian blended there.  The FP32 and FP64
+# oracles disagree the same way.  end_to_end_check therefore masks exactly
+# the decisions that differ, found with the oracle's blend signatures: the
+# oracle run on the GPU's preprocess outputs blends the pairs the GPU blends
+# (the stage tests show its render equals the GPU's), so a Gaussian or pixel
+# whose signature differs from the oracle's own took another decision.
+# Pixels whose decision margin is below DECISION_MARGIN on either side are
+# masked too (exp implementations differ there).  Every pixel past
+# 1e-4 + 8 |o32 - o64| must be masked (the second term covers pixels near the
+# poles, where the projection is ill-conditioned and the FP32 oracle itself
+# strays).  Gaussians blended in a masked pixel, and those whose signature
+# differs, are masked (at most 2%); every other gradient entry is held to
+#     |gpu - o32| <= 1e-3 |o32| + 4 u e,
+# e being the running-error bound of the backward with the forward state's
+# measured deviations (GPU and FP32 oracle from FP64) as input uncertainty.
+# The old fraction-based constants remain for callers outside this file.
 FLIP_PIXEL_FRACTION = 2e-4
 FLIP_MAX_ABS = 0.05
 FLIP_GRAD_FRACTION = 2e-3
+E2E_FWD_K = 8.0
+
+
+
+
+def _gpu_fwd(st):
+    return {"xy": st["means2D"], "conic": st["conic_opacity"], "rgb": st["rgb"], "final_T": st["final_T"]}
+
+
+def oracle_on_gpu_preprocess(sc, st, radii, tile=(16, 16)):
+    """The FP32 oracle's binning and render run on the GPU's preprocess
+    outputs: it takes the GPU's discrete decisions (up to exp rounding at
+    DECISION_MARGIN), so comparing its blend signatures with the oracle's own
+    finds the pairs whose decisions differ."""
+    o = _oracle(sc, 32) if tile == (16, 16) else __import__("oracle").Oracle(sc, 32, block=tile)
+    o.run("preprocess")
+    for k, v in (("radii", radii), ("xy", st["means2D"]), ("depth", st["depth"]),
+                 ("tiles_touched", st["tiles_touched"]), ("conic", st["conic_opacity"]), ("rgb", st["rgb"])):
+        o.set(k, v)
+    return o.run("binning").run("render")
+
+
+def end_to_end_check(case, sc, img, radii, g, o32, o64, names, o_gpu, gpu_state, tile=(16, 16), max_masked=0.02):
+    """Decision-masked end-to-end parity (see above).  g: GPU gradients by the
+    oracle's names (numpy); img [3,H,W]; radii [P]; o_gpu: the oracle on the
+    GPU's preprocess outputs; gpu_state: the GPU's xy, conic, rgb, final_T.
+    Records the measured errors against FP32 and FP64 and the masked counts."""
+    from tests.gpu_util import fp64_errors, record
+    from tests.helpers import per_gaussian_mask, rounding_check, U32, ROUNDING_K
+    from oracle import RUNNING_ERROR, Oracle
+    W, H = int(sc["width"]), int(sc["height"])
+    P = int(np.asarray(sc["means3D"]).shape[0])
+    bx, by = tile
+    gx = (W + bx - 1) // bx
+    N = W * H
+    # radii agree except at the ceil boundary
+    rmis = radii != o32.get("radii")
+    assert rmis.mean() <= 1e-3, (case, int(rmis.sum()))
+    # decisions that differ: blend signatures, and pairs within rounding of a threshold
+    g_mask = o_gpu.get("gsig") != o32.get("gsig")
+    pix_mask = (o_gpu.get("psig") != o32.get("psig")) | (o32.get("margin") < DECISION_MARGIN) | \
+        (o_gpu.get("margin") < DECISION_MARGIN)
+    d = np.abs(img.reshape(3, N) - o32.get("out_color").reshape(3, N)).max(0)
+    # where the projection is ill-conditioned (near the poles) the FP32 oracle
+    # itself strays from FP64; a pixel is allowed that much (times 8) on top of 1e-4
+    d_ref = np.abs(o32.get("out_color").reshape(3, N).astype(np.float64) - o64.get("out_color").reshape(3, N)).max(0)
+    pix_bad = d > IMG_ATOL + E2E_FWD_K * d_ref
+    unexplained = pix_bad & ~pix_mask
+    assert not unexplained.any(), (case, np.nonzero(unexplained)[0][:10], d[unexplained][:10])
+    assert d.max() <= FLIP_MAX_ABS, (case, float(d.max()))
+    # Gaussians blended in a masked pixel, on either side
+    taint = g_mask.copy()
+    for oo in (o32, o_gpu):
+        pl, rg, nc = oo.get("point_list"), oo.get("ranges").reshape(-1, 2), oo.get("n_contrib")
+        xy_o, co_o = oo.get("xy").reshape(-1, 2).astype(np.float64), oo.get("conic").reshape(-1, 4).astype(np.float64)
+        for pid in np.nonzero(pix_mask)[0]:
+            py, px = divmod(int(pid), W)
+            t = (py // by) * gx + px // bx
+            b, e = rg[t]
+            ids = pl[b:min(e, b + int(nc[pid]) + 1)]
+            dx, dy = xy_o[ids, 0] - px, xy_o[ids, 1] - py
+            if sc.get("seam_wrap"):
+                dx = dx - W * np.round(dx / W)
+            c = co_o[ids]
+            power = -0.5 * (c[:, 0] * dx * dx + c[:, 2] * dy * dy) - c[:, 1] * dx * dy
+            alpha = np.minimum(0.99, c[:, 3] * np.exp(np.minimum(power, 0.0)))
+            taint[ids[alpha >= (1.0 / 255.0) * (1.0 - 1e-3)]] = True
+    assert taint.mean() <= max_masked, (case, taint.mean())
+    # running-error bound of the backward on the FP64 forward state, each
+    # continuous forward quantity given as input uncertainty the larger of the
+    # GPU's and the FP32 oracle's measured deviation from FP64 (so the bound
+    # carries sum |d grad / d x| |dx| of the actual differences, to first order)
+    oe = Oracle(sc, RUNNING_ERROR)
+    if sc.get("scales") is not None and sc.get("cov3D_precomp") is None:
+        oe.run("cov3d")
+    for k in ("radii", "point_list", "ranges", "n_contrib", "clamped"):
+        oe.set(k, o32.get(k))
+    for k in ("xy", "conic", "rgb", "final_T"):
+        v64 = o64.get(k).astype(np.float64)
+        oe.set(k, v64)
+        dev = np.maximum(np.abs(np.asarray(gpu_state[k], np.float64).ravel() - v64),
+                         np.abs(o32.get(k).astype(np.float64) - v64))
+        oe.set("err:" + k, dev / U32)
+    oe.run("render_backward", sc["dL_dpix"]).run("cov2d_backward").run("preprocess_backward")
+    m = {"masked_gaussians": int(taint.sum()), "signature_gaussians": int(g_mask.sum()),
+         "masked_pixels": int(pix_mask.sum()),
+         "pixels_past_1e-4": int(pix_bad.sum()), "image_abs_vs_fp32": float(d.max()),
+         "image_abs_vs_fp64": float(np.abs(img.reshape(-1) - o64.get("out_color")).max()),
+         "image_abs_vs_fp64_unmasked": float(np.abs(img.reshape(3, N) - o64.get("out_color").reshape(3, N))
+                                             .max(0)[~pix_mask].max())}
+    for name in names:
+        a = np.asarray(g[name], np.float64).ravel()
+        o = o32.get(name).astype(np.float64)
+        w = a.size // P
+        ok, worst, n = rounding_check(a, o, oe.get("err:" + name), skip=per_gaussian_mask(taint, w))
+        assert ok, f"{case} {name}: worst ratio {worst:.3g} over {n} unmasked entries"
+        m[f"{name}_worst_ratio"] = round(worst, 4)
+        keep = ~per_gaussian_mask(taint, w)
+        m.update(fp64_errors(name, a[keep], o64.get(name)[keep]))
+    record(case, **m)
+    return m
 
 
 def _frac_outside(a, b, rtol, scale_atol):
     a = np.asarray(a, dtype=np.float64).ravel(); b = np.asarray(b, dtype=np.float64).ravel()
     tol = rtol * np.abs(b) + scale_atol * max(np.abs(b).max(), 1e-30)
     return float(np.mean(np.abs(a - b) > tol))
+
+
+_GRAD_NAMES = (("dL_dmeans3D", "dL_dmeans3D"), ("dL_dsh", "dL_dsh"), ("dL_dscales", "dL_dscales"),
+               ("dL_drotations", "dL_drot"), ("dL_dopacity", "dL_dopacity"), ("dL_dmeans2D", "dL_dmean2D"),
+               ("dL_dconic", "dL_dconic"), ("dL_dcolors", "dL_dcolors"))
 
 
 @pytest.mark.parametrize("P,W,H,seed", [(20000, 512, 256, 0), (20000, 512, 256, 1)])
@@ -210,23 +364,9 @@ def test_end_to_end_against_fp32_and_fp64(gpu, P, W, H, seed):
     o32 = _oracle(sc, 32).forward().backward(sc["dL_dpix"])
     o64 = _oracle(sc, 64).forward().backward(sc["dL_dpix"])
     assert abs(R - o32.num_rendered) <= max(2, 1e-4 * R)
-    im = np_(img).reshape(-1)
-    d32 = np.abs(im - o32.get("out_color"))
-    d64 = np.abs(im - o64.get("out_color"))
-    d3264 = np.abs(o32.get("out_color") - o64.get("out_color"))
-    print(f"image: max|gpu-fp32|={d32.max():.3g} (frac>1e-4: {np.mean(d32 > IMG_ATOL):.2e}); "
-          f"max|gpu-fp64|={d64.max():.3g}; max|fp32-fp64| oracle={d3264.max():.3g} "
-          f"(frac>1e-4: {np.mean(d3264 > IMG_ATOL):.2e})")
-    assert np.mean(d32 > IMG_ATOL) <= FLIP_PIXEL_FRACTION
-    assert d32.max() <= FLIP_MAX_ABS
-    for mine, ref in (("dL_dmeans3D", "dL_dmeans3D"), ("dL_dsh", "dL_dsh"), ("dL_dscales", "dL_dscales"),
-                      ("dL_drotations", "dL_drot"), ("dL_dopacity", "dL_dopacity"), ("dL_dmeans2D", "dL_dmean2D")):
-        a = np_(g[mine]).reshape(-1)
-        f32 = _frac_outside(a, o32.get(ref), GRAD_RTOL, GRAD_SCALE_ATOL)
-        f64 = _frac_outside(a, o64.get(ref), GRAD_RTOL, GRAD_SCALE_ATOL)
-        fo = _frac_outside(o32.get(ref), o64.get(ref), GRAD_RTOL, GRAD_SCALE_ATOL)
-        print(f"{mine}: frac outside tol vs fp32 {f32:.2e}, vs fp64 {f64:.2e} (fp32 oracle vs fp64: {fo:.2e})")
-        assert f32 <= FLIP_GRAD_FRACTION, (mine, f32)
+    gg = {ref: np_(g[mine]) for mine, ref in _GRAD_NAMES}
+    end_to_end_check(f"end_to_end P={P} {W}x{H} seed {seed}", sc, np_(img), np_(radii), gg, o32, o64,
+                     [ref for _, ref in _GRAD_NAMES], oracle_on_gpu_preprocess(sc, st, np_(radii)), _gpu_fwd(st))
 
 
 def test_backward_repeatable_with_records_rewritten(gpu):
@@ -319,19 +459,18 @@ def test_variants_end_to_end(gpu, variant):
     ds, R, img, radii, bufs, st = _setup(sc)
     g = run_backward(ds, R, radii, bufs, torch.from_numpy(sc["dL_dpix"]).to(gpu))
     o = _oracle(sc, 32).forward().backward(sc["dL_dpix"])
+    o64 = _oracle(sc, 64).forward().backward(sc["dL_dpix"])
     assert abs(R - o.num_rendered) <= max(2, 1e-4 * R)
-    d = np.abs(np_(img).reshape(-1) - o.get("out_color"))
-    assert np.mean(d > IMG_ATOL) <= FLIP_PIXEL_FRACTION and d.max() <= FLIP_MAX_ABS, (d.max(), np.mean(d > IMG_ATOL))
-    names = [("dL_dmeans3D", "dL_dmeans3D"), ("dL_dopacity", "dL_dopacity"), ("dL_dcov3D", "dL_dcov3D")]
-    if variant == "colors_precomp":
-        names.append(("dL_dcolors", "dL_dcolors"))
-    else:
+    names = [("dL_dmeans3D", "dL_dmeans3D"), ("dL_dopacity", "dL_dopacity"), ("dL_dcov3D", "dL_dcov3D"),
+             ("dL_dmeans2D", "dL_dmean2D"), ("dL_dconic", "dL_dconic"), ("dL_dcolors", "dL_dcolors")]
+    if variant != "colors_precomp":
         names.append(("dL_dsh", "dL_dsh"))
     if variant != "cov3D_precomp":
         names += [("dL_dscales", "dL_dscales"), ("dL_drotations", "dL_drot")]
-    for mine, ref in names:
-        f = _frac_outside(np_(g[mine]).reshape(-1), o.get(ref), GRAD_RTOL, GRAD_SCALE_ATOL)
-        assert f <= FLIP_GRAD_FRACTION, (variant, mine, f)
+    gg = {ref: np_(g[mine]) for mine, ref in names}
+    end_to_end_check(f"variant {variant}", sc, np_(img), np_(radii), gg, o, o64, [ref for _, ref in names],
+                     oracle_on_gpu_preprocess(sc, st, np_(radii)), _gpu_fwd(st),
+                     max_masked=0.05 if variant == "whale" else 0.02)
 
 
 def test_full_size_properties(gpu):

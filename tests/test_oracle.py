@@ -364,3 +364,81 @@ def test_openmp_baseline_matches_serial_and_is_thread_count_independent(oracle_l
         a, b = outs[0].get(name), ser.get(name)
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * max(np.abs(b).max(), 1e-300), err_msg=name)
         np.testing.assert_array_equal(outs[0].get(name), outs[1].get(name), err_msg=name)
+
+
+def _fp32_state(sc):
+    from oracle import Oracle
+    o64 = Oracle(sc, 64).forward()
+    keys = ("radii", "xy", "conic", "rgb", "point_list", "ranges", "final_T", "n_contrib", "clamped")
+    st = {k: o64.get(k) for k in keys}
+    for k in ("xy", "conic", "rgb", "final_T"):
+        st[k] = st[k].astype(np.float32).astype(np.float64)
+    return st
+
+
+def test_running_error_oracle_values_and_bound(oracle_lib):
+    """The running-error handle computes the FP64 oracle's values, and the
+    FP32 oracle's actual error on the same FP32 inputs stays within its
+    first-order bound u e for every backward output (a dropped propagation
+    term in Rerr would let some entry's error exceed it)."""
+    from oracle import RUNNING_ERROR, Oracle
+    sc = scenes.small_scene(P=3000, width=200, height=90, seed=1)
+    sc["background"] = np.array([0.1, 0.2, 0.3], dtype=np.float32)
+    st = _fp32_state(sc)
+    hs = {}
+    for prec in (32, 64, RUNNING_ERROR):
+        o = Oracle(sc, prec)
+        o.run("cov3d" if prec == RUNNING_ERROR else "preprocess")
+        for k, v in st.items():
+            o.set(k, v)
+        hs[prec] = o.backward(sc["dL_dpix"])
+    u = 2.0 ** -24
+    for k in ("dL_dmean2D", "dL_dconic", "dL_dopacity", "dL_dcolors", "dL_dmeans3D", "dL_dsh", "dL_dscales",
+              "dL_drot", "dL_dcov3D", "dpx_dt"):
+        v64, ve = hs[64].get(k), hs[RUNNING_ERROR].get(k)
+        np.testing.assert_allclose(ve, v64, rtol=1e-12, atol=1e-300)
+        err = hs[RUNNING_ERROR].get("err:" + k)
+        d = np.abs(hs[32].get(k).astype(np.float64) - v64)
+        assert np.all(d <= u * err), (k, float((d / np.maximum(u * err, 1e-300)).max()))
+        assert np.all(err >= 0) and err.max() > 0, k
+    # input uncertainty propagates: errors grow when the state carries some
+    oe2 = Oracle(sc, RUNNING_ERROR).run("cov3d")
+    for k, v in st.items():
+        oe2.set(k, v)
+    oe2.set("err:xy", np.abs(st["xy"]) * 4.0)
+    oe2.backward(sc["dL_dpix"])
+    assert np.all(oe2.get("err:dL_dmean2D") >= hs[RUNNING_ERROR].get("err:dL_dmean2D"))
+    assert oe2.get("err:dL_dmean2D").sum() > 2 * hs[RUNNING_ERROR].get("err:dL_dmean2D").sum()
+
+
+def test_blend_signatures(oracle_lib):
+    """Blend signatures do not depend on the thread count, and moving one
+    Gaussian so that its blended pixels change changes its signature and
+    those pixels' signatures only (Gaussians sharing those pixels keep
+    theirs unless their own blends change)."""
+    from oracle import Oracle
+    sc = scenes.small_scene(P=3000, width=200, height=90, seed=2)
+    a = Oracle(sc, 32).forward()
+    b = Oracle(sc, 32)
+    b.set_threads(4)
+    b.forward()
+    assert np.array_equal(a.get("gsig"), b.get("gsig")) and np.array_equal(a.get("psig"), b.get("psig"))
+    c = Oracle(sc, 32).run("preprocess")
+    xy = c.get("xy").reshape(-1, 2).copy()
+    vis = np.nonzero(c.get("radii") > 3)[0]
+    gi = int(vis[0])
+    xy[gi] += 0.5  # half a pixel: its blended pixel set changes
+    c.set("xy", xy.reshape(-1))
+    c.run("binning").run("render")
+    dg = np.nonzero(c.get("gsig") != a.get("gsig"))[0]
+    assert gi in dg
+    dp = c.get("psig") != a.get("psig")
+    assert dp.any()
+    # pixels whose signature changed lie within the moved Gaussian's reach,
+    # and pixels beyond it render exactly as before
+    n = 200 * 90
+    yy, xx = np.divmod(np.arange(n), 200)
+    r = int(c.get("radii")[gi]) + 2
+    near = (np.abs(xx - xy[gi, 0]) <= r) & (np.abs(yy - xy[gi, 1]) <= r)
+    assert not (dp & ~near).any()
+    assert np.array_equal(c.get("out_color").reshape(3, n)[:, ~near], a.get("out_color").reshape(3, n)[:, ~near])

@@ -146,3 +146,50 @@ def test_reader_errors(tmp_path):
         ply.load_ply(g, 3)
     with pytest.raises(RuntimeError, match="cannot open"):
         ply.load_ply(tmp_path / "missing.ply", 3)
+
+
+@pytest.mark.gpu
+def test_gpu_render_of_a_ply_in_the_reference_layout(gpu, tmp_path):
+    """A scene written in the reference's layout (raw parameters: log scales,
+    unnormalised quaternions, opacity logits, channel-major f_rest) is read
+    back onto the GPU and rendered through renderLonlat's contract (model
+    activations, autograd Function); image and gradients match the CPU oracle
+    run on the original parameters, and the gradients reach the loaded
+    tensors (end-to-end bars of the GPU parity tests)."""
+    from oracle import Oracle
+    from omnigs_amd import scenes
+    from omnigs_amd.rasterizer import render_lonlat
+    from tests.gpu_util import announce, np_
+    from tests.helpers import activate64, chain_activations, tiny_scene
+    from tests.test_gpu_parity import FLIP_GRAD_FRACTION, FLIP_MAX_ABS, FLIP_PIXEL_FRACTION, GRAD_RTOL, \
+        GRAD_SCALE_ATOL, IMG_ATOL, _frac_outside
+    announce("PLY in the reference layout -> GPU render vs oracle, 2000 Gaussians 256x128")
+    sc = tiny_scene(P=2000, W=256, H=128, seed=12, log_scale=(-3.6, 0.3), radius=(1.5, 4.0))
+    raw = {"xyz": torch.tensor(sc["means3D"], dtype=torch.float32),
+           "features_dc": torch.tensor(sc["shs"][:, :1], dtype=torch.float32),
+           "features_rest": torch.tensor(sc["shs"][:, 1:], dtype=torch.float32),
+           "opacity": torch.tensor(sc["logits"], dtype=torch.float32),
+           "scaling": torch.tensor(sc["log_scales"], dtype=torch.float32),
+           "rotation": torch.tensor(sc["raw_rotations"], dtype=torch.float32)}
+    f = tmp_path / "scene.ply"
+    ply.save_ply(f, raw, 3)
+    params = {k: v.requires_grad_(True) for k, v in ply.load_ply(f, 3, device=gpu).items()}
+    for k in raw:
+        assert torch.equal(params[k].detach().cpu(), raw[k]), k
+    vm = torch.tensor(sc["viewmatrix"], dtype=torch.float32, device=gpu)
+    cp = torch.tensor(sc["campos"], dtype=torch.float32, device=gpu)
+    bg = torch.tensor(sc["background"], dtype=torch.float32, device=gpu)
+    img, vs, vis, radii = render_lonlat(params, vm, cp, 128, 256, bg, 3)
+    (img * torch.tensor(sc["dL_dpix"], dtype=torch.float32, device=gpu)).sum().backward()
+    s32 = {k: v for k, v in activate64(sc).items() if k not in ("log_scales", "raw_rotations", "logits")}
+    o = Oracle(s32, 32).forward().backward(sc["dL_dpix"])
+    d = np.abs(np_(img).reshape(-1) - o.get("out_color"))
+    assert np.mean(d > IMG_ATOL) <= FLIP_PIXEL_FRACTION and d.max() <= FLIP_MAX_ABS
+    ch = chain_activations(sc, {k: o.get(k) for k in ("dL_dmeans3D", "dL_dscales", "dL_drot", "dL_dopacity",
+                                                      "dL_dsh")})
+    for mine, ref in (("xyz", "means3D"), ("scaling", "log_scales"), ("rotation", "raw_rotations"),
+                      ("opacity", "logits")):
+        f_out = _frac_outside(np_(params[mine].grad).reshape(-1), ch[ref].reshape(-1), GRAD_RTOL, GRAD_SCALE_ATOL)
+        assert f_out <= FLIP_GRAD_FRACTION, (mine, f_out)
+    sh = torch.cat([params["features_dc"].grad, params["features_rest"].grad], 1)
+    assert _frac_outside(np_(sh).reshape(-1), ch["shs"].reshape(-1), GRAD_RTOL, GRAD_SCALE_ATOL) <= FLIP_GRAD_FRACTION
