@@ -270,7 +270,9 @@ template <class R> struct State {
   // running-error mode (R = Rerr) only: Gaussians blended in a pixel where a
   // render-backward decision (power > 0, alpha >= 1/255) lies within rounding
   // of its threshold, so that another correct FP32 evaluation may decide it
-  // the other way and change the pixel's recovered T for every contributor
+  // the other way (for information: the error bounds already carry the
+  // effect of such a flip -- the pair's own terms, and the change of T and of
+  // the colour behind for every instance in front of it)
   std::vector<uint8_t> fragile;
   uint64_t fragile_pixels = 0;
 };
@@ -619,6 +621,8 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
     R bg_dot = 0;
     for (int ch = 0; ch < kChannels; ++ch) bg_dot += in.bg[ch] * dpix[ch];
     bool frag = false;
+    double pend[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // error bounds of an ambiguous pair blended here
+    bool have_pend = false;
     // contributor index k - b (0-based); only those before last are blended.
     for (uint32_t kk = e; kk > b; --kk) {
       const uint32_t k = kk - 1;
@@ -629,17 +633,59 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
       if (in.seam_wrap) dx = seam_dx(dx, W);
       const R* co = &st.conic[4 * g];
       const R power = R(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
+      bool ambiguous = false;
       if constexpr (is_rerr<R>::value) {
+        // a decision within rounding of its threshold may go the other way in
+        // another correct FP32 evaluation (power > 0 near the mean, alpha
+        // near 1/255): its possible effect enters the error bounds below
         constexpr double u = 0x1p-24;
-        if (std::fabs(power.v) <= 8.0 * power.e * u + 64.0 * u) frag = true;
+        if (std::fabs(power.v) <= 8.0 * power.e * u + 64.0 * u) ambiguous = true;
         const R oG = co[3] * oexp(power);
         const double thr = 1.0 / 255.0;
-        if (std::fabs(oG.v - thr) <= 8.0 * oG.e * u + thr * u * (64.0 + 8.0 * std::fabs(power.v))) frag = true;
+        if (std::fabs(oG.v - thr) <= 8.0 * oG.e * u + thr * u * (64.0 + 8.0 * std::fabs(power.v))) ambiguous = true;
+        if (ambiguous) frag = true;
       }
-      if (power > R(0)) continue;
+      const bool skip = power > R(0) || std::min(R(0.99), co[3] * oexp(power)) < R(1) / R(255);
+      if (skip && !ambiguous) continue;
+      if constexpr (is_rerr<R>::value) {
+        if (ambiguous) {
+          // the pair as if blended (G at most 1, alpha at least the threshold)
+          constexpr double u = 0x1p-24;
+          const double Gv = std::min(1.0, std::exp(std::min(0.0, power.v)));
+          const double av = std::max(1.0 / 255.0, std::min(0.99, co[3].v * Gv));
+          const double Th = T.v / (1.0 - av), r = av / (1.0 - av);
+          double dLa = 0.0;
+          for (int ch = 0; ch < kChannels; ++ch) {
+            const double c = col[kChannels * g + ch].v;
+            const double acc_h = last_alpha.v * last_color[ch].v + (1.0 - last_alpha.v) * accum[ch].v;
+            dLa += std::fabs((c - acc_h) * dpix[ch].v);
+          }
+          dLa = dLa * Th + std::fabs(T_final.v / (1.0 - av) * bg_dot.v);
+          const double dG = co[3].v * dLa, gdx = Gv * std::fabs(dx.v), gdy = Gv * std::fabs(dy.v);
+          R vh[9];
+          for (int ch = 0; ch < kChannels; ++ch) vh[ch] = R(0.0, av * Th * std::fabs(dpix[ch].v) / u);
+          vh[3] = R(0.0, dG * (gdx * std::fabs(co[0].v) + gdy * std::fabs(co[1].v)) * std::fabs(ddelx_dx.v) / u);
+          vh[4] = R(0.0, dG * (gdy * std::fabs(co[2].v) + gdx * std::fabs(co[1].v)) * std::fabs(ddely_dy.v) / u);
+          vh[5] = R(0.0, 0.5 * gdx * std::fabs(dx.v) * dG / u);
+          vh[6] = R(0.0, 0.5 * gdx * std::fabs(dy.v) * dG / u);
+          vh[7] = R(0.0, 0.5 * gdy * std::fabs(dy.v) * dG / u);
+          vh[8] = R(0.0, Gv * dLa / u);
+          // the instances in front see T and the colour behind them change by
+          // up to alpha / (1 - alpha) and alpha (c - acc)
+          T.e += std::fabs(T.v) * r / u;
+          for (int ch = 0; ch < kChannels; ++ch)
+            accum[ch].e += av * (std::fabs(col[kChannels * g + ch].v) + std::fabs(accum[ch].v)) / u;
+          if (skip) {
+            sink(k, g, vh);
+            continue;
+          }
+          // blended here: its own terms may be absent on the other side
+          for (int j = 0; j < 9; ++j) pend[j] = vh[j].e;
+          have_pend = true;
+        }
+      }
       const R G = oexp(power);
       const R alpha = std::min(R(0.99), co[3] * G);
-      if (alpha < R(1) / R(255)) continue;
       T = T / (R(1) - alpha);
       const R dch_dcolor = alpha * T;
       R dL_dalpha = 0;
@@ -664,6 +710,11 @@ template <class R> void render_backward(const Inputs<R>& in, State<R>& st, const
       v[6] = R(-0.5) * gdx * dy * dL_dG;
       v[7] = R(-0.5) * gdy * dy * dL_dG;
       v[8] = G * dL_dalpha;
+      if constexpr (is_rerr<R>::value) {
+        if (have_pend)
+          for (int j = 0; j < 9; ++j) v[j].e += pend[j];
+        have_pend = false;
+      }
       sink(k, g, v);
     }
     if constexpr (is_rerr<R>::value) {

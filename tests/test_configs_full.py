@@ -84,11 +84,10 @@ def test_room_scene_full_size_stages(gpu):
         oe.set(k, o.get(k))
     oe.run("render_backward", dLn)
     fragile = oe.get("fragile").astype(bool)
-    assert fragile.mean() <= 0.05, fragile.mean()
     m.update(fragile_gaussians=int(fragile.sum()), fragile_pixels=oe.fragile_pixels)
     for mine, ref, w in (("dL_dmeans2D", "dL_dmean2D", 3), ("dL_dconic", "dL_dconic", 4),
                          ("dL_dopacity", "dL_dopacity", 1), ("dL_dcolors", "dL_dcolors", 3)):
-        _rounding_grad_check(ref, np_(g[mine]).reshape(-1), o.get(ref), oe, w, fragile, m)
+        _rounding_grad_check(ref, np_(g[mine]).reshape(-1), o.get(ref), oe, w, None, m)
     del oe
     # preprocess backward given identical inputs
     g2 = {"dL_dmean2D": np_(g["dL_dmeans2D"]), "dL_dconic": np_(g["dL_dconic"]),
@@ -270,13 +269,12 @@ def test_batched_views_against_oracle(gpu, V, tile):
         oe = running_error_oracle(scv, dict(state, radii=it["radii"], clamped=_clamp3(it["clamped"])), dLv,
                                   threads=host_threads(), block=tile)
         fragile = oe.get("fragile").astype(bool)
-        assert fragile.mean() <= 0.05
         sl = slice(v * P, (v + 1) * P)
         g2 = {"dL_dmean2D": st["dL_dmean2D"].reshape(-1, 3)[sl], "dL_dconic": st["dL_dconic"].reshape(-1, 4)[sl],
               "dL_dopacity": st["dL_dopacity"][sl], "dL_dcolors": st["dL_dcolor"].reshape(-1, 3)[sl]}
         vm_ = {}
         for ref, w in (("dL_dmean2D", 3), ("dL_dconic", 4), ("dL_dopacity", 1), ("dL_dcolors", 3)):
-            _rounding_grad_check(ref, g2[ref].reshape(-1), o.get(ref), oe, w, fragile, vm_)
+            _rounding_grad_check(ref, g2[ref].reshape(-1), o.get(ref), oe, w, None, vm_)
         for k, x in vm_.items():
             if k.endswith("worst_ratio"):
                 worst[k] = max(worst.get(k, 0.0), x)

@@ -6,9 +6,9 @@ Bars:
 * image / final_T: |gpu - oracle| <= 1e-4 absolute;
 * gradients given identical inputs: |gpu - o32| <= 1e-3 |o32| + 4 u e per
   element, e being the running-error oracle's bound of that entry's FP32
-  rounding error (tests/helpers.py); Gaussians blended in a pixel whose
-  backward decision lies within rounding of its threshold ("fragile", a few
-  per cent at most) are held to 1e-3 |o32| + 1e-4 max|o32| instead;
+  rounding error (tests/helpers.py), which also carries the possible effect
+  of a backward decision within rounding of its threshold (the pair's own
+  terms, and the recovered T of the instances in front of it);
 * the GPU's error against the FP64 oracle is recorded (gpu_util.record).
 """
 from __future__ import annotations
@@ -138,8 +138,9 @@ def _grad_check(name, g, o_val):
 
 
 def _rounding_grad_check(name, g, o32, oerr, width=1, fragile=None, metrics=None):
-    """Rounding-aware bar (tests/helpers.py) on every entry; entries of
-    fragile Gaussians are held to the looser 1e-3 |o| + 1e-4 max|o| bar."""
+    """Rounding-aware bar (tests/helpers.py) on every entry (entries of
+    Gaussians flagged in ``fragile``, when given, are held to the looser
+    1e-3 |o| + 1e-4 max|o| bar instead)."""
     from tests.gpu_util import fp64_errors
     from tests.helpers import per_gaussian_mask, rounding_check
     skip = per_gaussian_mask(fragile, width) if fragile is not None else None
@@ -176,11 +177,10 @@ def test_render_backward_given_identical_forward(gpu, P, W, H, seed):
         oe.set(k, o.get(k))
     oe.run("render_backward", sc["dL_dpix"])
     fragile = oe.get("fragile").astype(bool)
-    assert fragile.mean() <= 0.05, fragile.mean()
     m = {"fragile_gaussians": int(fragile.sum()), "fragile_pixels": oe.fragile_pixels}
     for mine, ref, w in (("dL_dmeans2D", "dL_dmean2D", 3), ("dL_dconic", "dL_dconic", 4),
                          ("dL_dopacity", "dL_dopacity", 1), ("dL_dcolors", "dL_dcolors", 3)):
-        _rounding_grad_check(ref, np_(g[mine]).reshape(-1), o.get(ref), oe, w, fragile, m)
+        _rounding_grad_check(ref, np_(g[mine]).reshape(-1), o.get(ref), oe, w, None, m)
     record(f"render_backward_given_identical_forward P={P} {W}x{H}", **m)
 
 
