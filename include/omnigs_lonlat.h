@@ -178,6 +178,22 @@ int omnigs_photometric_loss(const float* image, const float* gt_image, const flo
                             int width, int skip_bottom, float lambda_dssim, void* workspace, size_t workspace_bytes,
                             float* dL_dimage, float* loss_out, void* stream);
 
+/* omnigs_photometric_loss without a workspace argument and without any
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight training or serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a file-scope device
+ * array of OMNIGS_LOSS_NOALLOC_MAX_BYTES that every GPU the library runs on
+ * holds until the process ends; its device address is looked up on the first
+ * call on each device.  Images whose omnigs_loss_workspace_bytes() exceeds it
+ * (beyond 3 x 1024 x 2048 with lambda_dssim != 0) are refused with
+ * OMNIGS_ERR_INVALID_ARGUMENT.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two threads) race.  Same
+ * arguments and results as omnigs_photometric_loss. */
+#define OMNIGS_LOSS_NOALLOC_MAX_BYTES (80u << 20)
+int omnigs_photometric_loss_noalloc(const float* image, const float* gt_image, const float* mask, int channels,
+                                    int height, int width, int skip_bottom, float lambda_dssim, float* dL_dimage,
+                                    float* loss_out, void* stream);
+
 /* Batched multi-view forward / backward (extension; the reference renders one
  * view per RasterizeGaussiansCUDA call, rasterize_points.cu:49-164).  V <= 32
  * equirectangular views of one Gaussian set, each with its own size and pose,

@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
     "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
     "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr", "omnigs_lonlat_batch_state",
+    "omnigs_photometric_loss_noalloc",
 )
 NUM_STAGES = 11
 
@@ -171,6 +172,9 @@ def load() -> ctypes.CDLL:
     lib.omnigs_photometric_loss.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [
         ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.omnigs_photometric_loss.restype = ctypes.c_int
+    lib.omnigs_photometric_loss_noalloc.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [
+        ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.omnigs_photometric_loss_noalloc.restype = ctypes.c_int
     lib.omnigs_mlp_param_count.restype = ctypes.c_int
     lib.omnigs_mlp_packed_bytes.restype = ctypes.c_size_t
     lib.omnigs_mlp_pack.argtypes = [ctypes.c_void_p] * 3

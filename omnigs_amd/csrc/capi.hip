@@ -189,6 +189,23 @@ int omnigs_photometric_loss(const float* image, const float* gt_image, const flo
   return OMNIGS_OK;
 }
 
+int omnigs_photometric_loss_noalloc(const float* image, const float* gt_image, const float* mask, int channels,
+                                    int height, int width, int skip_bottom, float lambda_dssim, float* dL_dimage,
+                                    float* loss_out, void* stream) {
+  if (channels <= 0 || height <= 0 || width <= 0) return fail_arg("empty image");
+  if (skip_bottom < 0 || skip_bottom >= height) return fail_arg("skip_bottom outside [0, height)");
+  if (!(lambda_dssim >= 0.f && lambda_dssim <= 1.f)) return fail_arg("lambda_dssim outside [0, 1]");
+  if (!image || !gt_image || !dL_dimage || !loss_out) return fail_arg("missing required pointer");
+  if (loss_workspace_floats(channels, height, width, skip_bottom) > kLossScratchFloats)
+    return fail_arg("image too large for the allocation-free loss (workspace over OMNIGS_LOSS_NOALLOC_MAX_BYTES)");
+  float* ws = nullptr;
+  OMNIGS_TRY(loss_scratch(&ws), "loss scratch address");
+  OMNIGS_TRY(launch_loss(image, gt_image, mask, channels, height, width, skip_bottom, lambda_dssim, ws, dL_dimage,
+                         loss_out, (hipStream_t)stream),
+             "photometric loss (no-alloc)");
+  return OMNIGS_OK;
+}
+
 int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream) {
   if (P < 0) return fail_arg("P < 0");
   if (P > 0 && !present) return fail_arg("present is NULL");

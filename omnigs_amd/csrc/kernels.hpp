@@ -202,6 +202,11 @@ float expon_lr(int step, float lr_init, float lr_final, int lr_delay_steps, floa
 // Floats of workspace the fused loss needs for a [C,H,W] image with `skip`
 // bottom rows dropped (0 when nothing is kept).
 size_t loss_workspace_floats(int C, int H, int W, int skip);
+// The allocation-free loss's workspace: a file-scope device array of
+// kLossScratchFloats floats (80 MiB) per GPU; loss_scratch gives its device
+// address on the current device (looked up once per device).
+constexpr size_t kLossScratchFloats = (size_t)80 << 18;
+hipError_t loss_scratch(float** out);
 hipError_t launch_loss(const float* x, const float* y, const float* mask, int C, int H, int W, int skip, float lambda,
                        float* workspace, float* dL_dx, float* out, hipStream_t s);
 

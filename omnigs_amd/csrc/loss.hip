@@ -20,6 +20,7 @@
 // pass (float4 when W % 4 == 0) writes the gradient and the L1 block sums.
 // A one-block kernel adds the block sums in a fixed order (deterministic).
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -331,6 +332,38 @@ hipError_t launch_loss(const float* x, const float* y, const float* mask, int C,
     hipLaunchKernelGGL(loss_final_kernel, dim3(1), dim3(kFinalThreads), 0, s, partial, blocks, n, lambda, out);
   }
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form: the workspace is this file-scope device array instead
+// of caller memory.  Every GPU the library runs on holds its own copy (80 MiB)
+// until the process ends; a call writes every workspace word it later reads
+// (the SSIM maps and the block partials), so nothing needs clearing, and
+// calls that overlap on one GPU race on it.  In host code `scratch` names a
+// host-side stand-in, so its device address is looked up once per device
+// with hipGetSymbolAddress and only that address is passed to the kernels.
+static __device__ __align__(256) float scratch[kLossScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t loss_scratch(float** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
 }
 
 }  // namespace omnigs
