@@ -383,6 +383,9 @@ typedef struct omnigs_lonlat_state_view {
   const float* final_T;          /* [H,W] */
   const uint32_t* n_contrib;     /* [H,W] */
   const uint32_t* ranges;        /* [tiles,2] */
+  /* binning buffer, per sorted instance: bit w set when wave w of its tile (8x8 quadrant w, row-major
+     over the tile's quadrants) blended it in the render forward; valid once a render forward ran */
+  const uint8_t* blend;
 } omnigs_lonlat_state_view;
 
 int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, const char* image_buffer, int P, int R,

@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
 NUM_STAGES = 11
 
 _STATE_FIELDS = ("depth_key", "radii", "render_records", "clamped", "tiles_touched",
-                 "depth_order", "offsets", "point_list", "tile_keys", "final_T", "n_contrib", "ranges")
+                 "depth_order", "offsets", "point_list", "tile_keys", "final_T", "n_contrib", "ranges", "blend")
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -350,6 +350,7 @@ def state(buffers, P: int, R: int, width: int, height: int) -> dict:
         "depth_order": (geom, torch.int32, P), "offsets": (geom, torch.int32, P),
         "point_list": (binning, torch.int32, R), "tile_keys": (binning, torch.int32, R),
         "final_T": (image, torch.float32, N), "n_contrib": (image, torch.int32, N), "ranges": (image, torch.int32, 2 * tiles),
+        "blend": (binning, torch.uint8, R),
     }
     out = {}
     for name, (buf, dt, n) in spec.items():

@@ -46,6 +46,22 @@ std::mutex g_prof_mu;
 int g_prof_mask = 0;  // bit s: time stage s
 std::vector<StageEvent> g_prof_events;
 
+// Events are kept in a pool and reused: creating one costs the calling thread
+// microseconds, which would delay the launches between two timed stages.
+std::vector<hipEvent_t> g_prof_pool;
+
+bool take_event(hipEvent_t* e) {
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_pool.empty()) {
+      *e = g_prof_pool.back();
+      g_prof_pool.pop_back();
+      return true;
+    }
+  }
+  return hipEventCreate(e) == hipSuccess;
+}
+
 struct StageTimer {
   int stage;
   hipStream_t s;
@@ -58,8 +74,7 @@ struct StageTimer {
     }
     if (on) {
       ev.stage = st;
-      if (hipEventCreate(&ev.start) != hipSuccess || hipEventCreate(&ev.stop) != hipSuccess ||
-          hipEventRecord(ev.start, s) != hipSuccess)
+      if (!take_event(&ev.start) || !take_event(&ev.stop) || hipEventRecord(ev.start, s) != hipSuccess)
         on = false;
     }
   }
@@ -84,7 +99,7 @@ using namespace omnigs;
 extern "C" {
 
 size_t omnigs_lonlat_geometry_bytes(int P) { return carve_geom(nullptr, (size_t)(P < 0 ? 0 : P)).bytes; }
-size_t omnigs_lonlat_binning_bytes(int R) { return carve_bin(nullptr, (size_t)(R < 0 ? 0 : R)).bytes; }
+size_t omnigs_lonlat_binning_bytes(int R) { return carve_bin(nullptr, bin_capacity((uint32_t)(R < 0 ? 0 : R))).bytes; }
 size_t omnigs_lonlat_image_bytes(int width, int height) {
   if (width <= 0 || height <= 0) return 0;
   return carve_img(nullptr, width, height).bytes;
@@ -127,8 +142,9 @@ int omnigs_profile_read(double* ms, int* count) {
       if (ms) ms[e.stage] += t;
       if (count) count[e.stage] += 1;
     }
-    (void)hipEventDestroy(e.start);
-    (void)hipEventDestroy(e.stop);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_pool.push_back(e.start);
+    g_prof_pool.push_back(e.stop);
   }
   return status;
 }
@@ -155,10 +171,11 @@ int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, con
     out->offsets = g.offsets;
   }
   if (binning_buffer && R > 0) {
-    const BinState b = carve_bin(const_cast<char*>(binning_buffer), R);
+    const BinState b = carve_bin(const_cast<char*>(binning_buffer), bin_capacity((uint32_t)R));
     const bool alt = radix_passes(bit_width_u32((uint32_t)tile_grid(width, height).tiles())) % 2;
     out->point_list = alt ? b.gid_alt : b.gid;
     out->tile_keys = alt ? b.tile_key_alt : b.tile_key;
+    out->blend = b.blend;
   }
   if (image_buffer) {
     const ImgState s = carve_img(const_cast<char*>(image_buffer), width, height);
@@ -231,6 +248,64 @@ uint32_t* pinned_word() {
   return word;
 }
 
+// One event per calling thread for the forward's read-back of R (created on
+// first use, kept for the thread's lifetime).
+hipEvent_t readback_event() {
+  thread_local hipEvent_t ev = nullptr;
+  thread_local bool tried = false;
+  if (!tried) {
+    tried = true;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+  }
+  return ev;
+}
+
+// Expansion, tile sort, tile ranges and (render = true) the render forward,
+// into a binning buffer carved for bin_capacity(n) instances.  R_dev NULL:
+// n is the exact instance count R.  Otherwise n is a capacity guess and the
+// kernels read the live count {R, overflow flag} at R_dev, stopping there
+// (their grids cover the capacity).
+int bin_and_render(const omnigs_lonlat_forward_args* a, omnigs_buffer binning_buffer, const GeomState& g,
+                   const ImgState& img, const TileGrid& grid, uint32_t n, const uint32_t* R_dev, hipStream_t s,
+                   bool render) {
+  const uint32_t C = bin_capacity(n);
+  char* bbuf = grow(binning_buffer, carve_bin(nullptr, C).bytes);
+  if (!bbuf) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "binning buffer allocation failed");
+    return OMNIGS_ERR_ALLOC;
+  }
+  const BinState b = carve_bin(bbuf, C);
+  const uint32_t grid_n = R_dev ? C : n;  // instances the grids cover
+  const int tile_bits = bit_width_u32((uint32_t)grid.tiles());
+  {
+    // the expansion also counts the tile sort's first digit
+    StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
+    OMNIGS_TRY(launch_duplicate(a->P, grid_n, R_dev, g.order, g.offsets, g.rect_depth, grid, b.tile_key, b.gid,
+                                reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, img.ranges,
+                                b.blend_ok, s),
+               "duplicate");
+  }
+  SortResult ts{b.tile_key, b.gid};
+  if (grid_n > 0) {
+    StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
+    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, grid_n, tile_bits,
+                                 b.sort_counts, b.scan_partials, s, &ts, R_dev, true),
+               "tile sort");
+  }
+  {
+    StageTimer tm(OMNIGS_STAGE_RANGES, s);
+    OMNIGS_TRY(launch_tile_ranges(ts.keys, grid_n, img.ranges, grid.tiles(), s, R_dev, grid_n > 0), "ranges");
+  }
+  if (!render) return OMNIGS_OK;
+  {
+    StageTimer tm(OMNIGS_STAGE_RENDER, s);
+    OMNIGS_TRY(launch_render_forward(grid, img.ranges, ts.vals, a->width, a->height, g.rec, img.final_T,
+                                     img.n_contrib, a->background, a->out_color, b.blend, b.blend_ok, s),
+               "render");
+  }
+  return OMNIGS_OK;
+}
+
 // Shared by forward (render = true) and recompute (render = false, image
 // buffer given and kept).
 int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer, omnigs_buffer binning_buffer,
@@ -293,17 +368,32 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
                "scan");
   }
 
-  // The instance count sizes the binning buffer, so the host needs it before
-  // the rest of the forward can be queued.  It is copied into a pinned word
-  // and the stream is polled rather than waited on: the GPU idles until the
-  // next launch, and a blocking wait adds the thread's wake-up to that.
-  // The scan leaves {total, 32-bit overflow flag} after its block partials.
+  // The instance count R sizes the binning buffer, and the host needs it to
+  // return it.  Its read-back ({R, overflow flag} after the scan's block
+  // partials) is queued first; the binning and the render are then queued
+  // against the capacity bucket of the previous call's R (the kernels read
+  // the live count on the device and stop there), and only then does the host
+  // wait for R -- so the GPU has the rest of the forward to run meanwhile.  A
+  // call whose R falls in another bucket (and the first call on a thread)
+  // bins again with the exact bucket: the backward carves the buffer from R.
   const uint32_t* total = g.scan_partials + scan_blocks((size_t)P);
-  uint32_t Rw[2] = {0u, 0u};
-  if (uint32_t* pw = pinned_word()) {
+  uint32_t* pw = pinned_word();
+  hipEvent_t ev = readback_event();
+  const bool async = pw && ev;
+  if (async) {
     OMNIGS_TRY(hipMemcpyAsync(pw, total, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
+    OMNIGS_TRY(hipEventRecord(ev, s), "read R event");
+  }
+  thread_local uint32_t last_R = 0;
+  const uint32_t guess = async ? bin_capacity(last_R) : 0u;
+  if (guess > 0) {
+    const int st = bin_and_render(a, binning_buffer, g, img, grid, guess, total, s, render);
+    if (st != OMNIGS_OK) return st;
+  }
+  uint32_t Rw[2] = {0u, 0u};
+  if (async) {
     hipError_t q;
-    while ((q = hipStreamQuery(s)) == hipErrorNotReady) {
+    while ((q = hipEventQuery(ev)) == hipErrorNotReady) {
     }
     OMNIGS_TRY(q, "sync R");
     Rw[0] = pw[0];
@@ -314,41 +404,12 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   }
   if (Rw[1] != 0u || Rw[0] > 0x7fffffffu) return fail_arg("instance count exceeds int range");
   const uint32_t R = Rw[0];
-
-  char* bbuf = grow(binning_buffer, carve_bin(nullptr, R).bytes);
-  if (!bbuf) {
-    std::snprintf(g_last_error, sizeof(g_last_error), "binning buffer allocation failed");
-    return OMNIGS_ERR_ALLOC;
+  if (bin_capacity(R) != guess) {
+    const int st = bin_and_render(a, binning_buffer, g, img, grid, R, nullptr, s, render);
+    if (st != OMNIGS_OK) return st;
   }
-  const BinState b = carve_bin(bbuf, R);
-  const int tile_bits = bit_width_u32((uint32_t)grid.tiles());
-  {
-    // the expansion also counts the tile sort's first digit
-    StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
-    OMNIGS_TRY(launch_duplicate(P, R, ds.vals, g.offsets, g.rect_depth, grid, b.tile_key, b.gid,
-                                reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, img.ranges,
-                                b.blend_ok, s),
-               "duplicate");
-  }
-  SortResult ts{b.tile_key, b.gid};
-  if (R > 0) {
-    StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
-    OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, R, tile_bits, b.sort_counts,
-                                 b.scan_partials, s, &ts, nullptr, true),
-               "tile sort");
-  }
-  {
-    StageTimer tm(OMNIGS_STAGE_RANGES, s);
-    OMNIGS_TRY(launch_tile_ranges(ts.keys, R, img.ranges, grid.tiles(), s, nullptr, R > 0), "ranges");
-  }
+  last_R = R;
   *num_rendered = (int)R;
-  if (!render) return OMNIGS_OK;
-  {
-    StageTimer tm(OMNIGS_STAGE_RENDER, s);
-    OMNIGS_TRY(launch_render_forward(grid, img.ranges, ts.vals, W, H, g.rec, img.final_T, img.n_contrib,
-                                     a->background, a->out_color, b.blend, b.blend_ok, s),
-               "render");
-  }
   return OMNIGS_OK;
 }
 
@@ -399,7 +460,7 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   const uint8_t* blend = nullptr;
   const uint32_t* blend_ok = nullptr;
   if (R > 0) {
-    const BinState b = carve_bin(a->binning_buffer, R);
+    const BinState b = carve_bin(a->binning_buffer, bin_capacity((uint32_t)R));
     const int passes = radix_passes(bit_width_u32((uint32_t)grid.tiles()));
     point_list = (passes % 2) ? b.gid_alt : b.gid;
     grad_rec = b.grad_rec;
@@ -680,6 +741,7 @@ int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* a, omnigs_lonlat_b
   const bool alt = radix_passes(bit_width_u32(vt.tile_base[vt.V])) % 2;
   v.point_list = alt ? bs.b.gid_alt : bs.b.gid;
   v.tile_keys = alt ? bs.b.tile_key_alt : bs.b.tile_key;
+  v.blend = bs.b.blend;
   v.final_T = bs.final_T;
   v.n_contrib = bs.n_contrib;
   v.ranges = reinterpret_cast<const uint32_t*>(bs.ranges);

@@ -317,6 +317,22 @@ __device__ __forceinline__ void store_rec(float4* __restrict__ rec, uint32_t g, 
   rec[3 * g + 2] = make_float4(rgb.z, __int_as_float(radius), __uint_as_float(slot), 0.f);
 }
 
+// Instance capacity of a single-view binning buffer: the instance count R
+// rounded up to a multiple of 2^(bit_width(R) - 5) (at least 1024), a bucket
+// of at most about 3%.  A function of R alone, so that the backward carves
+// the forward's layout from the R the forward returned, while the forward can
+// carve and queue its binning against the previous call's bucket before it
+// reads R back.
+inline uint32_t bin_capacity(uint32_t R) {
+  if (R == 0) return 0;
+  int bw = 0;
+  for (uint32_t v = R; v; v >>= 1) ++bw;
+  const int sh = bw - 5 > 10 ? bw - 5 : 10;
+  const uint64_t g = 1ull << sh;
+  const uint64_t c = ((uint64_t)R + g - 1) / g * g;
+  return c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
+}
+
 inline BinState carve_bin(char* base, size_t R) {
   BinState b;
   Carver<char> c(base);

@@ -98,8 +98,9 @@ uint32_t radix_first_mask(int bits);
 // counts (optional): receives the first-pass count table of a key_bits-bit
 // tile sort of the expanded keys (launch_radix_sort's first_counted).
 // ranges: cleared for launch_tile_ranges (its `zeroed`).  blend_ok: cleared
-// (BinState::blend).
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
+// (BinState::blend).  R_dev (optional): the live count {R, overflow flag} on
+// the device; R is then the capacity the grid covers.
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* R_dev, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
                             uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok,
                             hipStream_t s);

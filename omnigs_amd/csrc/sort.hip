@@ -613,14 +613,14 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   }
 }
 
-hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* order, const uint32_t* offsets,
+hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* R_dev, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
                             uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok,
                             hipStream_t s) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
-  hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, (const uint32_t*)nullptr,
+  hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, R_dev,
                      (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record, counts,
                      radix_first_mask(key_bits), (uint32_t)sort_blocks(R), reinterpret_cast<uint32_t*>(ranges),
                      (uint32_t)(2 * grid.tiles()), blend_ok);
