@@ -278,13 +278,19 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
         g = capi.backward_scene(ds, R, radii, bufs, dL, mlp=mlp, mlp_workspace=ws)
         return R, radii, g
 
-    median_ms, ms_per_step, stages, _, _, _ = time_steps(step, args.warmup, args.steps, dev)
+    median_ms, ms_per_step, stages, _, _, (R_dec, _, _) = time_steps(step, args.warmup, args.steps, dev)
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items() if v[1]}
+    # compulsory traffic with the decoder: the 192 B of SH per Gaussian become 32 B of BF16
+    # latents in the forward read, the backward read and the backward gradient write
+    alg_dec = 368 * P + 124 * int(R_dec) + 40 * W * H
     # training pass FLOPs per Gaussian: forward, backward chain and weight gradients, 5888 MACs each
     flop_per_pass = 3 * 2 * 5888 * P
     dec_ms = stage_ms.get("decoder_backward", 0.0)
     return {
         "ms_per_iter": round(median_ms, 4),
+        "roofline": {"bound": "hbm", "algorithmic_bytes": alg_dec,
+                     "achieved_GBps": round(alg_dec / (median_ms * 1e-3) / 1e9, 1),
+                     "frac": round(alg_dec / (median_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "ms_per_step_wall": round(ms_per_step, 4),
         "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "distill": {"passes": args.distill_steps, "ms_total": round(distill_ms, 1),
@@ -304,14 +310,19 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
 
 
 def random_views(rng, V, sizes_choice, cube: float):
-    """V seeded equirectangular views: size drawn from sizes_choice, random
-    rotation, centre uniform in a cube of side `cube` m around the origin."""
+    """V seeded equirectangular views: sizes drawn from sizes_choice in equal
+    shares (each size V / len(sizes_choice) times, rounded, in a seeded
+    shuffled order -- so that batches of different V hold the same mix and
+    their views/s compare), random rotation, centre uniform in a cube of side
+    `cube` m around the origin."""
     import numpy as np
 
     from omnigs_amd import scenes
     sizes, vms, cps = [], [], []
-    for _ in range(V):
-        sizes.append(tuple(sizes_choice[rng.integers(len(sizes_choice))]))
+    draw = [sizes_choice[i % len(sizes_choice)] for i in range(V)]
+    order = rng.permutation(V)
+    for i in range(V):
+        sizes.append(tuple(draw[order[i]]))
         R = random_rotation(rng)
         c = rng.uniform(-cube / 2, cube / 2, size=3)
         vms.append(scenes.viewmatrix_from_pose(R, -R @ c))
@@ -344,8 +355,17 @@ def batched_decoder_leg(sc, ds, dec, args, dev) -> dict:
         g = capture(make_step(rs, ts, lfs, dls), warmup=args.warmup)
         ms = time_graph(g, args.steps)
         assert not any(r.overflowed() for r in rs)
+        inst = sum(r.needed()[0] for r in rs)
+        # the compulsory-traffic model of one view (848 P + 124 R + 40 N bytes with SH,
+        # 368 P + ... with the decoder's 16 BF16 latents), charged per view as if each view
+        # were its own call, against HBM peak
+        per_g = 848 if mlp is None else 368
+        alg = sum(per_g * P + 40 * w * h for w, h in sizes) + 124 * inst
         out[name] = {"ms_per_step": round(ms, 3), "ms_per_view": round(ms / V, 4),
-                     "views_per_sec": round(1000.0 * V / ms, 1), "instances": rs[0].needed()[0]}
+                     "views_per_sec": round(1000.0 * V / ms, 1), "instances": inst,
+                     "roofline_per_view_model": {"bound": "hbm", "algorithmic_bytes": alg,
+                                                 "achieved_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+                                                 "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
         del g, rs, ts, lfs, dls
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
@@ -381,9 +401,11 @@ def batch_workload(dev, args) -> int:
         res = autotune(P, sizes, inputs, t(vms), t(cps), target, device=dev, reps=max(3, args.steps // 4),
                        views_per_launch=tuple(v for v in (1, 2, 4, 8, 16, 32) if v <= V),
                        log=lambda r: print(json.dumps(r), file=sys.stderr, flush=True))
+        pix = sum(w * h for w, h in sizes)
         rows[str(V)] = {"best_tile": f"{res.tile[0]}x{res.tile[1]}", "best_views_per_launch": res.views_per_launch,
                         "views_per_sec": round(res.views_per_sec, 1), "ms_per_view": round(res.ms_per_view, 4),
-                        "pixels": sum(w * h for w, h in sizes), "table": res.table}
+                        "pixels": pix, "gigapixels_per_sec": round(pix / (res.ms_per_view * V * 1e-3) / 1e9, 3),
+                        "table": res.table}
     print(json.dumps({"metric": "batched fwd+bwd views/sec (autotuned)", "unit": "views/s",
                       "config": {"workload": "1.5M Gaussians, uneven equirectangular views, L1 step per launch",
                                  "gaussians": P, "sizes_from": ["960x480", "1920x960", "2048x1024"]},
