@@ -68,6 +68,8 @@ def main() -> int:
                          "autotuned tile shape and views per launch (views/sec)")
     ap.add_argument("--memory-compare", action="store_true",
                     help="only the checkpointed-vs-stored backward comparison (3M Gaussians at 4096x2048)")
+    ap.add_argument("--memory-modes", default="stored,checkpointed,checkpointed_one_band",
+                    help="with --memory-compare: the modes to run (comma-separated)")
     ap.add_argument("--distill-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (known HBM bytes) to scale rocprofv3 FETCH/WRITE_SIZE")
@@ -438,14 +440,18 @@ def memory_compare(dev, args) -> int:
     }
     vm, cp, bg, target = t(sc["viewmatrix"]), t(sc["campos"]), t(sc["background"]), t(sc["target"])
     res = {}
-    for mode in ("stored", "checkpointed"):
-        ck = mode == "checkpointed"
+    # checkpointed: the backward bins in bands of tile rows (the library's
+    # default budget, R / 8 instances here); one_band: the same with a budget
+    # above R, i.e. the whole binning at once
+    modes = (("stored", False, 0), ("checkpointed", True, 0), ("checkpointed_one_band", True, 2**31 - 1))
+    wanted = set(args.memory_modes.split(","))
+    for mode, ck, budget in (m for m in modes if m[0] in wanted):
 
         def it():
             for p in params.values():
                 p.grad = None
             base = torch.cuda.memory_allocated(dev)
-            img, _, _, _ = rasterizer.render_lonlat(params, vm, cp, H, W, bg, checkpoint=ck)
+            img, _, _, _ = rasterizer.render_lonlat(params, vm, cp, H, W, bg, checkpoint=ck, band_instances=budget)
             held = torch.cuda.memory_allocated(dev) - base
             (img - target).abs().mean().backward()
             return held

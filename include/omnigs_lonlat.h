@@ -157,6 +157,36 @@ int omnigs_lonlat_recompute(const omnigs_lonlat_forward_args* args, omnigs_buffe
  * SH and scale/rotation terms in one kernel). */
 int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* args, void* stream);
 
+/* Checkpointed backward in tile-row bands, for a forward whose geometry and
+ * binning buffers were released: re-runs preprocess and the depth sort into
+ * a fresh geometry buffer, then bins and runs the render backward one band of
+ * whole tile rows at a time -- at most `band_instances` instances per band (0:
+ * max(2^20, R / 8); a row holding more is a band of its own) -- adding each
+ * band's per-Gaussian sums to the 2D gradient outputs, and ends with the
+ * preprocess backward.  The binning buffer is grown to the largest band only,
+ * so the gradient records (36 of the 53 bytes per instance) never exist for
+ * all R instances at once.
+ *   fwd: the forward's arguments (out_color unused; radii, if given, is
+ *        rewritten with the forward's values).
+ *   bwd: as for omnigs_lonlat_backward with R = the forward's R and
+ *        image_buffer = the forward's kept image buffer (its per-pixel state
+ *        is read; its per-tile ranges are overwritten, so afterwards it
+ *        serves further checkpointed backwards only); geom_buffer and
+ *        binning_buffer are unused (the callbacks grow fresh ones).
+ * The outputs equal omnigs_lonlat_backward's up to the order in which a
+ * Gaussian's per-tile terms are summed across bands.  The recomputed R must
+ * equal bwd->R (else OMNIGS_ERR_INVALID_ARGUMENT).  Synchronises `stream`
+ * once, to read the per-row instance counts.  *num_bands (optional): bands
+ * run.  Has no counterpart in the reference, whose autograd wrapper keeps
+ * all three buffers from the forward to the backward
+ * (src/gaussian_rasterizer.cpp:85-98); it serves memory-bound training
+ * callers, for whom the peak HBM of the backward is what bounds the image
+ * size. */
+int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* fwd,
+                                        const omnigs_lonlat_backward_args* bwd, omnigs_buffer geometry_buffer,
+                                        omnigs_buffer binning_buffer, unsigned int band_instances, void* stream,
+                                        int* num_bands);
+
 /* markVisible for camera_type 3: every Gaussian is visible (present[i] = 1). */
 int omnigs_lonlat_mark_visible(int P, uint8_t* present, void* stream);
 

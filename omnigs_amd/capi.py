@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
     "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
     "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr", "omnigs_lonlat_batch_state",
-    "omnigs_photometric_loss_noalloc",
+    "omnigs_photometric_loss_noalloc", "omnigs_lonlat_backward_checkpointed",
 )
 NUM_STAGES = 11
 
@@ -146,6 +146,9 @@ def load() -> ctypes.CDLL:
                                             ctypes.c_void_p, _ip]
     lib.omnigs_lonlat_recompute.restype = ctypes.c_int
     lib.omnigs_lonlat_backward.argtypes = [ctypes.POINTER(BackwardArgs), ctypes.c_void_p]
+    lib.omnigs_lonlat_backward_checkpointed.argtypes = [ctypes.POINTER(ForwardArgs), ctypes.POINTER(BackwardArgs),
+                                                        Buffer, Buffer, ctypes.c_uint, ctypes.c_void_p, _ip]
+    lib.omnigs_lonlat_backward_checkpointed.restype = ctypes.c_int
     lib.omnigs_lonlat_backward.restype = ctypes.c_int
     lib.omnigs_lonlat_mark_visible.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     lib.omnigs_lonlat_mark_visible.restype = ctypes.c_int
@@ -282,9 +285,14 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
 
 def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buffers, shs=None,
              colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0,
-             want_jacobian=False, seam_wrap=False, mlp=None, mlp_workspace=None):
+             want_jacobian=False, seam_wrap=False, mlp=None, mlp_workspace=None, checkpointed=None):
     """Runs omnigs_lonlat_backward; returns a dict of gradient tensors (plus
-    dL_dlatent [P,16] and dL_dmlp [param_count] when mlp = (packed, latents))."""
+    dL_dlatent [P,16] and dL_dmlp [param_count] when mlp = (packed, latents)).
+    checkpointed: {"opacities": [P] (activated), "band_instances": budget (0 =
+    the library's default)} runs omnigs_lonlat_backward_checkpointed instead,
+    which needs only the forward's image buffer (buffers[2]); the dict then
+    also holds "_bands" (bands run) and "_buffers" (the geometry and binning
+    buffers it grew)."""
     lib = load()
     dev = means3D.device
     P = means3D.shape[0]
@@ -312,10 +320,11 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
         if mlp_workspace is None or mlp_workspace.numel() < nbytes:
             mlp_workspace = torch.empty((nbytes,), dtype=torch.uint8, device=dev)
     geom, binning, image = buffers
+    geom_ptr = geom.tensor.data_ptr() if geom is not None else None
+    bin_ptr = binning.tensor.data_ptr() if binning is not None and binning.tensor.numel() else None
     a = BackwardArgs(P, degree, M, R, W, H, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
                      _ptr(scales), float(scale_modifier), _ptr(rotations), _ptr(cov3D_precomp), _ptr(viewmatrix),
-                     _ptr(campos), _ptr(radii), geom.tensor.data_ptr(),
-                     binning.tensor.data_ptr() if binning.tensor.numel() else None, image.tensor.data_ptr(),
+                     _ptr(campos), _ptr(radii), geom_ptr, bin_ptr, image.tensor.data_ptr(),
                      _ptr(dL_dpix.contiguous()), _ptr(g["dL_dmeans2D"]), _ptr(g["dL_dconic"]),
                      _ptr(g["dL_dopacity"]), _ptr(g["dL_dcolors"]), _ptr(g["dL_dmeans3D"]), _ptr(g["dL_dcov3D"]),
                      _ptr(g["dL_dsh"]) if shs is not None else None,
@@ -325,6 +334,21 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
                      _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None, _ptr(g.get("dL_dlatent")),
                      _ptr(g.get("dL_dmlp")), mlp_workspace.data_ptr() if mlp else None,
                      mlp_workspace.numel() if mlp else 0)
+    if checkpointed is not None:
+        fa = ForwardArgs(P, degree, M, W, H, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
+                         _ptr(checkpointed["opacities"]), _ptr(scales), float(scale_modifier), _ptr(rotations),
+                         _ptr(cov3D_precomp), _ptr(viewmatrix), _ptr(campos), 0, None, _ptr(radii),
+                         int(bool(seam_wrap)), _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None)
+        nb = ctypes.c_int(0)
+        fresh = (TorchBuffer(dev), TorchBuffer(dev))
+        st = lib.omnigs_lonlat_backward_checkpointed(ctypes.byref(fa), ctypes.byref(a), fresh[0].c, fresh[1].c,
+                                                     int(checkpointed.get("band_instances", 0)), stream_handle(dev),
+                                                     ctypes.byref(nb))
+        if st != OMNIGS_OK:
+            raise OmnigsError(st, "omnigs_lonlat_backward_checkpointed")
+        g["_bands"] = nb.value
+        g["_buffers"] = fresh
+        return g
     st = lib.omnigs_lonlat_backward(ctypes.byref(a), stream_handle(dev))
     if st != OMNIGS_OK:
         raise OmnigsError(st, "omnigs_lonlat_backward")

@@ -3,6 +3,7 @@
 // -> render) and the backward (render backward -> fused preprocess backward).
 // Stage order and buffer contract follow the reference's
 // LonlatRasterizer::forward / backward (rasterizer_impl.cu:540-697, :701-795).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -306,6 +307,48 @@ int bin_and_render(const omnigs_lonlat_forward_args* a, omnigs_buffer binning_bu
   return OMNIGS_OK;
 }
 
+// Checks a forward's arguments (P > 0; out_color only when render) and
+// returns its tile grid.
+int check_forward_args(const omnigs_lonlat_forward_args* a, bool render, TileGrid* grid) {
+  const int W = a->width, H = a->height;
+  if (a->P > kMaxGaussians) return fail_arg("more than 89478485 Gaussians in one call");
+  if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || (render && !a->out_color))
+    return fail_arg("missing required pointer");
+  const bool mlp = a->mlp_packed && a->latents;
+  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
+  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  if (!mlp && !a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
+    return fail_arg("SH degree / coefficient count");
+  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
+  *grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
+  if (grid->gx > kMaxGridCols || grid->gy > 65535) return fail_arg("tile grid over 32766 columns or 65535 rows");
+  if (grid->tiles() > kMaxTiles) return fail_arg("more than 2^21 tiles in one image");
+  if (a->seam_wrap && W % grid->bx != 0) return fail_arg(kWrapWidthMsg);
+  return OMNIGS_OK;
+}
+
+// Preprocess, then the Gaussians by (depth bits, id) into g.order: a 32-bit
+// stable radix sort with index values (an even pass count ends in g.order).
+int preprocess_and_sort(const omnigs_lonlat_forward_args* a, const GeomState& g, int* radii, hipStream_t s) {
+  const bool mlp = a->mlp_packed && a->latents;
+  PreprocessArgs pa{a->P, a->D, a->M, a->width, a->height, a->means3D, a->shs, a->colors_precomp, a->opacities,
+                    a->scales, a->scale_modifier, a->rotations, a->cov3D_precomp, a->viewmatrix, a->campos,
+                    a->seam_wrap ? 1 : 0, mlp ? a->mlp_packed : nullptr, mlp ? a->latents : nullptr};
+  {
+    StageTimer tm(OMNIGS_STAGE_PREPROCESS, s);
+    OMNIGS_TRY(launch_preprocess(pa, g, radii, s), "preprocess");
+  }
+  static_assert(32 % (2 * kRadixBits) == 0, "the depth sort must end in g.order");
+  SortResult ds;
+  {
+    StageTimer tm(OMNIGS_STAGE_DEPTH_SORT, s);
+    OMNIGS_TRY(launch_radix_sort(g.depth_key, g.key_alt, nullptr, g.order, g.order_alt, (size_t)a->P, 32,
+                                 g.sort_counts, g.scan_partials, s, &ds),
+               "depth sort");
+  }
+  return OMNIGS_OK;
+}
+
 // Shared by forward (render = true) and recompute (render = false, image
 // buffer given and kept).
 int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer, omnigs_buffer binning_buffer,
@@ -314,22 +357,14 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   *num_rendered = 0;
   const int P = a->P, W = a->width, H = a->height;
   if (P < 0 || W <= 0 || H <= 0) return fail_arg("P < 0 or empty image");
-  if (P > kMaxGaussians) return fail_arg("more than 89478485 Gaussians in one call");
   if (P == 0) return OMNIGS_OK;  // the reference boundary skips the rasterizer entirely
-  if (!a->means3D || !a->opacities || !a->viewmatrix || !a->campos || !a->background || (render && !a->out_color))
-    return fail_arg("missing required pointer");
   if (!render && !kept_image) return fail_arg("recompute needs the forward's image buffer");
-  const bool mlp = a->mlp_packed && a->latents;
-  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
-  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
-  if (!mlp && !a->colors_precomp && (a->D < 0 || a->D > 3 || a->M < (a->D + 1) * (a->D + 1)))
-    return fail_arg("SH degree / coefficient count");
-  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
+  TileGrid grid;
+  {
+    const int st = check_forward_args(a, render, &grid);
+    if (st != OMNIGS_OK) return st;
+  }
   hipStream_t s = (hipStream_t)stream_;
-  const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
-  if (grid.gx > kMaxGridCols || grid.gy > 65535) return fail_arg("tile grid over 32766 columns or 65535 rows");
-  if (grid.tiles() > kMaxTiles) return fail_arg("more than 2^21 tiles in one image");
-  if (a->seam_wrap && W % grid.bx != 0) return fail_arg(kWrapWidthMsg);
 
   char* gbuf = grow(geometry_buffer, carve_geom(nullptr, P).bytes);
   char* ibuf = render ? grow(image_buffer, carve_img(nullptr, W, H).bytes) : kept_image;
@@ -340,22 +375,9 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   const GeomState g = carve_geom(gbuf, P);
   const ImgState img = carve_img(ibuf, W, H);
   int* radii = a->radii ? a->radii : g.radii;
-
-  PreprocessArgs pa{P, a->D, a->M, W, H, a->means3D, a->shs, a->colors_precomp, a->opacities, a->scales,
-                    a->scale_modifier, a->rotations, a->cov3D_precomp, a->viewmatrix, a->campos,
-                    a->seam_wrap ? 1 : 0, mlp ? a->mlp_packed : nullptr, mlp ? a->latents : nullptr};
   {
-    StageTimer tm(OMNIGS_STAGE_PREPROCESS, s);
-    OMNIGS_TRY(launch_preprocess(pa, g, radii, s), "preprocess");
-  }
-
-  // Gaussians by (depth bits, id): 32-bit stable radix sort with index values.
-  SortResult ds;
-  {
-    StageTimer tm(OMNIGS_STAGE_DEPTH_SORT, s);
-    OMNIGS_TRY(launch_radix_sort(g.depth_key, g.key_alt, nullptr, g.order, g.order_alt, (size_t)P, 32,
-                                 g.sort_counts, g.scan_partials, s, &ds),
-               "depth sort");
+    const int st = preprocess_and_sort(a, g, radii, s);
+    if (st != OMNIGS_OK) return st;
   }
   // Instance offsets in depth order.
   {
@@ -363,7 +385,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     // key_alt is free after the depth sort: it holds tiles_touched in depth order.
     // The same launches scan the preprocess blocks' slot counts into the first
     // gradient slot of every block (the backward's records).
-    OMNIGS_TRY(launch_scan(g.tiles_touched, ds.vals, g.offsets, (size_t)P, false, g.scan_partials, s, g.key_alt,
+    OMNIGS_TRY(launch_scan(g.tiles_touched, g.order, g.offsets, (size_t)P, false, g.scan_partials, s, g.key_alt,
                            g.block_slots, slot_blocks(P), g.slot_base, g.dup_rect, g.rect_depth),
                "scan");
   }
@@ -413,6 +435,55 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   return OMNIGS_OK;
 }
 
+// Checks a backward's arguments; buffers: the forward's three buffers must
+// be given (else only the image buffer).
+int check_backward_args(const omnigs_lonlat_backward_args* a, bool buffers) {
+  const int P = a->P, W = a->width, H = a->height, R = a->R;
+  if (P < 0 || R < 0 || W <= 0 || H <= 0) return fail_arg("negative size or empty image");
+  if (P == 0) return OMNIGS_OK;
+  if (!a->means3D || !a->viewmatrix || !a->campos || !a->background || !a->dL_dpix || !a->image_buffer ||
+      (buffers && (!a->geom_buffer || (R > 0 && !a->binning_buffer))))
+    return fail_arg("missing required pointer");
+  if (!a->dL_dmean2D || !a->dL_dconic || !a->dL_dopacity || !a->dL_dcolor || !a->dL_dmean3D || !a->dL_dcov3D)
+    return fail_arg("missing gradient output");
+  const bool mlp = a->mlp_packed && a->latents;
+  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
+  if (mlp && (!a->dL_dlatent || !a->dL_dmlp || !a->mlp_workspace)) return fail_arg("decoder gradient output missing");
+  if (mlp && a->mlp_workspace_bytes < omnigs_mlp_workspace_bytes(P)) return fail_arg("mlp_workspace too small");
+  if (!mlp && a->shs && !a->dL_dsh) return fail_arg("dL_dsh is NULL while shs is given");
+  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
+  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
+  if (a->scales && (!a->dL_dscale || !a->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
+  const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
+  if (a->seam_wrap && W % grid.bx != 0) return fail_arg(kWrapWidthMsg);
+  return OMNIGS_OK;
+}
+
+// The preprocess backward (reading the per-Gaussian 2D gradients, or with
+// `gather` summing them from the records itself) and the decoder's backward.
+int backward_tail(const omnigs_lonlat_backward_args* a, const GeomState& g, const GatherSrc& gather, hipStream_t s) {
+  const int P = a->P;
+  const bool mlp = a->mlp_packed && a->latents;
+  const int* radii = a->radii ? a->radii : g.radii;
+  PreprocessBackwardArgs pb{P, a->D, a->M, a->width, a->height, a->means3D, mlp ? nullptr : a->shs, a->scales,
+                            a->scale_modifier, a->rotations,
+                            a->cov3D_precomp, a->viewmatrix, a->campos, radii, g.clamped, a->dL_dmean2D,
+                            a->dL_dconic, a->dL_dcolor, a->dL_dmean3D, a->dL_dcov3D, a->dL_dsh,
+                            a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt, gather,
+                            a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor};
+  {
+    StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
+    OMNIGS_TRY(launch_preprocess_backward(pb, s), "preprocess backward");
+  }
+  if (mlp) {
+    StageTimer tm(OMNIGS_STAGE_DECODER_BACKWARD, s);
+    OMNIGS_TRY(launch_mlp_backward(P, a->means3D, a->campos, radii, a->latents, a->mlp_packed, a->dL_dcolor,
+                                   a->dL_dmean3D, a->dL_dlatent, a->dL_dmlp, static_cast<float*>(a->mlp_workspace), s),
+               "decoder backward");
+  }
+  return OMNIGS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -430,28 +501,15 @@ int omnigs_lonlat_recompute(const omnigs_lonlat_forward_args* a, omnigs_buffer g
 
 int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) {
   if (!a) return fail_arg("NULL args");
+  {
+    const int st = check_backward_args(a, true);
+    if (st != OMNIGS_OK || a->P == 0) return st;
+  }
   const int P = a->P, W = a->width, H = a->height, R = a->R;
-  if (P < 0 || R < 0 || W <= 0 || H <= 0) return fail_arg("negative size or empty image");
-  if (P == 0) return OMNIGS_OK;
-  if (!a->means3D || !a->viewmatrix || !a->campos || !a->background || !a->dL_dpix || !a->geom_buffer ||
-      !a->image_buffer || (R > 0 && !a->binning_buffer))
-    return fail_arg("missing required pointer");
-  if (!a->dL_dmean2D || !a->dL_dconic || !a->dL_dopacity || !a->dL_dcolor || !a->dL_dmean3D || !a->dL_dcov3D)
-    return fail_arg("missing gradient output");
-  const bool mlp = a->mlp_packed && a->latents;
-  if (!mlp && (a->mlp_packed || a->latents)) return fail_arg("mlp_packed and latents must be given together");
-  if (mlp && (!a->dL_dlatent || !a->dL_dmlp || !a->mlp_workspace)) return fail_arg("decoder gradient output missing");
-  if (mlp && a->mlp_workspace_bytes < omnigs_mlp_workspace_bytes(P)) return fail_arg("mlp_workspace too small");
-  if (!mlp && a->shs && !a->dL_dsh) return fail_arg("dL_dsh is NULL while shs is given");
-  if (!mlp && !a->colors_precomp && !a->shs) return fail_arg("neither shs nor colors_precomp");
-  if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
-  if (a->scales && (!a->dL_dscale || !a->dL_drot)) return fail_arg("dL_dscale / dL_drot is NULL");
   hipStream_t s = (hipStream_t)stream_;
   const TileGrid grid = tile_grid(W, H, a->seam_wrap ? 1 : 0);
-  if (a->seam_wrap && W % grid.bx != 0) return fail_arg(kWrapWidthMsg);
   const GeomState g = carve_geom(a->geom_buffer, P);
   const ImgState img = carve_img(a->image_buffer, W, H);
-  const int* radii = a->radii ? a->radii : g.radii;
 
   // The point list ends in gid (even pass count) or gid_alt (odd).
   const uint32_t* point_list = nullptr;
@@ -479,25 +537,166 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   // The per-Gaussian sums of the records (the four 2D gradient outputs, every
   // entry written, zeros where no tile touched the Gaussian) are gathered
   // inside the preprocess backward, which uses them from registers.
-  PreprocessBackwardArgs pb{P, a->D, a->M, W, H, a->means3D, mlp ? nullptr : a->shs, a->scales, a->scale_modifier,
-                            a->rotations,
-                            a->cov3D_precomp, a->viewmatrix, a->campos, radii, g.clamped, a->dL_dmean2D,
-                            a->dL_dconic, a->dL_dcolor, a->dL_dmean3D, a->dL_dcov3D, a->dL_dsh,
-                            a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt,
-                            GatherSrc{P, (uint32_t)R, nullptr, g.tiles_touched, g.slot_local, g.slot_base, grad_rec,
-                                      grad_opacity},
-                            a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor};
+  return backward_tail(
+      a, g, GatherSrc{P, (uint32_t)R, nullptr, g.tiles_touched, g.slot_local, g.slot_base, grad_rec, grad_opacity},
+      s);
+}
+
+int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, const omnigs_lonlat_backward_args* a,
+                                        omnigs_buffer geometry_buffer, omnigs_buffer binning_buffer,
+                                        unsigned int band_instances, void* stream_, int* num_bands) {
+  if (!f || !a) return fail_arg("NULL args");
+  if (num_bands) *num_bands = 0;
+  {
+    const int st = check_backward_args(a, false);
+    if (st != OMNIGS_OK || a->P == 0) return st;
+  }
+  if (f->P != a->P || f->width != a->width || f->height != a->height || (f->seam_wrap != 0) != (a->seam_wrap != 0))
+    return fail_arg("forward and backward arguments describe different calls");
+  TileGrid grid;
+  {
+    const int st = check_forward_args(f, false, &grid);
+    if (st != OMNIGS_OK) return st;
+  }
+  const int P = a->P, W = a->width, H = a->height;
+  hipStream_t s = (hipStream_t)stream_;
+
+  // The geometry buffer, then a band's compacted owner rects and the per-row
+  // instance counts' difference array.  The band's compacted owner ids and
+  // offsets reuse order_alt and depth_key, free after the depth sort.
+  const size_t geom_bytes = carve_geom(nullptr, P).bytes;
+  const size_t rect_bytes = align_up((size_t)P * sizeof(uint2));
+  const size_t part_bytes = align_up((scan_blocks((size_t)P) + 2) * sizeof(uint32_t));
+  char* gbuf = grow(geometry_buffer,
+                    geom_bytes + rect_bytes + part_bytes + align_up(((size_t)grid.gy + 1) * sizeof(uint32_t)));
+  if (!gbuf) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "geometry buffer allocation failed");
+    return OMNIGS_ERR_ALLOC;
+  }
+  const GeomState g = carve_geom(gbuf, P);
+  // the whole rects in depth order (once), and per band its owners' ids,
+  // inclusive offsets and rects (free arrays after the depth sort, and the
+  // tail), with the per-scan-tile partials of its instances and owners
+  uint2* rect_d = g.rect_depth;
+  uint32_t* order_c = g.order_alt;
+  uint32_t* offsets_c = g.offsets;
+  uint2* rect_c = reinterpret_cast<uint2*>(gbuf + geom_bytes);
+  uint32_t* part_area = g.scan_partials;
+  uint32_t* part_count = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes);
+  uint32_t* row_diff = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes + part_bytes);
+  const uint32_t* band_total = part_area + scan_blocks((size_t)P);    // {R_b, overflow flag}
+  const uint32_t* band_owners = part_count + scan_blocks((size_t)P);  // owners with an instance
+  const ImgState img = carve_img(a->image_buffer, W, H);
+  {
+    const int st = preprocess_and_sort(f, g, f->radii ? f->radii : g.radii, s);
+    if (st != OMNIGS_OK) return st;
+  }
+  std::vector<uint32_t> diff((size_t)grid.gy + 1);
+  {
+    StageTimer tm(OMNIGS_STAGE_SCAN, s);
+    OMNIGS_TRY(launch_row_counts(P, g.dup_rect, grid.gy, row_diff, s), "row counts");
+    OMNIGS_TRY(launch_gather_rects(g.dup_rect, g.order, P, rect_d, s), "rects in depth order");
+  }
+  OMNIGS_TRY(hipMemcpyAsync(diff.data(), row_diff, diff.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+             "read row counts");
+  OMNIGS_TRY(hipStreamSynchronize(s), "sync row counts");
+
+  // Bands of whole tile rows, each of at most `budget` instances (a single
+  // row above it makes a band of its own).
+  std::vector<uint64_t> rows((size_t)grid.gy);
+  uint32_t run = 0;
+  uint64_t R = 0;
+  for (int y = 0; y < grid.gy; ++y) {
+    run += diff[(size_t)y];  // mod 2^32; each row holds at most R < 2^31 instances
+    rows[(size_t)y] = run;
+    R += run;
+  }
+  if (R != (uint64_t)a->R) return fail_arg("recomputed instance count differs from the forward's R");
+  const uint64_t budget = band_instances ? (uint64_t)band_instances
+                                         : std::max<uint64_t>((uint64_t)1 << 20, (R + 7) / 8);
+  struct Band {
+    int y0, y1;
+    uint32_t n;
+  };
+  std::vector<Band> bands;
+  uint64_t biggest = 0;
+  for (int y = 0; y < grid.gy;) {
+    const int y0 = y;
+    uint64_t n = rows[(size_t)y++];
+    while (y < grid.gy && n + rows[(size_t)y] <= budget) n += rows[(size_t)y++];
+    if (n > 0) bands.push_back({y0, y, (uint32_t)n});
+    biggest = std::max(biggest, n);
+  }
+  BinState b{};
+  if (biggest > 0) {
+    const uint32_t C = bin_capacity((uint32_t)biggest);
+    char* bbuf = grow(binning_buffer, carve_bin(nullptr, C).bytes);
+    if (!bbuf) {
+      std::snprintf(g_last_error, sizeof(g_last_error), "binning buffer allocation failed");
+      return OMNIGS_ERR_ALLOC;
+    }
+    b = carve_bin(bbuf, C);
+  }
+
+  // The 2D gradient outputs start at zero; every band adds its records' sums.
   {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
-    OMNIGS_TRY(launch_preprocess_backward(pb, s), "preprocess backward");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dmean2D), 3 * (size_t)P, 0u, s), "zero dmean2D");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dconic), 4 * (size_t)P, 0u, s), "zero dconic");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dopacity), (size_t)P, 0u, s), "zero dopacity");
+    OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dcolor), 3 * (size_t)P, 0u, s), "zero dcolor");
   }
-  if (mlp) {
-    StageTimer tm(OMNIGS_STAGE_DECODER_BACKWARD, s);
-    OMNIGS_TRY(launch_mlp_backward(P, a->means3D, a->campos, radii, a->latents, a->mlp_packed, a->dL_dcolor,
-                                   a->dL_dmean3D, a->dL_dlatent, a->dL_dmlp, static_cast<float*>(a->mlp_workspace), s),
-               "decoder backward");
+  // Per band: its owners (offsets, rects, and the first gradient slot of
+  // each, in owner order), expansion, tile sort and ranges (the lists of its
+  // tiles are the forward's: every Gaussian over a tile, in (depth, id)
+  // order), the render backward over its tiles, and the records' sums added
+  // to the 2D gradients.
+  const int tile_bits = bit_width_u32((uint32_t)grid.tiles());
+  for (const Band& band : bands) {
+    {
+      // the Gaussians outside the band (zero area, interleaved in depth
+      // order) are dropped from the expansion's owner list
+      StageTimer tm(OMNIGS_STAGE_SCAN, s);
+      OMNIGS_TRY(launch_band_owners(rect_d, g.order, (size_t)P, (uint32_t)band.y0, (uint32_t)band.y1, part_area,
+                                    part_count, order_c, offsets_c, rect_c, g.rec, s),
+                 "band owners");
+    }
+    {
+      // band.n (from the row counts) equals the owners' total; the kernel
+      // also stops at the device total
+      StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
+      OMNIGS_TRY(launch_duplicate(P, band.n, band_total, order_c, offsets_c, rect_c, grid, b.tile_key, b.gid,
+                                  reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, img.ranges,
+                                  b.blend_ok, s, band_owners),
+                 "band duplicate");
+    }
+    SortResult ts;
+    {
+      StageTimer tm(OMNIGS_STAGE_TILE_SORT, s);
+      OMNIGS_TRY(launch_radix_sort(b.tile_key, b.tile_key_alt, b.gid, b.gid, b.gid_alt, band.n, tile_bits,
+                                   b.sort_counts, b.scan_partials, s, &ts, nullptr, true),
+                 "band tile sort");
+    }
+    {
+      StageTimer tm(OMNIGS_STAGE_RANGES, s);
+      OMNIGS_TRY(launch_tile_ranges(ts.keys, band.n, img.ranges, grid.tiles(), s, nullptr, true), "band ranges");
+    }
+    {
+      StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
+      OMNIGS_TRY(launch_render_backward(grid, img.ranges, ts.vals, W, H, a->background, g.rec, img.final_T,
+                                        img.n_contrib, a->dL_dpix, nullptr, band.n, b.grad_rec, b.grad_opacity,
+                                        nullptr, nullptr, s, band.y0, band.y1),
+                 "band render backward");
+    }
+    {
+      StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
+      OMNIGS_TRY(launch_band_gather(P, g.dup_rect, (uint32_t)band.y0, (uint32_t)band.y1, g.rec, band.n, b.grad_rec,
+                                    b.grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor, s),
+                 "band gather");
+    }
   }
-  return OMNIGS_OK;
+  if (num_bands) *num_bands = (int)bands.size();
+  return backward_tail(a, g, GatherSrc{P, 0u, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -110,6 +110,15 @@ __device__ __forceinline__ uint32_t dup_rect_area(uint2 r) { return (r.x >> 16) 
 __device__ __forceinline__ uint2 dup_rect_xy(uint2 r) {
   return make_uint2((uint32_t)(int32_t)(int16_t)(r.x & 0xFFFFu), r.y);
 }
+// The record of the rect's rows inside the tile-row band [by0, by1) (same
+// columns), or the empty record when it has none there.  The band [0, 65535)
+// holds every rect whole (rows fit 16 bits).
+__device__ __forceinline__ uint2 band_dup_rect(uint2 r, uint32_t by0, uint32_t by1) {
+  const uint32_t y0 = r.y & 0xFFFFu, y1 = y0 + (r.x >> 16);
+  const uint32_t b0 = max(y0, by0), b1 = min(y1, by1);
+  if (b1 <= b0) return make_uint2(0u, 1u << 16);
+  return make_uint2((r.x & 0xFFFFu) | ((b1 - b0) << 16), b0 | (r.y & 0xFFFF0000u));
+}
 // Tile column of rect column x (identity in clamp mode).
 __device__ __forceinline__ int wrap_col(int x, TileGrid g) { return g.wrap ? ((x % g.gx) + g.gx) % g.gx : x; }
 

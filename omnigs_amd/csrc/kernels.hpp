@@ -76,6 +76,24 @@ hipError_t launch_scan(const uint32_t* in, const uint32_t* idx, uint32_t* out, s
                        uint32_t* partials, hipStream_t s, uint32_t* gather_buf = nullptr,
                        const uint32_t* side_in = nullptr, size_t side_n = 0, uint32_t* side_out = nullptr,
                        const uint2* rect_in = nullptr, uint2* rect_out = nullptr);
+// Per tile row y: (row_diff[0] + ... + row_diff[y]) mod 2^32 is the number of
+// instances in row y (the rect widths of the Gaussians whose rects span it);
+// row_diff has gy + 1 entries, cleared here.
+hipError_t launch_row_counts(int P, const uint2* dup_rect, int gy, uint32_t* row_diff, hipStream_t s);
+// rect_d[i] = dup_rect[order[i]] (the whole rects in depth order).
+hipError_t launch_gather_rects(const uint2* dup_rect, const uint32_t* order, int P, uint2* rect_d, hipStream_t s);
+// A band's expansion input: of the n depth-ordered rects rect_d, those with
+// rows in [by0, by1), in order -- their Gaussians (from order), inclusive
+// instance offsets and band rects {x0, y0 | w << 16} -> order_c / offsets_c /
+// rect_c.  part_area / part_count: scan_blocks(n) + 2 entries each; the
+// band's instance count and overflow flag land in part_area[scan_blocks(n)]
+// and [+ 1], its owner count in part_count[scan_blocks(n)].  The band's
+// gradient slots follow its owners: each owner's render record gets its first
+// slot (its exclusive offset) as slot word, for a render backward without
+// slot_base.
+hipError_t launch_band_owners(const uint2* rect_d, const uint32_t* order, size_t n, uint32_t by0, uint32_t by1,
+                              uint32_t* part_area, uint32_t* part_count, uint32_t* order_c, uint32_t* offsets_c,
+                              uint2* rect_c, float4* rec, hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned
@@ -99,11 +117,14 @@ uint32_t radix_first_mask(int bits);
 // tile sort of the expanded keys (launch_radix_sort's first_counted).
 // ranges: cleared for launch_tile_ranges (its `zeroed`).  blend_ok: cleared
 // (BinState::blend).  R_dev (optional): the live count {R, overflow flag} on
-// the device; R is then the capacity the grid covers.
+// the device; R is then the capacity the grid covers.  Every owner (entry of
+// order / offsets / rect_depth) must hold at least one instance, save for
+// owners past the last instance.  P_dev (optional): the owner count on the
+// device, P then being its bound.
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* R_dev, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
                             uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok,
-                            hipStream_t s);
+                            hipStream_t s, const uint32_t* P_dev = nullptr);
 // zeroed: ranges were cleared by the expansion (else they are cleared here).
 hipError_t launch_tile_ranges(const uint32_t* sorted_tile_keys, size_t R, uint2* ranges, int tiles,
                               hipStream_t s, const uint32_t* R_dev = nullptr, bool zeroed = false);
@@ -131,14 +152,25 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
 // view (items = V P); R_dev (optional): the live slot count.  blend (with
 // *blend_ok == 1): the forward's per-wave blend bits, which select the
 // (wave, instance) pairs to evaluate; else the quadrant masks do.
+// [band_y0, band_y1) (band_y1 < 0: to the last row): the tile rows rendered;
+// the lists and slots then hold the rect rows from band_y0 on.  slot_base
+// NULL: the record's slot word is the Gaussian's first slot.
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
-                                  float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s);
+                                  float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s,
+                                  int band_y0 = 0, int band_y1 = -1);
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
                                const uint32_t* tiles_touched, const uint32_t* slot_local, const uint32_t* slot_base,
                                const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
                                float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
+// One band's records added to the per-Gaussian sums (the four arrays in the
+// layouts launch_gather_grads writes): Gaussian i owns the slots from its
+// render record's slot word on (launch_band_owners writes it), as many as its
+// rect (dup_rect) has tiles in the rows [by0, by1).
+hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const float4* rec,
+                              uint32_t nslots, const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
+                              float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
 
 // Batched render over every tile of every view of vt; pixel arrays are the
 // views' slices laid out one after the other (colours [3, H_v, W_v] per view).

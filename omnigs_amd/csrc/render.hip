@@ -47,13 +47,14 @@ struct TileLoc {
   int W, H, gx, tile;
   size_t pix;
 };
+// tile0 (single view): the tile of block 0 (a grid over a band of tile rows).
 template <bool kMulti>
-__device__ __forceinline__ TileLoc locate(int W, int H, int gx, const ViewTable& vt) {
+__device__ __forceinline__ TileLoc locate(int W, int H, int gx, const ViewTable& vt, int tile0 = 0) {
   if constexpr (kMulti) {
     const int v = vt.view_of_tile(blockIdx.x);
     return {vt.W[v], vt.H[v], vt.gx[v], (int)(blockIdx.x - vt.tile_base[v]), (size_t)vt.pix_base[v]};
   } else {
-    return {W, H, gx, (int)blockIdx.x, 0};
+    return {W, H, gx, tile0 + (int)blockIdx.x, 0};
   }
 }
 
@@ -333,8 +334,10 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
     ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, const uint32_t* __restrict__ slot_base,
     uint32_t nslots, float4* __restrict__ grad_rec, float* __restrict__ grad_opacity,
-    const uint8_t* __restrict__ blend_in, const uint32_t* __restrict__ blend_ok) {
-  const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt);
+    const uint8_t* __restrict__ blend_in, const uint32_t* __restrict__ blend_ok, int band_y0) {
+  // band_y0 (single view): the grid covers the tile rows from band_y0 on, and
+  // the lists and gradient slots hold only the rect rows from band_y0 on
+  const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt, band_y0 * gx_);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
   const TileGrid tg{gx, (H + TY - 1) / TY, kWrap ? 1 : 0, TX, TY};
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
   pixel_of<TX, TY>(tile, gx, tid, px, py);
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
-  const uint2 range = ranges[blockIdx.x];
+  const uint2 range = ranges[kMulti ? (int)blockIdx.x : tile];
   const size_t N = (size_t)W * H;
   const size_t pid = (size_t)py * W + px;
   final_Ts += loc.pix;
@@ -493,9 +496,10 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
       // this instance's gradient slot: the Gaussian's first slot (its block's
       // base plus its offset in the block) plus the tile's position in its rect
       const float4 r2 = rec[3 * g + 2];
-      const Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
+      Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
+      rc.y0 = max(rc.y0, band_y0);
       const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
-      s_inst[tid] = slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z) +
+      s_inst[tid] = (slot_base ? slot_base[gauss / kSlotBlock] : 0u) + __float_as_uint(r2.z) +
                     rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
       s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
       s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
@@ -644,6 +648,30 @@ __global__ __launch_bounds__(256) void gather_grads_kernel(GatherSrc src, int it
   if (valid) store_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
 }
 
+// One band's records added to the per-Gaussian sums, one Gaussian per lane in
+// index order (as the whole-image gather): Gaussian i owns the slots
+// [first, first + its band area), first being its render record's slot word
+// (written by the band's owner pass).  Waves with no Gaussian in the band
+// leave at once.
+__global__ __launch_bounds__(256) void band_gather_kernel(const uint2* __restrict__ dup_rect, uint32_t by0,
+                                                          uint32_t by1, int P, const float* __restrict__ rec_words,
+                                                          uint32_t nslots, const float4* __restrict__ grad_rec,
+                                                          const float* __restrict__ grad_opacity,
+                                                          float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
+                                                          float* __restrict__ dL_dopacity,
+                                                          float* __restrict__ dL_dcolors) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t cnt = i < P ? dup_rect_area(band_dup_rect(dup_rect[i], by0, by1)) : 0u;
+  if (__ballot(cnt != 0u) == 0ull) return;  // wave-uniform
+  const uint32_t base = cnt ? __float_as_uint(rec_words[12 * (size_t)i + 10]) : 0u;
+  float acc[kGradTerms];
+#pragma unroll
+  for (int q = 0; q < kGradTerms; ++q) acc[q] = 0.f;
+  if (cnt) load_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+  gather_range<false>(grad_rec, grad_opacity, nslots, base, cnt, acc);
+  if (cnt) store_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+}
+
 namespace {
 
 // Dispatches a kernel template over (tile shape, wrap).
@@ -692,13 +720,16 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
-                                  float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s) {
-  const int tiles = grid.tiles();
+                                  float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s,
+                                  int band_y0, int band_y1) {
+  if (band_y1 < 0 || band_y1 > grid.gy) band_y1 = grid.gy;
+  if (band_y0 < 0 || band_y0 > band_y1) return hipErrorInvalidValue;
+  const int tiles = (band_y1 - band_y0) * grid.gx;
   if (tiles == 0) return hipSuccess;
   ViewTable none{};
   return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
                                      point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, slot_base, nslots,
-                                     grad_rec, grad_opacity, blend, blend_ok);
+                                     grad_rec, grad_opacity, blend, blend_ok, band_y0);
 }
 
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
@@ -709,6 +740,16 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
   const GatherSrc src{P, nslots, R_dev, tiles_touched, slot_local, slot_base, grad_rec, grad_opacity};
   hipLaunchKernelGGL(gather_grads_kernel, dim3((items + 255) / 256), dim3(256), 0, s, src, items, dL_dmean2D,
                      dL_dconic, dL_dopacity, dL_dcolors);
+  return hipGetLastError();
+}
+
+hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const float4* rec,
+                              uint32_t nslots, const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
+                              float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
+  if (P == 0 || nslots == 0) return hipSuccess;
+  hipLaunchKernelGGL(band_gather_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, dup_rect, by0, by1, P,
+                     reinterpret_cast<const float*>(rec), nslots, grad_rec, grad_opacity, dL_dmean2D, dL_dconic,
+                     dL_dopacity, dL_dcolors);
   return hipGetLastError();
 }
 
@@ -730,7 +771,7 @@ hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
                                     final_T, n_contrib, dL_dpix, slot_base, nslots, grad_rec, grad_opacity, blend,
-                                    blend_ok);
+                                    blend_ok, 0);
 }
 
 }  // namespace omnigs

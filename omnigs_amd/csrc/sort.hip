@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
 // bits, and a total past 2^32 - 1 leaves {0, 1} there (no usable instance,
 // overflow) instead of a wrapped count.  A single block's partial cannot
 // wrap: its kScanTile values are each at most kMaxTiles.
-__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ partials, size_t nb) {
+__device__ void scan_partials_block(uint32_t* __restrict__ partials, size_t nb) {
   __shared__ unsigned long long s_total;
   if (threadIdx.x == 0) s_total = 0ull;
   __syncthreads();
@@ -122,6 +122,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* _
     partials[nb] = ovf ? 0u : carry;
     partials[nb + 1] = ovf ? 1u : 0u;
   }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* __restrict__ partials, size_t nb) {
+  scan_partials_block(partials, nb);
 }
 
 // Each thread scans kScanItems consecutive elements (blocked layout); the
@@ -487,7 +491,8 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
                                                                 uint32_t* __restrict__ no_record,
                                                                 uint32_t* __restrict__ counts, uint32_t count_mask,
                                                                 uint32_t nblocks, uint32_t* __restrict__ zero,
-                                                                uint32_t zero_n, uint32_t* __restrict__ blend_ok) {
+                                                                uint32_t zero_n, uint32_t* __restrict__ blend_ok,
+                                                                const uint32_t* __restrict__ P_dev) {
   static_assert(kDupChunk == kSortTile, "expansion blocks are the tile sort's tiles");
   constexpr int kViewSlots = kMulti ? kDupChunk : 1;
   __shared__ uint32_t s_hist[kRadix];
@@ -525,9 +530,13 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   }
   const uint32_t k1 = min(R, k0 + (uint32_t)kDupChunk);
   for (int d = tid; d < kRadix; d += kDupThreads) s_hist[d] = 0u;
-  block_first_greater2(offsets, (uint32_t)P, k0, k1 - 1, s_span);
+  // P_dev: the owner count on the device (a band's compacted list)
+  const uint32_t Pn = P_dev ? min((uint32_t)P, *P_dev) : (uint32_t)P;
+  block_first_greater2(offsets, Pn, k0, k1 - 1, s_span);
   const uint32_t sb = s_span[0];
-  const uint32_t ns = s_span[1] - sb + 1;  // <= kDupChunk
+  // <= kDupChunk: every owner in the span holds an instance (the clamp only
+  // keeps the LDS staging in bounds should a caller break that)
+  const uint32_t ns = min(s_span[1] - sb + 1, (uint32_t)kDupChunk);
   const uint32_t begin0 = sb == 0 ? 0u : offsets[sb - 1];
   for (uint32_t j = tid; j < ns; j += kDupThreads) {
     const uint32_t s = sb + j;
@@ -616,14 +625,14 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
 hipError_t launch_duplicate(int P, uint32_t R, const uint32_t* R_dev, const uint32_t* order, const uint32_t* offsets,
                             const uint2* rect_depth, TileGrid grid, uint32_t* tile_key, uint32_t* gid,
                             uint32_t* no_record, uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok,
-                            hipStream_t s) {
+                            hipStream_t s, const uint32_t* P_dev) {
   if (P == 0 || R == 0) return hipSuccess;
   const uint32_t blocks = (R + kDupChunk - 1) / kDupChunk;
   ViewTable none{};
   hipLaunchKernelGGL(duplicate_kernel<false>, dim3(blocks), dim3(kDupThreads), 0, s, P, R, R_dev,
                      (uint32_t*)nullptr, order, offsets, rect_depth, grid, none, tile_key, gid, no_record, counts,
                      radix_first_mask(key_bits), (uint32_t)sort_blocks(R), reinterpret_cast<uint32_t*>(ranges),
-                     (uint32_t)(2 * grid.tiles()), blend_ok);
+                     (uint32_t)(2 * grid.tiles()), blend_ok, P_dev);
   return hipGetLastError();
 }
 
@@ -636,7 +645,149 @@ hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* 
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
                      order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record, counts,
                      radix_first_mask(key_bits), (uint32_t)sort_blocks(capacity), reinterpret_cast<uint32_t*>(ranges),
-                     2u * vt.tile_base[vt.V], blend_ok);
+                     2u * vt.tile_base[vt.V], blend_ok, (const uint32_t*)nullptr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tile-row bands (the checkpointed backward bins one band of tile rows at a
+// time).
+// ---------------------------------------------------------------------------
+// Instances per tile row as a difference array: a rect of width w over rows
+// [y0, y0 + h) adds w at y0 and takes it off at y0 + h (mod 2^32).  With kLds
+// each block counts its Gaussians (grid-stride) in LDS and adds its nonzero
+// entries once.
+constexpr int kRowLdsRows = 8192;  // rows + 1 counted in LDS (32 KiB)
+template <bool kLds>
+__global__ __launch_bounds__(256) void row_counts_kernel(int P, const uint2* __restrict__ dup_rect, int gy,
+                                                         uint32_t* __restrict__ row_diff) {
+  extern __shared__ uint32_t s_diff[];
+  if constexpr (kLds) {
+    for (int y = threadIdx.x; y <= gy; y += blockDim.x) s_diff[y] = 0u;
+    __syncthreads();
+  }
+  uint32_t* d = kLds ? s_diff : row_diff;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    const uint2 r = dup_rect[i];
+    const uint32_t h = r.x >> 16, w = r.y >> 16, y0 = r.y & 0xFFFFu;
+    if (h == 0) continue;  // culled: the empty record
+    atomicAdd(&d[y0], w);
+    atomicAdd(&d[y0 + h], 0u - w);
+  }
+  if constexpr (kLds) {
+    __syncthreads();
+    for (int y = threadIdx.x; y <= gy; y += blockDim.x)
+      if (s_diff[y]) atomicAdd(&row_diff[y], s_diff[y]);
+  }
+}
+
+hipError_t launch_row_counts(int P, const uint2* dup_rect, int gy, uint32_t* row_diff, hipStream_t s) {
+  const hipError_t e = launch_fill_u32(row_diff, (size_t)gy + 1, 0u, s);
+  if (e != hipSuccess || P == 0) return e;
+  const size_t want = ((size_t)P + 255) / 256;
+  const unsigned blocks = (unsigned)(want < 1024 ? want : 1024);
+  if (gy + 1 <= kRowLdsRows)
+    hipLaunchKernelGGL(row_counts_kernel<true>, dim3(blocks), dim3(256), (size_t)(gy + 1) * sizeof(uint32_t), s, P,
+                       dup_rect, gy, row_diff);
+  else
+    hipLaunchKernelGGL(row_counts_kernel<false>, dim3(blocks), dim3(256), 0, s, P, dup_rect, gy, row_diff);
+  return hipGetLastError();
+}
+
+// The whole rects in depth order (rect_d[i] = dup_rect[order[i]]), gathered
+// once for all bands.
+__global__ void gather_rects_kernel(const uint2* __restrict__ dup_rect, const uint32_t* __restrict__ order, int P,
+                                    uint2* __restrict__ rect_d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P) rect_d[i] = dup_rect[order[i]];
+}
+
+hipError_t launch_gather_rects(const uint2* dup_rect, const uint32_t* order, int P, uint2* rect_d, hipStream_t s) {
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_rects_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, dup_rect, order, P,
+                     rect_d);
+  return hipGetLastError();
+}
+
+// A band's owners: the depth-ordered Gaussians with rect rows in the band,
+// compacted in order with their inclusive instance offsets and band rects,
+// so that the expansion sees no owner without an instance.  Thread t of a
+// block takes kScanItems consecutive entries of the block's scan tile (so
+// that exclusive scans of the threads' sums keep them in order).
+__device__ __forceinline__ void band_items(const uint2* __restrict__ rect_d, size_t n, size_t base, uint32_t by0,
+                                           uint32_t by1, uint2 r[kScanItems], uint32_t& area, uint32_t& count) {
+  area = count = 0u;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t i = base + (size_t)k;
+    r[k] = band_dup_rect(i < n ? rect_d[i] : make_uint2(0u, 1u << 16), by0, by1);
+    const uint32_t a = dup_rect_area(r[k]);
+    area += a;
+    count += a ? 1u : 0u;
+  }
+}
+
+// Pass 1: per scan tile, the band's instances and owners.
+__global__ __launch_bounds__(kScanThreads) void band_reduce_kernel(const uint2* __restrict__ rect_d, size_t n,
+                                                                   uint32_t by0, uint32_t by1,
+                                                                   uint32_t* __restrict__ part_area,
+                                                                   uint32_t* __restrict__ part_count) {
+  uint2 r[kScanItems];
+  uint32_t area, count, tot_a, tot_c;
+  band_items(rect_d, n, (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems, by0, by1, r, area, count);
+  block_exclusive_scan(area, &tot_a);
+  block_exclusive_scan(count, &tot_c);
+  if (threadIdx.x == 0) {
+    part_area[blockIdx.x] = tot_a;
+    part_count[blockIdx.x] = tot_c;
+  }
+}
+
+// Pass 2: both partial arrays scanned at once (block 0: areas, block 1:
+// owner counts); totals at [nb] with the overflow flag at [nb + 1].
+__global__ __launch_bounds__(kScanThreads) void band_partials_kernel(uint32_t* __restrict__ part_area,
+                                                                     uint32_t* __restrict__ part_count, size_t nb) {
+  scan_partials_block(blockIdx.x == 0 ? part_area : part_count, nb);
+}
+
+// Pass 3: the owners' entries, compacted, and each owner's first slot into
+// its render record's slot word (word 2 of its third float4, store_rec).
+__global__ __launch_bounds__(kScanThreads) void band_owners_kernel(
+    const uint2* __restrict__ rect_d, const uint32_t* __restrict__ order, size_t n, uint32_t by0, uint32_t by1,
+    const uint32_t* __restrict__ part_area, const uint32_t* __restrict__ part_count, uint32_t* __restrict__ order_c,
+    uint32_t* __restrict__ offsets_c, uint2* __restrict__ rect_c, float* __restrict__ rec_words) {
+  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  uint2 r[kScanItems];
+  uint32_t area, count, tot;
+  band_items(rect_d, n, base, by0, by1, r, area, count);
+  uint32_t off = part_area[blockIdx.x] + block_exclusive_scan(area, &tot);
+  uint32_t pos = part_count[blockIdx.x] + block_exclusive_scan(count, &tot);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t a = dup_rect_area(r[k]);
+    if (a) {
+      const uint32_t g = order[base + (size_t)k];
+      rec_words[12 * (size_t)g + 10] = __uint_as_float(off);
+      off += a;
+      order_c[pos] = g;
+      offsets_c[pos] = off;
+      rect_c[pos] = dup_rect_xy(r[k]);
+      ++pos;
+    }
+  }
+}
+
+hipError_t launch_band_owners(const uint2* rect_d, const uint32_t* order, size_t n, uint32_t by0, uint32_t by1,
+                              uint32_t* part_area, uint32_t* part_count, uint32_t* order_c, uint32_t* offsets_c,
+                              uint2* rect_c, float4* rec, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t nb = scan_blocks(n);
+  hipLaunchKernelGGL(band_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, rect_d, n, by0, by1,
+                     part_area, part_count);
+  hipLaunchKernelGGL(band_partials_kernel, dim3(2), dim3(kScanThreads), 0, s, part_area, part_count, nb);
+  hipLaunchKernelGGL(band_owners_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, rect_d, order, n, by0, by1,
+                     (const uint32_t*)part_area, (const uint32_t*)part_count, order_c, offsets_c, rect_c,
+                     reinterpret_cast<float*>(rec));
   return hipGetLastError();
 }
 

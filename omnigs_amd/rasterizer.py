@@ -78,9 +78,12 @@ def _empty_like(ref: torch.Tensor) -> torch.Tensor:
 
 
 def rasterize_gaussians(means3D, means2D, opacities, settings: GaussianRasterizationSettings, shs=None,
-                        colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None, checkpoint=False):
+                        colors_precomp=None, scales=None, rotations=None, cov3D_precomp=None, checkpoint=False,
+                        band_instances=0):
     """Runs the C++ autograd Function (stored buffers, or checkpointed when
-    ``checkpoint``); returns (color [3,H,W], radii [P])."""
+    ``checkpoint``: the backward recomputes and bins in bands of tile rows of
+    at most ``band_instances`` instances, 0 = the library's default); returns
+    (color [3,H,W], radii [P])."""
     e = _empty_like(means3D)
     pm = settings.projmatrix if settings.projmatrix is not None else settings.viewmatrix
     return ext().rasterize_gaussians(
@@ -88,16 +91,18 @@ def rasterize_gaussians(means3D, means2D, opacities, settings: GaussianRasteriza
         scales if scales is not None else e, rotations if rotations is not None else e,
         cov3D_precomp if cov3D_precomp is not None else e, settings.image_height, settings.image_width, settings.bg,
         float(settings.scale_modifier), settings.viewmatrix, pm, int(settings.sh_degree), settings.campos,
-        bool(settings.prefiltered), int(settings.camera_type), bool(checkpoint), bool(settings.seam_wrap))
+        bool(settings.prefiltered), int(settings.camera_type), bool(checkpoint), bool(settings.seam_wrap),
+        int(band_instances))
 
 
 class GaussianRasterizer(torch.nn.Module):
     """Module form (reference include/gaussian_rasterizer.h:105-141)."""
 
-    def __init__(self, settings: GaussianRasterizationSettings, checkpoint: bool = False):
+    def __init__(self, settings: GaussianRasterizationSettings, checkpoint: bool = False, band_instances: int = 0):
         super().__init__()
         self.settings = settings
         self.checkpoint = checkpoint
+        self.band_instances = band_instances
 
     def mark_visible(self, positions: torch.Tensor) -> torch.Tensor:
         with torch.no_grad():
@@ -113,12 +118,12 @@ class GaussianRasterizer(torch.nn.Module):
             raise RuntimeError("Please provide exactly one of either scale/rotation pair or precomputed 3D covariance!")
         return rasterize_gaussians(means3D, means2D, opacities, self.settings, shs=shs, colors_precomp=colors_precomp,
                                    scales=scales, rotations=rotations, cov3D_precomp=cov3D_precomp,
-                                   checkpoint=self.checkpoint)
+                                   checkpoint=self.checkpoint, band_instances=self.band_instances)
 
 
 def render_lonlat(params: dict, viewmatrix: torch.Tensor, campos: torch.Tensor, image_height: int,
                   image_width: int, bg_color: torch.Tensor, active_sh_degree: int = 3, scaling_modifier: float = 1.0,
-                  checkpoint: bool = False, seam_wrap: bool = False):
+                  checkpoint: bool = False, seam_wrap: bool = False, band_instances: int = 0):
     """The caller contract of GaussianRenderer::renderLonlat
     (src/gaussian_renderer.cpp:166-291) over raw parameters
     {xyz, features_dc [P,1,3], features_rest [P,15,3], scaling (log), rotation
@@ -139,6 +144,6 @@ def render_lonlat(params: dict, viewmatrix: torch.Tensor, campos: torch.Tensor, 
     scales = torch.exp(params["scaling"])
     rotations = torch.nn.functional.normalize(params["rotation"])
     shs = torch.cat([params["features_dc"], params["features_rest"]], dim=1)
-    raster = GaussianRasterizer(settings, checkpoint=checkpoint)
+    raster = GaussianRasterizer(settings, checkpoint=checkpoint, band_instances=band_instances)
     image, radii = raster(xyz, screenspace_points, opacity, shs=shs, scales=scales, rotations=rotations)
     return image, screenspace_points, radii > 0, radii

@@ -13,13 +13,14 @@ std::tuple<torch::Tensor, torch::Tensor> rasterize_gaussians(
     torch::Tensor colors_precomp, torch::Tensor scales, torch::Tensor rotations, torch::Tensor cov3D_precomp,
     int image_height, int image_width, torch::Tensor bg, float scale_modifier, torch::Tensor viewmatrix,
     torch::Tensor projmatrix, int sh_degree, torch::Tensor campos, bool prefiltered, int camera_type,
-    bool checkpoint, bool seam_wrap) {
+    bool checkpoint, bool seam_wrap, int64_t band_instances) {
   torch::Tensor RwcT;
   GaussianRasterizationSettings rs(image_height, image_width, 0.f, 0.f, bg, scale_modifier, viewmatrix, projmatrix,
                                    RwcT, sh_degree, campos, prefiltered, (Camera::CameraModelType)camera_type, false);
   omnigs::RasterizerOptions opt;
   opt.checkpoint = checkpoint;
   opt.seam_wrap = seam_wrap;
+  opt.band_instances = band_instances;
   omnigs::LonlatRasterizer rasterizer(rs, opt);
   const bool has_shs = shs.defined() && shs.numel() > 0;
   const bool has_cols = colors_precomp.defined() && colors_precomp.numel() > 0;
@@ -89,5 +90,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("image_height"), py::arg("image_width"), py::arg("bg"), py::arg("scale_modifier"),
         py::arg("viewmatrix"), py::arg("projmatrix"), py::arg("sh_degree"), py::arg("campos"),
         py::arg("prefiltered") = false, py::arg("camera_type") = 3, py::arg("checkpoint") = false,
-        py::arg("seam_wrap") = false);
+        py::arg("seam_wrap") = false, py::arg("band_instances") = 0);
 }

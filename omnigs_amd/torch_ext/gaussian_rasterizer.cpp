@@ -6,7 +6,7 @@
 // "exactly one of" checks (their messages kept verbatim, typo included).
 // omnigs::StoredRasterizerFunction is the same stored form with the seam-wrap
 // switch; omnigs::CheckpointRasterizerFunction keeps only the image buffer
-// and re-runs preprocess and binning in the backward.
+// and re-runs preprocess and binning in the backward, in bands of tile rows.
 #include "gaussian_rasterizer.h"
 
 #include <c10/core/DeviceGuard.h>
@@ -193,12 +193,14 @@ tensor_list CheckpointRasterizerFunction::forward(AutogradContext* ctx, torch::T
                                                   torch::Tensor colors_precomp, torch::Tensor opacities,
                                                   torch::Tensor scales, torch::Tensor rotations,
                                                   torch::Tensor cov3Ds_precomp,
-                                                  GaussianRasterizationSettings rs, bool seam_wrap) {
+                                                  GaussianRasterizationSettings rs, bool seam_wrap,
+                                                  int64_t band_instances) {
   auto res = run_forward(rs, means3D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp, seam_wrap);
   const int R = std::get<0>(res);
   auto color = std::get<1>(res);
   auto radii = std::get<2>(res);
   save_settings(ctx, rs, R, seam_wrap);
+  ctx->saved_data["band_instances"] = band_instances;
   // geometry and binning buffers are dropped here; only the image buffer is kept
   ctx->save_for_backward({rs.bg_, rs.viewmatrix_, rs.projmatrix_, rs.campos_, colors_precomp, means3D, scales,
                           rotations, cov3Ds_precomp, radii, sh, opacities, std::get<5>(res)});
@@ -209,57 +211,19 @@ tensor_list CheckpointRasterizerFunction::forward(AutogradContext* ctx, torch::T
 tensor_list CheckpointRasterizerFunction::backward(AutogradContext* ctx, tensor_list grad_outputs) {
   const int R = ctx->saved_data["num_rendered"].toInt();
   const float scale_modifier = (float)ctx->saved_data["scale_modifier"].toDouble();
-  const float tanfovx = (float)ctx->saved_data["tanfovx"].toDouble();
-  const float tanfovy = (float)ctx->saved_data["tanfovy"].toDouble();
   const int sh_degree = ctx->saved_data["sh_degree"].toInt();
   const int camera_type = ctx->saved_data["camera_type"].toInt();
-  const int H = ctx->saved_data["image_height"].toInt();
-  const int W = ctx->saved_data["image_width"].toInt();
   const bool prefiltered = ctx->saved_data["prefiltered"].toBool();
   const bool seam_wrap = ctx->saved_data["seam_wrap"].toBool();
+  const int64_t band_instances = ctx->saved_data["band_instances"].toInt();
   auto s = ctx->get_saved_variables();
-  torch::Tensor bg = s[0].contiguous(), vm = s[1].contiguous(), cp = s[3].contiguous(), cols = s[4].contiguous(),
-                means3D = s[5].contiguous(), scales = s[6].contiguous(), rots = s[7].contiguous(),
-                cov = s[8].contiguous(), radii = s[9], sh = s[10].contiguous(), opac = s[11].contiguous(),
-                img = s[12];
-  const int P = (int)means3D.size(0);
-  torch::TensorOptions byte_opts = torch::TensorOptions(torch::kByte).device(means3D.device());
-  torch::Tensor geom = torch::empty({0}, byte_opts), binning = torch::empty({0}, byte_opts);
-  if (P != 0 && camera_type == kLonlat) {
-    const c10::DeviceGuard guard(means3D.device());
-    omnigs_lonlat_forward_args a{};
-    a.P = P;
-    a.D = sh_degree;
-    a.M = sh.size(0) != 0 ? (int)sh.size(1) : 0;
-    a.width = W;
-    a.height = H;
-    a.background = fptr(bg);
-    a.means3D = fptr(means3D);
-    a.shs = fptr(sh);
-    a.colors_precomp = fptr(cols);
-    a.opacities = fptr(opac);
-    a.scales = fptr(scales);
-    a.scale_modifier = scale_modifier;
-    a.rotations = fptr(rots);
-    a.cov3D_precomp = fptr(cov);
-    a.viewmatrix = fptr(vm);
-    a.campos = fptr(cp);
-    a.prefiltered = prefiltered ? 1 : 0;
-    a.seam_wrap = seam_wrap ? 1 : 0;
-    omnigs_buffer gb{&resize_tensor, &geom}, bb{&resize_tensor, &binning};
-    int R2 = 0;
-    void* stream = reinterpret_cast<void*>(c10::hip::getCurrentHIPStream(means3D.device().index()).stream());
-    const int st = omnigs_lonlat_recompute(&a, gb, bb, reinterpret_cast<char*>(img.data_ptr()), stream, &R2);
-    if (st != OMNIGS_OK)
-      throw std::runtime_error(std::string("checkpointed backward: ") + omnigs_status_string(st) + " (" +
-                               omnigs_last_error() + ")");
-    if (R2 != R) throw std::runtime_error("checkpointed backward: recomputed instance count differs from forward");
-  }
-  auto r = omnigs::RasterizeGaussiansBackwardCUDA(bg, means3D, radii, cols, scales, rots, scale_modifier, cov, vm,
-                                                  s[2], tanfovx, tanfovy, grad_outputs[0].contiguous(), sh,
-                                                  sh_degree, cp, geom, R, binning, img, camera_type, seam_wrap);
+  // preprocess, depth sort, then binning and render backward per band of tile rows
+  auto r = omnigs::RasterizeGaussiansBackwardCheckpointedCUDA(
+      s[0], s[5], s[9], s[4], s[11], s[6], s[7], scale_modifier, s[8], s[1], grad_outputs[0].contiguous(), s[10],
+      sh_degree, s[3], prefiltered, R, s[12], camera_type, seam_wrap, band_instances);
   auto g = grads_in_reference_order(r);
   g.emplace_back();  // the seam_wrap switch
+  g.emplace_back();  // the band budget
   return g;
 }
 

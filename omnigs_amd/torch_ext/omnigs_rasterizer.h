@@ -9,8 +9,10 @@
 //   the 16-pixel tile width.
 // * checkpointing: the autograd context keeps only the image buffer (per-pixel
 //   final T and contributor counts, per-tile ranges) and the radii; the
-//   backward re-runs preprocess and binning (the reference keeps all three
-//   buffers, src/gaussian_rasterizer.cpp:85-98).
+//   backward re-runs preprocess and the depth sort, then bins and runs the
+//   render backward in bands of tile rows, so that the per-instance buffers
+//   exist for one band at a time (the reference keeps all three buffers from
+//   the forward to the backward, src/gaussian_rasterizer.cpp:85-98).
 #pragma once
 
 #include <torch/torch.h>
@@ -45,9 +47,26 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
                                const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer,
                                const int camera_type, const bool seam_wrap);
 
+// The checkpointed backward: the forward's inputs (as given to
+// RasterizeGaussiansCUDA), its R and image buffer, and the band budget in
+// instances (0: the library's default); returns the eight gradients of
+// RasterizeGaussiansBackwardCUDA.
+std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,
+           torch::Tensor>
+RasterizeGaussiansBackwardCheckpointedCUDA(const torch::Tensor& background, const torch::Tensor& means3D,
+                                           const torch::Tensor& radii, const torch::Tensor& colors,
+                                           const torch::Tensor& opacity, const torch::Tensor& scales,
+                                           const torch::Tensor& rotations, const float scale_modifier,
+                                           const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+                                           const torch::Tensor& dL_dout_color, const torch::Tensor& sh,
+                                           const int degree, const torch::Tensor& campos, const bool prefiltered,
+                                           const int R, const torch::Tensor& imageBuffer, const int camera_type,
+                                           const bool seam_wrap, const int64_t band_instances);
+
 struct RasterizerOptions {
-  bool checkpoint = false;  // keep only the image buffer; recompute preprocess and binning in the backward
-  bool seam_wrap = false;   // wrap tile columns around the longitude seam
+  bool checkpoint = false;     // keep only the image buffer; recompute in the backward, in bands of tile rows
+  bool seam_wrap = false;      // wrap tile columns around the longitude seam
+  int64_t band_instances = 0;  // checkpointed backward: instances per band (0: the library's default)
 };
 
 // Stored form with the seam-wrap switch (GaussianRasterizerFunction is this
@@ -64,14 +83,16 @@ class StoredRasterizerFunction : public torch::autograd::Function<StoredRasteriz
                                                torch::autograd::tensor_list grad_out_color);
 };
 
-// Checkpointed form: same inputs, outputs and gradients as the stored form.
+// Checkpointed form: same inputs, outputs and gradients as the stored form
+// (backward: plus undefined ones for the two switches).
 class CheckpointRasterizerFunction : public torch::autograd::Function<CheckpointRasterizerFunction> {
  public:
   static torch::autograd::tensor_list forward(torch::autograd::AutogradContext* ctx, torch::Tensor means3D,
                                               torch::Tensor means2D, torch::Tensor sh, torch::Tensor colors_precomp,
                                               torch::Tensor opacities, torch::Tensor scales, torch::Tensor rotations,
                                               torch::Tensor cov3Ds_precomp,
-                                              GaussianRasterizationSettings raster_settings, bool seam_wrap);
+                                              GaussianRasterizationSettings raster_settings, bool seam_wrap,
+                                              int64_t band_instances);
   static torch::autograd::tensor_list backward(torch::autograd::AutogradContext* ctx,
                                                torch::autograd::tensor_list grad_out_color);
 };
@@ -84,7 +105,8 @@ inline torch::autograd::tensor_list rasterizeGaussians(torch::Tensor& means3D, t
                                                        const RasterizerOptions& options) {
   if (options.checkpoint)
     return CheckpointRasterizerFunction::apply(means3D, means2D, sh, colors_precomp, opacities, scales, rotations,
-                                               cov3Ds_precomp, raster_settings, options.seam_wrap);
+                                               cov3Ds_precomp, raster_settings, options.seam_wrap,
+                                               options.band_instances);
   return StoredRasterizerFunction::apply(means3D, means2D, sh, colors_precomp, opacities, scales, rotations,
                                          cov3Ds_precomp, raster_settings, options.seam_wrap);
 }

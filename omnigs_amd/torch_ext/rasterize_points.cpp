@@ -189,6 +189,101 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
                          dL_drotations);
 }
 
+std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,
+           torch::Tensor>
+RasterizeGaussiansBackwardCheckpointedCUDA(const torch::Tensor& background, const torch::Tensor& means3D,
+                                           const torch::Tensor& radii, const torch::Tensor& colors,
+                                           const torch::Tensor& opacity, const torch::Tensor& scales,
+                                           const torch::Tensor& rotations, const float scale_modifier,
+                                           const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+                                           const torch::Tensor& dL_dout_color, const torch::Tensor& sh,
+                                           const int degree, const torch::Tensor& campos, const bool prefiltered,
+                                           const int R, const torch::Tensor& imageBuffer, const int camera_type,
+                                           const bool seam_wrap, const int64_t band_instances) {
+  const int P = (int)means3D.size(0);
+  const int H = (int)dL_dout_color.size(1);
+  const int W = (int)dL_dout_color.size(2);
+  const int M = sh.size(0) != 0 ? (int)sh.size(1) : 0;
+  const c10::DeviceGuard guard(means3D.device());
+  auto opts = means3D.options();
+  torch::Tensor dL_dmeans3D = torch::zeros({P, 3}, opts);
+  torch::Tensor dL_dmeans2D = torch::zeros({P, 3}, opts);
+  torch::Tensor dL_dcolors = torch::zeros({P, 3}, opts);
+  torch::Tensor dL_dconic = torch::zeros({P, 2, 2}, opts);
+  torch::Tensor dL_dopacity = torch::zeros({P, 1}, opts);
+  torch::Tensor dL_dcov3D = torch::zeros({P, 6}, opts);
+  torch::Tensor dL_dsh = torch::zeros({P, M, 3}, opts);
+  torch::Tensor dL_dscales = torch::zeros({P, 3}, opts);
+  torch::Tensor dL_drotations = torch::zeros({P, 4}, opts);
+
+  if (P != 0) {
+    if (camera_type != kLonlat) bad_camera(camera_type);
+    if (band_instances < 0 || band_instances > 0xFFFFFFFFll)
+      throw std::runtime_error("RasterizeGaussiansBackwardCheckpointedCUDA: band_instances out of range");
+    F32 bg(background), m(means3D), s(sh), c(colors), o(opacity), sc(scales), r(rotations), cv(cov3D_precomp),
+        vm(viewmatrix), cp(campos), dpix(dL_dout_color);
+    torch::Tensor rad = radii.contiguous(), ibuf = imageBuffer.contiguous();
+    omnigs_lonlat_forward_args f{};
+    f.P = P;
+    f.D = degree;
+    f.M = M;
+    f.width = W;
+    f.height = H;
+    f.background = bg.ptr();
+    f.means3D = m.ptr();
+    f.shs = s.ptr();
+    f.colors_precomp = c.ptr();
+    f.opacities = o.ptr();
+    f.scales = sc.ptr();
+    f.scale_modifier = scale_modifier;
+    f.rotations = r.ptr();
+    f.cov3D_precomp = cv.ptr();
+    f.viewmatrix = vm.ptr();
+    f.campos = cp.ptr();
+    f.prefiltered = prefiltered ? 1 : 0;
+    f.radii = rad.numel() ? rad.data_ptr<int>() : nullptr;
+    f.seam_wrap = seam_wrap ? 1 : 0;
+    omnigs_lonlat_backward_args a{};
+    a.P = P;
+    a.D = degree;
+    a.M = M;
+    a.R = R;
+    a.width = W;
+    a.height = H;
+    a.background = f.background;
+    a.means3D = f.means3D;
+    a.shs = f.shs;
+    a.colors_precomp = f.colors_precomp;
+    a.scales = f.scales;
+    a.scale_modifier = scale_modifier;
+    a.rotations = f.rotations;
+    a.cov3D_precomp = f.cov3D_precomp;
+    a.viewmatrix = f.viewmatrix;
+    a.campos = f.campos;
+    a.radii = f.radii;
+    a.image_buffer = reinterpret_cast<char*>(ibuf.data_ptr());
+    a.dL_dpix = dpix.ptr();
+    a.dL_dmean2D = dL_dmeans2D.data_ptr<float>();
+    a.dL_dconic = dL_dconic.data_ptr<float>();
+    a.dL_dopacity = dL_dopacity.data_ptr<float>();
+    a.dL_dcolor = dL_dcolors.data_ptr<float>();
+    a.dL_dmean3D = dL_dmeans3D.data_ptr<float>();
+    a.dL_dcov3D = dL_dcov3D.data_ptr<float>();
+    a.dL_dsh = M ? dL_dsh.data_ptr<float>() : nullptr;
+    a.dL_dscale = dL_dscales.data_ptr<float>();
+    a.dL_drot = dL_drotations.data_ptr<float>();
+    a.seam_wrap = f.seam_wrap;
+    torch::TensorOptions byte_opts = torch::TensorOptions(torch::kByte).device(means3D.device());
+    torch::Tensor geom = torch::empty({0}, byte_opts), binning = torch::empty({0}, byte_opts);
+    omnigs_buffer gb{&resize_tensor, &geom}, bb{&resize_tensor, &binning};
+    check(omnigs_lonlat_backward_checkpointed(&f, &a, gb, bb, (unsigned int)band_instances, current_stream(means3D),
+                                              nullptr),
+          "checkpointed backward");
+  }
+  return std::make_tuple(dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales,
+                         dL_drotations);
+}
+
 }  // namespace omnigs
 
 std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> RasterizeGaussiansCUDA(

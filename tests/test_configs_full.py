@@ -7,8 +7,11 @@ oracle.
   render backward given identical forward and preprocess backward given
   identical inputs (both to the rounding-aware bar of tests/helpers.py).
 * 3M Gaussians at 4096x2048: the checkpointed autograd Function equals the
-  stored one; image(bg=1) - image(bg=0) = final T; binning bit-exact given
-  the GPU's preprocess.
+  stored one (to a summation-order bar), holds less HBM from forward to
+  backward and peaks below 0.8 of the stored form's peak (its backward bins
+  in bands of tile rows);
+  image(bg=1) - image(bg=0) = final T; binning bit-exact given the GPU's
+  preprocess.
 * Batches of 8, 16 and 32 uneven views per launch (tile 16x16, and 8 views at
   32x8 and 8x32), 20k Gaussians, views drawn from {240x120, 480x240,
   512x256} (the 960x480 / 1920x960 / 2048x1024 mix at a quarter scale) with
@@ -132,13 +135,27 @@ def test_checkpoint_3M_4096x2048(gpu):
                    held, torch.cuda.max_memory_allocated(gpu) - base)
         del params, img, vs
     assert torch.equal(out[False][0], out[True][0]) and torch.equal(out[False][1], out[True][1])
+    # The banded backward adds a Gaussian's per-tile terms band by band, so
+    # Gaussians spanning several bands sum them in another order than the
+    # stored form's one gather.  Summation order alone moves the stored form
+    # by up to 8e-7 max|g| between two runs (the render backward's in-tile
+    # LDS sums); the bar allows 1e-5 max|g| per entry and 1e-5 in L2.
+    diffs = {}
     for k in out[False][3]:
         a, b = out[False][3][k], out[True][3][k]
-        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * float(a.abs().max())), k
-    assert torch.allclose(out[False][2], out[True][2], rtol=1e-5, atol=1e-6 * float(out[False][2].abs().max()))
+        diffs[k] = float((a - b).abs().max() / a.abs().max())
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-5 * float(a.abs().max())), k
+        assert float((a - b).norm()) <= 1e-5 * float(a.norm()), k
+    assert torch.allclose(out[False][2], out[True][2], rtol=1e-5, atol=1e-5 * float(out[False][2].abs().max()))
+    # the checkpointed form holds less between forward and backward, and its
+    # peak (now set by autograd's parameter gradients after the rasterizer's
+    # backward, its binning running in bands of tile rows) is well below the
+    # stored form's
     assert out[True][4] < out[False][4]
+    assert out[True][5] < 0.8 * out[False][5], (out[True][5], out[False][5])
     record("checkpointed vs stored 3M 4096x2048", held_stored=out[False][4], held_checkpointed=out[True][4],
-           peak_stored=out[False][5], peak_checkpointed=out[True][5])
+           peak_stored=out[False][5], peak_checkpointed=out[True][5],
+           grad_maxdiff_over_max={k: round(v, 9) for k, v in diffs.items()})
     # background identity and binning bit-exact given the GPU's preprocess
     ds = to_dev(sc)
     R, img0, radii, bufs = run_forward(ds)

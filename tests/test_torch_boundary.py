@@ -161,10 +161,14 @@ def test_autograd_gradients_vs_fp64_finite_differences(gpu, checkpoint):
 
 
 @pytest.mark.gpu
-def test_checkpointed_equals_stored(gpu):
+@pytest.mark.parametrize("band_instances", [0, 4000])
+def test_checkpointed_equals_stored(gpu, band_instances):
+    """The checkpointed Function (backward binned in bands of tile rows: one
+    band at the default budget, several at 4000 instances) against the stored
+    one."""
     from omnigs_amd.rasterizer import render_lonlat
     from tests.gpu_util import announce
-    announce("checkpointed vs stored, 20k @ 512x256")
+    announce(f"checkpointed (band budget {band_instances}) vs stored, 20k @ 512x256")
     sc = scenes.small_scene(P=20000, width=512, height=256, seed=4)
     out = {}
     for ck in (False, True):
@@ -176,7 +180,8 @@ def test_checkpointed_equals_stored(gpu):
                   "opacity": torch.tensor(sc["logits"], device=gpu, requires_grad=True)}
         vm = torch.tensor(sc["viewmatrix"], device=gpu)
         img, vs, vis, radii = render_lonlat(params, vm, torch.tensor(sc["campos"], device=gpu), 256, 512,
-                                            torch.zeros(3, device=gpu), 3, checkpoint=ck)
+                                            torch.zeros(3, device=gpu), 3, checkpoint=ck,
+                                            band_instances=band_instances)
         (img * torch.tensor(sc["dL_dpix"], device=gpu)).sum().backward()
         out[ck] = (img.detach(), vs.grad.clone(), {k: v.grad.clone() for k, v in params.items()})
     assert torch.equal(out[False][0], out[True][0])
