@@ -66,6 +66,11 @@ def main() -> int:
     ap.add_argument("--batch", action="store_true",
                     help="only the batched multi-view workload: 1.5M Gaussians, 8/16/32 views of uneven size, "
                          "autotuned tile shape and views per launch (views/sec)")
+    ap.add_argument("--batch-views", default="8,16,32",
+                    help="with --batch: the batch sizes V to time (comma-separated)")
+    ap.add_argument("--batch-vpl", default="",
+                    help="with --batch: only these views per launch (comma-separated; default all), tile 16x16 only "
+                         "when given")
     ap.add_argument("--memory-compare", action="store_true",
                     help="only the checkpointed-vs-stored backward comparison (3M Gaussians at 4096x2048)")
     ap.add_argument("--memory-modes", default="stored,checkpointed,checkpointed_one_band",
@@ -395,14 +400,16 @@ def batch_workload(dev, args) -> int:
                   rotations=t(sc["rotations"]))
     choice = [(960, 480), (1920, 960), (2048, 1024)]
     rows = {}
-    for V in (8, 16, 32):
+    vpl_only = tuple(int(x) for x in args.batch_vpl.split(",") if x)
+    for V in (int(x) for x in args.batch_views.split(",") if x):
         rng = np.random.default_rng(100 + V)
         sizes, vms, cps = random_views(rng, V, choice, 2.0)
         target = torch.rand((3 * sum(w * h for w, h in sizes),), generator=torch.Generator(device=dev).manual_seed(V),
                             device=dev)
+        extra = {"shapes": ((16, 16),)} if vpl_only else {}
         res = autotune(P, sizes, inputs, t(vms), t(cps), target, device=dev, reps=max(3, args.steps // 4),
-                       views_per_launch=tuple(v for v in (1, 2, 4, 8, 16, 32) if v <= V),
-                       log=lambda r: print(json.dumps(r), file=sys.stderr, flush=True))
+                       views_per_launch=tuple(v for v in (vpl_only or (1, 2, 4, 8, 16, 32)) if v <= V),
+                       log=lambda r: print(json.dumps(r), file=sys.stderr, flush=True), **extra)
         pix = sum(w * h for w, h in sizes)
         rows[str(V)] = {"best_tile": f"{res.tile[0]}x{res.tile[1]}", "best_views_per_launch": res.views_per_launch,
                         "views_per_sec": round(res.views_per_sec, 1), "ms_per_view": round(res.ms_per_view, 4),

@@ -425,18 +425,26 @@ int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, con
  * _backward), for stage-by-stage parity tests.  Per-item arrays hold V P
  * items, item v P + g being Gaussian g in view v; point_list / tile_keys hold
  * the batch's sorted instances (their count is status[0] of the forward),
- * Gaussian ids being item ids and tile ids the views' global tiles, view v's
- * tiles starting at the sum of the earlier views' tile counts; final_T /
- * n_contrib hold every view's pixels in the layout of out_color (one plane
- * each); ranges is per global tile.  The 2D gradients are the backward's
- * per-item sums (the single-view backward's dL_dmean2D, dL_dconic [4],
- * dL_dopacity and dL_dcolor, per item).  No counterpart in the reference. */
+ * Gaussian ids being item ids; final_T / n_contrib hold every view's pixels in
+ * the layout of out_color (one plane each).  Tile t of view v has the key
+ * key_base(v) + t: with key_shift 0 (compact keys) key_base(v) is the sum of
+ * the earlier views' tile counts and the instances run view by view; with
+ * key_shift > 0 (padded keys, chosen when they save the tile sort a pass)
+ * key_base(v) = v << key_shift and the instances are ordered by the key's low
+ * key_shift bits, each (view, tile) group contiguous.  Either way each group
+ * is in depth order and ranges (key_count entries) holds its [start, end) in
+ * point_list per key.  depth_key / depth_order are the depth sort's (by
+ * depth, item id).  The 2D gradients are the backward's per-item sums (the
+ * single-view backward's dL_dmean2D, dL_dconic [4], dL_dopacity and
+ * dL_dcolor, per item).  No counterpart in the reference. */
 typedef struct omnigs_lonlat_batch_state_view {
   omnigs_lonlat_state_view s;
   const float* dL_dmean2D;  /* [V P, 3] */
   const float* dL_dconic;   /* [V P, 4] */
   const float* dL_dopacity; /* [V P] */
   const float* dL_dcolor;   /* [V P, 3] */
+  unsigned int key_shift;
+  unsigned int key_count;
 } omnigs_lonlat_batch_state_view;
 int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* args, omnigs_lonlat_batch_state_view* out);
 

@@ -168,7 +168,8 @@ class BatchRenderer:
     def state(self) -> dict:
         """Torch views of the workspace's arrays (omnigs_lonlat_batch_state),
         for stage-by-stage parity tests: per-item arrays over V P items,
-        point_list / tile_keys over the instances of the last forward."""
+        point_list / tile_keys over the instances of the last forward, ranges
+        per tile key; sets key_shift / key_count (the key layout)."""
         self._sync_args()
         sv = capi.BatchStateView()
         st = self.lib.omnigs_lonlat_batch_state(ctypes.byref(self.args), ctypes.byref(sv))
@@ -176,8 +177,8 @@ class BatchRenderer:
             raise capi.OmnigsError(st, "omnigs_lonlat_batch_state")
         n_items = self.V * self.P
         R = min(self.needed()[0], self.capacity)
-        tiles = int(sum(((w + self.tile[0] - 1) // self.tile[0]) * ((h + self.tile[1] - 1) // self.tile[1])
-                        for w, h in self.sizes))
+        tiles = int(sv.key_count)
+        self.key_shift, self.key_count = int(sv.key_shift), int(sv.key_count)
         spec = {  # field: (dtype, count)
             "depth_key": (torch.int32, n_items), "depth_order": (torch.int32, n_items),
             "radii": (torch.int32, n_items), "render_records": (torch.float32, 12 * n_items),

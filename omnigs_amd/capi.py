@@ -125,7 +125,8 @@ class StateView(ctypes.Structure):
 
 class BatchStateView(ctypes.Structure):
     _fields_ = [("s", StateView), ("dL_dmean2D", ctypes.c_void_p), ("dL_dconic", ctypes.c_void_p),
-                ("dL_dopacity", ctypes.c_void_p), ("dL_dcolor", ctypes.c_void_p)]
+                ("dL_dopacity", ctypes.c_void_p), ("dL_dcolor", ctypes.c_void_p), ("key_shift", ctypes.c_uint),
+                ("key_count", ctypes.c_uint)]
 
 
 _lib = None

@@ -767,6 +767,22 @@ int make_view_table(int P, int V, const int* widths, const int* heights, int bx,
   if ((uint64_t)V * (uint64_t)P > 0x7fffffffULL) return fail_arg("V * P exceeds int range");
   vt.tile_base[V] = (uint32_t)tiles;
   vt.pix_base[V] = (uint32_t)pix;
+  // Tile keys: compact global ids, unless padding every view to 2^b keys
+  // lets the tile sort order only the b low bits in fewer passes (a stable
+  // pass over the V P items then groups the depth order by view).
+  uint32_t widest = 0;
+  for (int v = 0; v < V; ++v) widest = std::max(widest, vt.tile_base[v + 1] - vt.tile_base[v]);
+  const int compact_bits = bit_width_u32((uint32_t)tiles);
+  const int local_bits = std::max(1, bit_width_u32(widest - 1u));
+  if (V > 1 && radix_passes(local_bits) < radix_passes(compact_bits)) {
+    vt.key_shift = local_bits;
+    vt.key_count = (uint32_t)V << local_bits;
+    vt.sort_bits = local_bits;
+  } else {
+    vt.key_shift = 0;
+    vt.key_count = (uint32_t)tiles;
+    vt.sort_bits = compact_bits;
+  }
   return OMNIGS_OK;
 }
 
@@ -776,7 +792,7 @@ size_t omnigs_lonlat_batch_workspace_bytes(int P, int V, const int* widths, cons
                                            int tile_y, unsigned int capacity) {
   ViewTable vt;
   if (P < 0 || make_view_table(P, V, widths, heights, tile_x, tile_y, 0, vt) != OMNIGS_OK) return 0;
-  return carve_batch(nullptr, (size_t)V * P, vt.pix_base[V], vt.tile_base[V], capacity).bytes;
+  return carve_batch(nullptr, (size_t)V * P, vt.pix_base[V], vt.key_count, capacity).bytes;
 }
 
 int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_) {
@@ -798,8 +814,8 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   if (!a->cov3D_precomp && (!a->scales || !a->rotations)) return fail_arg("neither scales+rotations nor cov3D");
   if (a->capacity == 0) return fail_arg("capacity 0");
   const size_t items = (size_t)a->V * P;
-  const BatchState bs = carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V],
-                                    vt.tile_base[vt.V], a->capacity);
+  const BatchState bs = carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V], vt.key_count,
+                                    a->capacity);
   if (a->workspace_bytes < bs.bytes) return fail_arg("workspace smaller than omnigs_lonlat_batch_workspace_bytes()");
   hipStream_t s = (hipStream_t)stream_;
   if (P == 0) {
@@ -820,6 +836,16 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
     OMNIGS_TRY(launch_radix_sort(g.depth_key, g.key_alt, nullptr, g.order, g.order_alt, items, 32, g.sort_counts,
                                  g.scan_partials, s, &ds),
                "batch depth sort");
+    if (vt.key_shift) {
+      // padded keys: the depth order regrouped view by view (a stable sort by
+      // view; key_alt and offsets are free until the scan), into order_alt
+      SortResult vs;
+      OMNIGS_TRY(launch_view_keys(ds.vals, items, (uint32_t)P, g.key_alt, s), "batch view keys");
+      OMNIGS_TRY(launch_radix_sort(g.key_alt, g.offsets, ds.vals, g.order_alt, g.order_alt, items,
+                                   bit_width_u32((uint32_t)vt.V - 1u), g.sort_counts, g.scan_partials, s, &vs),
+                 "batch view sort");
+      ds.vals = vs.vals;
+    }
   }
   {
     StageTimer tm(OMNIGS_STAGE_SCAN, s);
@@ -830,7 +856,7 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   // {instance count, 32-bit overflow flag} after the offset scan's block partials
   const uint32_t* R_dev = g.scan_partials + scan_blocks(items);
   const BinState& b = bs.b;
-  const int tile_bits = bit_width_u32(vt.tile_base[vt.V]);
+  const int tile_bits = vt.sort_bits;
   {
     StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
     OMNIGS_TRY(launch_duplicate_views((int)items, a->capacity, R_dev, a->status, ds.vals, g.offsets, g.rect_depth,
@@ -847,7 +873,7 @@ int omnigs_lonlat_batch_forward(const omnigs_lonlat_batch_args* a, void* stream_
   }
   {
     StageTimer tm(OMNIGS_STAGE_RANGES, s);
-    OMNIGS_TRY(launch_tile_ranges(ts.keys, a->capacity, bs.ranges, (int)vt.tile_base[vt.V], s, R_dev,
+    OMNIGS_TRY(launch_tile_ranges(ts.keys, a->capacity, bs.ranges, (int)vt.key_count, s, R_dev,
                                   items > 0 && a->capacity > 0),
                "batch ranges");
   }
@@ -877,13 +903,13 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
               gr->mlp_workspace_bytes < omnigs_mlp_workspace_bytes(P)))
     return fail_arg("decoder gradient outputs / workspace missing");
   const size_t items = (size_t)a->V * P;
-  const BatchState bs = carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V],
-                                    vt.tile_base[vt.V], a->capacity);
+  const BatchState bs = carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V], vt.key_count,
+                                    a->capacity);
   if (a->workspace_bytes < bs.bytes) return fail_arg("workspace smaller than omnigs_lonlat_batch_workspace_bytes()");
   hipStream_t s = (hipStream_t)stream_;
   const GeomState& g = bs.g;
   const BinState& b = bs.b;
-  const uint32_t* point_list = (radix_passes(bit_width_u32(vt.tile_base[vt.V])) % 2) ? b.gid_alt : b.gid;
+  const uint32_t* point_list = (radix_passes(vt.sort_bits) % 2) ? b.gid_alt : b.gid;
   {
     StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
     OMNIGS_TRY(launch_render_backward_views(vt, bs.ranges, point_list, a->background, g.rec, bs.final_T,
@@ -927,7 +953,7 @@ int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* a, omnigs_lonlat_b
   if (st != OMNIGS_OK) return st;
   const size_t items = (size_t)a->V * a->P;
   const BatchState bs =
-      carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V], vt.tile_base[vt.V], a->capacity);
+      carve_batch(static_cast<char*>(a->workspace), items, vt.pix_base[vt.V], vt.key_count, a->capacity);
   if (a->workspace_bytes < bs.bytes) return fail_arg("workspace smaller than omnigs_lonlat_batch_workspace_bytes()");
   omnigs_lonlat_state_view& v = out->s;
   v.depth_key = bs.g.depth_key;
@@ -937,7 +963,7 @@ int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* a, omnigs_lonlat_b
   v.tiles_touched = bs.g.tiles_touched;
   v.depth_order = bs.g.order;
   v.offsets = bs.g.offsets;
-  const bool alt = radix_passes(bit_width_u32(vt.tile_base[vt.V])) % 2;
+  const bool alt = radix_passes(vt.sort_bits) % 2;
   v.point_list = alt ? bs.b.gid_alt : bs.b.gid;
   v.tile_keys = alt ? bs.b.tile_key_alt : bs.b.tile_key;
   v.blend = bs.b.blend;
@@ -948,6 +974,8 @@ int omnigs_lonlat_batch_state(const omnigs_lonlat_batch_args* a, omnigs_lonlat_b
   out->dL_dconic = bs.dconic;
   out->dL_dopacity = bs.dopacity;
   out->dL_dcolor = bs.dcolor;
+  out->key_shift = (unsigned int)vt.key_shift;
+  out->key_count = vt.key_count;
   return OMNIGS_OK;
 }
 

@@ -54,6 +54,20 @@ struct ViewTable {
   int W[kMaxViews], H[kMaxViews], gx[kMaxViews], gy[kMaxViews];
   uint32_t tile_base[kMaxViews + 1];  // first global tile of view v; [V] = total tiles
   uint32_t pix_base[kMaxViews + 1];   // first pixel of view v; [V] = total pixels
+  // Tile keys of the expansion, tile sort and ranges: tile t of view v has
+  // key key_base(v) + t.  Compact (key_shift 0): the global tile ids.  Padded
+  // (key_shift > 0): v << key_shift | t, every view's tiles padded to
+  // 2^key_shift keys, used when a sort over those low bits alone takes fewer
+  // passes than over the compact ids; the instances are then expanded view by
+  // view (the depth order re-sorted by view) and the tile sort orders them by
+  // the low bits only, leaving each (view, tile) group contiguous and in depth
+  // order.
+  int key_shift;
+  uint32_t key_count;  // keys (range entries): tile_base[V], or V << key_shift
+  int sort_bits;       // key bits the tile sort orders by
+  __host__ __device__ uint32_t key_base(int v) const {
+    return key_shift ? (uint32_t)v << key_shift : tile_base[v];
+  }
   __host__ __device__ TileGrid grid(int v) const { return {gx[v], gy[v], wrap, bx, by}; }
   // view owning global tile t (V is small: a scan over constant kernel arguments)
   __device__ int view_of_tile(uint32_t t) const {

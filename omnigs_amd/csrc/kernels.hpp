@@ -136,6 +136,8 @@ hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* 
                                   const ViewTable& vt, uint32_t* tile_key, uint32_t* gid, uint32_t* no_record,
                                   uint32_t* counts, int key_bits, uint2* ranges, uint32_t* blend_ok, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+// keys[i] = order[i] / P: the view of each batch item in the given order.
+hipError_t launch_view_keys(const uint32_t* order, size_t n, uint32_t P, uint32_t* keys, hipStream_t s);
 
 // render.hip
 // blend (optional, R bytes): bit w of instance i set when wave w of its tile

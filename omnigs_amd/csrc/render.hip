@@ -46,15 +46,17 @@ __device__ __forceinline__ void pixel_of(int tile, int gx, int tid, int& px, int
 struct TileLoc {
   int W, H, gx, tile;
   size_t pix;
+  uint32_t key;  // the tile's entry in the ranges
 };
 // tile0 (single view): the tile of block 0 (a grid over a band of tile rows).
 template <bool kMulti>
 __device__ __forceinline__ TileLoc locate(int W, int H, int gx, const ViewTable& vt, int tile0 = 0) {
   if constexpr (kMulti) {
     const int v = vt.view_of_tile(blockIdx.x);
-    return {vt.W[v], vt.H[v], vt.gx[v], (int)(blockIdx.x - vt.tile_base[v]), (size_t)vt.pix_base[v]};
+    const int t = (int)(blockIdx.x - vt.tile_base[v]);
+    return {vt.W[v], vt.H[v], vt.gx[v], t, (size_t)vt.pix_base[v], vt.key_base(v) + (uint32_t)t};
   } else {
-    return {W, H, gx, tile0 + (int)blockIdx.x, 0};
+    return {W, H, gx, tile0 + (int)blockIdx.x, 0, (uint32_t)(tile0 + (int)blockIdx.x)};
   }
 }
 
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kTilePixels) void render_forward_kernel(
   pixel_of<TX, TY>(tile, gx, tid, px, py);
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
-  const uint2 range = ranges[blockIdx.x];
+  const uint2 range = ranges[loc.key];
   const int n = (int)(range.y - range.x);
 
   // Lanes that are done (outside the image, or stopped), as a wave-uniform
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
   pixel_of<TX, TY>(tile, gx, tid, px, py);
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
-  const uint2 range = ranges[kMulti ? (int)blockIdx.x : tile];
+  const uint2 range = ranges[loc.key];
   const size_t N = (size_t)W * H;
   const size_t pid = (size_t)py * W + px;
   final_Ts += loc.pix;

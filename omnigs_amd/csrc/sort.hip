@@ -577,7 +577,7 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
       if constexpr (kMulti) {
         const int v = s_view[j];
         const TileGrid gv = vt.grid(v);
-        keys[u] = vt.tile_base[v] + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, gv);
+        keys[u] = vt.key_base(v) + y * (uint32_t)gv.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, gv);
       } else {
         keys[u] = y * (uint32_t)grid.gx + (uint32_t)wrap_col(s_x0[j] + (int)r, grid);
       }
@@ -645,7 +645,7 @@ hipError_t launch_duplicate_views(int items, uint32_t capacity, const uint32_t* 
   hipLaunchKernelGGL(duplicate_kernel<true>, dim3(blocks), dim3(kDupThreads), 0, s, items, capacity, R_dev, status,
                      order, offsets, rect_depth, vt.grid(0), vt, tile_key, gid, no_record, counts,
                      radix_first_mask(key_bits), (uint32_t)sort_blocks(capacity), reinterpret_cast<uint32_t*>(ranges),
-                     2u * vt.tile_base[vt.V], blend_ok, (const uint32_t*)nullptr);
+                     2u * vt.key_count, blend_ok, (const uint32_t*)nullptr);
   return hipGetLastError();
 }
 
@@ -788,6 +788,19 @@ hipError_t launch_band_owners(const uint2* rect_d, const uint32_t* order, size_t
   hipLaunchKernelGGL(band_owners_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, rect_d, order, n, by0, by1,
                      (const uint32_t*)part_area, (const uint32_t*)part_count, order_c, offsets_c, rect_c,
                      reinterpret_cast<float*>(rec));
+  return hipGetLastError();
+}
+
+// The view of every item of a batch in its current order (item i is in view
+// i / P), as keys for the stable sort that regroups the depth order by view.
+__global__ void view_keys_kernel(const uint32_t* __restrict__ order, size_t n, uint32_t P, uint32_t* __restrict__ keys) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = order[i] / P;
+}
+
+hipError_t launch_view_keys(const uint32_t* order, size_t n, uint32_t P, uint32_t* keys, hipStream_t s) {
+  if (n == 0 || P == 0) return hipSuccess;
+  hipLaunchKernelGGL(view_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, order, n, P, keys);
   return hipGetLastError();
 }
 

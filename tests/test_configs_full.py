@@ -216,22 +216,28 @@ def _view_items(st, v, P):
             "tiles_touched": st["tiles_touched"].view(np.uint32)[sl], "clamped": st["clamped"][sl]}
 
 
-def _view_binning(st, v, P, sizes, tile, base):
-    """View v's sorted list (Gaussian ids) and tile ranges from the batch's
-    global arrays; base = instances of the views before it."""
+def _view_binning(st, v, P, sizes, tile, key_shift):
+    """View v's sorted list (Gaussian ids) and tile ranges (relative to that
+    list) from the batch's arrays: its tiles' keys start at v << key_shift
+    (padded keys) or after the earlier views' tiles (compact keys), and each
+    key's range holds that (view, tile) group of the point list in depth
+    order."""
     ntiles = [((w + tile[0] - 1) // tile[0]) * ((h + tile[1] - 1) // tile[1]) for w, h in sizes]
-    tb = int(sum(ntiles[:v]))
-    it = _view_items(st, v, P)
-    Rv = int(it["tiles_touched"].astype(np.int64).sum())
-    pl = st["point_list"].view(np.uint32)[base:base + Rv].astype(np.int64) - v * P
-    rg = st["ranges"].view(np.uint32).reshape(-1, 2)[tb:tb + ntiles[v]].astype(np.int64)
-    empty = rg[:, 1] <= rg[:, 0]
-    rg = np.where(empty[:, None], 0, rg - base)
-    return pl, rg.astype(np.uint32), Rv
+    kb = v << key_shift if key_shift else int(sum(ntiles[:v]))
+    rg = st["ranges"].view(np.uint32).reshape(-1, 2)[kb:kb + ntiles[v]].astype(np.int64)
+    cnt = np.maximum(rg[:, 1] - rg[:, 0], 0)
+    plist = st["point_list"].view(np.uint32).astype(np.int64)
+    pl = np.concatenate([plist[a:a + c] for (a, _), c in zip(rg, cnt) if c > 0] or [np.zeros(0, np.int64)]) - v * P
+    start = np.cumsum(cnt) - cnt
+    rel = np.where((cnt > 0)[:, None], np.stack([start, start + cnt], axis=1), 0)
+    return pl, rel.astype(np.uint32), int(cnt.sum())
 
 
-@pytest.mark.parametrize("V,tile", [(8, (16, 16)), (16, (16, 16)), (32, (16, 16)), (8, (32, 8)), (8, (8, 32))])
-def test_batched_views_against_oracle(gpu, V, tile):
+@pytest.mark.parametrize("V,tile,full", [(8, (16, 16), False), (16, (16, 16), False), (32, (16, 16), False),
+                                         (8, (32, 8), False), (8, (8, 32), False), (16, (16, 16), True)])
+def test_batched_views_against_oracle(gpu, V, tile, full):
+    """full: 16 views of the full sizes, whose 13-bit tile ids per view sort
+    in one pass fewer than the batch's compact 17-bit ids: padded keys."""
     import torch
 
     from tests.gpu_util import np_, record
@@ -239,13 +245,14 @@ def test_batched_views_against_oracle(gpu, V, tile):
     from tests.test_gpu_parity import _check_image_with_margin, _radii_agree, _rounding_grad_check
     P = 20000
     sc = scenes.small_scene(P=P, seed=11)
-    sizes, vms, cps = _views(V, QUARTER_SIZES, 200 + V + tile[0])
+    sizes, vms, cps = _views(V, FULL_SIZES if full else QUARTER_SIZES, 200 + V + tile[0])
     r, ds = _bind(gpu, sc, sizes, vms, cps, tile, P)
     rng = np.random.default_rng(V)
     dL = torch.from_numpy(rng.normal(0, 1e-2, size=r.out.numel()).astype(np.float32)).to(gpu)
     g = r.backward(dL)
     torch.cuda.synchronize()
     st = {k: np_(v) for k, v in r.state().items()}
+    assert (r.key_shift > 0) == full, r.key_shift
     need, over = r.needed()
     assert not over
     dLn = np_(dL)
@@ -253,8 +260,7 @@ def test_batched_views_against_oracle(gpu, V, tile):
              ("dL_drotations", "dL_drot"), ("dL_dopacity", "dL_dopacity"), ("dL_dcov3D", "dL_dcov3D"))
     sums = {ref: 0.0 for _, ref in names}
     errs = {ref: 0.0 for _, ref in names}
-    base = 0
-    m = {"views": V, "tile": f"{tile[0]}x{tile[1]}", "instances": need}
+    m = {"views": V, "tile": f"{tile[0]}x{tile[1]}", "instances": need, "key_shift": r.key_shift}
     worst = {}
     for v, (W, H) in enumerate(sizes):
         scv = {k: x for k, x in sc.items() if k != "dL_dpix"}
@@ -265,11 +271,10 @@ def test_batched_views_against_oracle(gpu, V, tile):
         for k in ("radii", "xy", "depth", "tiles_touched"):
             o.set(k, it[k])
         o.run("binning")
-        pl, rg, Rv = _view_binning(st, v, P, sizes, tile, base)
+        pl, rg, Rv = _view_binning(st, v, P, sizes, tile, r.key_shift)
         assert o.num_rendered == Rv, (v, o.num_rendered, Rv)
         np.testing.assert_array_equal(pl, o.get("point_list"), err_msg=f"view {v}")
         np.testing.assert_array_equal(rg.reshape(-1), o.get("ranges"), err_msg=f"view {v}")
-        base += Rv
         # render given identical binning
         o.set("conic", it["conic"])
         o.set("rgb", it["rgb"])
@@ -329,7 +334,6 @@ def test_batched_full_size_properties(gpu):
     st = {k: np_(v) for k, v in r.state().items()}
     assert need == int(st["tiles_touched"].view(np.uint32).astype(np.int64).sum())
     # per-view binning bit-exact given the GPU's preprocess
-    base = 0
     for v, (W, H) in enumerate(sizes):
         scv = {k: x for k, x in sc.items() if k != "dL_dpix"}
         scv.update(viewmatrix=vms[v], campos=cps[v], width=W, height=H)
@@ -338,11 +342,10 @@ def test_batched_full_size_properties(gpu):
         for k in ("radii", "xy", "depth", "tiles_touched"):
             o.set(k, it[k])
         o.run("binning")
-        pl, rg, Rv = _view_binning(st, v, P, sizes, (16, 16), base)
+        pl, rg, Rv = _view_binning(st, v, P, sizes, (16, 16), r.key_shift)
         assert o.num_rendered == Rv
         np.testing.assert_array_equal(pl, o.get("point_list"), err_msg=f"view {v}")
         np.testing.assert_array_equal(rg.reshape(-1), o.get("ranges"), err_msg=f"view {v}")
-        base += Rv
         del o
     # determinism
     img0 = r.out.clone()
