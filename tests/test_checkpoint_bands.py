@@ -132,3 +132,43 @@ def test_banded_rejects_wrong_R(gpu):
     R, img, radii, bufs = capi.forward_scene(ds)
     with pytest.raises(capi.OmnigsError, match="differs"):
         _banded(ds, R + 1, radii, bufs[2], torch.from_numpy(sc["dL_dpix"]).to(gpu), 0)
+
+
+def test_banded_with_decoder_colours(gpu):
+    """Decoder colours (omnigs_mlp_*): the banded backward recomputes the
+    preprocess with the decoder and ends with the decoder's backward, equal to
+    the stored backward's (2D and covariance gradients to FP32 rounding, the
+    decoder-dependent ones in norm)."""
+    import torch
+
+    from omnigs_amd import capi
+    from tests.gpu_util import announce, to_dev
+    from tests.test_mlp import _bf16_bits, om
+    announce("banded backward with decoder colours, 20k @ 512x256")
+    P = 20000
+    sc = scenes.small_scene(P=P, width=512, height=256, seed=2)
+    ds = to_dev(sc)
+    rng = np.random.default_rng(3)
+    lat_bits = torch.from_numpy(_bf16_bits(rng.normal(0, 0.7, size=(P, 16)).astype(np.float32))).to(gpu)
+    packed = capi.mlp_pack(torch.from_numpy(om.init_theta(8)).to(gpu))
+    common = dict(background=ds["background"], means3D=ds["means3D"], opacities=ds["opacities"].reshape(-1),
+                  viewmatrix=ds["viewmatrix"], campos=ds["campos"], width=512, height=256, scales=ds["scales"],
+                  rotations=ds["rotations"])
+    R, img, radii, bufs = capi.forward(mlp=(packed, lat_bits), **common)
+    dL = torch.from_numpy(sc["dL_dpix"]).to(gpu)
+    bc = dict(background=ds["background"], means3D=ds["means3D"], viewmatrix=ds["viewmatrix"], campos=ds["campos"],
+              dL_dpix=dL, scales=ds["scales"], rotations=ds["rotations"], mlp=(packed, lat_bits))
+    ref = capi.backward(R=R, radii=radii, buffers=bufs, **bc)
+    got = capi.backward(R=R, radii=radii, buffers=(None, None, bufs[2]),
+                        checkpointed={"opacities": common["opacities"].contiguous(), "band_instances": 3000}, **bc)
+    torch.cuda.synchronize()
+    assert got["_bands"] > 4
+    for k in ("dL_dmeans2D", "dL_dconic", "dL_dopacity", "dL_dcolors", "dL_dcov3D", "dL_dscales", "dL_drotations"):
+        a, b = ref[k], got[k]
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * float(a.abs().max())), (k, float((a - b).abs().max()))
+    # the decoder's backward rounds dL/dcolor-derived adjoints to BF16, so a
+    # last-bit difference in a summed dL/dcolor can move one Gaussian's
+    # contribution by a BF16 step: compared in norm
+    for k in ("dL_dmeans3D", "dL_dlatent", "dL_dmlp"):
+        a, b = ref[k], got[k]
+        assert float((a - b).norm()) <= 1e-3 * float(a.norm()), (k, float((a - b).norm() / a.norm()))
