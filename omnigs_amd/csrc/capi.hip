@@ -561,9 +561,9 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
   const int P = a->P, W = a->width, H = a->height;
   hipStream_t s = (hipStream_t)stream_;
 
-  // The geometry buffer, then a band's compacted owner rects and the per-row
-  // instance counts' difference array.  The band's compacted owner ids and
-  // offsets reuse order_alt and depth_key, free after the depth sort.
+  // The geometry buffer, then a tail: a band's compacted owner rects, the
+  // per-scan-tile owner counts, and the per-row instance counts' difference
+  // array.
   const size_t geom_bytes = carve_geom(nullptr, P).bytes;
   const size_t rect_bytes = align_up((size_t)P * sizeof(uint2));
   const size_t part_bytes = align_up((scan_blocks((size_t)P) + 2) * sizeof(uint32_t));
