@@ -562,13 +562,14 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
   hipStream_t s = (hipStream_t)stream_;
 
   // The geometry buffer, then a tail: a band's compacted owner rects, the
-  // per-scan-tile owner counts, and the per-row instance counts' difference
-  // array.
+  // per-scan-tile owner counts, the per-row instance counts' difference
+  // array, and each Gaussian's first gradient slot in the band.
   const size_t geom_bytes = carve_geom(nullptr, P).bytes;
   const size_t rect_bytes = align_up((size_t)P * sizeof(uint2));
   const size_t part_bytes = align_up((scan_blocks((size_t)P) + 2) * sizeof(uint32_t));
+  const size_t diff_bytes = align_up(((size_t)grid.gy + 1) * sizeof(uint32_t));
   char* gbuf = grow(geometry_buffer,
-                    geom_bytes + rect_bytes + part_bytes + align_up(((size_t)grid.gy + 1) * sizeof(uint32_t)));
+                    geom_bytes + rect_bytes + part_bytes + diff_bytes + align_up((size_t)P * sizeof(uint32_t)));
   if (!gbuf) {
     std::snprintf(g_last_error, sizeof(g_last_error), "geometry buffer allocation failed");
     return OMNIGS_ERR_ALLOC;
@@ -584,6 +585,7 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
   uint32_t* part_area = g.scan_partials;
   uint32_t* part_count = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes);
   uint32_t* row_diff = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes + part_bytes);
+  uint32_t* slot_words = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes + part_bytes + diff_bytes);
   const uint32_t* band_total = part_area + scan_blocks((size_t)P);    // {R_b, overflow flag}
   const uint32_t* band_owners = part_count + scan_blocks((size_t)P);  // owners with an instance
   const ImgState img = carve_img(a->image_buffer, W, H);
@@ -658,7 +660,7 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
       // order) are dropped from the expansion's owner list
       StageTimer tm(OMNIGS_STAGE_SCAN, s);
       OMNIGS_TRY(launch_band_owners(rect_d, g.order, (size_t)P, (uint32_t)band.y0, (uint32_t)band.y1, part_area,
-                                    part_count, order_c, offsets_c, rect_c, g.rec, s),
+                                    part_count, order_c, offsets_c, rect_c, slot_words, s),
                  "band owners");
     }
     {
@@ -685,12 +687,12 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
       StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
       OMNIGS_TRY(launch_render_backward(grid, img.ranges, ts.vals, W, H, a->background, g.rec, img.final_T,
                                         img.n_contrib, a->dL_dpix, nullptr, band.n, b.grad_rec, b.grad_opacity,
-                                        nullptr, nullptr, s, band.y0, band.y1),
+                                        nullptr, nullptr, s, band.y0, band.y1, slot_words),
                  "band render backward");
     }
     {
       StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
-      OMNIGS_TRY(launch_band_gather(P, g.dup_rect, (uint32_t)band.y0, (uint32_t)band.y1, g.rec, band.n, b.grad_rec,
+      OMNIGS_TRY(launch_band_gather(P, g.dup_rect, (uint32_t)band.y0, (uint32_t)band.y1, slot_words, band.n, b.grad_rec,
                                     b.grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor, s),
                  "band gather");
     }

@@ -88,12 +88,12 @@ hipError_t launch_gather_rects(const uint2* dup_rect, const uint32_t* order, int
 // rect_c.  part_area / part_count: scan_blocks(n) + 2 entries each; the
 // band's instance count and overflow flag land in part_area[scan_blocks(n)]
 // and [+ 1], its owner count in part_count[scan_blocks(n)].  The band's
-// gradient slots follow its owners: each owner's render record gets its first
-// slot (its exclusive offset) as slot word, for a render backward without
-// slot_base.
+// gradient slots follow its owners: slot_words[g] = owner g's first slot (its
+// exclusive offset), for the render backward and the gather (other entries
+// are left alone).
 hipError_t launch_band_owners(const uint2* rect_d, const uint32_t* order, size_t n, uint32_t by0, uint32_t by1,
                               uint32_t* part_area, uint32_t* part_count, uint32_t* order_c, uint32_t* offsets_c,
-                              uint2* rect_c, float4* rec, hipStream_t s);
+                              uint2* rect_c, uint32_t* slot_words, hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs over key bits [0, bits).  When
 // vals_in is NULL the values are the element indices.  The result is in
 // (keys_out, vals_out) if the pass count is even... callers use the returned
@@ -155,22 +155,23 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
 // *blend_ok == 1): the forward's per-wave blend bits, which select the
 // (wave, instance) pairs to evaluate; else the quadrant masks do.
 // [band_y0, band_y1) (band_y1 < 0: to the last row): the tile rows rendered;
-// the lists and slots then hold the rect rows from band_y0 on.  slot_base
-// NULL: the record's slot word is the Gaussian's first slot.
+// the lists and slots then hold the rect rows from band_y0 on.  slot_words
+// (optional): each Gaussian's first slot, in place of slot_base plus the
+// record's slot word.
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
                                   float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s,
-                                  int band_y0 = 0, int band_y1 = -1);
+                                  int band_y0 = 0, int band_y1 = -1, const uint32_t* slot_words = nullptr);
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
                                const uint32_t* tiles_touched, const uint32_t* slot_local, const uint32_t* slot_base,
                                const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
                                float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
 // One band's records added to the per-Gaussian sums (the four arrays in the
-// layouts launch_gather_grads writes): Gaussian i owns the slots from its
-// render record's slot word on (launch_band_owners writes it), as many as its
-// rect (dup_rect) has tiles in the rows [by0, by1).
-hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const float4* rec,
+// layouts launch_gather_grads writes): Gaussian i owns the slots from
+// slot_words[i] on (launch_band_owners writes it), as many as its rect
+// (dup_rect) has tiles in the rows [by0, by1).
+hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const uint32_t* slot_words,
                               uint32_t nslots, const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
                               float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
 

@@ -336,9 +336,12 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
     ViewTable vt, const float* __restrict__ bg, const float4* __restrict__ rec, const float* __restrict__ final_Ts,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, const uint32_t* __restrict__ slot_base,
     uint32_t nslots, float4* __restrict__ grad_rec, float* __restrict__ grad_opacity,
-    const uint8_t* __restrict__ blend_in, const uint32_t* __restrict__ blend_ok, int band_y0) {
+    const uint8_t* __restrict__ blend_in, const uint32_t* __restrict__ blend_ok, int band_y0,
+    const uint32_t* __restrict__ slot_words) {
   // band_y0 (single view): the grid covers the tile rows from band_y0 on, and
-  // the lists and gradient slots hold only the rect rows from band_y0 on
+  // the lists and gradient slots hold only the rect rows from band_y0 on;
+  // slot_words (a band's): each Gaussian's first slot, in place of slot_base
+  // plus the record's slot word
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt, band_y0 * gx_);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
@@ -501,8 +504,9 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
       Rect rc = tile_rect(m, __float_as_int(r2.y), tg);
       rc.y0 = max(rc.y0, band_y0);
       const uint32_t gauss = kMulti ? g % (uint32_t)vt.P : g;
-      s_inst[tid] = (slot_base ? slot_base[gauss / kSlotBlock] : 0u) + __float_as_uint(r2.z) +
-                    rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
+      const uint32_t first =
+          slot_words ? slot_words[gauss] : slot_base[gauss / kSlotBlock] + __float_as_uint(r2.z);
+      s_inst[tid] = first + rect_local(rc, max(1, rc.x1 - rc.x0), tile % gx, tile / gx, tg);
       s_pa[tid] = make_float4(m.x, m.y, q.x, q.y);
       s_pb[tid] = make_float4(q.z, co.w, rr.rgb.x, rr.rgb.y);
       s_blue[tid] = rr.rgb.z;
@@ -652,11 +656,10 @@ __global__ __launch_bounds__(256) void gather_grads_kernel(GatherSrc src, int it
 
 // One band's records added to the per-Gaussian sums, one Gaussian per lane in
 // index order (as the whole-image gather): Gaussian i owns the slots
-// [first, first + its band area), first being its render record's slot word
-// (written by the band's owner pass).  Waves with no Gaussian in the band
-// leave at once.
+// [slot_words[i], + its band area) (written by the band's owner pass).  Waves
+// with no Gaussian in the band leave at once.
 __global__ __launch_bounds__(256) void band_gather_kernel(const uint2* __restrict__ dup_rect, uint32_t by0,
-                                                          uint32_t by1, int P, const float* __restrict__ rec_words,
+                                                          uint32_t by1, int P, const uint32_t* __restrict__ slot_words,
                                                           uint32_t nslots, const float4* __restrict__ grad_rec,
                                                           const float* __restrict__ grad_opacity,
                                                           float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
@@ -665,7 +668,7 @@ __global__ __launch_bounds__(256) void band_gather_kernel(const uint2* __restric
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t cnt = i < P ? dup_rect_area(band_dup_rect(dup_rect[i], by0, by1)) : 0u;
   if (__ballot(cnt != 0u) == 0ull) return;  // wave-uniform
-  const uint32_t base = cnt ? __float_as_uint(rec_words[12 * (size_t)i + 10]) : 0u;
+  const uint32_t base = cnt ? slot_words[i] : 0u;
   float acc[kGradTerms];
 #pragma unroll
   for (int q = 0; q < kGradTerms; ++q) acc[q] = 0.f;
@@ -723,7 +726,7 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
                                   float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s,
-                                  int band_y0, int band_y1) {
+                                  int band_y0, int band_y1, const uint32_t* slot_words) {
   if (band_y1 < 0 || band_y1 > grid.gy) band_y1 = grid.gy;
   if (band_y0 < 0 || band_y0 > band_y1) return hipErrorInvalidValue;
   const int tiles = (band_y1 - band_y0) * grid.gx;
@@ -731,7 +734,7 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
   ViewTable none{};
   return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
                                      point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, slot_base, nslots,
-                                     grad_rec, grad_opacity, blend, blend_ok, band_y0);
+                                     grad_rec, grad_opacity, blend, blend_ok, band_y0, slot_words);
 }
 
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
@@ -745,13 +748,12 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const float4* rec,
+hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const uint32_t* slot_words,
                               uint32_t nslots, const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
                               float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
   if (P == 0 || nslots == 0) return hipSuccess;
   hipLaunchKernelGGL(band_gather_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, dup_rect, by0, by1, P,
-                     reinterpret_cast<const float*>(rec), nslots, grad_rec, grad_opacity, dL_dmean2D, dL_dconic,
-                     dL_dopacity, dL_dcolors);
+                     slot_words, nslots, grad_rec, grad_opacity, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
   return hipGetLastError();
 }
 
@@ -773,7 +775,7 @@ hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
                                     final_T, n_contrib, dL_dpix, slot_base, nslots, grad_rec, grad_opacity, blend,
-                                    blend_ok, 0);
+                                    blend_ok, 0, (const uint32_t*)nullptr);
 }
 
 }  // namespace omnigs

@@ -750,12 +750,11 @@ __global__ __launch_bounds__(kScanThreads) void band_partials_kernel(uint32_t* _
   scan_partials_block(blockIdx.x == 0 ? part_area : part_count, nb);
 }
 
-// Pass 3: the owners' entries, compacted, and each owner's first slot into
-// its render record's slot word (word 2 of its third float4, store_rec).
+// Pass 3: the owners' entries, compacted, and each owner's first slot.
 __global__ __launch_bounds__(kScanThreads) void band_owners_kernel(
     const uint2* __restrict__ rect_d, const uint32_t* __restrict__ order, size_t n, uint32_t by0, uint32_t by1,
     const uint32_t* __restrict__ part_area, const uint32_t* __restrict__ part_count, uint32_t* __restrict__ order_c,
-    uint32_t* __restrict__ offsets_c, uint2* __restrict__ rect_c, float* __restrict__ rec_words) {
+    uint32_t* __restrict__ offsets_c, uint2* __restrict__ rect_c, uint32_t* __restrict__ slot_words) {
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
   uint2 r[kScanItems];
   uint32_t area, count, tot;
@@ -767,7 +766,7 @@ __global__ __launch_bounds__(kScanThreads) void band_owners_kernel(
     const uint32_t a = dup_rect_area(r[k]);
     if (a) {
       const uint32_t g = order[base + (size_t)k];
-      rec_words[12 * (size_t)g + 10] = __uint_as_float(off);
+      slot_words[g] = off;
       off += a;
       order_c[pos] = g;
       offsets_c[pos] = off;
@@ -779,7 +778,7 @@ __global__ __launch_bounds__(kScanThreads) void band_owners_kernel(
 
 hipError_t launch_band_owners(const uint2* rect_d, const uint32_t* order, size_t n, uint32_t by0, uint32_t by1,
                               uint32_t* part_area, uint32_t* part_count, uint32_t* order_c, uint32_t* offsets_c,
-                              uint2* rect_c, float4* rec, hipStream_t s) {
+                              uint2* rect_c, uint32_t* slot_words, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const size_t nb = scan_blocks(n);
   hipLaunchKernelGGL(band_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, rect_d, n, by0, by1,
@@ -787,7 +786,7 @@ hipError_t launch_band_owners(const uint2* rect_d, const uint32_t* order, size_t
   hipLaunchKernelGGL(band_partials_kernel, dim3(2), dim3(kScanThreads), 0, s, part_area, part_count, nb);
   hipLaunchKernelGGL(band_owners_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, rect_d, order, n, by0, by1,
                      (const uint32_t*)part_area, (const uint32_t*)part_count, order_c, offsets_c, rect_c,
-                     reinterpret_cast<float*>(rec));
+                     slot_words);
   return hipGetLastError();
 }
 
