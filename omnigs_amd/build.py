@@ -38,6 +38,13 @@ HIP_FLAGS = [
 ]
 
 
+# Per-source additions.  mlp.hip: MFMA results in ordinary VGPRs (its
+# training kernel runs one wave per SIMD; left to itself the compiler writes
+# every MFMA result to accumulation registers and copies each back before
+# use).
+HIP_FILE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _run(cmd: list[str]) -> None:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -53,7 +60,7 @@ def build_hip(jobs: int = 8) -> str:
     for src in HIP_SOURCES:
         obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *HIP_FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for f in [ex.submit(_run, c) for c in cmds]:
             f.result()

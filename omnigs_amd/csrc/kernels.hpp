@@ -196,7 +196,7 @@ hipError_t launch_preprocess_backward_views(const PreprocessBackwardArgs& a, con
                                             float* dL_dopacity, hipStream_t s);
 
 // mlp.hip
-constexpr int kMlpTrainBlocks = 512;  // persistent grid of the decoder training pass
+constexpr int kMlpTrainBlocks = 256;  // persistent grid of the decoder training pass (one block per CU)
 constexpr int kThetaReduceRows = 32;  // rows per stage-1 block of the weight-gradient reduction
 int mlp_train_blocks(int P);
 size_t mlp_workspace_floats(int P);

@@ -3,15 +3,15 @@
 // colours came from the decoder), the deterministic reduction of the weight
 // gradient, Adam, and FP32 -> BF16 conversion.  Device math in mlp.hpp.
 //
-// Training pass layout: 256-thread blocks (4 waves) walk chunks of 64
-// Gaussians.  Wave w runs the forward and backward chain of column block w
-// (Gaussians 16w..16w+15 of the chunk), then stages its activations and
-// adjoints as BF16 [unit][Gaussian] rows in LDS; after a barrier each wave
-// computes a quarter of the weight-gradient tiles with K = the chunk's 64
-// Gaussians (two MFMA K steps) and keeps them in registers for the whole
-// kernel.  Bias gradients come from MFMAs against an all-ones B operand.
-// Each block finally writes its partial gradient row; a two-stage kernel adds
-// the rows in a fixed order.
+// Training pass layout: every wave walks its own pairs of 16-Gaussian
+// column blocks.  Per block it runs the forward and backward chains, turns
+// the activations and adjoints into K-over-Gaussians operands on the matrix
+// cores (an MFMA against a 0/1 selector transposes a tile exactly), and adds
+// the block's weight-gradient tiles into registers it keeps for the whole
+// kernel: no LDS staging and no barrier inside the loop.  Bias gradients come
+// from MFMAs against all-ones rows.  At the end the four waves of a block add
+// their gradients in a fixed order through LDS, each block writes one partial
+// row, and a two-stage kernel adds the rows in a fixed order.
 #include "kernels.hpp"
 #include "lonlat_math.hpp"
 #include "mlp.hpp"
@@ -77,12 +77,15 @@ __device__ __forceinline__ void stage_weights(char* dst, const char* __restrict_
   for (int i = threadIdx.x; i < n / 16; i += blockDim.x) d[i] = s[i];
 }
 
-// Unit direction of a world-space vector (0, 0, 1 for a zero vector).
+// Unit direction of a world-space vector (0, 0, 1 for a zero vector),
+// computed by every lane (selects, no branch).
 __device__ __forceinline__ float3 unit_dir(float3 v, float* len_out) {
   const float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
   *len_out = len;
-  if (!(len > 0.f)) return make_float3(0.f, 0.f, 1.f);
-  return make_float3(v.x / len, v.y / len, v.z / len);
+  const bool pos = len > 0.f;
+  const float l = pos ? len : 1.f;
+  const float3 d = make_float3(v.x / l, v.y / l, v.z / l);
+  return pos ? d : make_float3(0.f, 0.f, 1.f);
 }
 
 __device__ __forceinline__ bf16x8 load_latent_half(const float* __restrict__ lat, size_t G, int q, bool ok) {
@@ -156,245 +159,506 @@ struct MlpTrainArgs {
   float* theta_partial;  // [gridDim.x][kThetaStride]; column kThetaSize holds the block's squared error
 };
 
-constexpr int kStageRow = 64 + 8;  // bf16 per staged row (64 Gaussians + pad), 144 B
-constexpr int kRowsX = 32, kRowsH = 64, kRowsO = 16;
-constexpr int kStageRows = kRowsX + 4 * kRowsH + kRowsO;  // X, H1, H2, dH1, dH2, dO
+// Weight-gradient operands need the Gaussians on the MFMA K index, while the
+// chains above hold them on the lane index (D tiles, one Gaussian per lane
+// c).  The matrix cores transpose: mfma(z, S) = M(z) M(S)^T, where M(r) is the
+// 16x32 matrix a register holds in A-operand order.  When row j of M(S) has a
+// single 1, at slot k(j), lane (j, q) element e of the result holds slot
+// k(j) of Gaussian 4q + e: unit j of a tile on the lane, four Gaussians on
+// its elements.  The values are BF16 operands times 1 plus zeros, so the
+// FP32 result is exact and packs back to BF16 without rounding.  Selectors,
+// in B-operand lane order:
+enum : int {
+  kSelLo,    // hidden pack, even tile: unit j at slot (q = j/4, e = j%4)
+  kSelHi,    // hidden pack, odd tile: slot (q = j/4, e = 4 + j%4)
+  kSelInLo,  // layer-1 input, inputs 0..15: slot j
+  kSelInHi,  // inputs 16..31: slot 16 + j
+  kSelDO,    // output adjoint: row j = 11..13 takes output j - 11
+  kSelOne9,  // all ones in row 9 (a K = 16 operand: elements 0..3)
+  kSelOne10,
+  kSelOne14,
+  kNumSel
+};
+
+__device__ __forceinline__ bf16x8 make_selector(int sel, int lane) {
+  const int j = lane & 15, q = lane >> 4;
+  int e = -1;
+  bool all = false;
+  if (sel == kSelLo) e = q == (j >> 2) ? (j & 3) : -1;
+  else if (sel == kSelHi) e = q == (j >> 2) ? 4 + (j & 3) : -1;
+  else if (sel == kSelInLo) e = q == (j >> 3) ? (j & 7) : -1;
+  else if (sel == kSelInHi) e = q == 2 + (j >> 3) ? (j & 7) : -1;
+  else if (sel == kSelDO) e = (q == 0 && j >= 11 && j < 14) ? j - 11 : -1;
+  else all = j == (sel == kSelOne9 ? 9 : sel == kSelOne10 ? 10 : 14);
+  bf16x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = (__bf16)((all || k == e) ? 1.f : 0.f);
+  return r;
+}
+
+// Four BF16-exact FP32 values as BF16 bits (two dwords).
+__device__ __forceinline__ uint2 bf16x4_bits(f32x4 v) {
+  return make_uint2(cvt_pk_bf16(v[0], v[1]), cvt_pk_bf16(v[2], v[3]));
+}
+
+// One column block's transposed tile (four Gaussians per lane).
+__device__ __forceinline__ uint2 tr_half(const bf16x8& z, const bf16x8& sel) {
+  return bf16x4_bits(mfma16(z, sel, f32x4{0.f, 0.f, 0.f, 0.f}));
+}
+
+// A weight-gradient operand over 32 Gaussians: slot (q, e) is Gaussian
+// 4q + e of the first column block (e < 4) or 4q + e - 4 of the second.
+__device__ __forceinline__ bf16x8 join_halves(uint2 a, uint2 b) {
+  return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, b.x, b.y));
+}
+
+// ReLU derivative on BF16 packs: element e of g where element e of the
+// post-ReLU activation h (relu_pack_bf16: +0 or positive) is nonzero, else
+// zero.  Per dword, min(h, 1) as unsigned 16-bit halves is the 0/1 mask and
+// a 16-bit multiply applies it: two packed ops (v_pk_min_u16,
+// v_pk_mul_lo_u16), written out because the compiler turns the same vector
+// code into per-half compares and selects.
+__device__ __forceinline__ bf16x8 relu_mask(const bf16x8& h, const bf16x8& g) {
+  const uint4 hb = __builtin_bit_cast(uint4, h), gb = __builtin_bit_cast(uint4, g);
+  const uint32_t hv[4] = {hb.x, hb.y, hb.z, hb.w}, gv[4] = {gb.x, gb.y, gb.z, gb.w};
+  const uint32_t one = 0x00010001u;
+  uint32_t r[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t m;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(hv[i]), "v"(one));
+    asm("v_pk_mul_lo_u16 %0, %1, %2" : "=v"(r[i]) : "v"(gv[i]), "v"(m));
+  }
+  return __builtin_bit_cast(bf16x8, make_uint4(r[0], r[1], r[2], r[3]));
+}
+
+__device__ __forceinline__ float sigmoid_rcp(float v) { return __builtin_amdgcn_rcpf(1.f + __expf(-v)); }
+
+// 32-bit cross-row moves: the value lane (c, 1) holds (row), or lane (c, 2)
+// (row2), delivered to lane (c, 0).  A permlane swap of a register with
+// itself leaves {own, other} in the two results of row 0 (in an order the
+// XOR with the own value makes irrelevant).
+__device__ __forceinline__ float from_row1(float v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0] ^ r[1] ^ __float_as_int(v));
+}
+__device__ __forceinline__ float from_row2(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0] ^ r[1] ^ __float_as_int(v));
+}
+// Sum over the four 16-lane rows, on every lane.
+__device__ __forceinline__ float row_sum(float v) {
+  auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+  const float a = __int_as_float(r[0]) + __int_as_float(r[1]);
+  r = __builtin_amdgcn_permlane16_swap(__float_as_int(a), __float_as_int(a), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+
+// Column blocks a wave carries through each step: independent chains that
+// hide one another's MFMA and LDS latency (one wave per SIMD holds them).
+constexpr int kTrainCB = 4;
 
 template <bool kDistill>
-__global__ __launch_bounds__(256) void mlp_train_kernel(MlpTrainArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void mlp_train_kernel(MlpTrainArgs a) {
+  constexpr int NCB = kTrainCB, NP = NCB / 2;  // column blocks, and pairs of them (K = 32 Gaussians)
   __shared__ __attribute__((aligned(16))) char s_w[kMlpPackedBytes];
-  __shared__ __attribute__((aligned(16))) __bf16 s_st[kStageRows][kStageRow];
+  __shared__ __attribute__((aligned(16))) bf16x8 s_sel[kNumSel][64];
   __shared__ float s_err[4];
-  __bf16 (*sX)[kStageRow] = s_st;
-  __bf16 (*sH1)[kStageRow] = s_st + kRowsX;
-  __bf16 (*sH2)[kStageRow] = sH1 + kRowsH;
-  __bf16 (*sD1)[kStageRow] = sH2 + kRowsH;
-  __bf16 (*sD2)[kStageRow] = sD1 + kRowsH;
-  __bf16 (*sO)[kStageRow] = sD2 + kRowsH;
 
   stage_weights(s_w, a.packed, kMlpPackedBytes);
-  for (int i = threadIdx.x; i < kRowsO * kStageRow; i += blockDim.x) (&sO[0][0])[i] = (__bf16)0.f;
+  for (int i = threadIdx.x; i < kNumSel * 64; i += blockDim.x) s_sel[i >> 6][i & 63] = make_selector(i >> 6, i & 63);
   __syncthreads();
 
   const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
   const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
-  bf16x8 ones;
+  const bf16x8* W1A = reinterpret_cast<const bf16x8*>(s_w + kPkW1A);
+  const bf16x8* W2A = reinterpret_cast<const bf16x8*>(s_w + kPkW2A);
+  const bf16x8* W3A = reinterpret_cast<const bf16x8*>(s_w + kPkW3A);
+  const bf16x8* W3T = reinterpret_cast<const bf16x8*>(s_w + kPkW3T);
+  const bf16x8* W2T = reinterpret_cast<const bf16x8*>(s_w + kPkW2T);
+  const bf16x8* W1T = reinterpret_cast<const bf16x8*>(s_w + kPkW1T);
+  const f32x4* B1 = reinterpret_cast<const f32x4*>(s_w + kPkB1);
+  const f32x4* B2 = reinterpret_cast<const f32x4*>(s_w + kPkB2);
+  // The wave's whole weight gradient, in D-tile order: gW2[nt][mt] holds
+  // dW2 rows 16nt.., columns 16mt..; gW1[nt][it] dW1 rows 16nt.., inputs
+  // 16it.. (it = 1: inputs 16..24 in columns 0..8), and in the free columns
+  // of gW1[nt][1]: 9 = db1, 10 = db2, 11..13 = dW3^T, and (nt = 0, rows
+  // 11..13) 14 = db3.
+  f32x4 gW2[4][4], gW1[4][2];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.f;
-  // weight-gradient tiles owned by this wave: dW2 row tile w (4 column tiles),
-  // dW1 row tile w (2), db1 / db2 row tile w, dW3 column tile w, db3 (wave 0)
-  f32x4 gW2[4] = {z4, z4, z4, z4}, gW1[2] = {z4, z4}, gB1 = z4, gB2 = z4, gW3 = z4, gB3 = z4;
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gW2[i][k] = z4;
+    gW1[i][0] = z4;
+    gW1[i][1] = z4;
+  }
   float sq_err = 0.f;
   const float inv3P = 1.f / (3.f * (float)a.P);
 
-  // Per-lane inputs of one chunk, loaded a chunk ahead so that the HBM
-  // latency overlaps the previous chunk's MFMA work.
-  struct ChunkIn {
-    bf16x8 lat;    // latent inputs 8q..8q+7 (q < 2)
-    float3 m;      // mean (render backward)
-    float3 dc;     // dL/dcolor (render backward, q = 0)
-    float3 dm;     // dL/dmean3D so far (render backward, q = 0)
-    int rad;       // radius (render backward, q = 0)
-    float sh[12];  // SH coefficients 4q..4q+3 (distillation)
+  const int ngroups = (a.P + 16 * NCB - 1) / (16 * NCB);
+  const int nwaves = (int)gridDim.x * 4;
+  const int nviews = kDistill ? 1 : a.V;
+  // Per column block: Gaussian G (lane c) and its inputs, loaded a group
+  // ahead.  Loads use a valid index for every lane; results of Gaussians past
+  // P are masked.
+  struct GroupIn {
+    bf16x8 lat[NCB];
+    float3 m[NCB], dm[NCB];
   };
-  auto load_chunk = [&](int chunk, ChunkIn& in) {
-    const size_t G = (size_t)chunk * 64 + 16 * w + c;
-    const bool ok = G < (size_t)a.P;
-    if constexpr (kDistill) {
-      in.lat = load_latent_half(a.latent_f32, G, q & 1, ok && q < 2);
-      const float* sh = a.shs + (G * a.M + 4 * q) * 3;
+  auto load_group = [&](int grp, GroupIn& in) {
 #pragma unroll
-      for (int i = 0; i < 12; ++i) in.sh[i] = (ok && 4 * q + i / 3 < a.M) ? sh[i] : 0.f;
-    } else {
-      in.lat = load_latent_half(a.latent_bf16, G, q & 1, ok && q < 2);
-      in.m = ok ? make_float3(a.means3D[3 * G], a.means3D[3 * G + 1], a.means3D[3 * G + 2])
-                : make_float3(0.f, 0.f, 0.f);
-      const bool own = ok && q == 0;
-      in.dc = own ? make_float3(a.dL_dcolor[3 * G], a.dL_dcolor[3 * G + 1], a.dL_dcolor[3 * G + 2])
-                  : make_float3(0.f, 0.f, 0.f);
-      in.dm = own ? make_float3(a.dL_dmean3D[3 * G], a.dL_dmean3D[3 * G + 1], a.dL_dmean3D[3 * G + 2])
-                  : make_float3(0.f, 0.f, 0.f);
-      in.rad = own ? a.radii[G] : 0;
+    for (int cb = 0; cb < NCB; ++cb) {
+      const size_t G = (size_t)grp * 16 * NCB + 16 * cb + c;
+      const size_t Gv = G < (size_t)a.P ? G : 0;
+      if constexpr (kDistill) {
+        const float4 x = *reinterpret_cast<const float4*>(a.latent_f32 + Gv * kLatent + 8 * (q & 1));
+        const float4 y = *reinterpret_cast<const float4*>(a.latent_f32 + Gv * kLatent + 8 * (q & 1) + 4);
+        in.lat[cb] = pack_bf16(f32x4{x.x, x.y, x.z, x.w}, f32x4{y.x, y.y, y.z, y.w});
+        in.m[cb] = in.dm[cb] = make_float3(0.f, 0.f, 0.f);
+      } else {
+        in.lat[cb] = *reinterpret_cast<const bf16x8*>(a.latent_bf16 + Gv * kLatent + 8 * (q & 1));
+        in.m[cb] = make_float3(a.means3D[3 * Gv], a.means3D[3 * Gv + 1], a.means3D[3 * Gv + 2]);
+        in.dm[cb] = make_float3(a.dL_dmean3D[3 * Gv], a.dL_dmean3D[3 * Gv + 1], a.dL_dmean3D[3 * Gv + 2]);
+      }
     }
   };
-
-  const int nchunks = (a.P + 63) / 64;
-  const int nviews = kDistill ? 1 : a.V;
-  ChunkIn cur{}, nxt{};
-  if ((int)blockIdx.x < nchunks) load_chunk(blockIdx.x, cur);
-  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    if (chunk + (int)gridDim.x < nchunks) load_chunk(chunk + gridDim.x, nxt);
-    const size_t G = (size_t)chunk * 64 + 16 * w + c;
-    const bool ok = G < (size_t)a.P;
-    float4 dlat = make_float4(0.f, 0.f, 0.f, 0.f);
-    float3 dmv = make_float3(0.f, 0.f, 0.f);
-    bool any_dir = false;
+  GroupIn nxt;
+  if ((int)blockIdx.x * 4 + w < ngroups) load_group((int)blockIdx.x * 4 + w, nxt);
+  for (int grp = (int)blockIdx.x * 4 + w; grp < ngroups; grp += nwaves) {
+    const GroupIn cur = nxt;
+    if (grp + nwaves < ngroups) load_group(grp + nwaves, nxt);
+    size_t Gs[NCB];
+    bool ok[NCB];
+    bf16x8 lat[NCB];
+    float3 m[NCB], dm[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const size_t G = (size_t)grp * 16 * NCB + 16 * cb + c;
+      ok[cb] = G < (size_t)a.P;
+      Gs[cb] = ok[cb] ? G : 0;
+      lat[cb] = (ok[cb] && q < 2) ? cur.lat[cb] : zero_bf16x8();
+      m[cb] = cur.m[cb];
+      dm[cb] = cur.dm[cb];
+    }
+    float4 dlat[NCB];
+    float3 dmv[NCB];
+    bool any_dir[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      dlat[cb] = make_float4(0.f, 0.f, 0.f, 0.f);
+      dmv[cb] = make_float3(0.f, 0.f, 0.f);
+      any_dir[cb] = false;
+    }
     for (int vi = 0; vi < nviews; ++vi) {
-      // view vi: items vi P + G (one view in distillation)
-      const size_t item = (size_t)vi * a.P + G;
-      const float3 dcv3 = vi == 0 ? cur.dc
-                                  : ((ok && q == 0) ? make_float3(a.dL_dcolor[3 * item], a.dL_dcolor[3 * item + 1],
-                                                                  a.dL_dcolor[3 * item + 2])
-                                                    : make_float3(0.f, 0.f, 0.f));
-      const int rad = vi == 0 ? cur.rad : ((ok && q == 0) ? a.radii[item] : 0);
-      // direction (every q lane of Gaussian c holds it)
-      float3 d = make_float3(0.f, 0.f, 1.f), v = make_float3(0.f, 0.f, 0.f);
-      float len = 1.f;
-      if (ok) {
-        if (kDistill) {
-          d = seeded_direction(a.seed, (uint32_t)G, distill_dir_index(a.seed, a.step, (uint32_t)G, a.num_dirs));
+      // keeps the weight and selector reads from LDS inside the loop (hoisted,
+      // they would hold 150 registers for the whole kernel)
+      asm volatile("" ::: "memory");
+      // direction (every q lane of Gaussian c holds it), layer-1 input, and
+      // the loads the adjoint needs
+      float3 d[NCB];
+      float len[NCB];
+      bf16x8 xin[NCB];
+      float3 dcv[NCB];
+      int rad[NCB];
+      float sh[NCB][12];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        len[cb] = 1.f;
+        rad[cb] = 0;
+        dcv[cb] = make_float3(0.f, 0.f, 0.f);
+        if constexpr (kDistill) {
+          d[cb] = seeded_direction(a.seed, (uint32_t)Gs[cb],
+                                   distill_dir_index(a.seed, a.step, (uint32_t)Gs[cb], a.num_dirs));
+          const float* src = a.shs + (Gs[cb] * a.M + 4 * q) * 3;
+#pragma unroll
+          for (int i = 0; i < 12; ++i) sh[cb][i] = (4 * q + i / 3 < a.M) ? src[i] : 0.f;
         } else {
           const float3 cp = make_float3(a.campos[3 * vi], a.campos[3 * vi + 1], a.campos[3 * vi + 2]);
-          v = make_float3(cur.m.x - cp.x, cur.m.y - cp.y, cur.m.z - cp.z);
-          d = unit_dir(v, &len);
+          d[cb] = unit_dir(make_float3(m[cb].x - cp.x, m[cb].y - cp.y, m[cb].z - cp.z), &len[cb]);
+          const size_t item = (size_t)vi * a.P + Gs[cb];  // view vi's copy of Gaussian G
+          dcv[cb] = make_float3(a.dL_dcolor[3 * item], a.dL_dcolor[3 * item + 1], a.dL_dcolor[3 * item + 2]);
+          rad[cb] = a.radii[item];
         }
+        xin[cb] = input_operand(q, lat[cb], d[cb].x, d[cb].y, d[cb].z);
       }
-      const bf16x8 xin = input_operand(q, cur.lat, d.x, d.y, d.z);
-      MlpAct act;
-      mlp_forward_block(s_w, lane, xin, act);
 
-      // output adjoint (lanes q = 0, rows 0..2)
-      f32x4 dO = z4;
-      if (kDistill) {
-        // teacher: degree-3 SH colour + 0.5, clamped at 0; lane q adds coefficients 4q..4q+3
-        float b[16];
-        sh_basis(3, d.x, d.y, d.z, b);
-        float3 t = make_float3(0.f, 0.f, 0.f);
-  #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          t.x += b[4 * q + i] * cur.sh[3 * i];
-          t.y += b[4 * q + i] * cur.sh[3 * i + 1];
-          t.z += b[4 * q + i] * cur.sh[3 * i + 2];
-        }
-        t.x += __shfl_xor(t.x, 16);
-        t.y += __shfl_xor(t.y, 16);
-        t.z += __shfl_xor(t.z, 16);
-        t.x += __shfl_xor(t.x, 32);
-        t.y += __shfl_xor(t.y, 32);
-        t.z += __shfl_xor(t.z, 32);
-        t = make_float3(fmaxf(t.x + 0.5f, 0.f), fmaxf(t.y + 0.5f, 0.f), fmaxf(t.z + 0.5f, 0.f));
-        if (ok && q == 0) {
-          const float tt[3] = {t.x, t.y, t.z};
-  #pragma unroll
+      // forward chains (units on the elements, Gaussian c on the lane); each
+      // weight operand read once for all column blocks
+      bf16x8 x2[NCB][2], x3[NCB][2];
+      f32x4 t[NCB][4];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const bf16x8 wa = W1A[rt * 64 + lane];
+        const f32x4 bias = B1[rt * 4 + q];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) t[cb][rt] = mfma16(wa, xin[cb], bias);
+      }
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        x2[cb][0] = relu_pack_bf16(t[cb][0], t[cb][1]);
+        x2[cb][1] = relu_pack_bf16(t[cb][2], t[cb][3]);
+      }
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const bf16x8 wa0 = W2A[(rt * 2) * 64 + lane], wa1 = W2A[(rt * 2 + 1) * 64 + lane];
+        const f32x4 bias = B2[rt * 4 + q];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) t[cb][rt] = mfma16(wa1, x2[cb][1], mfma16(wa0, x2[cb][0], bias));
+      }
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        x3[cb][0] = relu_pack_bf16(t[cb][0], t[cb][1]);
+        x3[cb][1] = relu_pack_bf16(t[cb][2], t[cb][3]);
+      }
+      f32x4 o[NCB];
+      {
+        const bf16x8 wa0 = W3A[lane], wa1 = W3A[64 + lane];
+        const f32x4 b3 = q == 0 ? *reinterpret_cast<const f32x4*>(s_w + kPkB3) : z4;
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) o[cb] = mfma16(wa1, x3[cb][1], mfma16(wa0, x3[cb][0], b3));
+      }
+
+      // output adjoints (lanes q = 0, rows 0..2) as the B operand bo
+      bf16x8 bo[NCB];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        const bool own = ok[cb] && q == 0;
+        float dO[3];
+        if constexpr (kDistill) {
+          // teacher: degree-3 SH colour + 0.5, clamped at 0; lane q adds coefficients 4q..4q+3
+          float b[16];
+          sh_basis(3, d[cb].x, d[cb].y, d[cb].z, b);
+          float3 tc = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            tc.x += b[4 * q + i] * sh[cb][3 * i];
+            tc.y += b[4 * q + i] * sh[cb][3 * i + 1];
+            tc.z += b[4 * q + i] * sh[cb][3 * i + 2];
+          }
+          const float tt[3] = {fmaxf(row_sum(tc.x) + 0.5f, 0.f), fmaxf(row_sum(tc.y) + 0.5f, 0.f),
+                               fmaxf(row_sum(tc.z) + 0.5f, 0.f)};
+#pragma unroll
           for (int e = 0; e < 3; ++e) {
-            const float s = sigmoidf(act.o[e]);
-            const float err = s - tt[e];
+            const float s = sigmoid_rcp(o[cb][e]);
+            const float err = own ? s - tt[e] : 0.f;
             sq_err += err * err;
             dO[e] = 2.f * err * inv3P * s * (1.f - s);
           }
+        } else {
+          const float dc[3] = {dcv[cb].x, dcv[cb].y, dcv[cb].z};
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            const float s = sigmoid_rcp(o[cb][e]);
+            dO[e] = own ? dc[e] * s * (1.f - s) : 0.f;
+          }
         }
-      } else if (ok && q == 0) {
-        const float dcv[3] = {dcv3.x, dcv3.y, dcv3.z};
-  #pragma unroll
-        for (int e = 0; e < 3; ++e) {
-          const float s = sigmoidf(act.o[e]);
-          dO[e] = dcv[e] * s * (1.f - s);
-        }
-      }
-      MlpGrad gr;
-      mlp_backward_block(s_w, lane, act, dO, gr);
-
-      dlat.x += gr.dx[0][0];
-      dlat.y += gr.dx[0][1];
-      dlat.z += gr.dx[0][2];
-      dlat.w += gr.dx[0][3];
-      if (!kDistill) {
-        // basis adjoint 4q..4q+3 held here -> partial direction gradient, summed over q
-        float gb[9];
-  #pragma unroll
-        for (int i = 0; i < 9; ++i) gb[i] = 0.f;
-  #pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (4 * q + e < 9) gb[4 * q + e] = gr.dx[1][e];
-        float3 gd = sh2_basis_vjp(d.x, d.y, d.z, gb);
-        gd.x += __shfl_xor(gd.x, 16);
-        gd.y += __shfl_xor(gd.y, 16);
-        gd.z += __shfl_xor(gd.z, 16);
-        gd.x += __shfl_xor(gd.x, 32);
-        gd.y += __shfl_xor(gd.y, 32);
-        gd.z += __shfl_xor(gd.z, 32);
-        if (ok && q == 0 && rad > 0 && len > 0.f) {
-          // d(v/|v|)/dv applied to gd (dnormvdv, auxiliary.h:134-144)
-          const float dot = d.x * gd.x + d.y * gd.y + d.z * gd.z;
-          const float il = 1.f / len;
-          dmv.x += (gd.x - d.x * dot) * il;
-          dmv.y += (gd.y - d.y * dot) * il;
-          dmv.z += (gd.z - d.z * dot) * il;
-          any_dir = true;
-        }
+        bo[cb] = q == 0 ? pack_bf16(f32x4{dO[0], dO[1], dO[2], 0.f}, z4) : zero_bf16x8();
       }
 
-      // stage this column block (columns 16w + c) for the weight gradients
-      const int col = 16 * w + c;
-  #pragma unroll
-      for (int e = 0; e < 8; ++e) sX[8 * q + e][col] = xin[e];
-  #pragma unroll
+      // backward chains, each adjoint's weight-gradient products taken as soon
+      // as its transposed tiles exist; column blocks 2p and 2p + 1 make one
+      // K = 32 operand (slots e < 4: block 2p)
+      bf16x8 b2[NCB][2];
+#pragma unroll
       for (int rt = 0; rt < 4; ++rt) {
-  #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int r = 16 * rt + 4 * q + e;
-          sH1[r][col] = (__bf16)act.h1[rt][e];
-          sH2[r][col] = (__bf16)act.h2[rt][e];
-          sD1[r][col] = (__bf16)gr.dh1[rt][e];
-          sD2[r][col] = (__bf16)gr.dh2[rt][e];
+        const bf16x8 wt = W3T[rt * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) t[cb][rt] = mfma16(wt, bo[cb], z4);
+      }
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        b2[cb][0] = relu_mask(x3[cb][0], pack_bf16(t[cb][0], t[cb][1]));
+        b2[cb][1] = relu_mask(x3[cb][1], pack_bf16(t[cb][2], t[cb][3]));
+      }
+      // dW2 = dH2 H1^T, db2 (column 10 of gW1[.][1])
+      {
+        const bf16x8 sLo = s_sel[kSelLo][lane], sHi = s_sel[kSelHi][lane];
+        bf16x8 d2[NP][4];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) {
+            const bf16x8& s = (nt & 1) ? sHi : sLo;
+            d2[p][nt] = join_halves(tr_half(b2[2 * p][nt >> 1], s), tr_half(b2[2 * p + 1][nt >> 1], s));
+          }
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          const bf16x8& s = (mt & 1) ? sHi : sLo;
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            const bf16x8 h1 = join_halves(tr_half(x2[2 * p][mt >> 1], s), tr_half(x2[2 * p + 1][mt >> 1], s));
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) gW2[nt][mt] = mfma16(d2[p][nt], h1, gW2[nt][mt]);
+          }
+        }
+        const bf16x8 one10 = s_sel[kSelOne10][lane];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) gW1[nt][1] = mfma16(d2[p][nt], one10, gW1[nt][1]);
         }
       }
-      if (q == 0) {
-  #pragma unroll
-        for (int e = 0; e < 3; ++e) sO[e][col] = (__bf16)dO[e];
+      bf16x8 b1[NCB][2];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const bf16x8 wt0 = W2T[(rt * 2) * 64 + lane], wt1 = W2T[(rt * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) t[cb][rt] = mfma16(wt1, b2[cb][1], mfma16(wt0, b2[cb][0], z4));
       }
-      __syncthreads();
-  #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        const int k0 = 32 * ks + 8 * q;
-        const bf16x8 aD2 = *reinterpret_cast<const bf16x8*>(&sD2[16 * w + c][k0]);
-        const bf16x8 aD1 = *reinterpret_cast<const bf16x8*>(&sD1[16 * w + c][k0]);
-  #pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
-          gW2[ct] = mfma16(aD2, *reinterpret_cast<const bf16x8*>(&sH1[16 * ct + c][k0]), gW2[ct]);
-  #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-          gW1[ct] = mfma16(aD1, *reinterpret_cast<const bf16x8*>(&sX[16 * ct + c][k0]), gW1[ct]);
-        gB1 = mfma16(aD1, ones, gB1);
-        gB2 = mfma16(aD2, ones, gB2);
-        const bf16x8 aO = *reinterpret_cast<const bf16x8*>(&sO[c][k0]);
-        gW3 = mfma16(aO, *reinterpret_cast<const bf16x8*>(&sH2[16 * w + c][k0]), gW3);
-        if (w == 0) gB3 = mfma16(aO, ones, gB3);
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        b1[cb][0] = relu_mask(x2[cb][0], pack_bf16(t[cb][0], t[cb][1]));
+        b1[cb][1] = relu_mask(x2[cb][1], pack_bf16(t[cb][2], t[cb][3]));
       }
-      __syncthreads();
+      f32x4 dx[NCB][2];
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt) {
+        const bf16x8 wt0 = W1T[(rt * 2) * 64 + lane], wt1 = W1T[(rt * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) dx[cb][rt] = mfma16(wt1, b1[cb][1], mfma16(wt0, b1[cb][0], z4));
+      }
+      // dW1 = dH1 X^T, db1 (column 9)
+      {
+        const bf16x8 sLo = s_sel[kSelLo][lane], sHi = s_sel[kSelHi][lane];
+        const bf16x8 sIL = s_sel[kSelInLo][lane], sIH = s_sel[kSelInHi][lane], one9 = s_sel[kSelOne9][lane];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const bf16x8 xt0 = join_halves(tr_half(xin[2 * p], sIL), tr_half(xin[2 * p + 1], sIL));
+          const bf16x8 xt1 = join_halves(tr_half(xin[2 * p], sIH), tr_half(xin[2 * p + 1], sIH));
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) {
+            const bf16x8& s = (nt & 1) ? sHi : sLo;
+            const bf16x8 d1 = join_halves(tr_half(b1[2 * p][nt >> 1], s), tr_half(b1[2 * p + 1][nt >> 1], s));
+            gW1[nt][0] = mfma16(d1, xt0, gW1[nt][0]);
+            gW1[nt][1] = mfma16(d1, xt1, gW1[nt][1]);
+            gW1[nt][1] = mfma16(d1, one9, gW1[nt][1]);
+          }
+        }
+      }
+      // dW3^T = H2 dO^T (columns 11..13), db3 (rows 11..13 of column 14, tile 0)
+      {
+        const bf16x8 sLo = s_sel[kSelLo][lane], sHi = s_sel[kSelHi][lane], sDO = s_sel[kSelDO][lane];
+        const bf16x8 one14 = s_sel[kSelOne14][lane];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const bf16x8 dot_ = join_halves(tr_half(bo[2 * p], sDO), tr_half(bo[2 * p + 1], sDO));
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) {
+            const bf16x8& s = (nt & 1) ? sHi : sLo;
+            const bf16x8 h2 = join_halves(tr_half(x3[2 * p][nt >> 1], s), tr_half(x3[2 * p + 1][nt >> 1], s));
+            gW1[nt][1] = mfma16(h2, dot_, gW1[nt][1]);
+          }
+          gW1[0][1] = mfma16(dot_, one14, gW1[0][1]);
+        }
+      }
+
+      // per-Gaussian adjoints: latent, and (render backward) the direction's
+      // term of dL/dmean3D
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        dlat[cb].x += dx[cb][0][0];
+        dlat[cb].y += dx[cb][0][1];
+        dlat[cb].z += dx[cb][0][2];
+        dlat[cb].w += dx[cb][0][3];
+        if constexpr (!kDistill) {
+          // basis adjoints: lane (c, q) holds basis 4q..4q+3 (q < 2) and basis 8
+          // (q = 2, element 0); lane (c, 0) gathers all nine
+          float gb[9];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            gb[e] = dx[cb][1][e];
+            gb[4 + e] = from_row1(dx[cb][1][e]);
+          }
+          gb[8] = from_row2(dx[cb][1][0]);
+          const float3 gd = sh2_basis_vjp(d[cb].x, d[cb].y, d[cb].z, gb);
+          // d(v/|v|)/dv applied to gd (dnormvdv, auxiliary.h:134-144)
+          const float dot = d[cb].x * gd.x + d[cb].y * gd.y + d[cb].z * gd.z;
+          const float il = __builtin_amdgcn_rcpf(len[cb]);
+          const bool use = ok[cb] && q == 0 && rad[cb] > 0 && len[cb] > 0.f;
+          dmv[cb].x += use ? (gd.x - d[cb].x * dot) * il : 0.f;
+          dmv[cb].y += use ? (gd.y - d[cb].y * dot) * il : 0.f;
+          dmv[cb].z += use ? (gd.z - d[cb].z * dot) * il : 0.f;
+          any_dir[cb] = any_dir[cb] || use;
+        }
+      }
     }
-    if (ok) *reinterpret_cast<float4*>(a.dL_dlatent + G * kLatent + 4 * q) = dlat;
-    if (!kDistill && ok && q == 0 && any_dir) {
-      a.dL_dmean3D[3 * G] = cur.dm.x + dmv.x;
-      a.dL_dmean3D[3 * G + 1] = cur.dm.y + dmv.y;
-      a.dL_dmean3D[3 * G + 2] = cur.dm.z + dmv.z;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      if (ok[cb]) {
+        *reinterpret_cast<float4*>(a.dL_dlatent + Gs[cb] * kLatent + 4 * q) = dlat[cb];
+        if (!kDistill && any_dir[cb]) {
+          a.dL_dmean3D[3 * Gs[cb]] = dm[cb].x + dmv[cb].x;
+          a.dL_dmean3D[3 * Gs[cb] + 1] = dm[cb].y + dmv[cb].y;
+          a.dL_dmean3D[3 * Gs[cb] + 2] = dm[cb].z + dmv[cb].z;
+        }
+      }
     }
-    cur = nxt;
   }
 
-  // block partial row (each wave writes its own tiles)
-  float* row = a.theta_partial + (size_t)blockIdx.x * kThetaStride;
+  // Block sum of the four waves' gradients in a fixed order (wave 0, + 1,
+  // + 2, + 3), through LDS (the weights are no longer read), then wave 3
+  // writes the block's partial row.
+  __syncthreads();
+  f32x4* red = reinterpret_cast<f32x4*>(s_w);  // 24 tiles x 64 lanes
+  auto tile_ref = [&](int t) -> f32x4& {
+    return t < 16 ? gW2[t >> 2][t & 3] : gW1[(t - 16) >> 1][t & 1];
+  };
+  for (int src = 0; src < 3; ++src) {
+    if (w == src) {
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int n = 16 * w + 4 * q + e;  // hidden unit of this lane's D rows
+      for (int t = 0; t < 24; ++t) {
+        f32x4 v = tile_ref(t);
+        if (src > 0) {
+          const f32x4 p = red[t * 64 + lane];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) row[kThW2 + n * kHidden + 16 * ct + c] = gW2[ct][e];
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
-      if (16 * ct + c < kMlpIn) row[kThW1 + n * kMlpIn + 16 * ct + c] = gW1[ct][e];
-    if (c == 0) {
-      row[kThB1 + n] = gB1[e];
-      row[kThB2 + n] = gB2[e];
+          for (int e = 0; e < 4; ++e) v[e] = p[e] + v[e];
+        }
+        red[t * 64 + lane] = v;
+      }
     }
-    if (q == 0 && e < kMlpOut) {
-      row[kThW3 + e * kHidden + 16 * w + c] = gW3[e];
-      if (w == 0 && c == 0) row[kThB3 + e] = gB3[e];
-    }
+    __syncthreads();
   }
   for (int o = 32; o > 0; o >>= 1) sq_err += __shfl_xor(sq_err, o);
   if (lane == 0) s_err[w] = sq_err;
   __syncthreads();
-  if (threadIdx.x == 0) row[kThetaSize] = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
+  if (w != 3) return;
+  float* row = a.theta_partial + (size_t)blockIdx.x * kThetaStride;
+#pragma unroll
+  for (int t = 0; t < 24; ++t) {
+    const f32x4 p = red[t * 64 + lane];
+    f32x4& v = tile_ref(t);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = p[e] + v[e];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      const int n = 16 * nt + 4 * q + e;  // hidden unit of this lane's D row
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) row[kThW2 + n * kHidden + 16 * mt + c] = gW2[nt][mt][e];
+      row[kThW1 + n * kMlpIn + c] = gW1[nt][0][e];
+      const float v = gW1[nt][1][e];
+      if (16 + c < kMlpIn) row[kThW1 + n * kMlpIn + 16 + c] = v;
+      else if (c == 9) row[kThB1 + n] = v;
+      else if (c == 10) row[kThB2 + n] = v;
+      else if (c < 11 + kMlpOut) row[kThW3 + (c - 11) * kHidden + n] = v;
+      else if (c == 14 && nt == 0 && 4 * q + e >= 11 && 4 * q + e < 11 + kMlpOut) row[kThB3 + 4 * q + e - 11] = v;
+    }
+  }
+  if (lane == 0) row[kThetaSize] = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
+}
+// Sum of rows [r0, r1) of column col with eight loads in flight, in a fixed
+// order (eight strided partial sums, then added pairwise).
+__device__ __forceinline__ float sum_rows(const float* __restrict__ p, int r0, int r1, int col) {
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int r = r0; r < r1; r += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (r + k < r1) s[k] += p[(size_t)(r + k) * kThetaStride + col];
+  }
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 // Stage 1: sums rows [r0, r0 + rows) of each column.  Stage 2 then sums the
@@ -403,10 +667,8 @@ __global__ __launch_bounds__(256) void theta_reduce1_kernel(const float* __restr
                                                             float* __restrict__ out1) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col > kThetaSize) return;
-  const int r0 = blockIdx.y * rows, r1 = min(nrows, r0 + rows);
-  float s = 0.f;
-  for (int r = r0; r < r1; ++r) s += part[(size_t)r * kThetaStride + col];
-  out1[(size_t)blockIdx.y * kThetaStride + col] = s;
+  const int r0 = blockIdx.y * rows;
+  out1[(size_t)blockIdx.y * kThetaStride + col] = sum_rows(part, r0, min(nrows, r0 + rows), col);
 }
 
 __global__ __launch_bounds__(256) void theta_reduce2_kernel(const float* __restrict__ in1, int nrows,
@@ -414,8 +676,7 @@ __global__ __launch_bounds__(256) void theta_reduce2_kernel(const float* __restr
                                                             float* __restrict__ loss) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col > kThetaSize) return;
-  float s = 0.f;
-  for (int r = 0; r < nrows; ++r) s += in1[(size_t)r * kThetaStride + col];
+  const float s = sum_rows(in1, 0, nrows, col);
   if (col < kThetaSize) grad[col] = s;
   else if (loss) *loss = s * err_scale;
 }
@@ -452,7 +713,7 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ 
 // Launchers
 
 int mlp_train_blocks(int P) {
-  const int chunks = (P + 63) / 64;
+  const int chunks = (P + 64 * kTrainCB - 1) / (64 * kTrainCB);  // 4 waves x 16 kTrainCB Gaussians
   return chunks < kMlpTrainBlocks ? (chunks > 0 ? chunks : 1) : kMlpTrainBlocks;
 }
 

@@ -55,21 +55,32 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ f32x4 relu4(f32x4 v) {
-  f32x4 r;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) r[e] = fmaxf(v[e], 0.f);
-  return r;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+// Two FP32 values as two BF16 (round to nearest even) in one dword, element
+// 0 in the low half: one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
 
 __device__ __forceinline__ bf16x8 pack_bf16(f32x4 lo, f32x4 hi) {
-  bf16x8 r;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    r[e] = (__bf16)lo[e];
-    r[e + 4] = (__bf16)hi[e];
-  }
-  return r;
+  return __builtin_bit_cast(bf16x8, make_uint4(cvt_pk_bf16(lo[0], lo[1]), cvt_pk_bf16(lo[2], lo[3]),
+                                               cvt_pk_bf16(hi[0], hi[1]), cvt_pk_bf16(hi[2], hi[3])));
+}
+
+// ReLU of two BF16 in a dword: the bits as signed 16-bit integers, maxed
+// with 0 (v_pk_max_i16), so every negative value and -0 become +0.  Rounding
+// to BF16 keeps the sign, so relu(bf16(v)) = bf16(relu(v)).
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(short2v, v), (short2v){0, 0}));
+}
+
+// bf16(relu(lo)), bf16(relu(hi)) in pack_bf16's order.
+__device__ __forceinline__ bf16x8 relu_pack_bf16(f32x4 lo, f32x4 hi) {
+  const uint4 u = __builtin_bit_cast(uint4, pack_bf16(lo, hi));
+  return __builtin_bit_cast(bf16x8, make_uint4(relu_bf16x2(u.x), relu_bf16x2(u.y), relu_bf16x2(u.z), relu_bf16x2(u.w)));
 }
 
 __device__ __forceinline__ bf16x8 zero_bf16x8() {
@@ -107,24 +118,22 @@ __device__ __forceinline__ float3 sh2_basis_vjp(float x, float y, float z, const
 // B operand of layer 1 for lane (c, q): inputs 8q..8q+7 of its Gaussian:
 // q = 0, 1: latent; q = 2: basis 0..7; q = 3: basis 8, then zeros.
 __device__ __forceinline__ bf16x8 input_operand(int q, const bf16x8& latent_half, float x, float y, float z) {
-  if (q < 2) return latent_half;
   float b[9];
   sh2_basis(x, y, z, b);
-  bf16x8 r = zero_bf16x8();
-  if (q == 2) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) r[e] = (__bf16)b[e];
-  } else {
-    r[0] = (__bf16)b[8];
-  }
-  return r;
+  const uint4 lat = __builtin_bit_cast(uint4, latent_half);
+  const uint4 bas = q == 2 ? make_uint4(cvt_pk_bf16(b[0], b[1]), cvt_pk_bf16(b[2], b[3]), cvt_pk_bf16(b[4], b[5]),
+                                        cvt_pk_bf16(b[6], b[7]))
+                           : make_uint4(cvt_pk_bf16(b[8], 0.f), 0u, 0u, 0u);
+  // selects, not branches: the operand is built by every lane alike
+  return __builtin_bit_cast(bf16x8, q < 2 ? lat : bas);
 }
 
-// Activations of one 16-Gaussian column block.
+// Activations of one 16-Gaussian column block: the post-ReLU layers as the
+// next layer's BF16 B operands (units permuted as above), and the logits.
 struct MlpAct {
-  f32x4 h1[4];  // post-ReLU layer 1, D tiles rt = 0..3
-  f32x4 h2[4];  // post-ReLU layer 2
-  f32x4 o;      // output logits: rows 0..2 valid in lanes q = 0
+  bf16x8 x2[2];  // layer 1 (D tiles 0, 1 | 2, 3)
+  bf16x8 x3[2];  // layer 2
+  f32x4 o;       // output logits: rows 0..2 valid in lanes q = 0
 };
 
 // Forward of one column block.  w: packed image (LDS or global), lane: 0..63.
@@ -136,61 +145,24 @@ __device__ __forceinline__ void mlp_forward_block(const char* w, int lane, const
   const f32x4* B2 = reinterpret_cast<const f32x4*>(w + kPkB2);
   const f32x4 b3 = *reinterpret_cast<const f32x4*>(w + kPkB3);
   const int q = lane >> 4;
+  f32x4 t[4];
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt) a.h1[rt] = relu4(mfma16(W1A[rt * 64 + lane], xin, B1[rt * 4 + q]));
-  const bf16x8 x2[2] = {pack_bf16(a.h1[0], a.h1[1]), pack_bf16(a.h1[2], a.h1[3])};
+  for (int rt = 0; rt < 4; ++rt) t[rt] = mfma16(W1A[rt * 64 + lane], xin, B1[rt * 4 + q]);
+  a.x2[0] = relu_pack_bf16(t[0], t[1]);
+  a.x2[1] = relu_pack_bf16(t[2], t[3]);
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt) {
     f32x4 acc = B2[rt * 4 + q];
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) acc = mfma16(W2A[(rt * 2 + kb) * 64 + lane], x2[kb], acc);
-    a.h2[rt] = relu4(acc);
+    for (int kb = 0; kb < 2; ++kb) acc = mfma16(W2A[(rt * 2 + kb) * 64 + lane], a.x2[kb], acc);
+    t[rt] = acc;
   }
-  const bf16x8 x3[2] = {pack_bf16(a.h2[0], a.h2[1]), pack_bf16(a.h2[2], a.h2[3])};
+  a.x3[0] = relu_pack_bf16(t[0], t[1]);
+  a.x3[1] = relu_pack_bf16(t[2], t[3]);
   f32x4 acc = q == 0 ? b3 : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kb = 0; kb < 2; ++kb) acc = mfma16(W3A[kb * 64 + lane], x3[kb], acc);
+  for (int kb = 0; kb < 2; ++kb) acc = mfma16(W3A[kb * 64 + lane], a.x3[kb], acc);
   a.o = acc;
-}
-
-// Backward of one column block from dO (rows 0..2 in lanes q = 0, zero
-// elsewhere): hidden-unit adjoints and the input adjoint dx (D tiles: dx[0]
-// rows = latent dims, dx[1] rows 16 + 4q + e = basis index 4q + e).
-struct MlpGrad {
-  f32x4 dh1[4], dh2[4];
-  f32x4 dx[2];
-};
-
-__device__ __forceinline__ void mlp_backward_block(const char* w, int lane, const MlpAct& a, f32x4 dO, MlpGrad& g) {
-  const bf16x8* W3T = reinterpret_cast<const bf16x8*>(w + kPkW3T);
-  const bf16x8* W2T = reinterpret_cast<const bf16x8*>(w + kPkW2T);
-  const bf16x8* W1T = reinterpret_cast<const bf16x8*>(w + kPkW1T);
-  const int q = lane >> 4;
-  const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
-  const bf16x8 bo = q == 0 ? pack_bf16(f32x4{dO[0], dO[1], dO[2], 0.f}, z4) : zero_bf16x8();
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
-    const f32x4 t = mfma16(W3T[rt * 64 + lane], bo, z4);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) g.dh2[rt][e] = a.h2[rt][e] > 0.f ? t[e] : 0.f;
-  }
-  const bf16x8 b2[2] = {pack_bf16(g.dh2[0], g.dh2[1]), pack_bf16(g.dh2[2], g.dh2[3])};
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
-    f32x4 t = z4;
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) t = mfma16(W2T[(rt * 2 + kb) * 64 + lane], b2[kb], t);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) g.dh1[rt][e] = a.h1[rt][e] > 0.f ? t[e] : 0.f;
-  }
-  const bf16x8 b1[2] = {pack_bf16(g.dh1[0], g.dh1[1]), pack_bf16(g.dh1[2], g.dh1[3])};
-#pragma unroll
-  for (int rt = 0; rt < 2; ++rt) {
-    f32x4 t = z4;
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) t = mfma16(W1T[(rt * 2 + kb) * 64 + lane], b1[kb], t);
-    g.dx[rt] = t;
-  }
 }
 
 // Seeded direction k of Gaussian g for distillation: uniform on the sphere
