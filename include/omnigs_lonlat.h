@@ -141,6 +141,19 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* args, omnigs_buffer 
                           omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream,
                           int* num_rendered);
 
+/* Forward for the checkpointed backward: as omnigs_lonlat_forward (same
+ * image, radii, R and image buffer), but the binning buffer is carved without
+ * the backward's per-instance gradient records -- 17 of its 53 bytes per
+ * instance are allocated -- so the forward's peak HBM drops with the
+ * backward's.  Its geometry and binning buffers serve no stored backward:
+ * pass the image buffer to omnigs_lonlat_backward_checkpointed.  Has no
+ * counterpart in the reference, whose autograd wrapper keeps all three
+ * buffers for its backward (src/gaussian_rasterizer.cpp:85-98); it serves
+ * memory-bound training callers. */
+int omnigs_lonlat_forward_checkpointed(const omnigs_lonlat_forward_args* args, omnigs_buffer geometry_buffer,
+                                       omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream,
+                                       int* num_rendered);
+
 /* Checkpointed form: re-runs preprocess and binning of a forward whose
  * geometry and binning buffers were released, into fresh buffers, against
  * the forward's kept image buffer (its per-pixel final_T and n_contrib are

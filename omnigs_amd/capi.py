@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "omnigs_adam_step", "omnigs_f32_to_bf16", "omnigs_lonlat_batch_workspace_bytes", "omnigs_lonlat_batch_forward",
     "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
     "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr", "omnigs_lonlat_batch_state",
-    "omnigs_photometric_loss_noalloc", "omnigs_lonlat_backward_checkpointed",
+    "omnigs_photometric_loss_noalloc", "omnigs_lonlat_backward_checkpointed", "omnigs_lonlat_forward_checkpointed",
 )
 NUM_STAGES = 11
 
@@ -143,6 +143,8 @@ def load() -> ctypes.CDLL:
     lib.omnigs_lonlat_forward.argtypes = [ctypes.POINTER(ForwardArgs), Buffer, Buffer, Buffer, ctypes.c_void_p,
                                           _ip]
     lib.omnigs_lonlat_forward.restype = ctypes.c_int
+    lib.omnigs_lonlat_forward_checkpointed.argtypes = lib.omnigs_lonlat_forward.argtypes
+    lib.omnigs_lonlat_forward_checkpointed.restype = ctypes.c_int
     lib.omnigs_lonlat_recompute.argtypes = [ctypes.POINTER(ForwardArgs), Buffer, Buffer, ctypes.c_void_p,
                                             ctypes.c_void_p, _ip]
     lib.omnigs_lonlat_recompute.restype = ctypes.c_int
@@ -258,9 +260,12 @@ def stream_handle(device: torch.device | None = None) -> int:
 
 def forward(*, background, means3D, opacities, viewmatrix, campos, width, height, shs=None, colors_precomp=None,
             scales=None, rotations=None, cov3D_precomp=None, degree=3, scale_modifier=1.0, prefiltered=False,
-            buffers=None, seam_wrap=False, mlp=None):
+            buffers=None, seam_wrap=False, mlp=None, checkpointed=False):
     """Runs omnigs_lonlat_forward.  Returns (R, out_color [3,H,W], radii [P], (geom, binning, image)).
-    mlp: (packed image, BF16 latents [P,16] as int16/uint16 tensor) to take colours from the decoder."""
+    mlp: (packed image, BF16 latents [P,16] as int16/uint16 tensor) to take colours from the decoder.
+    checkpointed: run omnigs_lonlat_forward_checkpointed instead (no gradient
+    records in the binning buffer; the image buffer serves the checkpointed
+    backward)."""
     lib = load()
     dev = means3D.device
     P = means3D.shape[0]
@@ -277,10 +282,10 @@ def forward(*, background, means3D, opacities, viewmatrix, campos, width, height
                     _ptr(viewmatrix), _ptr(campos), int(prefiltered), _ptr(out), _ptr(radii), int(bool(seam_wrap)),
                     _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None)
     R = ctypes.c_int(0)
-    st = lib.omnigs_lonlat_forward(ctypes.byref(a), bufs[0].c, bufs[1].c, bufs[2].c, stream_handle(dev),
-                                   ctypes.byref(R))
+    fn = lib.omnigs_lonlat_forward_checkpointed if checkpointed else lib.omnigs_lonlat_forward
+    st = fn(ctypes.byref(a), bufs[0].c, bufs[1].c, bufs[2].c, stream_handle(dev), ctypes.byref(R))
     if st != OMNIGS_OK:
-        raise OmnigsError(st, "omnigs_lonlat_forward")
+        raise OmnigsError(st, fn.__name__)
     return R.value, out, radii, bufs
 
 

@@ -268,14 +268,14 @@ hipEvent_t readback_event() {
 // (their grids cover the capacity).
 int bin_and_render(const omnigs_lonlat_forward_args* a, omnigs_buffer binning_buffer, const GeomState& g,
                    const ImgState& img, const TileGrid& grid, uint32_t n, const uint32_t* R_dev, hipStream_t s,
-                   bool render) {
+                   bool render, bool records) {
   const uint32_t C = bin_capacity(n);
-  char* bbuf = grow(binning_buffer, carve_bin(nullptr, C).bytes);
+  char* bbuf = grow(binning_buffer, carve_bin(nullptr, C, records).bytes);
   if (!bbuf) {
     std::snprintf(g_last_error, sizeof(g_last_error), "binning buffer allocation failed");
     return OMNIGS_ERR_ALLOC;
   }
-  const BinState b = carve_bin(bbuf, C);
+  const BinState b = carve_bin(bbuf, C, records);
   const uint32_t grid_n = R_dev ? C : n;  // instances the grids cover
   const int tile_bits = bit_width_u32((uint32_t)grid.tiles());
   {
@@ -352,7 +352,8 @@ int preprocess_and_sort(const omnigs_lonlat_forward_args* a, const GeomState& g,
 // Shared by forward (render = true) and recompute (render = false, image
 // buffer given and kept).
 int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer, omnigs_buffer binning_buffer,
-                 omnigs_buffer image_buffer, char* kept_image, void* stream_, int* num_rendered, bool render) {
+                 omnigs_buffer image_buffer, char* kept_image, void* stream_, int* num_rendered, bool render,
+                 bool records = true) {
   if (!a || !num_rendered) return fail_arg("NULL args");
   *num_rendered = 0;
   const int P = a->P, W = a->width, H = a->height;
@@ -409,7 +410,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   thread_local uint32_t last_R = 0;
   const uint32_t guess = async ? bin_capacity(last_R) : 0u;
   if (guess > 0) {
-    const int st = bin_and_render(a, binning_buffer, g, img, grid, guess, total, s, render);
+    const int st = bin_and_render(a, binning_buffer, g, img, grid, guess, total, s, render, records);
     if (st != OMNIGS_OK) return st;
   }
   uint32_t Rw[2] = {0u, 0u};
@@ -427,7 +428,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   if (Rw[1] != 0u || Rw[0] > 0x7fffffffu) return fail_arg("instance count exceeds int range");
   const uint32_t R = Rw[0];
   if (bin_capacity(R) != guess) {
-    const int st = bin_and_render(a, binning_buffer, g, img, grid, R, nullptr, s, render);
+    const int st = bin_and_render(a, binning_buffer, g, img, grid, R, nullptr, s, render, records);
     if (st != OMNIGS_OK) return st;
   }
   last_R = R;
@@ -491,6 +492,12 @@ extern "C" {
 int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,
                           omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream, int* num_rendered) {
   return forward_impl(a, geometry_buffer, binning_buffer, image_buffer, nullptr, stream, num_rendered, true);
+}
+
+int omnigs_lonlat_forward_checkpointed(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,
+                                       omnigs_buffer binning_buffer, omnigs_buffer image_buffer, void* stream,
+                                       int* num_rendered) {
+  return forward_impl(a, geometry_buffer, binning_buffer, image_buffer, nullptr, stream, num_rendered, true, false);
 }
 
 int omnigs_lonlat_recompute(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buffer,

@@ -356,19 +356,23 @@ inline uint32_t bin_capacity(uint32_t R) {
   return c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
 }
 
-inline BinState carve_bin(char* base, size_t R) {
+// records = false: the forward-only layout, the same arrays up to the
+// backward's gradient records (36 of the 53 bytes per instance), which it
+// leaves out -- a prefix of the full layout, for a forward whose backward
+// bins again (the checkpointed form).
+inline BinState carve_bin(char* base, size_t R, bool records = true) {
   BinState b;
   Carver<char> c(base);
   b.tile_key = c.take<uint32_t>(R);
   b.tile_key_alt = c.take<uint32_t>(R);
   b.gid = c.take<uint32_t>(R);
   b.gid_alt = c.take<uint32_t>(R);
-  b.grad_rec = c.take<float4>(2 * (size_t)R);
-  b.grad_opacity = c.take<float>(R);
   b.sort_counts = c.take<uint32_t>(sort_workspace_words(R));
   b.scan_partials = c.take<uint32_t>(1);
   b.blend = c.take<uint8_t>(R);
   b.blend_ok = c.take<uint32_t>(1);
+  b.grad_rec = records ? c.take<float4>(2 * (size_t)R) : nullptr;
+  b.grad_opacity = records ? c.take<float>(R) : nullptr;
   b.bytes = align_up(c.off) + kAlign;
   return b;
 }

@@ -600,23 +600,26 @@ __global__ __launch_bounds__(kDupThreads) void duplicate_kernel(int P, uint32_t 
   if (kb >= k1) return;
   // The gradient slots number as many as the instances: this pass also marks
   // every one as holding no record (kNoRecord in its opacity word) for the
-  // backward, which overwrites the slots it stages.
+  // backward, which overwrites the slots it stages.  no_record is NULL in a
+  // binning without gradient records (a forward whose backward bins again).
   if (kb + kDupRun <= k1) {  // whole run: 16-byte stores (kb is a multiple of kDupRun)
     uint4* tk = reinterpret_cast<uint4*>(tile_key + kb);
     uint4* gd = reinterpret_cast<uint4*>(gid + kb);
-    uint4* nr = reinterpret_cast<uint4*>(no_record + kb);
     tk[0] = make_uint4(keys[0], keys[1], keys[2], keys[3]);
     tk[1] = make_uint4(keys[4], keys[5], keys[6], keys[7]);
     gd[0] = make_uint4(ids[0], ids[1], ids[2], ids[3]);
     gd[1] = make_uint4(ids[4], ids[5], ids[6], ids[7]);
-    nr[0] = nr[1] = make_uint4(kNoRecord, kNoRecord, kNoRecord, kNoRecord);
+    if (no_record) {
+      uint4* nr = reinterpret_cast<uint4*>(no_record + kb);
+      nr[0] = nr[1] = make_uint4(kNoRecord, kNoRecord, kNoRecord, kNoRecord);
+    }
   } else {
 #pragma unroll
     for (int u = 0; u < kDupRun; ++u) {
       if (kb + (uint32_t)u < k1) {
         tile_key[kb + u] = keys[u];
         gid[kb + u] = ids[u];
-        no_record[kb + u] = kNoRecord;
+        if (no_record) no_record[kb + u] = kNoRecord;
       }
     }
   }

@@ -195,7 +195,11 @@ tensor_list CheckpointRasterizerFunction::forward(AutogradContext* ctx, torch::T
                                                   torch::Tensor cov3Ds_precomp,
                                                   GaussianRasterizationSettings rs, bool seam_wrap,
                                                   int64_t band_instances) {
-  auto res = run_forward(rs, means3D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp, seam_wrap);
+  // the binning buffer is carved without gradient records: the backward bins again
+  auto res = omnigs::RasterizeGaussiansCheckpointedCUDA(
+      rs.bg_, means3D, colors_precomp, opacities, scales, rotations, rs.scale_modifier_, cov3Ds_precomp,
+      rs.viewmatrix_, rs.image_height_, rs.image_width_, sh, rs.sh_degree_, rs.campos_, rs.prefiltered_,
+      (int)rs.camera_type_, seam_wrap);
   const int R = std::get<0>(res);
   auto color = std::get<1>(res);
   auto radii = std::get<2>(res);

@@ -47,6 +47,19 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
                                const torch::Tensor& binningBuffer, const torch::Tensor& imageBuffer,
                                const int camera_type, const bool seam_wrap);
 
+// The checkpointed form's forward: RasterizeGaussiansCUDA's outputs, but the
+// binning buffer holds no gradient records (17 of 53 bytes per instance);
+// its buffers serve RasterizeGaussiansBackwardCheckpointedCUDA only, which
+// reads the image buffer alone.
+std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor>
+RasterizeGaussiansCheckpointedCUDA(const torch::Tensor& background, const torch::Tensor& means3D,
+                                   const torch::Tensor& colors, const torch::Tensor& opacity,
+                                   const torch::Tensor& scales, const torch::Tensor& rotations,
+                                   const float scale_modifier, const torch::Tensor& cov3D_precomp,
+                                   const torch::Tensor& viewmatrix, const int image_height, const int image_width,
+                                   const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
+                                   const bool prefiltered, const int camera_type, const bool seam_wrap);
+
 // The checkpointed backward: the forward's inputs (as given to
 // RasterizeGaussiansCUDA), its R and image buffer, and the band budget in
 // instances (0: the library's default); returns the eight gradients of

@@ -63,13 +63,18 @@ void* current_stream(const torch::Tensor& ref) {
 
 namespace omnigs {
 
-std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> RasterizeGaussiansCUDA(
+namespace {
+
+// The forward of RasterizeGaussiansCUDA; checkpointed: through
+// omnigs_lonlat_forward_checkpointed (a binning buffer without the
+// backward's gradient records).
+std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> forward_common(
     const torch::Tensor& background, const torch::Tensor& means3D, const torch::Tensor& colors,
     const torch::Tensor& opacity, const torch::Tensor& scales, const torch::Tensor& rotations,
     const float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
-    const torch::Tensor& /*projmatrix*/, const float /*tan_fovx*/, const float /*tan_fovy*/, const int image_height,
-    const int image_width, const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
-    const bool prefiltered, const int camera_type, const bool /*render_depth*/, const bool seam_wrap) {
+    const int image_height, const int image_width, const torch::Tensor& sh, const int degree,
+    const torch::Tensor& campos, const bool prefiltered, const int camera_type, const bool seam_wrap,
+    const bool checkpointed) {
   if (means3D.ndimension() != 2 || means3D.size(1) != 3) {
     AT_ERROR("means3D must have dimensions (num_points, 3)");
   }
@@ -113,9 +118,38 @@ std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torc
     a.radii = radii.data_ptr<int>();
     a.seam_wrap = seam_wrap ? 1 : 0;
     omnigs_buffer gb{&resize_tensor, &geomBuffer}, bb{&resize_tensor, &binningBuffer}, ib{&resize_tensor, &imgBuffer};
-    check(omnigs_lonlat_forward(&a, gb, bb, ib, current_stream(means3D), &rendered), "RasterizeGaussiansCUDA");
+    check(checkpointed ? omnigs_lonlat_forward_checkpointed(&a, gb, bb, ib, current_stream(means3D), &rendered)
+                       : omnigs_lonlat_forward(&a, gb, bb, ib, current_stream(means3D), &rendered),
+          "RasterizeGaussiansCUDA");
   }
   return std::make_tuple(rendered, out_color, radii, geomBuffer, binningBuffer, imgBuffer);
+}
+
+}  // namespace
+
+std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> RasterizeGaussiansCUDA(
+    const torch::Tensor& background, const torch::Tensor& means3D, const torch::Tensor& colors,
+    const torch::Tensor& opacity, const torch::Tensor& scales, const torch::Tensor& rotations,
+    const float scale_modifier, const torch::Tensor& cov3D_precomp, const torch::Tensor& viewmatrix,
+    const torch::Tensor& /*projmatrix*/, const float /*tan_fovx*/, const float /*tan_fovy*/, const int image_height,
+    const int image_width, const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
+    const bool prefiltered, const int camera_type, const bool /*render_depth*/, const bool seam_wrap) {
+  return forward_common(background, means3D, colors, opacity, scales, rotations, scale_modifier, cov3D_precomp,
+                        viewmatrix, image_height, image_width, sh, degree, campos, prefiltered, camera_type,
+                        seam_wrap, false);
+}
+
+std::tuple<int, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor>
+RasterizeGaussiansCheckpointedCUDA(const torch::Tensor& background, const torch::Tensor& means3D,
+                                   const torch::Tensor& colors, const torch::Tensor& opacity,
+                                   const torch::Tensor& scales, const torch::Tensor& rotations,
+                                   const float scale_modifier, const torch::Tensor& cov3D_precomp,
+                                   const torch::Tensor& viewmatrix, const int image_height, const int image_width,
+                                   const torch::Tensor& sh, const int degree, const torch::Tensor& campos,
+                                   const bool prefiltered, const int camera_type, const bool seam_wrap) {
+  return forward_common(background, means3D, colors, opacity, scales, rotations, scale_modifier, cov3D_precomp,
+                        viewmatrix, image_height, image_width, sh, degree, campos, prefiltered, camera_type,
+                        seam_wrap, true);
 }
 
 std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor,

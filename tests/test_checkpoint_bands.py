@@ -172,3 +172,40 @@ def test_banded_with_decoder_colours(gpu):
     for k in ("dL_dmeans3D", "dL_dlatent", "dL_dmlp"):
         a, b = ref[k], got[k]
         assert float((a - b).norm()) <= 1e-3 * float(a.norm()), (k, float((a - b).norm() / a.norm()))
+
+
+@pytest.mark.parametrize("wrap", [False, True])
+def test_forward_checkpointed_matches_forward(gpu, wrap):
+    """omnigs_lonlat_forward_checkpointed: the same R, image, radii and
+    per-pixel state as omnigs_lonlat_forward, tile lists bit-exact, with a
+    binning buffer that leaves out the gradient records (17 of 53 bytes per
+    instance); its image buffer feeds the banded backward, which then equals
+    the stored backward of the plain forward."""
+    import torch
+
+    from omnigs_amd import capi
+    from tests.gpu_util import announce, gpu_state, to_dev
+    announce(f"forward without gradient records, wrap {wrap}, 20k @ 512x256")
+    P = 20000
+    sc = scenes.small_scene(P=P, width=512, height=256, seed=11)
+    ds = to_dev(sc)
+    W, H = sc["width"], sc["height"]
+    R, img, radii, bufs = capi.forward_scene(ds, seam_wrap=wrap)
+    Rc, imgc, radiic, bufsc = capi.forward_scene(ds, seam_wrap=wrap, checkpointed=True)
+    torch.cuda.synchronize()
+    assert Rc == R
+    assert torch.equal(img, imgc) and torch.equal(radii, radiic)
+    st, stc = gpu_state(bufs, P, R, W, H), gpu_state(bufsc, P, R, W, H)
+    for k in ("final_T", "n_contrib", "ranges", "point_list", "tile_keys"):
+        assert np.array_equal(st[k], stc[k]), k
+    full = capi.load().omnigs_lonlat_binning_bytes(R)
+    got_bytes = bufsc[1].tensor.numel()
+    assert got_bytes < 0.4 * full, (got_bytes, full)
+    dL = torch.from_numpy(sc["dL_dpix"]).to(gpu)
+    ref = capi.backward_scene(ds, R, radii, bufs, dL, seam_wrap=wrap)
+    got = _banded(ds, R, radii, bufsc[2], dL, 3000, wrap)
+    torch.cuda.synchronize()
+    assert got["_bands"] > 4
+    for k in _GRADS:
+        a, b = ref[k], got[k]
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * float(a.abs().max())), (k, float((a - b).abs().max()))
