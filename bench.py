@@ -217,6 +217,10 @@ def time_steps(step, warmup: int, steps: int, dev):
             dom[0] / dom[1], res)
 
 
+# Camera-centre box of the single-view workload (lo xyz, hi xyz): the room
+# scene's camera and the decoder's held-out poses are drawn from it.
+VIEW_BOX = (-0.5, -0.5, -0.5, 0.5, 0.5, 0.5)
+
 # PSNR of the reference on 360Roam (supp.pdf p.5, Table 3): the quality level
 # at which the decoder's extra error is priced as a PSNR loss.
 REFERENCE_PSNR_DB = 25.464
@@ -234,10 +238,12 @@ def random_rotation(rng):
 
 def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
     """The learned-colour form: distils the decoder from the scene's own SH
-    colours (2000 passes, 64 seeded directions per Gaussian, Adam with network
-    lr 1e-3 and latent lr 1e-2), judges it against the SH render on 16
-    held-out random poses, then times fwd+bwd with decoder colours and the
-    decoder's backward (latent and weight gradients) on the timed workload."""
+    colours (2000 passes, 64 seeded directions per Gaussian from seeded camera
+    centres in VIEW_BOX, Adam with network lr 1e-3 and latent lr 1e-2), judges
+    it against the SH render on 16 held-out random poses (other centres in the
+    same box, random rotations), then times fwd+bwd with decoder colours and
+    the decoder's backward (latent and weight gradients) on the timed
+    workload."""
     import math
 
     import numpy as np
@@ -251,7 +257,11 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
     dec = ColourDecoder(P, dev, seed=0)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
-    losses = dec.distill(shs, steps=args.distill_steps, lr=1e-3, num_dirs=64, seed=1)
+    # the cameras of this workload stand in the 1 m cube around the origin (the
+    # scene's own camera and the held-out poses below): the 64 directions per
+    # Gaussian come from seeded centres in that cube
+    losses = dec.distill(shs, steps=args.distill_steps, lr=1e-3, num_dirs=64, seed=1, means3D=ds["means3D"],
+                         view_box=VIEW_BOX)
     e1.record()
     torch.cuda.synchronize()
     distill_ms = e0.elapsed_time(e1)

@@ -320,7 +320,15 @@ size_t omnigs_mlp_workspace_bytes(int P);
  * view directions (chosen by seed and step), the teacher colour is its SH
  * colour there (degree 3, + 0.5, clamped at 0, as computeColorFromSH), and
  * the loss is the mean over P x 3 of (decoder - teacher)^2.  Writes the
- * gradients of the FP32 latents and of theta, and the loss. */
+ * gradients of the FP32 latents and of theta, and the loss.
+ * Directions: with means3D NULL, uniform on the sphere
+ * (omnigs_mlp_seeded_direction); with means3D [P,3] given, direction k of
+ * Gaussian g is the view direction unit(mean_g - c) from the seeded view
+ * centre c = omnigs_mlp_seeded_view_centre(seed, g, k, view_box), uniform in
+ * the box view_box = {lo x, lo y, lo z, hi x, hi y, hi z} where the cameras
+ * that will render the scene stand -- the directions the colours are
+ * actually seen from, as the reference's SH are fitted on its training
+ * views. */
 typedef struct omnigs_mlp_distill_args {
   int P, M;                /* shs is [P,M,3], M >= 16 */
   const float* shs;
@@ -334,9 +342,15 @@ typedef struct omnigs_mlp_distill_args {
   float* loss;             /* 1 float */
   void* workspace;         /* omnigs_mlp_workspace_bytes(P) */
   size_t workspace_bytes;
+  const float* means3D;    /* [P,3] or NULL (directions on the sphere) */
+  float view_box[6];       /* camera-centre box with means3D: lo xyz, hi xyz */
 } omnigs_mlp_distill_args;
 int omnigs_mlp_distill_step(const omnigs_mlp_distill_args* args, void* stream);
 
+/* Seeded view centre k of Gaussian g in view_box (lo xyz, hi xyz), used by
+ * the distillation when means3D is given. */
+void omnigs_mlp_seeded_view_centre(unsigned int seed, unsigned int g, unsigned int k, const float view_box[6],
+                                   float out[3]);
 /* Seeded direction k of Gaussian g used by the distillation (unit vector). */
 void omnigs_mlp_seeded_direction(unsigned int seed, unsigned int g, unsigned int k, float out[3]);
 

@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "omnigs_lonlat_batch_backward", "omnigs_ply_write", "omnigs_ply_count", "omnigs_ply_read",
     "omnigs_ply_last_error", "omnigs_densification_stats", "omnigs_expon_lr", "omnigs_lonlat_batch_state",
     "omnigs_photometric_loss_noalloc", "omnigs_lonlat_backward_checkpointed", "omnigs_lonlat_forward_checkpointed",
+    "omnigs_mlp_seeded_view_centre",
 )
 NUM_STAGES = 11
 
@@ -108,6 +109,7 @@ class DistillArgs(ctypes.Structure):
         ("mlp_packed", ctypes.c_void_p), ("seed", ctypes.c_uint), ("step", ctypes.c_int), ("num_dirs", ctypes.c_int),
         ("dL_dlatent", ctypes.c_void_p), ("dL_dmlp", ctypes.c_void_p), ("loss", ctypes.c_void_p),
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+        ("means3D", ctypes.c_void_p), ("view_box", ctypes.c_float * 6),
     ]
 
 
@@ -194,6 +196,9 @@ def load() -> ctypes.CDLL:
     lib.omnigs_mlp_seeded_direction.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                                 ctypes.POINTER(ctypes.c_float)]
     lib.omnigs_mlp_seeded_direction.restype = None
+    lib.omnigs_mlp_seeded_view_centre.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    lib.omnigs_mlp_seeded_view_centre.restype = None
     lib.omnigs_adam_step.argtypes = [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_void_p]
     lib.omnigs_adam_step.restype = ctypes.c_int
@@ -433,12 +438,16 @@ def mlp_eval(latents_bf16: torch.Tensor, dirs: torch.Tensor, packed: torch.Tenso
     return rgb
 
 
-def mlp_distill_step(shs, latents, packed, seed, step, num_dirs, dL_dlatent, dL_dmlp, loss, workspace) -> None:
+def mlp_distill_step(shs, latents, packed, seed, step, num_dirs, dL_dlatent, dL_dmlp, loss, workspace,
+                     means3D=None, view_box=None) -> None:
+    """One distillation pass; means3D [P,3] with view_box (lo xyz, hi xyz)
+    takes the directions from seeded view centres in the box."""
     lib = load()
     P, M = shs.shape[0], shs.shape[1]
+    box = (ctypes.c_float * 6)(*([float(v) for v in view_box] if means3D is not None else [0.0] * 6))
     a = DistillArgs(P, M, _ptr(shs), _ptr(latents), packed.data_ptr(), int(seed) & 0xFFFFFFFF, int(step),
                     int(num_dirs), _ptr(dL_dlatent), _ptr(dL_dmlp), _ptr(loss), workspace.data_ptr(),
-                    workspace.numel())
+                    workspace.numel(), _ptr(means3D), box)
     _check(lib.omnigs_mlp_distill_step(ctypes.byref(a), stream_handle(shs.device)), "omnigs_mlp_distill_step")
 
 
@@ -449,6 +458,13 @@ def mlp_workspace(P: int, device) -> torch.Tensor:
 def seeded_direction(seed: int, g: int, k: int):
     out = (ctypes.c_float * 3)()
     load().omnigs_mlp_seeded_direction(seed & 0xFFFFFFFF, g, k, out)
+    return tuple(out)
+
+
+def seeded_view_centre(seed: int, g: int, k: int, view_box):
+    out = (ctypes.c_float * 3)()
+    box = (ctypes.c_float * 6)(*[float(v) for v in view_box])
+    load().omnigs_mlp_seeded_view_centre(seed & 0xFFFFFFFF, g, k, box, out)
     return tuple(out)
 
 

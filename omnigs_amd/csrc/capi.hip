@@ -1011,11 +1011,23 @@ int omnigs_mlp_distill_step(const omnigs_mlp_distill_args* a, void* stream) {
   if (!a->shs || !a->latents || !a->mlp_packed || !a->dL_dlatent || !a->dL_dmlp || !a->loss || !a->workspace)
     return fail_arg("missing pointer");
   if (a->workspace_bytes < omnigs_mlp_workspace_bytes(a->P)) return fail_arg("workspace too small");
+  if (a->means3D)
+    for (int i = 0; i < 3; ++i)
+      if (!(a->view_box[i] <= a->view_box[3 + i])) return fail_arg("view_box: lo above hi");
   OMNIGS_TRY(launch_mlp_distill(a->P, a->M, a->shs, a->latents, a->mlp_packed, a->seed, a->step, a->num_dirs,
                                 a->dL_dlatent, a->dL_dmlp, a->loss, static_cast<float*>(a->workspace),
-                                (hipStream_t)stream),
+                                (hipStream_t)stream, a->means3D, a->view_box),
              "mlp distill");
   return OMNIGS_OK;
+}
+
+void omnigs_mlp_seeded_view_centre(unsigned int seed, unsigned int g, unsigned int k, const float view_box[6],
+                                   float out[3]) {
+  const float3 c = seeded_view_centre(seed, g, k, make_float3(view_box[0], view_box[1], view_box[2]),
+                                      make_float3(view_box[3], view_box[4], view_box[5]));
+  out[0] = c.x;
+  out[1] = c.y;
+  out[2] = c.z;
 }
 
 void omnigs_mlp_seeded_direction(unsigned int seed, unsigned int g, unsigned int k, float out[3]) {

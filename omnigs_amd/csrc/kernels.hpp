@@ -205,7 +205,7 @@ hipError_t launch_mlp_eval(int N, const uint16_t* latents, const float* dirs, co
                            hipStream_t s);
 hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* latents, const void* packed, uint32_t seed,
                               int step, int num_dirs, float* dL_dlatent, float* dL_dtheta, float* loss, float* ws,
-                              hipStream_t s);
+                              hipStream_t s, const float* means3D = nullptr, const float* view_box = nullptr);
 hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos, const int* radii,
                                const uint16_t* latents, const void* packed, const float* dL_dcolor, float* dL_dmean3D,
                                float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V = 1);

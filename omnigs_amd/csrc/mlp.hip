@@ -148,9 +148,11 @@ struct MlpTrainArgs {
   const float* dL_dcolor;  // [V P, 3]
   float* dL_dmean3D;       // [P,3], += direction term of every view with radius > 0
   int V;                   // views (items v P + g); campos is [V,3]
-  // distillation
+  // distillation (means3D given: directions from seeded view centres in
+  // [view_lo, view_hi]; NULL: seeded directions uniform on the sphere)
   const float* shs;  // [P,M,3] teacher, degree 3
   int M;
+  float3 view_lo, view_hi;
   uint32_t seed;
   int step;
   int num_dirs;
@@ -314,7 +316,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const float4 x = *reinterpret_cast<const float4*>(a.latent_f32 + Gv * kLatent + 8 * (q & 1));
         const float4 y = *reinterpret_cast<const float4*>(a.latent_f32 + Gv * kLatent + 8 * (q & 1) + 4);
         in.lat[cb] = pack_bf16(f32x4{x.x, x.y, x.z, x.w}, f32x4{y.x, y.y, y.z, y.w});
-        in.m[cb] = in.dm[cb] = make_float3(0.f, 0.f, 0.f);
+        in.m[cb] = a.means3D ? make_float3(a.means3D[3 * Gv], a.means3D[3 * Gv + 1], a.means3D[3 * Gv + 2])
+                             : make_float3(0.f, 0.f, 0.f);
+        in.dm[cb] = make_float3(0.f, 0.f, 0.f);
       } else {
         in.lat[cb] = *reinterpret_cast<const bf16x8*>(a.latent_bf16 + Gv * kLatent + 8 * (q & 1));
         in.m[cb] = make_float3(a.means3D[3 * Gv], a.means3D[3 * Gv + 1], a.means3D[3 * Gv + 2]);
@@ -367,8 +371,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         rad[cb] = 0;
         dcv[cb] = make_float3(0.f, 0.f, 0.f);
         if constexpr (kDistill) {
-          d[cb] = seeded_direction(a.seed, (uint32_t)Gs[cb],
-                                   distill_dir_index(a.seed, a.step, (uint32_t)Gs[cb], a.num_dirs));
+          const uint32_t k = distill_dir_index(a.seed, a.step, (uint32_t)Gs[cb], a.num_dirs);
+          if (a.means3D) {
+            const float3 cv = seeded_view_centre(a.seed, (uint32_t)Gs[cb], k, a.view_lo, a.view_hi);
+            d[cb] = unit_dir(make_float3(m[cb].x - cv.x, m[cb].y - cv.y, m[cb].z - cv.z), &len[cb]);
+          } else {
+            d[cb] = seeded_direction(a.seed, (uint32_t)Gs[cb], k);
+          }
           const float* src = a.shs + (Gs[cb] * a.M + 4 * q) * 3;
 #pragma unroll
           for (int i = 0; i < 12; ++i) sh[cb][i] = (4 * q + i / 3 < a.M) ? src[i] : 0.f;
@@ -748,9 +757,14 @@ static hipError_t reduce_theta(int nb, float* ws, float* dL_dtheta, float err_sc
 
 hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* latents, const void* packed, uint32_t seed,
                               int step, int num_dirs, float* dL_dlatent, float* dL_dtheta, float* loss, float* ws,
-                              hipStream_t s) {
+                              hipStream_t s, const float* means3D, const float* view_box) {
   MlpTrainArgs a{};
   a.P = P;
+  a.means3D = means3D;
+  if (means3D) {
+    a.view_lo = make_float3(view_box[0], view_box[1], view_box[2]);
+    a.view_hi = make_float3(view_box[3], view_box[4], view_box[5]);
+  }
   a.packed = (const char*)packed;
   a.latent_f32 = latents;
   a.shs = shs;

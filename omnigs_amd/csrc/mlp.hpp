@@ -186,6 +186,19 @@ __host__ __device__ __forceinline__ float3 seeded_direction(uint32_t seed, uint3
   return make_float3(s * cosf(ph), s * sinf(ph), z);
 }
 
+// Seeded view centre k of Gaussian g: uniform in the box [lo, hi] (three
+// more hash words of the same counter).  The distillation direction is then
+// the view direction from that centre, unit(mean - centre), as a render from
+// a camera there sees the Gaussian.
+__host__ __device__ __forceinline__ float3 seeded_view_centre(uint32_t seed, uint32_t g, uint32_t k, float3 lo,
+                                                              float3 hi) {
+  const uint32_t h0 = mix32(seed ^ mix32(g * 0x9E3779B9U + k * 0x85EBCA6BU + 0x27d4eb2fU));
+  const uint32_t h1 = mix32(h0 ^ 0x5bd1e995U), h2 = mix32(h1 ^ 0x165667b1U), h3 = mix32(h2 ^ 0xc2b2ae35U);
+  const float s = 1.0f / 16777216.0f;
+  return make_float3(fmaf((float)(h1 >> 8) * s, hi.x - lo.x, lo.x), fmaf((float)(h2 >> 8) * s, hi.y - lo.y, lo.y),
+                     fmaf((float)(h3 >> 8) * s, hi.z - lo.z, lo.z));
+}
+
 // Direction index a distillation step gives Gaussian g.
 __host__ __device__ __forceinline__ uint32_t distill_dir_index(uint32_t seed, int step, uint32_t g, int num_dirs) {
   return mix32(seed * 0x632BE5ABU + (uint32_t)step * 0x9E3779B9U + g) % (uint32_t)num_dirs;

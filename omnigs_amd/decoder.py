@@ -84,11 +84,20 @@ class ColourDecoder:
         capi.mlp_pack(self.theta, self.packed)
 
     def distill(self, shs: torch.Tensor, steps: int = 2000, lr: float = 1e-3, num_dirs: int = 64, seed: int = 1,
-                lr_latent: float | None = 1e-2, log_every: int = 0):
+                lr_latent: float | None = 1e-2, log_every: int = 0, means3D: torch.Tensor | None = None,
+                view_box=None):
         """Distills the decoder from SH colours (shs [P,16,3]); returns the
         per-step losses as a tensor (kept on the device until the end).
         lr is the network's Adam rate; the latents default to 1e-2 because
-        each one sees a single sample per pass."""
+        each one sees a single sample per pass.
+
+        Directions: uniform on the sphere, or, with means3D [P,3] and
+        view_box (lo xyz, hi xyz), the view directions from ``num_dirs``
+        seeded camera centres per Gaussian in the box where the cameras that
+        render the scene stand (the colours a render can show)."""
+        if means3D is not None:
+            assert view_box is not None and len(view_box) == 6
+            means3D = means3D.contiguous()
         assert shs.shape[0] == self.P and shs.dtype == torch.float32
         shs = shs.contiguous()
         st = self._adam_state()
@@ -96,7 +105,7 @@ class ColourDecoder:
         losses = torch.empty((steps,), dtype=torch.float32, device=self.device)
         for i in range(steps):
             capi.mlp_distill_step(shs, self.latents, self.packed, seed, self.step_count, num_dirs, st["g_l"],
-                                  st["g_t"], losses[i:i + 1], ws)
+                                  st["g_t"], losses[i:i + 1], ws, means3D=means3D, view_box=view_box)
             self.apply_gradients(st["g_t"], st["g_l"], lr=lr, lr_latent=lr_latent)
             if log_every and (i + 1) % log_every == 0:
                 print(f"distill step {i + 1}: loss {float(losses[i]):.6f}", flush=True)

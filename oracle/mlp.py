@@ -147,6 +147,34 @@ def seeded_directions(seed: int, g: np.ndarray, k: np.ndarray) -> np.ndarray:
     return np.stack([s * np.cos(ph), s * np.sin(ph), z], axis=1).astype(np.float32)
 
 
+def seeded_view_centres(seed: int, g: np.ndarray, k: np.ndarray, view_box) -> np.ndarray:
+    """View centres uniform in view_box = (lo xyz, hi xyz): three more words
+    of the direction hash's counter; lo + u (hi - lo) with one FP32 rounding
+    (the kernels' fmaf; exact product in FP64, then rounded)."""
+    g = np.asarray(g, dtype=np.uint64)
+    k = np.asarray(k, dtype=np.uint64)
+    inner = (g * 0x9E3779B9 + k * 0x85EBCA6B + 0x27d4eb2f) & 0xFFFFFFFF
+    h0 = _mix32(np.uint64(seed & 0xFFFFFFFF) ^ _mix32(inner))
+    h1 = _mix32(h0 ^ 0x5bd1e995)
+    h2 = _mix32(h1 ^ 0x165667b1)
+    h3 = _mix32(h2 ^ 0xc2b2ae35)
+    box = np.asarray(view_box, dtype=np.float32)
+    out = []
+    for i, h in enumerate((h1, h2, h3)):
+        u = (h >> 8).astype(np.float32) * np.float32(1.0 / 16777216.0)
+        span = np.float32(box[3 + i] - box[i])
+        out.append((u.astype(np.float64) * np.float64(span) + np.float64(box[i])).astype(np.float32))
+    return np.stack(out, axis=1)
+
+
+def view_box_directions(means: np.ndarray, centres: np.ndarray) -> np.ndarray:
+    """unit(mean - centre) in FP32 ((0, 0, 1) for a zero vector)."""
+    v = (means.astype(np.float32) - centres.astype(np.float32)).astype(np.float32)
+    n = np.sqrt(np.sum(v.astype(np.float64) ** 2, axis=1, keepdims=True)).astype(np.float32)
+    d = np.where(n > 0, v / np.where(n > 0, n, 1), np.array([0, 0, 1], dtype=np.float32))
+    return d.astype(np.float32)
+
+
 def distill_dir_index(seed: int, step: int, g: np.ndarray, num_dirs: int) -> np.ndarray:
     g = np.asarray(g, dtype=np.uint64)
     x = ((seed * 0x632BE5AB) + (step * 0x9E3779B9) + g) & 0xFFFFFFFF
@@ -169,12 +197,17 @@ def sh3_colour(shs: np.ndarray, dirs: np.ndarray) -> np.ndarray:
     return np.maximum(rgb, 0.0)
 
 
-def distill_step(theta, latents, shs, seed: int, step: int, num_dirs: int):
-    """One distillation pass as the kernel runs it: (loss, dL/dlatent, dL/dtheta)."""
+def distill_step(theta, latents, shs, seed: int, step: int, num_dirs: int, means=None, view_box=None):
+    """One distillation pass as the kernel runs it: (loss, dL/dlatent, dL/dtheta).
+    With means [P,3] and view_box, the directions run from seeded view
+    centres in the box to the means; else they are uniform on the sphere."""
     P = latents.shape[0]
     g = np.arange(P)
     k = distill_dir_index(seed, step, g, num_dirs)
-    dirs = seeded_directions(seed, g, k)
+    if means is None:
+        dirs = seeded_directions(seed, g, k)
+    else:
+        dirs = view_box_directions(means, seeded_view_centres(seed, g, k, view_box))
     target = sh3_colour(shs, dirs)
     rgb, cache = forward(theta, latents, dirs)
     err = rgb - target
