@@ -472,6 +472,13 @@ int backward_tail(const omnigs_lonlat_backward_args* a, const GeomState& g, cons
                             a->dL_dconic, a->dL_dcolor, a->dL_dmean3D, a->dL_dcov3D, a->dL_dsh,
                             a->dL_dscale, a->dL_drot, a->dpx_dt, a->dpy_dt, gather,
                             a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor};
+  if (mlp) {
+    // the decoder's backward runs on the Gaussians with a colour gradient
+    // only, listed by the preprocess backward
+    pb.act_list = mlp_act_list(static_cast<float*>(a->mlp_workspace), P);
+    pb.act_count = mlp_act_count(static_cast<float*>(a->mlp_workspace), P);
+    pb.dL_dlatent = a->dL_dlatent;
+  }
   {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
     OMNIGS_TRY(launch_preprocess_backward(pb, s), "preprocess backward");
@@ -479,7 +486,8 @@ int backward_tail(const omnigs_lonlat_backward_args* a, const GeomState& g, cons
   if (mlp) {
     StageTimer tm(OMNIGS_STAGE_DECODER_BACKWARD, s);
     OMNIGS_TRY(launch_mlp_backward(P, a->means3D, a->campos, radii, a->latents, a->mlp_packed, a->dL_dcolor,
-                                   a->dL_dmean3D, a->dL_dlatent, a->dL_dmlp, static_cast<float*>(a->mlp_workspace), s),
+                                   a->dL_dmean3D, a->dL_dlatent, a->dL_dmlp, static_cast<float*>(a->mlp_workspace), s,
+                                   1, true),
                "decoder backward");
   }
   return OMNIGS_OK;

@@ -54,6 +54,13 @@ struct PreprocessBackwardArgs {
   float* out_dconic;
   float* out_dopacity;
   float* out_dcolor;
+  // Colour decoder (act_list set): the Gaussians of each 64-Gaussian wave
+  // whose dL/dcolor is not zero, in index order, at act_list[64 w ..], their
+  // number at act_count[w]; dL/dlatent of the others is zeroed here (the
+  // decoder's backward writes it for the listed ones only).
+  uint32_t* act_list;
+  uint32_t* act_count;
+  float* dL_dlatent;
 };
 
 // preprocess.hip
@@ -197,7 +204,7 @@ hipError_t launch_preprocess_backward_views(const PreprocessBackwardArgs& a, con
 
 // mlp.hip
 constexpr int kMlpTrainBlocks = 256;  // persistent grid of the decoder training pass (one block per CU)
-constexpr int kThetaReduceRows = 32;  // rows per stage-1 block of the weight-gradient reduction
+constexpr int kThetaReduceRows = 16;  // rows per stage-1 block of the weight-gradient reduction
 int mlp_train_blocks(int P);
 size_t mlp_workspace_floats(int P);
 hipError_t launch_mlp_pack(const float* theta, void* packed, hipStream_t s);
@@ -208,7 +215,13 @@ hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* laten
                               hipStream_t s, const float* means3D = nullptr, const float* view_box = nullptr);
 hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos, const int* radii,
                                const uint16_t* latents, const void* packed, const float* dL_dcolor, float* dL_dmean3D,
-                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V = 1);
+                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V = 1,
+                               bool listed = false);
+// The active-Gaussian list inside the decoder workspace (see
+// PreprocessBackwardArgs::act_list): list [P rounded up to 256], then the
+// per-wave counts.
+uint32_t* mlp_act_list(float* ws, int P);
+uint32_t* mlp_act_count(float* ws, int P);
 struct AdamGroupDev {
   float* param;
   const float* grad;

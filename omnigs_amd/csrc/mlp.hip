@@ -148,6 +148,10 @@ struct MlpTrainArgs {
   const float* dL_dcolor;  // [V P, 3]
   float* dL_dmean3D;       // [P,3], += direction term of every view with radius > 0
   int V;                   // views (items v P + g); campos is [V,3]
+  // V = 1: the Gaussians with a colour gradient, per 64-Gaussian wave (the
+  // preprocess backward's list), or NULL for every Gaussian
+  const uint32_t* act_list;
+  const uint32_t* act_count;
   // distillation (means3D given: directions from seeded view centres in
   // [view_lo, view_hi]; NULL: seeded directions uniform on the sphere)
   const float* shs;  // [P,M,3] teacher, degree 3
@@ -259,6 +263,9 @@ __device__ __forceinline__ float row_sum(float v) {
 // Column blocks a wave carries through each step: independent chains that
 // hide one another's MFMA and LDS latency (one wave per SIMD holds them).
 constexpr int kTrainCB = 4;
+// 64-Gaussian segments whose listed Gaussians a wave of the training pass
+// holds in LDS at once.
+constexpr int kSegBatch = 24;
 
 template <bool kDistill>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void mlp_train_kernel(MlpTrainArgs a) {
@@ -266,10 +273,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   __shared__ __attribute__((aligned(16))) char s_w[kMlpPackedBytes];
   __shared__ __attribute__((aligned(16))) bf16x8 s_sel[kNumSel][64];
   __shared__ float s_err[4];
-
-  stage_weights(s_w, a.packed, kMlpPackedBytes);
-  for (int i = threadIdx.x; i < kNumSel * 64; i += blockDim.x) s_sel[i >> 6][i & 63] = make_selector(i >> 6, i & 63);
-  __syncthreads();
+  __shared__ uint32_t s_ids[4][kSegBatch * 64];  // per wave: its listed Gaussians (render backward)
 
   const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
   const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
@@ -297,21 +301,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   float sq_err = 0.f;
   const float inv3P = 1.f / (3.f * (float)a.P);
 
-  const int ngroups = (a.P + 16 * NCB - 1) / (16 * NCB);
   const int nwaves = (int)gridDim.x * 4;
   const int nviews = kDistill ? 1 : a.V;
+  // Work.  Listed (render backward with the preprocess backward's list):
+  // wave wid takes the 64-Gaussian segments [seg0, seg1), an equal share,
+  // and copies the listed Gaussians of up to kSegBatch of them at a time into
+  // its LDS id array, in index order; groups are 64 consecutive entries, so
+  // only a batch's last group is partial.  Otherwise units of 64 consecutive
+  // Gaussians, wave wid taking units wid, wid + nwaves, ...
+  const bool listed = !kDistill && a.act_list != nullptr;
+  const int wid = (int)blockIdx.x * 4 + w;
+  const int nseg = (a.P + 63) / 64;
+  const int seg0 = (int)((int64_t)wid * nseg / nwaves), seg1 = (int)((int64_t)(wid + 1) * nseg / nwaves);
+  const int nunits = (a.P + 63) / 64;
+  uint32_t* ids = s_ids[w];
+  struct Cursor {
+    int u, gi, n;  // listed: first segment of the batch; else the unit.  Group in it, Gaussians in it
+  };
+  auto more = [&](const Cursor& k) { return listed ? k.u < seg1 : k.u < nunits; };
+  // the listed Gaussians of segments [sb, se) (se - sb <= kSegBatch) into ids
+  auto list_batch = [&](int sb, int se) -> int {
+    const int ns = se - sb;
+    const uint32_t cl = lane < ns ? a.act_count[sb + lane] : 0u;
+    uint32_t inc = cl;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    const uint32_t excl = inc - cl;
+#pragma unroll
+    for (int i0 = 0; i0 < kSegBatch; i0 += 8) {
+      uint32_t v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t ci = (uint32_t)__shfl((int)cl, i0 + u, 64);
+        v[u] = (uint32_t)lane < ci ? a.act_list[(size_t)64 * (sb + i0 + u) + lane] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t ci = (uint32_t)__shfl((int)cl, i0 + u, 64), ei = (uint32_t)__shfl((int)excl, i0 + u, 64);
+        if ((uint32_t)lane < ci) ids[ei + lane] = v[u];
+      }
+    }
+    // the wave's later reads of ids (other lanes) follow these writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return __shfl((int)inc, ns - 1, 64);
+  };
+  auto unit_open = [&](Cursor& k) {
+    if (listed) k.n = list_batch(k.u, min(seg1, k.u + kSegBatch));
+    else k.n = min(64, a.P - 64 * k.u);
+  };
+  auto step = [&](Cursor& k) {
+    if (64 * (k.gi + 1) < k.n) {
+      ++k.gi;
+      return;
+    }
+    k.gi = 0;
+    for (k.u += listed ? kSegBatch : nwaves; more(k); k.u += listed ? kSegBatch : nwaves) {
+      unit_open(k);
+      if (k.n > 0) return;
+    }
+  };
   // Per column block: Gaussian G (lane c) and its inputs, loaded a group
-  // ahead.  Loads use a valid index for every lane; results of Gaussians past
-  // P are masked.
+  // ahead.  Loads use a valid index for every lane; results of lanes past
+  // the group's Gaussians are masked.
   struct GroupIn {
+    uint32_t G[NCB];
+    bool ok[NCB];
     bf16x8 lat[NCB];
     float3 m[NCB], dm[NCB];
   };
-  auto load_group = [&](int grp, GroupIn& in) {
+  auto load_group = [&](const Cursor& k, GroupIn& in) {
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
-      const size_t G = (size_t)grp * 16 * NCB + 16 * cb + c;
-      const size_t Gv = G < (size_t)a.P ? G : 0;
+      const int j = 64 * k.gi + 16 * cb + c;
+      in.ok[cb] = j < k.n;
+      in.G[cb] = !in.ok[cb] ? 0u : listed ? ids[j] : (uint32_t)(64 * k.u + j);
+      const size_t Gv = in.G[cb];
       if constexpr (kDistill) {
         const float4 x = *reinterpret_cast<const float4*>(a.latent_f32 + Gv * kLatent + 8 * (q & 1));
         const float4 y = *reinterpret_cast<const float4*>(a.latent_f32 + Gv * kLatent + 8 * (q & 1) + 4);
@@ -326,20 +395,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
   };
+  // The first group's list and inputs are requested before the weights are
+  // staged, so that their latencies overlap.
+  Cursor nk{listed ? seg0 - kSegBatch : wid - nwaves, 0, 0};
+  step(nk);
   GroupIn nxt;
-  if ((int)blockIdx.x * 4 + w < ngroups) load_group((int)blockIdx.x * 4 + w, nxt);
-  for (int grp = (int)blockIdx.x * 4 + w; grp < ngroups; grp += nwaves) {
+  if (more(nk)) load_group(nk, nxt);
+  stage_weights(s_w, a.packed, kMlpPackedBytes);
+  for (int i = threadIdx.x; i < kNumSel * 64; i += blockDim.x) s_sel[i >> 6][i & 63] = make_selector(i >> 6, i & 63);
+  __syncthreads();
+  while (more(nk)) {
     const GroupIn cur = nxt;
-    if (grp + nwaves < ngroups) load_group(grp + nwaves, nxt);
+    step(nk);
+    if (more(nk)) load_group(nk, nxt);
     size_t Gs[NCB];
     bool ok[NCB];
     bf16x8 lat[NCB];
     float3 m[NCB], dm[NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
-      const size_t G = (size_t)grp * 16 * NCB + 16 * cb + c;
-      ok[cb] = G < (size_t)a.P;
-      Gs[cb] = ok[cb] ? G : 0;
+      ok[cb] = cur.ok[cb];
+      Gs[cb] = cur.G[cb];
       lat[cb] = (ok[cb] && q < 2) ? cur.lat[cb] : zero_bf16x8();
       m[cb] = cur.m[cb];
       dm[cb] = cur.dm[cb];
@@ -658,16 +734,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   if (lane == 0) row[kThetaSize] = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
 }
-// Sum of rows [r0, r1) of column col with eight loads in flight, in a fixed
-// order (eight strided partial sums, then added pairwise).
+// Sum of rows [r0, r1) of column col with sixteen loads in flight, in a
+// fixed order (sixteen strided partial sums, then added pairwise).
 __device__ __forceinline__ float sum_rows(const float* __restrict__ p, int r0, int r1, int col) {
-  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int r = r0; r < r1; r += 8) {
+  float s[16];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
+  for (int k = 0; k < 16; ++k) s[k] = 0.f;
+  for (int r = r0; r < r1; r += 16) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
       if (r + k < r1) s[k] += p[(size_t)(r + k) * kThetaStride + col];
   }
-  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+#pragma unroll
+  for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+    for (int k = 0; k < h; ++k) s[k] = s[k] + s[k + h];
+  return s[0];
 }
 
 // Stage 1: sums rows [r0, r0 + rows) of each column.  Stage 2 then sums the
@@ -726,11 +808,24 @@ int mlp_train_blocks(int P) {
   return chunks < kMlpTrainBlocks ? (chunks > 0 ? chunks : 1) : kMlpTrainBlocks;
 }
 
-size_t mlp_workspace_floats(int P) {
+// Workspace: the weight-gradient partial rows (nb + n1 rows), then the
+// active-Gaussian list (P rounded up to 256 entries) and its per-block counts.
+static size_t mlp_theta_floats(int P) {
   const int nb = mlp_train_blocks(P);
   const int n1 = (nb + kThetaReduceRows - 1) / kThetaReduceRows;
   return (size_t)(nb + n1) * kThetaStride;
 }
+
+size_t mlp_workspace_floats(int P) {
+  const size_t segs = ((size_t)P + 255) / 256 * 4;  // the preprocess backward's waves
+  return align_up(mlp_theta_floats(P) * sizeof(float)) / sizeof(float) + segs * 64 + segs;
+}
+
+uint32_t* mlp_act_list(float* ws, int P) {
+  return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + align_up(mlp_theta_floats(P) * sizeof(float)));
+}
+
+uint32_t* mlp_act_count(float* ws, int P) { return mlp_act_list(ws, P) + ((size_t)P + 255) / 256 * 256; }
 
 hipError_t launch_mlp_pack(const float* theta, void* packed, hipStream_t s) {
   hipLaunchKernelGGL(mlp_pack_kernel, dim3((1920 + 132 + 255) / 256), dim3(256), 0, s, theta, (char*)packed);
@@ -782,11 +877,16 @@ hipError_t launch_mlp_distill(int P, int M, const float* shs, const float* laten
 
 hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos, const int* radii,
                                const uint16_t* latents, const void* packed, const float* dL_dcolor, float* dL_dmean3D,
-                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V) {
+                               float* dL_dlatent, float* dL_dtheta, float* ws, hipStream_t s, int V,
+                               bool listed) {
   if (P == 0) return hipSuccess;
   MlpTrainArgs a{};
   a.P = P;
   a.V = V;
+  if (listed && V == 1) {
+    a.act_list = mlp_act_list(ws, P);
+    a.act_count = mlp_act_count(ws, P);
+  }
   a.packed = (const char*)packed;
   a.latent_bf16 = latents;
   a.means3D = means3D;

@@ -312,6 +312,23 @@ __global__ __launch_bounds__(256) void preprocess_backward_kernel(PreprocessBack
     gather_sum(a.gather, idx, idx < a.P, gs);
     if (idx < a.P) store_gathered((size_t)idx, gs, a.out_dmean2D, a.out_dconic, a.out_dopacity, a.out_dcolor);
   }
+  if (a.act_list) {
+    // The decoder's work list: this wave's Gaussians with a colour gradient,
+    // in index order, from one ballot (no block barrier).
+    float3 dc = make_float3(0.f, 0.f, 0.f);
+    if (idx < a.P)
+      dc = kGather ? make_float3(gs[0], gs[1], gs[2])
+                   : make_float3(a.dL_dcolor[3 * idx], a.dL_dcolor[3 * idx + 1], a.dL_dcolor[3 * idx + 2]);
+    const bool act = idx < a.P && (dc.x != 0.f || dc.y != 0.f || dc.z != 0.f);
+    const uint64_t bal = __ballot(act);
+    const size_t wave_base = (size_t)idx - (size_t)lane;
+    if (act) a.act_list[wave_base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)idx;
+    if (lane == 0) a.act_count[wave_base / kWave] = (uint32_t)__popcll(bal);
+    if (idx < a.P && !act) {
+      float4* dl = reinterpret_cast<float4*>(a.dL_dlatent + (size_t)idx * 16);
+      dl[0] = dl[1] = dl[2] = dl[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
   const int M = a.M;
   const bool in = idx < a.P;
   const bool staged = a.shs && a.dL_dsh && M == 16;  // uniform over the launch
