@@ -295,13 +295,15 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
         g = capi.backward_scene(ds, R, radii, bufs, dL, mlp=mlp, mlp_workspace=ws)
         return R, radii, g
 
-    median_ms, ms_per_step, stages, _, _, (R_dec, _, _) = time_steps(step, args.warmup, args.steps, dev)
+    median_ms, ms_per_step, stages, _, _, (R_dec, _, g_dec) = time_steps(step, args.warmup, args.steps, dev)
+    # the training pass runs on the Gaussians with a colour gradient only
+    trained = int((g_dec["dL_dcolors"] != 0).any(dim=1).sum())
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items() if v[1]}
     # compulsory traffic with the decoder: the 192 B of SH per Gaussian become 32 B of BF16
     # latents in the forward read, the backward read and the backward gradient write
     alg_dec = 368 * P + 124 * int(R_dec) + 40 * W * H
-    # training pass FLOPs per Gaussian: forward, backward chain and weight gradients, 5888 MACs each
-    flop_per_pass = 3 * 2 * 5888 * P
+    # training pass FLOPs per trained Gaussian: forward, backward chain and weight gradients, 5888 MACs each
+    flop_per_pass = 3 * 2 * 5888 * trained
     dec_ms = stage_ms.get("decoder_backward", 0.0)
     return {
         "ms_per_iter": round(median_ms, 4),
@@ -320,7 +322,7 @@ def decoder_leg(sc, ds, bufs, target, l1, dL, args, dev) -> dict:
         "gate_db": 0.1,
         "gate_passed": bool(loss_db <= 0.1),
         "colour_path_used": "decoder" if loss_db <= 0.1 else "SH (fallback: gate failed)",
-        "decoder_backward_mfma": {"flop": flop_per_pass, "ms": round(dec_ms, 5),
+        "decoder_backward_mfma": {"gaussians_trained": trained, "flop": flop_per_pass, "ms": round(dec_ms, 5),
                                   "TFLOPs": round(flop_per_pass / (dec_ms * 1e-3) / 1e12, 2) if dec_ms else None,
                                   "peak_TFLOPs_bf16": 2500.0},
     }, dec
