@@ -167,6 +167,7 @@ def main() -> int:
         "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
     }
 
+    out["pose_cycle"] = pose_cycle_leg(ds, bufs, target, l1, dL, dev)
     if not args.no_decoder:
         out["mlp_colour"], dec = decoder_leg(sc, ds, bufs, target, l1, dL, args, dev)
         out["batched_mlp_colour"] = batched_decoder_leg(sc, ds, dec, args, dev)
@@ -175,6 +176,54 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out))
     return 0
+
+
+def pose_cycle_leg(ds, bufs, target, l1, dL, dev, n_poses: int = 8, steps: int = 40) -> dict:
+    """The same fwd + L1 + bwd over a cycle of seeded poses (centres uniform in
+    the 1 m cube, as a training loop moves from view to view), so that R
+    changes from call to call: ms per step, and how many forwards kept their
+    speculative binning (a layout of the previous call's capacity with
+    headroom) rather than binning again for the new R."""
+    import statistics
+
+    import numpy as np
+    import torch
+
+    from omnigs_amd import capi, scenes
+    rng = np.random.default_rng(99)
+    poses = []
+    for _ in range(n_poses):
+        vm, cc = scenes.camera_at(rng.uniform(-0.5, 0.5, size=3))
+        poses.append((torch.from_numpy(vm).to(dev), torch.from_numpy(cc).to(dev)))
+    lib = capi.load()
+    kept = []
+
+    def step(k):
+        d = dict(ds)
+        d["viewmatrix"], d["campos"] = poses[k % n_poses]
+        R, img, radii, _ = capi.forward_scene(d, buffers=bufs)
+        kept.append(bufs[1].requested != lib.omnigs_lonlat_binning_bytes(R))
+        l1(img, target, dL_dimage=dL)
+        capi.backward_scene(d, R, radii, bufs, dL)
+        return R
+
+    for k in range(n_poses):
+        step(k)
+    kept.clear()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    Rs = []
+    torch.cuda.synchronize()
+    for k in range(steps):
+        evs[k][0].record()
+        Rs.append(step(k))
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    ts = [a.elapsed_time(b) for a, b in evs]
+    # restore the headline pose's buffers for the legs that follow
+    capi.forward_scene(ds, buffers=bufs)
+    return {"poses": n_poses, "steps": steps, "ms_per_step_median": round(statistics.median(ts), 4),
+            "ms_per_step_mean": round(statistics.mean(ts), 4), "num_rendered_range": [min(Rs), max(Rs)],
+            "forwards_on_larger_capacity": int(sum(kept))}
 
 
 def time_steps(step, warmup: int, steps: int, dev):

@@ -134,6 +134,16 @@ typedef struct omnigs_lonlat_backward_args {
   float* dL_dmlp;              /* [omnigs_mlp_param_count()] output: gradient of the FP32 parameter vector */
   void* mlp_workspace;         /* device scratch of omnigs_mlp_workspace_bytes(P) bytes */
   size_t mlp_workspace_bytes;
+  /* The size in bytes the forward's last call of the binning buffer's
+   * resize callback asked for (the reference's resize functor makes it the
+   * tensor's size, resize_({N}); a caller whose allocation only grows must
+   * remember the request, not its allocation).  The forward bins against the
+   * previous call's capacity with headroom and keeps that binning whenever it
+   * covers R, so the buffer's layout is that of a capacity this size
+   * identifies; 0 means the layout of omnigs_lonlat_binning_bytes(R) (true
+   * when the forward re-binned for R).  A size that matches no layout for R
+   * is OMNIGS_ERR_INVALID_ARGUMENT. */
+  size_t binning_bytes;
 } omnigs_lonlat_backward_args;
 
 /* Forward: preprocess, binning, render.  On success *num_rendered = R. */
@@ -446,7 +456,8 @@ typedef struct omnigs_lonlat_state_view {
 } omnigs_lonlat_state_view;
 
 int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, const char* image_buffer, int P, int R,
-                        int width, int height, omnigs_lonlat_state_view* out);
+                        int width, int height, omnigs_lonlat_state_view* out,
+                        size_t binning_bytes /* as omnigs_lonlat_backward_args::binning_bytes */);
 
 /* The same view of a batched workspace (omnigs_lonlat_batch_forward /
  * _backward), for stage-by-stage parity tests.  Per-item arrays hold V P

@@ -77,6 +77,7 @@ class BackwardArgs(ctypes.Structure):
         ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
         ("mlp_packed", ctypes.c_void_p), ("latents", ctypes.c_void_p), ("dL_dlatent", ctypes.c_void_p),
         ("dL_dmlp", ctypes.c_void_p), ("mlp_workspace", ctypes.c_void_p), ("mlp_workspace_bytes", ctypes.c_size_t),
+        ("binning_bytes", ctypes.c_size_t),
     ]
 
 
@@ -163,7 +164,8 @@ def load() -> ctypes.CDLL:
     lib.omnigs_lonlat_image_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.omnigs_lonlat_image_bytes.restype = ctypes.c_size_t
     lib.omnigs_lonlat_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(StateView)]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(StateView),
+                                        ctypes.c_size_t]
     lib.omnigs_lonlat_state.restype = ctypes.c_int
     lib.omnigs_profile_enable.argtypes = [ctypes.c_int]
     lib.omnigs_profile_enable.restype = ctypes.c_int
@@ -241,16 +243,21 @@ def _ptr(t: torch.Tensor | None) -> int | None:
 
 class TorchBuffer:
     """A uint8 device tensor grown by the library through the resize callback
-    (the C form of the reference's resizeFunctional, rasterize_points.cu:41-47)."""
+    (the C form of the reference's resizeFunctional, rasterize_points.cu:41-47).
+    The allocation only grows; ``requested`` is the size the library last
+    asked for, which identifies the layout the forward chose (the backward's
+    binning_bytes)."""
 
     def __init__(self, device: torch.device):
         self.device = device
         self.tensor = torch.empty(0, dtype=torch.uint8, device=device)
+        self.requested = 0
 
         def _resize(_ctx, nbytes):
             try:
                 if self.tensor.numel() < nbytes:
                     self.tensor = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+                self.requested = int(nbytes)
                 return self.tensor.data_ptr() if nbytes else self.tensor.data_ptr() or None
             except Exception:  # allocation failure surfaces as OMNIGS_ERR_ALLOC
                 return None
@@ -344,7 +351,8 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
                      _ptr(g.get("dpx_dt")), _ptr(g.get("dpy_dt")), int(bool(seam_wrap)),
                      _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None, _ptr(g.get("dL_dlatent")),
                      _ptr(g.get("dL_dmlp")), mlp_workspace.data_ptr() if mlp else None,
-                     mlp_workspace.numel() if mlp else 0)
+                     mlp_workspace.numel() if mlp else 0,
+                     binning.requested if bin_ptr is not None else 0)
     if checkpointed is not None:
         fa = ForwardArgs(P, degree, M, W, H, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
                          _ptr(checkpointed["opacities"]), _ptr(scales), float(scale_modifier), _ptr(rotations),
@@ -373,7 +381,8 @@ def state(buffers, P: int, R: int, width: int, height: int) -> dict:
     geom, binning, image = buffers
     sv = StateView()
     bases = [b.tensor.data_ptr() if b.tensor.numel() else None for b in buffers]
-    st = lib.omnigs_lonlat_state(bases[0], bases[1], bases[2], P, R, width, height, ctypes.byref(sv))
+    st = lib.omnigs_lonlat_state(bases[0], bases[1], bases[2], P, R, width, height, ctypes.byref(sv),
+                                 binning.requested)
     if st != OMNIGS_OK:
         raise OmnigsError(st, "omnigs_lonlat_state")
     tiles = ((width + 15) // 16) * ((height + 15) // 16)

@@ -158,7 +158,7 @@ const char* omnigs_stage_name(int stage) {
 }
 
 int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, const char* image_buffer, int P, int R,
-                        int width, int height, omnigs_lonlat_state_view* out) {
+                        int width, int height, omnigs_lonlat_state_view* out, size_t binning_bytes) {
   if (!out || P < 0 || R < 0 || width <= 0 || height <= 0) return fail_arg("state view");
   std::memset(out, 0, sizeof(*out));
   if (geom_buffer) {
@@ -172,7 +172,15 @@ int omnigs_lonlat_state(const char* geom_buffer, const char* binning_buffer, con
     out->offsets = g.offsets;
   }
   if (binning_buffer && R > 0) {
-    const BinState b = carve_bin(const_cast<char*>(binning_buffer), bin_capacity((uint32_t)R));
+    // the arrays viewed precede the gradient records, so a forward-only
+    // layout (omnigs_lonlat_forward_checkpointed) is viewed the same way
+    uint32_t C = bin_capacity((uint32_t)R);
+    if (binning_bytes) {
+      C = bin_capacity_of_bytes((uint32_t)R, binning_bytes, true);
+      if (C == 0) C = bin_capacity_of_bytes((uint32_t)R, binning_bytes, false);
+      if (C == 0) return fail_arg("binning_bytes is not the size of a forward's binning buffer for this R");
+    }
+    const BinState b = carve_bin(const_cast<char*>(binning_buffer), C);
     const bool alt = radix_passes(bit_width_u32((uint32_t)tile_grid(width, height).tiles())) % 2;
     out->point_list = alt ? b.gid_alt : b.gid;
     out->tile_keys = alt ? b.tile_key_alt : b.tile_key;
@@ -394,11 +402,13 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   // The instance count R sizes the binning buffer, and the host needs it to
   // return it.  Its read-back ({R, overflow flag} after the scan's block
   // partials) is queued first; the binning and the render are then queued
-  // against the capacity bucket of the previous call's R (the kernels read
-  // the live count on the device and stop there), and only then does the host
-  // wait for R -- so the GPU has the rest of the forward to run meanwhile.  A
-  // call whose R falls in another bucket (and the first call on a thread)
-  // bins again with the exact bucket: the backward carves the buffer from R.
+  // against the capacity bucket of the previous call's R with 1/16 headroom
+  // (the kernels read the live count on the device and stop there), and only
+  // then does the host wait for R -- so the GPU has the rest of the forward
+  // to run meanwhile.  That binning stands whenever its capacity covers R
+  // (and is under twice R): the backward recovers the capacity from the
+  // binning buffer's size.  A call whose R outgrew it (and the first call on
+  // a thread) bins again with bin_capacity(R).
   const uint32_t* total = g.scan_partials + scan_blocks((size_t)P);
   uint32_t* pw = pinned_word();
   hipEvent_t ev = readback_event();
@@ -408,7 +418,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     OMNIGS_TRY(hipEventRecord(ev, s), "read R event");
   }
   thread_local uint32_t last_R = 0;
-  const uint32_t guess = async ? bin_capacity(last_R) : 0u;
+  const uint32_t guess = async && last_R ? bin_capacity(last_R + last_R / 16u) : 0u;
   if (guess > 0) {
     const int st = bin_and_render(a, binning_buffer, g, img, grid, guess, total, s, render, records);
     if (st != OMNIGS_OK) return st;
@@ -427,7 +437,8 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   }
   if (Rw[1] != 0u || Rw[0] > 0x7fffffffu) return fail_arg("instance count exceeds int range");
   const uint32_t R = Rw[0];
-  if (bin_capacity(R) != guess) {
+  const bool covered = guess > 0 && R > 0 && guess >= R && (uint64_t)guess <= 2ull * R;
+  if (!covered) {
     const int st = bin_and_render(a, binning_buffer, g, img, grid, R, nullptr, s, render, records);
     if (st != OMNIGS_OK) return st;
   }
@@ -533,7 +544,10 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   const uint8_t* blend = nullptr;
   const uint32_t* blend_ok = nullptr;
   if (R > 0) {
-    const BinState b = carve_bin(a->binning_buffer, bin_capacity((uint32_t)R));
+    const uint32_t C = a->binning_bytes ? bin_capacity_of_bytes((uint32_t)R, a->binning_bytes)
+                                        : bin_capacity((uint32_t)R);
+    if (C == 0) return fail_arg("binning_bytes is not the size of a forward's binning buffer for this R");
+    const BinState b = carve_bin(a->binning_buffer, C);
     const int passes = radix_passes(bit_width_u32((uint32_t)grid.tiles()));
     point_list = (passes % 2) ? b.gid_alt : b.gid;
     grad_rec = b.grad_rec;

@@ -377,6 +377,24 @@ inline BinState carve_bin(char* base, size_t R, bool records = true) {
   return b;
 }
 
+// The capacity whose binning layout occupies `bytes` bytes: a forward may
+// bin against a larger capacity than bin_capacity(R) (the previous call's
+// bucket with headroom), so the backward recovers it from the buffer's size.
+// The capacities a forward uses are bin_capacity values (multiples of at
+// least 1024), on which the layout's size strictly increases.  0 when no
+// such capacity at or above R has that size.
+inline uint32_t bin_capacity_of_bytes(uint32_t R, size_t bytes, bool records = true) {
+  if (R == 0) return 0;
+  for (uint32_t C = bin_capacity(R);;) {
+    const size_t b = carve_bin(nullptr, C, records).bytes;
+    if (b == bytes) return C;
+    if (b > bytes) return 0;
+    const uint32_t next = bin_capacity(C + 1u);
+    if (next <= C) return 0;
+    C = next;
+  }
+}
+
 // Per-pixel / per-tile state.
 struct ImgState {
   float* final_T;

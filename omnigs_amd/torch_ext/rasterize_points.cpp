@@ -205,6 +205,7 @@ RasterizeGaussiansBackwardCUDA(const torch::Tensor& background, const torch::Ten
     a.radii = rad.numel() ? rad.data_ptr<int>() : nullptr;
     a.geom_buffer = reinterpret_cast<char*>(gbuf.data_ptr());
     a.binning_buffer = bbuf.numel() ? reinterpret_cast<char*>(bbuf.data_ptr()) : nullptr;
+    a.binning_bytes = (size_t)bbuf.numel();  // the forward's layout, identified by its size
     a.image_buffer = reinterpret_cast<char*>(ibuf.data_ptr());
     a.dL_dpix = dpix.ptr();
     a.dL_dmean2D = dL_dmeans2D.data_ptr<float>();

@@ -107,9 +107,14 @@ def test_state_view_offsets_are_inside_buffers(lib):
     P, R, W, H = 1000, 5000, 200, 90
     base_g, base_b, base_i = 0x100000, 0x900000, 0x2000000
     sv = capi.StateView()
-    assert lib.omnigs_lonlat_state(base_g, base_b, base_i, P, R, W, H, ctypes.byref(sv)) == 0
+    assert lib.omnigs_lonlat_state(base_g, base_b, base_i, P, R, W, H, ctypes.byref(sv), 0) == 0
     gsz = lib.omnigs_lonlat_geometry_bytes(P)
     bsz = lib.omnigs_lonlat_binning_bytes(R)
+    # a binning buffer laid out for a larger capacity (a forward's speculative
+    # binning that covered R) is recognised by its size; other sizes refused
+    assert lib.omnigs_lonlat_state(base_g, base_b, base_i, P, R, W, H, ctypes.byref(sv), bsz + 4) == \
+        capi.OMNIGS_ERR_INVALID_ARGUMENT
+    assert lib.omnigs_lonlat_state(base_g, base_b, base_i, P, R, W, H, ctypes.byref(sv), bsz) == 0
     isz = lib.omnigs_lonlat_image_bytes(W, H)
     for f in ("depth_key", "radii", "render_records", "clamped", "tiles_touched",
               "depth_order", "offsets"):
