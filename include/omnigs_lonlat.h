@@ -140,9 +140,8 @@ typedef struct omnigs_lonlat_backward_args {
    * remember the request, not its allocation).  The forward bins against the
    * previous call's capacity with headroom and keeps that binning whenever it
    * covers R, so the buffer's layout is that of a capacity this size
-   * identifies; 0 means the layout of omnigs_lonlat_binning_bytes(R) (true
-   * when the forward re-binned for R).  A size that matches no layout for R
-   * is OMNIGS_ERR_INVALID_ARGUMENT. */
+   * identifies.  Required when R > 0: a size that matches no layout for R
+   * (0 included) is OMNIGS_ERR_INVALID_ARGUMENT. */
   size_t binning_bytes;
 } omnigs_lonlat_backward_args;
 

@@ -544,8 +544,9 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* a, void* stream_) 
   const uint8_t* blend = nullptr;
   const uint32_t* blend_ok = nullptr;
   if (R > 0) {
-    const uint32_t C = a->binning_bytes ? bin_capacity_of_bytes((uint32_t)R, a->binning_bytes)
-                                        : bin_capacity((uint32_t)R);
+    // the forward's layout, named by the size it asked for (required: a
+    // forward may keep a layout larger than R's own)
+    const uint32_t C = bin_capacity_of_bytes((uint32_t)R, a->binning_bytes);
     if (C == 0) return fail_arg("binning_bytes is not the size of a forward's binning buffer for this R");
     const BinState b = carve_bin(a->binning_buffer, C);
     const int passes = radix_passes(bit_width_u32((uint32_t)grid.tiles()));

@@ -98,6 +98,13 @@ def test_argument_validation_needs_no_gpu(lib):
     b.D, b.M, b.width, b.height, b.seam_wrap = 0, 1, 1000, 500, 1
     assert lib.omnigs_lonlat_backward(ctypes.byref(b), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
     assert b"multiple of the tile width" in lib.omnigs_last_error()
+    # the binning buffer's size names the forward's layout: required, refused
+    # before any device work when it names none
+    b.seam_wrap = 0
+    for nb in (0, lib.omnigs_lonlat_binning_bytes(5) + 4):
+        b.binning_bytes = nb
+        assert lib.omnigs_lonlat_backward(ctypes.byref(b), None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
+        assert b"binning_bytes" in lib.omnigs_last_error()
     assert lib.omnigs_lonlat_mark_visible(-1, None, None) == capi.OMNIGS_ERR_INVALID_ARGUMENT
     assert lib.omnigs_lonlat_mark_visible(0, None, None) == capi.OMNIGS_OK
 
