@@ -509,9 +509,10 @@ def memory_compare(dev, args) -> int:
     vm, cp, bg, target = t(sc["viewmatrix"]), t(sc["campos"]), t(sc["background"]), t(sc["target"])
     res = {}
     # checkpointed: the backward bins in bands of tile rows (the library's
-    # default budget, R / 8 instances here); one_band: the same with a budget
+    # default budget, R / 4 instances here); one_band: the same with a budget
     # above R, i.e. the whole binning at once
-    modes = (("stored", False, 0), ("checkpointed", True, 0), ("checkpointed_one_band", True, 2**31 - 1))
+    modes = (("stored", False, 0), ("checkpointed", True, 0), ("checkpointed_one_band", True, 2**31 - 1),
+             ("checkpointed_budget_8M", True, 8_000_000), ("checkpointed_budget_13M", True, 13_000_000))
     wanted = set(args.memory_modes.split(","))
     for mode, ck, budget in (m for m in modes if m[0] in wanted):
 

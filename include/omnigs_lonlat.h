@@ -183,7 +183,7 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* args, void* stream
  * binning buffers were released: re-runs preprocess and the depth sort into
  * a fresh geometry buffer, then bins and runs the render backward one band of
  * whole tile rows at a time -- at most `band_instances` instances per band (0:
- * max(2^20, R / 8); a row holding more is a band of its own) -- adding each
+ * max(2^20, R / 4); a row holding more is a band of its own) -- adding each
  * band's per-Gaussian sums to the 2D gradient outputs, and ends with the
  * preprocess backward.  The binning buffer is grown to the largest band only,
  * so the gradient records (36 of the 53 bytes per instance) never exist for

@@ -645,7 +645,7 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
   }
   if (R != (uint64_t)a->R) return fail_arg("recomputed instance count differs from the forward's R");
   const uint64_t budget = band_instances ? (uint64_t)band_instances
-                                         : std::max<uint64_t>((uint64_t)1 << 20, (R + 7) / 8);
+                                         : std::max<uint64_t>((uint64_t)1 << 20, (R + 3) / 4);
   struct Band {
     int y0, y1;
     uint32_t n;

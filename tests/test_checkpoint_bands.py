@@ -39,7 +39,7 @@ def _expected_bands(tile_keys: np.ndarray, gx: int, gy: int, budget: int, R: int
     instances per band unless one row holds more), rows from the forward's
     own tile keys."""
     rows = np.bincount(tile_keys.astype(np.int64) // gx, minlength=gy)
-    b = budget if budget else max(1 << 20, (R + 7) // 8)
+    b = budget if budget else max(1 << 20, (R + 3) // 4)
     n_bands, y = 0, 0
     while y < gy:
         n = int(rows[y])
