@@ -403,7 +403,8 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   // return it.  Its read-back ({R, overflow flag} after the scan's block
   // partials) is queued first; the binning and the render are then queued
   // against the capacity bucket of the previous call's R with 1/16 headroom
-  // (the kernels read the live count on the device and stop there), and only
+  // when that call had the same Gaussian count and image size (the kernels
+  // read the live count on the device and stop there), and only
   // then does the host wait for R -- so the GPU has the rest of the forward
   // to run meanwhile.  That binning stands whenever its capacity covers R
   // (and is under twice R): the backward recovers the capacity from the
@@ -417,8 +418,17 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     OMNIGS_TRY(hipMemcpyAsync(pw, total, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read R");
     OMNIGS_TRY(hipEventRecord(ev, s), "read R event");
   }
-  thread_local uint32_t last_R = 0;
-  const uint32_t guess = async && last_R ? bin_capacity(last_R + last_R / 16u) : 0u;
+  // The previous call on this thread, if it rendered the same number of
+  // Gaussians at the same size (a training loop between densifications):
+  // another scene or size bins exactly, rather than against a capacity from
+  // an unrelated call.
+  struct LastCall {
+    uint32_t R;
+    int P, W, H;
+  };
+  thread_local LastCall last{0u, 0, 0, 0};
+  const bool same_shape = last.P == a->P && last.W == a->width && last.H == a->height;
+  const uint32_t guess = async && same_shape && last.R ? bin_capacity(last.R + last.R / 16u) : 0u;
   if (guess > 0) {
     const int st = bin_and_render(a, binning_buffer, g, img, grid, guess, total, s, render, records);
     if (st != OMNIGS_OK) return st;
@@ -442,7 +452,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     const int st = bin_and_render(a, binning_buffer, g, img, grid, R, nullptr, s, render, records);
     if (st != OMNIGS_OK) return st;
   }
-  last_R = R;
+  last = LastCall{R, a->P, a->width, a->height};
   *num_rendered = (int)R;
   return OMNIGS_OK;
 }

@@ -78,3 +78,27 @@ def test_backward_refuses_a_foreign_binning_size(gpu):
     bufs[1].requested += 256  # a size no forward carved
     with pytest.raises(capi.OmnigsError, match="binning_bytes"):
         capi.backward_scene(ds, R, radii, bufs, torch.from_numpy(sc["dL_dpix"]).to(gpu))
+
+
+def test_no_speculation_across_scenes(gpu):
+    """A forward of another scene (another Gaussian count or image size) bins
+    exactly for its R: its binning buffer is never grown for the previous
+    scene's capacity."""
+    import torch
+
+    from omnigs_amd import capi
+    from tests.gpu_util import announce, to_dev
+    announce("no speculative binning across scenes: 200k @ 1024x512 then 2k @ 128x64")
+    big = to_dev(scenes.small_scene(P=200_000, width=1024, height=512, seed=6))
+    small = to_dev(scenes.small_scene(P=2000, width=128, height=64, seed=6))
+    capi.forward_scene(big)
+    capi.forward_scene(big)
+    R, _, _, bufs = capi.forward_scene(small)
+    torch.cuda.synchronize()
+    exact = capi.load().omnigs_lonlat_binning_bytes(R)
+    assert bufs[1].requested == exact
+    assert bufs[1].tensor.numel() == exact  # the only size this fresh buffer was ever grown to
+    # the same scene again keeps a speculative layout that covers R
+    R2, _, _, bufs2 = capi.forward_scene(small)
+    torch.cuda.synchronize()
+    assert R2 == R and bufs2[1].requested >= exact
