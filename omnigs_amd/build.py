@@ -38,11 +38,13 @@ HIP_FLAGS = [
 ]
 
 
-# Per-source additions.  mlp.hip: MFMA results in ordinary VGPRs (its
-# training kernel runs one wave per SIMD; left to itself the compiler writes
-# every MFMA result to accumulation registers and copies each back before
-# use).
-HIP_FILE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# Per-source additions.  mlp.hip and preprocess.hip (whose decoder form
+# evaluates the MLP inline): MFMA results in ordinary VGPRs.  Left to itself
+# the compiler writes every MFMA result to accumulation registers and copies
+# each back before use (150 copies per block in the training kernel, 156 per
+# wave in the decoder preprocess).
+HIP_FILE_FLAGS = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+                  "preprocess.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd: list[str]) -> None:

@@ -204,7 +204,6 @@ hipError_t launch_preprocess_backward_views(const PreprocessBackwardArgs& a, con
 
 // mlp.hip
 constexpr int kMlpTrainBlocks = 256;  // persistent grid of the decoder training pass (one block per CU)
-constexpr int kThetaReduceRows = 16;  // rows per stage-1 block of the weight-gradient reduction
 int mlp_train_blocks(int P);
 size_t mlp_workspace_floats(int P);
 hipError_t launch_mlp_pack(const float* theta, void* packed, hipStream_t s);

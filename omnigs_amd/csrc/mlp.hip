@@ -238,8 +238,6 @@ __device__ __forceinline__ bf16x8 relu_mask(const bf16x8& h, const bf16x8& g) {
   return __builtin_bit_cast(bf16x8, make_uint4(r[0], r[1], r[2], r[3]));
 }
 
-__device__ __forceinline__ float sigmoid_rcp(float v) { return __builtin_amdgcn_rcpf(1.f + __expf(-v)); }
-
 // 32-bit cross-row moves: the value lane (c, 1) holds (row), or lane (c, 2)
 // (row2), delivered to lane (c, 0).  A permlane swap of a register with
 // itself leaves {own, other} in the two results of row 0 (in an order the
@@ -524,7 +522,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                                fmaxf(row_sum(tc.z) + 0.5f, 0.f)};
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
-            const float s = sigmoid_rcp(o[cb][e]);
+            const float s = sigmoidf(o[cb][e]);
             const float err = own ? s - tt[e] : 0.f;
             sq_err += err * err;
             dO[e] = 2.f * err * inv3P * s * (1.f - s);
@@ -533,7 +531,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
           const float dc[3] = {dcv[cb].x, dcv[cb].y, dcv[cb].z};
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
-            const float s = sigmoid_rcp(o[cb][e]);
+            const float s = sigmoidf(o[cb][e]);
             dO[e] = own ? dc[e] * s * (1.f - s) : 0.f;
           }
         }
@@ -736,40 +734,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 // Sum of rows [r0, r1) of column col with sixteen loads in flight, in a
 // fixed order (sixteen strided partial sums, then added pairwise).
-__device__ __forceinline__ float sum_rows(const float* __restrict__ p, int r0, int r1, int col) {
-  float s[16];
+// The weight-gradient sum over the nb partial rows, in one launch: block b
+// owns columns [64 b, 64 b + 64) (lane = column; column kThetaSize is the
+// distillation error), wave w of its 16 sums rows w, w + 16, ... with all of
+// its loads in flight together, and wave 0 adds the 16 wave sums.  The order
+// is fixed, so the result is deterministic.
+constexpr int kReduceWaves = 16;
+
+__global__ __launch_bounds__(64 * kReduceWaves) void theta_reduce_kernel(const float* __restrict__ part, int nrows,
+                                                                        float* __restrict__ grad, float err_scale,
+                                                                        float* __restrict__ loss) {
+  __shared__ float s_part[kReduceWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = blockIdx.x * 64 + lane;  // < kThetaStride: every load stays in its row
+  float acc = 0.f;
+  for (int r0 = w; r0 < nrows; r0 += 16 * kReduceWaves) {
+    float v[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) s[k] = 0.f;
-  for (int r = r0; r < r1; r += 16) {
+    for (int k = 0; k < 16; ++k) {
+      // clamped rows and a select after the load: no branch holds a load back
+      const int r = r0 + k * kReduceWaves;
+      const float x = part[(size_t)min(r, nrows - 1) * kThetaStride + col];
+      v[k] = r < nrows ? x : 0.f;
+    }
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
-      if (r + k < r1) s[k] += p[(size_t)(r + k) * kThetaStride + col];
+    for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+      for (int k = 0; k < h; ++k) v[k] = v[k] + v[k + h];
+    acc += v[0];
   }
+  s_part[w][lane] = acc;
+  __syncthreads();
+  if (w != 0 || col > kThetaSize) return;
+  float t[kReduceWaves];
 #pragma unroll
-  for (int h = 8; h > 0; h >>= 1)
+  for (int k = 0; k < kReduceWaves; ++k) t[k] = s_part[k][lane];
 #pragma unroll
-    for (int k = 0; k < h; ++k) s[k] = s[k] + s[k + h];
-  return s[0];
-}
-
-// Stage 1: sums rows [r0, r0 + rows) of each column.  Stage 2 then sums the
-// stage-1 rows into out (columns < kThetaSize) and scales the error column.
-__global__ __launch_bounds__(256) void theta_reduce1_kernel(const float* __restrict__ part, int nrows, int rows,
-                                                            float* __restrict__ out1) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col > kThetaSize) return;
-  const int r0 = blockIdx.y * rows;
-  out1[(size_t)blockIdx.y * kThetaStride + col] = sum_rows(part, r0, min(nrows, r0 + rows), col);
-}
-
-__global__ __launch_bounds__(256) void theta_reduce2_kernel(const float* __restrict__ in1, int nrows,
-                                                            float* __restrict__ grad, float err_scale,
-                                                            float* __restrict__ loss) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col > kThetaSize) return;
-  const float s = sum_rows(in1, 0, nrows, col);
-  if (col < kThetaSize) grad[col] = s;
-  else if (loss) *loss = s * err_scale;
+  for (int h = kReduceWaves / 2; h > 0; h >>= 1)
+#pragma unroll
+    for (int k = 0; k < h; ++k) t[k] = t[k] + t[k + h];
+  if (col < kThetaSize) grad[col] = t[0];
+  else if (loss) *loss = t[0] * err_scale;
 }
 
 // ----------------------------------------------------------------------------
@@ -808,13 +813,9 @@ int mlp_train_blocks(int P) {
   return chunks < kMlpTrainBlocks ? (chunks > 0 ? chunks : 1) : kMlpTrainBlocks;
 }
 
-// Workspace: the weight-gradient partial rows (nb + n1 rows), then the
+// Workspace: the weight-gradient partial rows (nb rows), then the
 // active-Gaussian list (P rounded up to 256 entries) and its per-block counts.
-static size_t mlp_theta_floats(int P) {
-  const int nb = mlp_train_blocks(P);
-  const int n1 = (nb + kThetaReduceRows - 1) / kThetaReduceRows;
-  return (size_t)(nb + n1) * kThetaStride;
-}
+static size_t mlp_theta_floats(int P) { return (size_t)mlp_train_blocks(P) * kThetaStride; }
 
 size_t mlp_workspace_floats(int P) {
   const size_t segs = ((size_t)P + 255) / 256 * 4;  // the preprocess backward's waves
@@ -841,12 +842,9 @@ hipError_t launch_mlp_eval(int N, const uint16_t* latents, const float* dirs, co
 }
 
 static hipError_t reduce_theta(int nb, float* ws, float* dL_dtheta, float err_scale, float* loss, hipStream_t s) {
-  const int n1 = (nb + kThetaReduceRows - 1) / kThetaReduceRows;
-  float* stage1 = ws + (size_t)nb * kThetaStride;
-  const dim3 g1((kThetaSize + 1 + 255) / 256, n1);
-  hipLaunchKernelGGL(theta_reduce1_kernel, g1, dim3(256), 0, s, (const float*)ws, nb, kThetaReduceRows, stage1);
-  hipLaunchKernelGGL(theta_reduce2_kernel, dim3((kThetaSize + 1 + 255) / 256), dim3(256), 0, s,
-                     (const float*)stage1, n1, dL_dtheta, err_scale, loss);
+  static_assert(((kThetaSize + 1 + 63) / 64) * 64 <= kThetaStride, "reduction columns stay inside a row");
+  hipLaunchKernelGGL(theta_reduce_kernel, dim3((kThetaSize + 1 + 63) / 64), dim3(64 * kReduceWaves), 0, s,
+                     (const float*)ws, nb, dL_dtheta, err_scale, loss);
   return hipGetLastError();
 }
 

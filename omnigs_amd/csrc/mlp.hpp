@@ -204,7 +204,9 @@ __host__ __device__ __forceinline__ uint32_t distill_dir_index(uint32_t seed, in
   return mix32(seed * 0x632BE5ABU + (uint32_t)step * 0x9E3779B9U + g) % (uint32_t)num_dirs;
 }
 
-__device__ __forceinline__ float sigmoidf(float v) { return 1.f / (1.f + __expf(-v)); }
+// Logistic sigmoid with the hardware reciprocal (1 ulp), the same in every
+// decoder kernel, so a training pass sees the colours the renders use.
+__device__ __forceinline__ float sigmoidf(float v) { return __builtin_amdgcn_rcpf(1.f + __expf(-v)); }
 
 // Latent inputs 8h..8h+7 of Gaussian G (BF16 [P,16]); zeros when !ok.
 __device__ __forceinline__ bf16x8 load_latent_half(const uint16_t* __restrict__ lat, size_t G, int q, bool ok) {
@@ -212,24 +214,47 @@ __device__ __forceinline__ bf16x8 load_latent_half(const uint16_t* __restrict__ 
   return *reinterpret_cast<const bf16x8*>(lat + G * kLatent + 8 * q);
 }
 
-// Decoder colours of the wave's 64 Gaussians (wave_base + lane), evaluated
-// inside the forward preprocess.  dir: this lane's own view direction.  All
-// 64 lanes must call it.  w: forward part of the packed image in LDS.
-__device__ __forceinline__ float3 mlp_colour_wave(const char* w, const uint16_t* __restrict__ latents, int P,
-                                                  size_t wave_base, float3 dir) {
+// Latents of the wave's 64 Gaussians (wave_base + 0..63) as the layer-1 B
+// operand halves of the four column blocks: lane (c, q < 2) holds inputs
+// 8q..8q+7 of Gaussian wave_base + 16 cb + c, other lanes zeros.  Issued
+// ahead of other work so that their HBM latency overlaps it.  P >= 1.
+__device__ __forceinline__ void mlp_wave_latents(const uint16_t* __restrict__ latents, int P, size_t wave_base,
+                                                 bf16x8 lat[4]) {
   const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
-  // all four column blocks' latent loads first: one HBM latency per wave
-  bf16x8 lat[4];
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
+    // every lane loads (from a clamped row) and masks afterwards: no branch
+    // around the load, so all four stay in flight together
     const size_t G = wave_base + 16 * cb + c;
-    lat[cb] = load_latent_half(latents, G, q & 1, G < (size_t)P && q < 2);
+    const bool ok = G < (size_t)P && q < 2;
+    const size_t Gc = G < (size_t)P ? G : (size_t)P - 1;
+    const uint4 v = *reinterpret_cast<const uint4*>(latents + Gc * kLatent + 8 * (q & 1));
+    const uint32_t m = ok ? 0xFFFFFFFFu : 0u;
+    lat[cb] = __builtin_bit_cast(bf16x8, make_uint4(v.x & m, v.y & m, v.z & m, v.w & m));
   }
+}
+
+// Decoder colours of the wave's 64 Gaussians, evaluated inside the forward
+// preprocess.  lat: from mlp_wave_latents; dir: this lane's own view
+// direction.  All 64 lanes must call it.  w: forward part of the packed image
+// in LDS.
+__device__ __forceinline__ float3 mlp_colour_wave(const char* w, const bf16x8 lat[4], float3 dir) {
+  const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
+  // each lane packs the basis of its own direction once; column block cb
+  // then fetches Gaussian 16 cb + c's packs (the same bits input_operand
+  // would compute there)
+  float b[9];
+  sh2_basis(dir.x, dir.y, dir.z, b);
+  const uint32_t p0 = cvt_pk_bf16(b[0], b[1]), p1 = cvt_pk_bf16(b[2], b[3]), p2 = cvt_pk_bf16(b[4], b[5]),
+                 p3 = cvt_pk_bf16(b[6], b[7]), p4 = cvt_pk_bf16(b[8], 0.f);
   float3 mine = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
-    const float x = __shfl(dir.x, 16 * cb + c), y = __shfl(dir.y, 16 * cb + c), z = __shfl(dir.z, 16 * cb + c);
-    const bf16x8 xin = input_operand(q, lat[cb], x, y, z);
+    const int src = 16 * cb + c;
+    const uint32_t v0 = __shfl(p0, src), v1 = __shfl(p1, src), v2 = __shfl(p2, src), v3 = __shfl(p3, src),
+                   v4 = __shfl(p4, src);
+    const uint4 bas = q == 2 ? make_uint4(v0, v1, v2, v3) : make_uint4(v4, 0u, 0u, 0u);
+    const bf16x8 xin = __builtin_bit_cast(bf16x8, q < 2 ? __builtin_bit_cast(uint4, lat[cb]) : bas);
     MlpAct a;
     mlp_forward_block(w, lane, xin, a);
     // lanes (c, 0) hold the logits of Gaussian 16cb + c; lane 16cb + c takes them

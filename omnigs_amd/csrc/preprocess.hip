@@ -129,13 +129,25 @@ template <bool kMlp>
 __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomState g, int* __restrict__ radii_out,
                                                          TileGrid grid) {
   __shared__ __attribute__((aligned(16))) char s_w[kMlp ? kMlpFwdBytes : 16];
-  if constexpr (kMlp) {
-    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_packed);
-    for (int i = threadIdx.x; i < kMlpFwdBytes / 16; i += blockDim.x) reinterpret_cast<uint4*>(s_w)[i] = src[i];
-    __syncthreads();
-  }
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in_range = idx < a.P;
+  // MLP: the decoder weights (for LDS) and the wave's latents are requested
+  // before the geometry, so their latencies overlap it; the weights reach
+  // LDS before the block scan's barriers, which publish them.
+  // (four named registers: an indexed array here would live in scratch)
+  constexpr int kWn = kMlpFwdBytes / 16;
+  static_assert(kWn > 768 && kWn <= 1024, "four 16-byte weight loads per thread of 256");
+  uint4 w0{}, w1{}, w2{}, w3{};
+  bf16x8 lat[kMlp ? 4 : 1];
+  if constexpr (kMlp) {
+    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_packed);
+    const int t = threadIdx.x;
+    w0 = src[t];
+    w1 = src[t + 256];
+    w2 = src[t + 512];
+    w3 = src[min(t + 768, kWn - 1)];
+    mlp_wave_latents(a.latents, a.P, (size_t)(idx & ~63), lat);
+  }
 
   if (in_range) {
     radii_out[idx] = 0;
@@ -144,6 +156,14 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
   }
   GeomOut o;
   const bool live = in_range && preprocess_geometry(a, idx, grid, o);
+  if constexpr (kMlp) {
+    uint4* dst = reinterpret_cast<uint4*>(s_w);
+    const int t = threadIdx.x;
+    dst[t] = w0;
+    dst[t + 256] = w1;
+    dst[t + 512] = w2;
+    if (t + 768 < kWn) dst[t + 768] = w3;
+  }
   // gradient slots: the block's Gaussians in index order, one slot per tile
   // of the rect (every thread of the block takes part in the scan)
   uint32_t block_total;
@@ -163,7 +183,7 @@ __global__ __launch_bounds__(256) void preprocess_kernel(PreprocessArgs a, GeomS
       const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
       if (len > 0.f) dir = make_float3(d.x / len, d.y / len, d.z / len);
     }
-    col = mlp_colour_wave(s_w, a.latents, a.P, (size_t)(idx & ~63), dir);
+    col = mlp_colour_wave(s_w, lat, dir);
     if (!live) return;
     g.clamped[idx] = 0;
   } else {
@@ -236,7 +256,9 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
         const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
         if (len > 0.f) dir = make_float3(d.x / len, d.y / len, d.z / len);
       }
-      col = mlp_colour_wave(s_w, a.latents, a.P, (size_t)(idx & ~63), dir);
+      bf16x8 lat[4];
+      mlp_wave_latents(a.latents, a.P, (size_t)(idx & ~63), lat);
+      col = mlp_colour_wave(s_w, lat, dir);
       if (!live) continue;
       g.clamped[item] = 0;
     } else {
