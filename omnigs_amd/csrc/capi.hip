@@ -971,6 +971,13 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
                             a->rotations, a->cov3D_precomp, nullptr, nullptr, g.radii, g.clamped, bs.dmean2D,
                             bs.dconic, bs.dcolor, gr->dL_dmean3D, gr->dL_dcov3D, mlp ? nullptr : gr->dL_dsh,
                             gr->dL_dscale, gr->dL_drot, nullptr, nullptr};
+  if (mlp) {
+    // the decoder's backward runs on the Gaussians with a colour gradient in
+    // some view, listed by the preprocess backward (all of their views)
+    pb.act_list = mlp_act_list(static_cast<float*>(gr->mlp_workspace), P);
+    pb.act_count = mlp_act_count(static_cast<float*>(gr->mlp_workspace), P);
+    pb.dL_dlatent = gr->dL_dlatent;
+  }
   {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
     OMNIGS_TRY(launch_preprocess_backward_views(pb, a->viewmatrices, a->campos, vt, bs.dopacity, gr->dL_dopacity, s),
@@ -980,7 +987,7 @@ int omnigs_lonlat_batch_backward(const omnigs_lonlat_batch_args* a, const omnigs
     StageTimer tm(OMNIGS_STAGE_DECODER_BACKWARD, s);
     OMNIGS_TRY(launch_mlp_backward(P, a->means3D, a->campos, g.radii, a->latents, a->mlp_packed, bs.dcolor,
                                    gr->dL_dmean3D, gr->dL_dlatent, gr->dL_dmlp, static_cast<float*>(gr->mlp_workspace),
-                                   s, a->V),
+                                   s, a->V, true),
                "batch decoder backward");
   }
   return OMNIGS_OK;

@@ -881,7 +881,7 @@ hipError_t launch_mlp_backward(int P, const float* means3D, const float* campos,
   MlpTrainArgs a{};
   a.P = P;
   a.V = V;
-  if (listed && V == 1) {
+  if (listed) {
     a.act_list = mlp_act_list(ws, P);
     a.act_count = mlp_act_count(ws, P);
   }

@@ -212,20 +212,38 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
                                                                const float* __restrict__ camposv, ViewTable vt,
                                                                GeomState g, int* __restrict__ radii_out) {
   __shared__ __attribute__((aligned(16))) char s_w[kMlp ? kMlpFwdBytes : 16];
-  if constexpr (kMlp) {
-    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_packed);
-    for (int i = threadIdx.x; i < kMlpFwdBytes / 16; i += blockDim.x) reinterpret_cast<uint4*>(s_w)[i] = src[i];
-    __syncthreads();
-  }
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in_range = idx < a.P;
   const bool wave_has = (idx & ~63) < a.P;  // MLP: whole waves evaluate the decoder
+  // MLP: weights and the wave's latents (the same for every view) requested
+  // first, as in preprocess_kernel; the first view's block scan publishes
+  // the weights in LDS before any evaluation
+  constexpr int kWn = kMlpFwdBytes / 16;
+  uint4 w0{}, w1{}, w2{}, w3{};
+  bf16x8 lat[kMlp ? 4 : 1];
+  if constexpr (kMlp) {
+    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_packed);
+    const int t = threadIdx.x;
+    w0 = src[t];
+    w1 = src[t + 256];
+    w2 = src[t + 512];
+    w3 = src[min(t + 768, kWn - 1)];
+    mlp_wave_latents(a.latents, a.P, (size_t)(idx & ~63), lat);
+  }
   GaussGeo gg{};
   float3 c[16];
   const int nc = (kMlp || a.colors_precomp) ? 0 : min(a.M, sh_coeffs(a.D));
   if (in_range) {
     gg = load_gauss(a, idx);
     if (!kMlp && !a.colors_precomp) load_sh(a.shs, a.M, idx, nc, c);
+  }
+  if constexpr (kMlp) {
+    uint4* dst = reinterpret_cast<uint4*>(s_w);
+    const int t = threadIdx.x;
+    dst[t] = w0;
+    dst[t + 256] = w1;
+    dst[t + 512] = w2;
+    if (t + 768 < kWn) dst[t + 768] = w3;
   }
   // gradient slots of the block: view by view, the block's Gaussians in index
   // order within each view (every thread of the block takes part in the scans)
@@ -256,8 +274,6 @@ __global__ __launch_bounds__(256) void preprocess_views_kernel(PreprocessArgs a,
         const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
         if (len > 0.f) dir = make_float3(d.x / len, d.y / len, d.z / len);
       }
-      bf16x8 lat[4];
-      mlp_wave_latents(a.latents, a.P, (size_t)(idx & ~63), lat);
       col = mlp_colour_wave(s_w, lat, dir);
       if (!live) continue;
       g.clamped[item] = 0;

@@ -473,9 +473,14 @@ __global__ __launch_bounds__(256) void preprocess_backward_views_kernel(Preproce
 #pragma unroll
   for (int k = 0; k < 48; ++k) fsum[k] = 0.f;
   float dop = 0.f;
+  bool act = false;  // a colour gradient in any view: the decoder trains this Gaussian
   for (int vi = 0; vi < vt.V; ++vi) {
     const size_t item = (size_t)vi * a.P + idx;
     dop += dopacity_items[item];
+    // read with the radius, before any branch, so the loads share one wait
+    // (the item's dL/dcolor is written, zero when it was not rendered)
+    const float3 dci = make_float3(a.dL_dcolor[3 * item], a.dL_dcolor[3 * item + 1], a.dL_dcolor[3 * item + 2]);
+    act |= (dci.x != 0.f) | (dci.y != 0.f) | (dci.z != 0.f);
     if (!(a.radii[item] > 0)) continue;
     const View v = load_view(viewmats + 16 * vi);
     float dc[6];
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(256) void preprocess_backward_views_kernel(Preproce
       float f[48];
       const float3 dsv = sh_backward(
           a.D, nc, c, p, make_float3(camposv[3 * vi], camposv[3 * vi + 1], camposv[3 * vi + 2]), a.clamped[item],
-          make_float3(a.dL_dcolor[3 * item], a.dL_dcolor[3 * item + 1], a.dL_dcolor[3 * item + 2]), f);
+          dci, f);
 #pragma unroll
       for (int k = 0; k < 48; ++k) fsum[k] += f[k];
       if (a.D > 0) {
@@ -500,6 +505,20 @@ __global__ __launch_bounds__(256) void preprocess_backward_views_kernel(Preproce
         dmean.y += dsv.y;
         dmean.z += dsv.z;
       }
+    }
+  }
+  if (a.act_list) {
+    // The decoder's work list: the wave's Gaussians with a colour gradient in
+    // any view, in index order (one ballot); the others' latent gradients are
+    // zero.  Lanes past P left above, so lane 0 holds a valid Gaussian.
+    const int lane = threadIdx.x & 63;
+    const uint64_t bal = __ballot(act);
+    const size_t wave_base = (size_t)idx - (size_t)lane;
+    if (act) a.act_list[wave_base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)idx;
+    if (lane == 0) a.act_count[wave_base / kWave] = (uint32_t)__popcll(bal);
+    if (!act) {
+      float4* dl = reinterpret_cast<float4*>(a.dL_dlatent + (size_t)idx * 16);
+      dl[0] = dl[1] = dl[2] = dl[3] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
   a.dL_dmean3D[3 * idx] = dmean.x;
