@@ -143,6 +143,14 @@ typedef struct omnigs_lonlat_backward_args {
    * identifies.  Required when R > 0: a size that matches no layout for R
    * (0 included) is OMNIGS_ERR_INVALID_ARGUMENT. */
   size_t binning_bytes;
+  /* The size in bytes the forward's last call of the image buffer's resize
+   * callback asked for.  Read by omnigs_lonlat_backward_checkpointed only:
+   * omnigs_lonlat_forward_checkpointed keeps the render's per-instance blend
+   * bytes at the image buffer's end, and a size that covers them lets the
+   * banded backward evaluate only the pairs the forward blended.  0 (or the
+   * plain forward's size) selects the pairs by each ellipse's reach instead;
+   * the gradients agree either way. */
+  size_t image_bytes;
 } omnigs_lonlat_backward_args;
 
 /* Forward: preprocess, binning, render.  On success *num_rendered = R. */
@@ -151,11 +159,14 @@ int omnigs_lonlat_forward(const omnigs_lonlat_forward_args* args, omnigs_buffer 
                           int* num_rendered);
 
 /* Forward for the checkpointed backward: as omnigs_lonlat_forward (same
- * image, radii, R and image buffer), but the binning buffer is carved without
- * the backward's per-instance gradient records -- 17 of its 53 bytes per
+ * image, radii and R), but the binning buffer is carved without the
+ * backward's per-instance gradient records -- 17 of its 53 bytes per
  * instance are allocated -- so the forward's peak HBM drops with the
- * backward's.  Its geometry and binning buffers serve no stored backward:
- * pass the image buffer to omnigs_lonlat_backward_checkpointed.  Has no
+ * backward's.  The image buffer grows by one byte per instance of the
+ * binning capacity (plus 256), which holds the render's record of which
+ * waves blended each instance.  Its geometry and binning buffers serve no
+ * stored backward: pass the image buffer (and its size, image_bytes) to
+ * omnigs_lonlat_backward_checkpointed.  Has no
  * counterpart in the reference, whose autograd wrapper keeps all three
  * buffers for its backward (src/gaussian_rasterizer.cpp:85-98); it serves
  * memory-bound training callers. */
@@ -191,9 +202,9 @@ int omnigs_lonlat_backward(const omnigs_lonlat_backward_args* args, void* stream
  *   fwd: the forward's arguments (out_color unused; radii, if given, is
  *        rewritten with the forward's values).
  *   bwd: as for omnigs_lonlat_backward with R = the forward's R and
- *        image_buffer = the forward's kept image buffer (its per-pixel state
- *        is read; its per-tile ranges are overwritten, so afterwards it
- *        serves further checkpointed backwards only); geom_buffer and
+ *        image_buffer = the forward's kept image buffer (read only: its
+ *        per-pixel state, per-tile ranges and, when image_bytes covers them,
+ *        the checkpointed forward's blend bytes); geom_buffer and
  *        binning_buffer are unused (the callbacks grow fresh ones).
  * The outputs equal omnigs_lonlat_backward's up to the order in which a
  * Gaussian's per-tile terms are summed across bands.  The recomputed R must

@@ -77,7 +77,7 @@ class BackwardArgs(ctypes.Structure):
         ("dpx_dt", ctypes.c_void_p), ("dpy_dt", ctypes.c_void_p), ("seam_wrap", ctypes.c_int),
         ("mlp_packed", ctypes.c_void_p), ("latents", ctypes.c_void_p), ("dL_dlatent", ctypes.c_void_p),
         ("dL_dmlp", ctypes.c_void_p), ("mlp_workspace", ctypes.c_void_p), ("mlp_workspace_bytes", ctypes.c_size_t),
-        ("binning_bytes", ctypes.c_size_t),
+        ("binning_bytes", ctypes.c_size_t), ("image_bytes", ctypes.c_size_t),
     ]
 
 
@@ -307,7 +307,9 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
     """Runs omnigs_lonlat_backward; returns a dict of gradient tensors (plus
     dL_dlatent [P,16] and dL_dmlp [param_count] when mlp = (packed, latents)).
     checkpointed: {"opacities": [P] (activated), "band_instances": budget (0 =
-    the library's default)} runs omnigs_lonlat_backward_checkpointed instead,
+    the library's default), "blend": False to withhold the image buffer's size,
+    so that the bands select pairs by ellipse reach instead of the forward's
+    blend bytes} runs omnigs_lonlat_backward_checkpointed instead,
     which needs only the forward's image buffer (buffers[2]); the dict then
     also holds "_bands" (bands run) and "_buffers" (the geometry and binning
     buffers it grew)."""
@@ -352,7 +354,8 @@ def backward(*, R, background, means3D, radii, viewmatrix, campos, dL_dpix, buff
                      _ptr(mlp[0]) if mlp else None, _ptr(mlp[1]) if mlp else None, _ptr(g.get("dL_dlatent")),
                      _ptr(g.get("dL_dmlp")), mlp_workspace.data_ptr() if mlp else None,
                      mlp_workspace.numel() if mlp else 0,
-                     binning.requested if bin_ptr is not None else 0)
+                     binning.requested if bin_ptr is not None else 0,
+                     image.requested if checkpointed is None or checkpointed.get("blend", True) else 0)
     if checkpointed is not None:
         fa = ForwardArgs(P, degree, M, W, H, _ptr(background), _ptr(means3D), _ptr(shs), _ptr(colors_precomp),
                          _ptr(checkpointed["opacities"]), _ptr(scales), float(scale_modifier), _ptr(rotations),

@@ -274,16 +274,30 @@ hipEvent_t readback_event() {
 // n is the exact instance count R.  Otherwise n is a capacity guess and the
 // kernels read the live count {R, overflow flag} at R_dev, stopping there
 // (their grids cover the capacity).
-int bin_and_render(const omnigs_lonlat_forward_args* a, omnigs_buffer binning_buffer, const GeomState& g,
-                   const ImgState& img, const TileGrid& grid, uint32_t n, const uint32_t* R_dev, hipStream_t s,
-                   bool render, bool records) {
+int bin_and_render(const omnigs_lonlat_forward_args* a, omnigs_buffer binning_buffer, omnigs_buffer image_buffer,
+                   const GeomState& g, ImgState& img, const TileGrid& grid, uint32_t n, const uint32_t* R_dev,
+                   hipStream_t s, bool render, bool records) {
   const uint32_t C = bin_capacity(n);
   char* bbuf = grow(binning_buffer, carve_bin(nullptr, C, records).bytes);
   if (!bbuf) {
     std::snprintf(g_last_error, sizeof(g_last_error), "binning buffer allocation failed");
     return OMNIGS_ERR_ALLOC;
   }
-  const BinState b = carve_bin(bbuf, C, records);
+  BinState b = carve_bin(bbuf, C, records);
+  if (render && !records) {
+    // checkpointed forward: the blend bytes go to the image buffer's tail,
+    // which the caller keeps for the backward (the image state is carved
+    // again: the buffer may have moved)
+    const int W = a->width, H = a->height;
+    char* ibuf = grow(image_buffer, img_blend_bytes(W, H, C));
+    if (!ibuf) {
+      std::snprintf(g_last_error, sizeof(g_last_error), "image buffer allocation failed");
+      return OMNIGS_ERR_ALLOC;
+    }
+    img = carve_img(ibuf, W, H);
+    b.blend_ok = reinterpret_cast<uint32_t*>(ibuf + img_blend_offset(W, H));
+    b.blend = reinterpret_cast<uint8_t*>(ibuf + img_blend_offset(W, H) + kAlign);
+  }
   const uint32_t grid_n = R_dev ? C : n;  // instances the grids cover
   const int tile_bits = bit_width_u32((uint32_t)grid.tiles());
   {
@@ -382,7 +396,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
     return OMNIGS_ERR_ALLOC;
   }
   const GeomState g = carve_geom(gbuf, P);
-  const ImgState img = carve_img(ibuf, W, H);
+  ImgState img = carve_img(ibuf, W, H);
   int* radii = a->radii ? a->radii : g.radii;
   {
     const int st = preprocess_and_sort(a, g, radii, s);
@@ -430,7 +444,8 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   const bool same_shape = last.P == a->P && last.W == a->width && last.H == a->height;
   const uint32_t guess = async && same_shape && last.R ? bin_capacity(last.R + last.R / 16u) : 0u;
   if (guess > 0) {
-    const int st = bin_and_render(a, binning_buffer, g, img, grid, guess, total, s, render, records);
+    const int st =
+        bin_and_render(a, binning_buffer, image_buffer, g, img, grid, guess, total, s, render, records);
     if (st != OMNIGS_OK) return st;
   }
   uint32_t Rw[2] = {0u, 0u};
@@ -449,7 +464,7 @@ int forward_impl(const omnigs_lonlat_forward_args* a, omnigs_buffer geometry_buf
   const uint32_t R = Rw[0];
   const bool covered = guess > 0 && R > 0 && guess >= R && (uint64_t)guess <= 2ull * R;
   if (!covered) {
-    const int st = bin_and_render(a, binning_buffer, g, img, grid, R, nullptr, s, render, records);
+    const int st = bin_and_render(a, binning_buffer, image_buffer, g, img, grid, R, nullptr, s, render, records);
     if (st != OMNIGS_OK) return st;
   }
   last = LastCall{R, a->P, a->width, a->height};
@@ -608,8 +623,9 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
   const size_t rect_bytes = align_up((size_t)P * sizeof(uint2));
   const size_t part_bytes = align_up((scan_blocks((size_t)P) + 2) * sizeof(uint32_t));
   const size_t diff_bytes = align_up(((size_t)grid.gy + 1) * sizeof(uint32_t));
-  char* gbuf = grow(geometry_buffer,
-                    geom_bytes + rect_bytes + part_bytes + diff_bytes + align_up((size_t)P * sizeof(uint32_t)));
+  const size_t slot_bytes = align_up((size_t)P * sizeof(uint32_t));
+  char* gbuf = grow(geometry_buffer, geom_bytes + rect_bytes + part_bytes + diff_bytes + slot_bytes +
+                                         align_up((size_t)grid.tiles() * sizeof(uint2)));
   if (!gbuf) {
     std::snprintf(g_last_error, sizeof(g_last_error), "geometry buffer allocation failed");
     return OMNIGS_ERR_ALLOC;
@@ -626,9 +642,23 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
   uint32_t* part_count = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes);
   uint32_t* row_diff = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes + part_bytes);
   uint32_t* slot_words = reinterpret_cast<uint32_t*>(gbuf + geom_bytes + rect_bytes + part_bytes + diff_bytes);
+  // a band's tile ranges (the image buffer keeps the forward's, through
+  // which the forward's blend bytes are read)
+  uint2* band_ranges = reinterpret_cast<uint2*>(gbuf + geom_bytes + rect_bytes + part_bytes + diff_bytes + slot_bytes);
   const uint32_t* band_total = part_area + scan_blocks((size_t)P);    // {R_b, overflow flag}
   const uint32_t* band_owners = part_count + scan_blocks((size_t)P);  // owners with an instance
   const ImgState img = carve_img(a->image_buffer, W, H);
+  // The forward's blend bytes, when the image buffer is a checkpointed
+  // forward's (its size covers them): each band's render backward then
+  // evaluates only the (wave, instance) pairs the forward blended, as the
+  // stored backward does (a band's list of a tile is the forward's list, so
+  // position p there is position p of the forward's).  Else the quadrant
+  // masks pick the pairs.
+  const bool fwd_blend = a->R > 0 && a->image_bytes >= img_blend_bytes(W, H, (uint32_t)a->R);
+  const uint32_t* blend_ok = fwd_blend ? reinterpret_cast<const uint32_t*>(a->image_buffer + img_blend_offset(W, H))
+                                       : nullptr;
+  const uint8_t* blend = fwd_blend ? reinterpret_cast<const uint8_t*>(a->image_buffer + img_blend_offset(W, H) + kAlign)
+                                   : nullptr;
   {
     const int st = preprocess_and_sort(f, g, f->radii ? f->radii : g.radii, s);
     if (st != OMNIGS_OK) return st;
@@ -708,7 +738,7 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
       // also stops at the device total
       StageTimer tm(OMNIGS_STAGE_DUPLICATE, s);
       OMNIGS_TRY(launch_duplicate(P, band.n, band_total, order_c, offsets_c, rect_c, grid, b.tile_key, b.gid,
-                                  reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, img.ranges,
+                                  reinterpret_cast<uint32_t*>(b.grad_opacity), b.sort_counts, tile_bits, band_ranges,
                                   b.blend_ok, s, band_owners),
                  "band duplicate");
     }
@@ -721,13 +751,14 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
     }
     {
       StageTimer tm(OMNIGS_STAGE_RANGES, s);
-      OMNIGS_TRY(launch_tile_ranges(ts.keys, band.n, img.ranges, grid.tiles(), s, nullptr, true), "band ranges");
+      OMNIGS_TRY(launch_tile_ranges(ts.keys, band.n, band_ranges, grid.tiles(), s, nullptr, true), "band ranges");
     }
     {
       StageTimer tm(OMNIGS_STAGE_RENDER_BACKWARD, s);
-      OMNIGS_TRY(launch_render_backward(grid, img.ranges, ts.vals, W, H, a->background, g.rec, img.final_T,
+      OMNIGS_TRY(launch_render_backward(grid, band_ranges, ts.vals, W, H, a->background, g.rec, img.final_T,
                                         img.n_contrib, a->dL_dpix, nullptr, band.n, b.grad_rec, b.grad_opacity,
-                                        nullptr, nullptr, s, band.y0, band.y1, slot_words),
+                                        blend, blend_ok, s, band.y0, band.y1, slot_words,
+                                        fwd_blend ? img.ranges : nullptr),
                  "band render backward");
     }
     {

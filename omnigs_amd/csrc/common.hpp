@@ -414,6 +414,15 @@ inline ImgState carve_img(char* base, int W, int H) {
   return s;
 }
 
+// The checkpointed forward keeps the render's per-instance blend bytes in
+// the image buffer, behind the per-pixel and per-tile state, since its
+// backward has no forward binning buffer: a kAlign-byte header holding the
+// blend_ok word, then one byte per instance of the binning capacity C.  The
+// backward reads them through the forward's tile ranges, which stay in the
+// image buffer.
+inline size_t img_blend_offset(int W, int H) { return carve_img(nullptr, W, H).bytes; }
+inline size_t img_blend_bytes(int W, int H, uint32_t C) { return img_blend_offset(W, H) + kAlign + align_up(C); }
+
 // Number of 8-bit passes for a key range of `bits` bits, and whether the
 // sorted result ends in the *_alt arrays (odd pass count).
 inline int radix_passes(int bits) { return (bits + kRadixBits - 1) / kRadixBits; }

@@ -164,12 +164,15 @@ hipError_t launch_render_forward(TileGrid grid, const uint2* ranges, const uint3
 // [band_y0, band_y1) (band_y1 < 0: to the last row): the tile rows rendered;
 // the lists and slots then hold the rect rows from band_y0 on.  slot_words
 // (optional): each Gaussian's first slot, in place of slot_base plus the
-// record's slot word.
+// record's slot word.  blend_ranges (optional): the tile ranges of the list
+// the blend bytes were recorded against (the forward's, when `ranges` are a
+// band's of the same per-tile lists); else `ranges`.
 hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint32_t* point_list, int W, int H,
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
                                   float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s,
-                                  int band_y0 = 0, int band_y1 = -1, const uint32_t* slot_words = nullptr);
+                                  int band_y0 = 0, int band_y1 = -1, const uint32_t* slot_words = nullptr,
+                                  const uint2* blend_ranges = nullptr);
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
                                const uint32_t* tiles_touched, const uint32_t* slot_local, const uint32_t* slot_base,
                                const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,

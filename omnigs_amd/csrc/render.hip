@@ -337,11 +337,13 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dpix, const uint32_t* __restrict__ slot_base,
     uint32_t nslots, float4* __restrict__ grad_rec, float* __restrict__ grad_opacity,
     const uint8_t* __restrict__ blend_in, const uint32_t* __restrict__ blend_ok, int band_y0,
-    const uint32_t* __restrict__ slot_words) {
+    const uint32_t* __restrict__ slot_words, const uint2* __restrict__ blend_ranges) {
   // band_y0 (single view): the grid covers the tile rows from band_y0 on, and
   // the lists and gradient slots hold only the rect rows from band_y0 on;
   // slot_words (a band's): each Gaussian's first slot, in place of slot_base
-  // plus the record's slot word
+  // plus the record's slot word; blend_ranges: the ranges of the list the
+  // blend bytes belong to (position p of a tile's list is entry
+  // blend_ranges[tile].x + p there)
   const TileLoc loc = locate<kMulti>(W_, H_, gx_, vt, band_y0 * gx_);
   const int W = loc.W, H = loc.H, gx = loc.gx;
   const float Wf = (float)W, invW = 1.0f / (float)W;
@@ -370,6 +372,7 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
   const bool inside = px < W && py < H;
   const float fx = (float)px, fy = (float)py;
   const uint2 range = ranges[loc.key];
+  const uint32_t blend_x = blend_ranges[loc.key].x;
   const size_t N = (size_t)W * H;
   const size_t pid = (size_t)py * W + px;
   final_Ts += loc.pix;
@@ -477,8 +480,8 @@ __global__ __launch_bounds__(kTilePixels) __attribute__((amdgpu_waves_per_eu(6, 
     while (q_len < kBatch && scanned < n) {
       const int k = scanned + 2 * tid;  // positions n - 1 - k and n - 2 - k
       uint32_t b0 = 0u, b1 = 0u;
-      if (k < n) b0 = use_blend ? blend_in[range.x + (n - 1 - k)] & 0xFu : 0xFu;
-      if (k + 1 < n) b1 = use_blend ? blend_in[range.x + (n - 2 - k)] & 0xFu : 0xFu;
+      if (k < n) b0 = use_blend ? blend_in[blend_x + (n - 1 - k)] & 0xFu : 0xFu;
+      if (k + 1 < n) b1 = use_blend ? blend_in[blend_x + (n - 2 - k)] & 0xFu : 0xFu;
       uint32_t added;
       uint32_t at = block_exclusive_scan((b0 ? 1u : 0u) + (b1 ? 1u : 0u), &added);
       if (b0) s_queue[(q_head + q_len + (int)at++) & kRing] = ((uint32_t)(n - 1 - k) << 4) | b0;
@@ -726,7 +729,7 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
                                   const float* bg, const float4* rec, const float* final_T, const uint32_t* n_contrib,
                                   const float* dL_dpix, const uint32_t* slot_base, uint32_t nslots, float4* grad_rec,
                                   float* grad_opacity, const uint8_t* blend, const uint32_t* blend_ok, hipStream_t s,
-                                  int band_y0, int band_y1, const uint32_t* slot_words) {
+                                  int band_y0, int band_y1, const uint32_t* slot_words, const uint2* blend_ranges) {
   if (band_y1 < 0 || band_y1 > grid.gy) band_y1 = grid.gy;
   if (band_y0 < 0 || band_y0 > band_y1) return hipErrorInvalidValue;
   const int tiles = (band_y1 - band_y0) * grid.gx;
@@ -734,7 +737,8 @@ hipError_t launch_render_backward(TileGrid grid, const uint2* ranges, const uint
   ViewTable none{};
   return dispatch_shape<false, BwdK>(grid.bx, grid.by, grid.wrap != 0, (unsigned)tiles, s, grid.gx, ranges,
                                      point_list, W, H, none, bg, rec, final_T, n_contrib, dL_dpix, slot_base, nslots,
-                                     grad_rec, grad_opacity, blend, blend_ok, band_y0, slot_words);
+                                     grad_rec, grad_opacity, blend, blend_ok, band_y0, slot_words,
+                                     blend_ranges ? blend_ranges : ranges);
 }
 
 hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t* R_dev,
@@ -775,7 +779,7 @@ hipError_t launch_render_backward_views(const ViewTable& vt, const uint2* ranges
   if (tiles == 0) return hipSuccess;
   return dispatch_shape<true, BwdK>(vt.bx, vt.by, vt.wrap != 0, tiles, s, 0, ranges, point_list, 0, 0, vt, bg, rec,
                                     final_T, n_contrib, dL_dpix, slot_base, nslots, grad_rec, grad_opacity, blend,
-                                    blend_ok, 0, (const uint32_t*)nullptr);
+                                    blend_ok, 0, (const uint32_t*)nullptr, ranges);
 }
 
 }  // namespace omnigs

@@ -297,6 +297,9 @@ RasterizeGaussiansBackwardCheckpointedCUDA(const torch::Tensor& background, cons
     a.campos = f.campos;
     a.radii = f.radii;
     a.image_buffer = reinterpret_cast<char*>(ibuf.data_ptr());
+    // resize_ made the tensor the size the forward asked for: with the
+    // checkpointed forward's blend bytes at its end
+    a.image_bytes = (size_t)ibuf.numel();
     a.dL_dpix = dpix.ptr();
     a.dL_dmean2D = dL_dmeans2D.data_ptr<float>();
     a.dL_dconic = dL_dconic.data_ptr<float>();

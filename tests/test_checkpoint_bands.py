@@ -24,14 +24,14 @@ _GRADS = ("dL_dmeans2D", "dL_dconic", "dL_dopacity", "dL_dcolors", "dL_dmeans3D"
           "dL_dscales", "dL_drotations")
 
 
-def _banded(ds, R, radii, image_buffer, dL, budget, wrap=False):
+def _banded(ds, R, radii, image_buffer, dL, budget, wrap=False, blend=True):
     from omnigs_amd import capi
     return capi.backward(R=R, background=ds["background"], means3D=ds["means3D"], radii=radii,
                          viewmatrix=ds["viewmatrix"], campos=ds["campos"], dL_dpix=dL,
                          buffers=(None, None, image_buffer), shs=ds.get("shs"), scales=ds.get("scales"),
                          rotations=ds.get("rotations"), degree=ds.get("degree", 3), seam_wrap=wrap,
                          checkpointed={"opacities": ds["opacities"].reshape(-1).contiguous(),
-                                       "band_instances": budget})
+                                       "band_instances": budget, "blend": blend})
 
 
 def _expected_bands(tile_keys: np.ndarray, gx: int, gy: int, budget: int, R: int) -> int:
@@ -177,10 +177,11 @@ def test_banded_with_decoder_colours(gpu):
 @pytest.mark.parametrize("wrap", [False, True])
 def test_forward_checkpointed_matches_forward(gpu, wrap):
     """omnigs_lonlat_forward_checkpointed: the same R, image, radii and
-    per-pixel state as omnigs_lonlat_forward, tile lists bit-exact, with a
-    binning buffer that leaves out the gradient records (17 of 53 bytes per
-    instance); its image buffer feeds the banded backward, which then equals
-    the stored backward of the plain forward."""
+    per-pixel state as omnigs_lonlat_forward, tile lists and blend bytes
+    bit-exact, with a binning buffer that leaves out the gradient records (17
+    of 53 bytes per instance); its image buffer feeds the banded backward,
+    which then equals the stored backward of the plain forward, with and
+    without the blend bytes."""
     import torch
 
     from omnigs_amd import capi
@@ -201,11 +202,21 @@ def test_forward_checkpointed_matches_forward(gpu, wrap):
     full = capi.load().omnigs_lonlat_binning_bytes(R)
     got_bytes = bufsc[1].requested
     assert got_bytes < 0.4 * full, (got_bytes, full)
+    # the render's blend bytes sit at the image buffer's end, equal to the
+    # plain forward's (which keeps them in its binning buffer)
+    off = capi.load().omnigs_lonlat_image_bytes(W, H) + 256
+    assert bufsc[2].requested >= off + R
+    tail = bufsc[2].tensor[off:off + R].cpu().numpy()
+    assert np.array_equal(tail & 0xF, st["blend"][:R] & 0xF)
     dL = torch.from_numpy(sc["dL_dpix"]).to(gpu)
     ref = capi.backward_scene(ds, R, radii, bufs, dL, seam_wrap=wrap)
-    got = _banded(ds, R, radii, bufsc[2], dL, 3000, wrap)
+    # bands on the forward's blend bytes (twice: the image buffer is only
+    # read), and on the ellipses' reach with the image size withheld
+    runs = [_banded(ds, R, radii, bufsc[2], dL, 3000, wrap), _banded(ds, R, radii, bufsc[2], dL, 3000, wrap),
+            _banded(ds, R, radii, bufsc[2], dL, 3000, wrap, blend=False)]
     torch.cuda.synchronize()
-    assert got["_bands"] > 4
-    for k in _GRADS:
-        a, b = ref[k], got[k]
-        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * float(a.abs().max())), (k, float((a - b).abs().max()))
+    for got in runs:
+        assert got["_bands"] > 4
+        for k in _GRADS:
+            a, b = ref[k], got[k]
+            assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * float(a.abs().max())), (k, float((a - b).abs().max()))
