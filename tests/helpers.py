@@ -81,7 +81,7 @@ def allclose_grad(a, b, rtol=1e-3, scale_atol=1e-4):
 # reference's operations accumulates, in units of u = 2^-24 (Higham, Sec. 3.3).
 # A GPU entry a passes against the FP32 oracle o when
 #     |a - o| <= GRAD_RTOL |o| + ROUNDING_K u e,
-# i.e. within the north-star 1e-3 relative, or within a few times the rounding
+# i.e. within the 1e-3 relative gradient bar, or within a few times the rounding
 # error both evaluations are entitled to.  ROUNDING_K = 4: the oracle's own
 # FP32 error stays below 0.25 u e on the test scenes, and the GPU evaluates
 # the same functions in another order (tree sums, FMAs, rcp).
