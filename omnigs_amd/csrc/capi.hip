@@ -710,8 +710,10 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
     b = carve_bin(bbuf, C);
   }
 
-  // The 2D gradient outputs start at zero; every band adds its records' sums.
-  {
+  // The 2D gradient outputs: a Gaussian's first band writes its sums and
+  // later bands add theirs; the first band also writes the zeros of the
+  // Gaussians without instances.  No band: all zeros.
+  if (bands.empty()) {
     StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
     OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dmean2D), 3 * (size_t)P, 0u, s), "zero dmean2D");
     OMNIGS_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a->dL_dconic), 4 * (size_t)P, 0u, s), "zero dconic");
@@ -764,7 +766,8 @@ int omnigs_lonlat_backward_checkpointed(const omnigs_lonlat_forward_args* f, con
     {
       StageTimer tm(OMNIGS_STAGE_PREPROCESS_BACKWARD, s);
       OMNIGS_TRY(launch_band_gather(P, g.dup_rect, (uint32_t)band.y0, (uint32_t)band.y1, slot_words, band.n, b.grad_rec,
-                                    b.grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor, s),
+                                    b.grad_opacity, a->dL_dmean2D, a->dL_dconic, a->dL_dopacity, a->dL_dcolor,
+                                    &band == &bands.front(), s),
                  "band gather");
     }
   }

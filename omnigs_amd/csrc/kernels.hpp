@@ -180,10 +180,12 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
 // One band's records added to the per-Gaussian sums (the four arrays in the
 // layouts launch_gather_grads writes): Gaussian i owns the slots from
 // slot_words[i] on (launch_band_owners writes it), as many as its rect
-// (dup_rect) has tiles in the rows [by0, by1).
+// (dup_rect) has tiles in the rows [by0, by1).  The band holding a rect's
+// first row writes the sums (bands run top to bottom); zero_empty (the first
+// band) also writes the zeros of Gaussians with an empty rect.
 hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const uint32_t* slot_words,
                               uint32_t nslots, const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
-                              float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s);
+                              float* dL_dconic, float* dL_dopacity, float* dL_dcolors, bool zero_empty, hipStream_t s);
 
 // Batched render over every tile of every view of vt; pixel arrays are the
 // views' slices laid out one after the other (colours [3, H_v, W_v] per view).

@@ -667,17 +667,23 @@ __global__ __launch_bounds__(256) void band_gather_kernel(const uint2* __restric
                                                           const float* __restrict__ grad_opacity,
                                                           float* __restrict__ dL_dmean2D, float* __restrict__ dL_dconic,
                                                           float* __restrict__ dL_dopacity,
-                                                          float* __restrict__ dL_dcolors) {
+                                                          float* __restrict__ dL_dcolors, bool zero_empty) {
+  // The band holding the first row of a Gaussian's rect writes its sums;
+  // later bands add theirs to them.  zero_empty (the first band launched):
+  // Gaussians with no instance at all get their zeros here, so no band
+  // reads an output that was not written.
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t cnt = i < P ? dup_rect_area(band_dup_rect(dup_rect[i], by0, by1)) : 0u;
-  if (__ballot(cnt != 0u) == 0ull) return;  // wave-uniform
+  const uint2 full = i < P ? dup_rect[i] : make_uint2(0u, 1u << 16);
+  const uint32_t cnt = dup_rect_area(band_dup_rect(full, by0, by1));
+  const bool zero = zero_empty && i < P && dup_rect_area(full) == 0u;
+  if (__ballot(cnt != 0u || zero) == 0ull) return;  // wave-uniform
   const uint32_t base = cnt ? slot_words[i] : 0u;
   float acc[kGradTerms];
 #pragma unroll
   for (int q = 0; q < kGradTerms; ++q) acc[q] = 0.f;
-  if (cnt) load_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+  if (cnt && (full.y & 0xFFFFu) < by0) load_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
   gather_range<false>(grad_rec, grad_opacity, nslots, base, cnt, acc);
-  if (cnt) store_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+  if (cnt || zero) store_gathered((size_t)i, acc, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
 }
 
 namespace {
@@ -754,10 +760,11 @@ hipError_t launch_gather_grads(int items, int P, uint32_t nslots, const uint32_t
 
 hipError_t launch_band_gather(int P, const uint2* dup_rect, uint32_t by0, uint32_t by1, const uint32_t* slot_words,
                               uint32_t nslots, const float4* grad_rec, const float* grad_opacity, float* dL_dmean2D,
-                              float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s) {
+                              float* dL_dconic, float* dL_dopacity, float* dL_dcolors, bool zero_empty, hipStream_t s) {
   if (P == 0 || nslots == 0) return hipSuccess;
   hipLaunchKernelGGL(band_gather_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, dup_rect, by0, by1, P,
-                     slot_words, nslots, grad_rec, grad_opacity, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+                     slot_words, nslots, grad_rec, grad_opacity, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors,
+                     zero_empty);
   return hipGetLastError();
 }
 
