@@ -461,7 +461,9 @@ typedef struct omnigs_lonlat_state_view {
   const uint32_t* n_contrib;     /* [H,W] */
   const uint32_t* ranges;        /* [tiles,2] */
   /* binning buffer, per sorted instance: bit w set when wave w of its tile (8x8 quadrant w, row-major
-     over the tile's quadrants) blended it in the render forward; valid once a render forward ran */
+     over the tile's quadrants) blended it in the render forward; valid once omnigs_lonlat_forward ran
+     (omnigs_lonlat_forward_checkpointed writes these bytes at its image buffer's end instead, after
+     omnigs_lonlat_image_bytes(width, height) + 256 bytes) */
   const uint8_t* blend;
 } omnigs_lonlat_state_view;
 
